@@ -1,0 +1,26 @@
+"""hellblast_amd - MI355X-native HEALPix spherical-harmonic transforms.
+
+Drop-in for the SHT hot path of hellblast: ``map2alm_gpu`` (sht.py),
+``alm2map_gpu`` (isht.py), ``anafast_ring`` (anafast.py), the ``hp_compat``
+SHT wrappers and ``HealpixRingTables``, running hand-written gfx950 HIP
+kernels (libhbx.so, C ABI in include/hbx.h) on PyTorch-ROCm tensors.
+
+Package aliases as in the reference (``__init__.py:125-127``):
+``map2alm``, ``alm2map``, ``anafast``.
+"""
+from .ring_tables import HealpixRingTables
+from .sht import map2alm_gpu
+from .isht import alm2map_gpu
+from .anafast import anafast_ring
+from .alm_ops import alm2cl, almxfl, synalm, alm_index, getidx, getsize, to_reference_layout, from_reference_layout
+from . import hp_compat
+
+map2alm = map2alm_gpu
+alm2map = alm2map_gpu
+anafast = anafast_ring
+
+__all__ = [
+    "HealpixRingTables", "map2alm_gpu", "alm2map_gpu", "anafast_ring", "alm2cl", "almxfl", "synalm",
+    "alm_index", "getidx", "getsize", "to_reference_layout", "from_reference_layout", "hp_compat",
+    "map2alm", "alm2map", "anafast",
+]

@@ -1,0 +1,294 @@
+// C ABI of libhbx.so: argument validation, plan construction and the
+// execute entry points.  Validation is host-only so it can be exercised
+// without a GPU; execute functions only enqueue kernels on the given stream.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hbx_internal.h"
+
+namespace hbx {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_error(hipError_t e, const char *what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return HBX_EDEVICE;
+}
+
+int64_t phase_elems_per_map(const Plan *p) { return (int64_t)p->nmb * p->nrings * MB; }
+int64_t scaled_elems_per_map(const Plan *p) { return p->nrow; }
+
+static uint64_t align256(uint64_t v) { return (v + 255) & ~uint64_t(255); }
+
+static uint64_t ws_bytes(const Plan *p, int nmaps) {
+    const uint64_t csz = p->precision == HBX_F64 ? 16 : 8;
+    return align256((uint64_t)nmaps * phase_elems_per_map(p) * csz) +
+           align256((uint64_t)nmaps * scaled_elems_per_map(p) * csz);
+}
+
+static int validate_geom(const hbx_geom *g, int32_t lmax, int32_t mmax) {
+    if (!g) return set_error(HBX_EINVAL, "geometry is NULL");
+    if (g->nrings < 1) return set_error(HBX_EINVAL, "nrings must be >= 1");
+    if (!g->ring_start || !g->nphi || !g->cos_theta || !g->sin_theta)
+        return set_error(HBX_EINVAL, "ring_start, nphi, cos_theta and sin_theta are required");
+    if (lmax < 0) return set_error(HBX_EINVAL, "lmax must be >= 0");
+    if (mmax < 0 || mmax > lmax) return set_error(HBX_EINVAL, "mmax must satisfy 0 <= mmax <= lmax");
+    if (lmax > 32767) return set_error(HBX_EUNSUPPORTED, "lmax above 32767");
+    const int nr = g->nrings;
+    if (g->ring_start[0] < 0) return set_error(HBX_EINVAL, "ring_start[0] must be >= 0");
+    for (int r = 0; r < nr; ++r) {
+        const int n = g->nphi[r];
+        if (n < 2 || (n & 1))
+            return set_error(HBX_EINVAL, "nphi[" + std::to_string(r) + "] must be even and >= 2");
+        if (g->ring_start[r + 1] - g->ring_start[r] != (int64_t)n)
+            return set_error(HBX_EINVAL, "ring_start/nphi mismatch at ring " + std::to_string(r));
+        const double c = g->cos_theta[r], s = g->sin_theta[r];
+        if (!std::isfinite(c) || !std::isfinite(s) || s < 0.0 || std::fabs(c * c + s * s - 1.0) > 1e-10)
+            return set_error(HBX_EINVAL, "cos/sin theta invalid at ring " + std::to_string(r));
+        if (g->mcut && g->mcut[r] < 0) return set_error(HBX_EINVAL, "mcut must be >= 0");
+        if (g->weight && !std::isfinite(g->weight[r])) return set_error(HBX_EINVAL, "weight not finite");
+        if (g->phi0 && !std::isfinite(g->phi0[r])) return set_error(HBX_EINVAL, "phi0 not finite");
+        const int q = nr - 1 - r;
+        if (q > r) {
+            if (g->nphi[q] != n)
+                return set_error(HBX_EUNSUPPORTED, "rings " + std::to_string(r) + " and " + std::to_string(q) +
+                                                       " are not mirror images (nphi differs)");
+            if (std::fabs(g->cos_theta[q] + c) > 1e-12 || std::fabs(g->sin_theta[q] - s) > 1e-12)
+                return set_error(HBX_EUNSUPPORTED, "rings " + std::to_string(r) + " and " + std::to_string(q) +
+                                                       " are not mirror images (theta)");
+        } else if (q == r && std::fabs(c) > 1e-12) {
+            return set_error(HBX_EUNSUPPORTED, "odd middle ring must lie on the equator");
+        }
+    }
+    return HBX_OK;
+}
+
+template <typename X>
+static int upload(Plan *p, X **dst, const X *src, size_t n) {
+    hipError_t e = hipMalloc(dst, n * sizeof(X));
+    if (e != hipSuccess) return set_error(HBX_ENOMEM, "hipMalloc failed for plan geometry");
+    p->allocations.push_back(*dst);
+    p->device_bytes += n * sizeof(X);
+    HBX_HIP(hipMemcpy(*dst, src, n * sizeof(X), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static void destroy_plan(Plan *p) {
+    int cur = 0;
+    const bool have = hipGetDevice(&cur) == hipSuccess;
+    if (have && cur != p->device) (void)hipSetDevice(p->device);
+    for (void *ptr : p->allocations) (void)hipFree(ptr);
+    p->allocations.clear();
+    if (have && cur != p->device) (void)hipSetDevice(cur);
+}
+
+static int build_plan(Plan *p, const hbx_geom *g) {
+    const int nr = p->nrings;
+    std::vector<int32_t> nphi(g->nphi, g->nphi + nr);
+    std::vector<int32_t> mcut(nr);
+    std::vector<double> scale(nr);
+    bool any_phi0 = false;
+    for (int r = 0; r < nr; ++r) {
+        mcut[r] = g->mcut ? g->mcut[r] : g->nphi[r];
+        scale[r] = (g->weight ? g->weight[r] : 1.0) / (double)g->nphi[r];
+        if (g->phi0 && g->phi0[r] != 0.0) any_phi0 = true;
+    }
+    int rc;
+    if ((rc = upload(p, &p->d_ring_start, g->ring_start, nr + 1))) return rc;
+    if ((rc = upload(p, &p->d_nphi, nphi.data(), nr))) return rc;
+    if ((rc = upload(p, &p->d_mcut, mcut.data(), nr))) return rc;
+    if ((rc = upload(p, &p->d_scale_ana, scale.data(), nr))) return rc;
+    p->d_scale_adj = p->d_scale_ana;
+    if (any_phi0 && (rc = upload(p, &p->d_phi0, g->phi0, nr))) return rc;
+    const int np = p->npairs;
+    std::vector<int32_t> pn(np), ps(np);
+    std::vector<double> px(np), psin(np);
+    for (int i = 0; i < np; ++i) {
+        pn[i] = i;
+        ps[i] = nr - 1 - i;
+        px[i] = g->cos_theta[i];
+        psin[i] = g->sin_theta[i];
+    }
+    if ((rc = upload(p, &p->d_pair_n, pn.data(), np))) return rc;
+    if ((rc = upload(p, &p->d_pair_s, ps.data(), np))) return rc;
+    if ((rc = upload(p, &p->d_px, px.data(), np))) return rc;
+    if ((rc = plan_build_fft_tables(p, nphi))) return rc;
+    // switch-on tolerance: FP64 plans 2^-64, FP32 plans 2^-40 (relative
+    // contributions far below each format's rounding error)
+    const double tol = p->precision == HBX_F64 ? -64.0 : -40.0;
+    if ((rc = plan_build_recursion(p, psin, tol))) return rc;
+    return 0;
+}
+
+static int check_exec(const hbx_plan *plan, int32_t nmaps, const void *a, const void *b, void *ws,
+                      uint64_t wsb, int need_ws) {
+    if (!plan) return set_error(HBX_EINVAL, "plan is NULL");
+    if (nmaps < 0 || nmaps > 65535) return set_error(HBX_EINVAL, "nmaps must be in [0, 65535]");
+    if (nmaps == 0) return HBX_OK;
+    if (!a || !b) return set_error(HBX_EINVAL, "data pointer is NULL");
+    if (need_ws) {
+        const uint64_t need = ws_bytes(&plan->impl, nmaps);
+        if (!ws || wsb < need)
+            return set_error(HBX_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+    }
+    int dev = -1;
+    HBX_HIP(hipGetDevice(&dev));
+    if (dev != plan->impl.device)
+        return set_error(HBX_EPLAN, "plan was created on device " + std::to_string(plan->impl.device) +
+                                        " but device " + std::to_string(dev) + " is current");
+    return HBX_OK;
+}
+
+static int check_strides(const Plan *p, int32_t map_dtype, int64_t map_stride, int64_t alm_stride) {
+    if (map_dtype != HBX_DT_F64 && map_dtype != HBX_DT_F32) return set_error(HBX_EINVAL, "map_dtype");
+    if (map_stride < p->npix) return set_error(HBX_EINVAL, "map_stride smaller than npix");
+    if (alm_stride < p->nlm) return set_error(HBX_EINVAL, "alm_stride smaller than the alm count");
+    return HBX_OK;
+}
+
+}  // namespace hbx
+
+using namespace hbx;
+
+extern "C" {
+
+int hbx_abi_version(void) { return HBX_ABI_VERSION; }
+
+const char *hbx_last_error(void) { return g_err.c_str(); }
+
+int hbx_validate_geom(const hbx_geom *g, int32_t lmax, int32_t mmax) { return validate_geom(g, lmax, mmax); }
+
+int hbx_plan_create(const hbx_geom *g, int32_t lmax, int32_t mmax, int32_t spin, int32_t precision,
+                    uint32_t flags, hbx_plan **out) {
+    if (!out) return set_error(HBX_EINVAL, "out is NULL");
+    *out = nullptr;
+    int rc = validate_geom(g, lmax, mmax);
+    if (rc) return rc;
+    if (precision != HBX_F64 && precision != HBX_F32) return set_error(HBX_EINVAL, "precision must be 0 or 1");
+    if (spin != 0) return set_error(HBX_EUNSUPPORTED, "spin-weighted plans are not available");
+    if (flags != 0) return set_error(HBX_EINVAL, "flags must be 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_error(HBX_EDEVICE, "no HIP device");
+    auto *hp = new hbx_plan();
+    Plan *p = &hp->impl;
+    if (hipGetDevice(&p->device) != hipSuccess) {
+        delete hp;
+        return set_error(HBX_EDEVICE, "hipGetDevice failed");
+    }
+    p->nrings = g->nrings;
+    p->npairs = (g->nrings + 1) / 2;
+    p->lmax = lmax;
+    p->mmax = mmax;
+    p->nmb = (mmax + 1 + MB - 1) / MB;
+    p->spin = spin;
+    p->precision = precision;
+    p->npix = g->ring_start[g->nrings];
+    p->nlm = nalm(lmax, mmax);
+    rc = build_plan(p, g);
+    if (rc) {
+        const std::string msg = g_err;
+        destroy_plan(p);
+        delete hp;
+        return set_error(rc, msg);
+    }
+    *out = hp;
+    return HBX_OK;
+}
+
+int hbx_plan_destroy(hbx_plan *plan) {
+    if (!plan) return HBX_OK;
+    destroy_plan(&plan->impl);
+    delete plan;
+    return HBX_OK;
+}
+
+int hbx_plan_get_info(const hbx_plan *plan, hbx_plan_info *info) {
+    if (!plan || !info) return set_error(HBX_EINVAL, "NULL argument");
+    const Plan &p = plan->impl;
+    info->nrings = p.nrings;
+    info->npairs = p.npairs;
+    info->lmax = p.lmax;
+    info->mmax = p.mmax;
+    info->precision = p.precision;
+    info->spin = p.spin;
+    info->npix = p.npix;
+    info->nalm = p.nlm;
+    info->device_bytes = p.device_bytes;
+    return HBX_OK;
+}
+
+int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes) {
+    if (!plan || !bytes) return set_error(HBX_EINVAL, "NULL argument");
+    if (nmaps < 0) return set_error(HBX_EINVAL, "nmaps must be >= 0");
+    *bytes = ws_bytes(&plan->impl, nmaps);
+    return HBX_OK;
+}
+
+int hbx_map2alm(const hbx_plan *plan, int32_t nmaps, const void *maps, int32_t map_dtype, int64_t map_stride,
+                void *alm, int64_t alm_stride, void *ws, uint64_t ws_bytes_, void *stream) {
+    int rc = check_exec(plan, nmaps, maps, alm, ws, ws_bytes_, 1);
+    if (rc || nmaps == 0) return rc;
+    const Plan *p = &plan->impl;
+    if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, p->d_scale_ana, s))) return rc;
+    return launch_leg_analysis(p, nmaps, ws, alm, alm_stride, 0, s);
+}
+
+int hbx_alm2map_adjoint(const hbx_plan *plan, int32_t nmaps, const void *maps, int32_t map_dtype,
+                        int64_t map_stride, void *alm, int64_t alm_stride, void *ws, uint64_t ws_bytes_,
+                        void *stream) {
+    int rc = check_exec(plan, nmaps, maps, alm, ws, ws_bytes_, 1);
+    if (rc || nmaps == 0) return rc;
+    const Plan *p = &plan->impl;
+    if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, nullptr, s))) return rc;
+    return launch_leg_analysis(p, nmaps, ws, alm, alm_stride, 1, s);
+}
+
+static int synth(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t alm_stride, void *maps,
+                 int32_t map_dtype, int64_t map_stride, void *ws, uint64_t ws_bytes_, void *stream,
+                 int adjoint) {
+    int rc = check_exec(plan, nmaps, alm, maps, ws, ws_bytes_, 1);
+    if (rc || nmaps == 0) return rc;
+    const Plan *p = &plan->impl;
+    if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t csz = p->precision == HBX_F64 ? 16 : 8;
+    void *phase = ws;
+    void *scaled = static_cast<char *>(ws) + align256((uint64_t)nmaps * phase_elems_per_map(p) * csz);
+    if ((rc = launch_prescale(p, nmaps, alm, alm_stride, scaled, adjoint, s))) return rc;
+    if ((rc = launch_leg_synthesis(p, nmaps, scaled, phase, s))) return rc;
+    return launch_ring_synthesis(p, nmaps, phase, maps, map_dtype, map_stride, adjoint ? p->d_scale_adj : nullptr, s);
+}
+
+int hbx_alm2map(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t alm_stride, void *maps,
+                int32_t map_dtype, int64_t map_stride, void *ws, uint64_t ws_bytes_, void *stream) {
+    return synth(plan, nmaps, alm, alm_stride, maps, map_dtype, map_stride, ws, ws_bytes_, stream, 0);
+}
+
+int hbx_map2alm_adjoint(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t alm_stride, void *maps,
+                        int32_t map_dtype, int64_t map_stride, void *ws, uint64_t ws_bytes_, void *stream) {
+    return synth(plan, nmaps, alm, alm_stride, maps, map_dtype, map_stride, ws, ws_bytes_, stream, 1);
+}
+
+int hbx_alm2cl(int32_t lmax, int32_t mmax, int32_t nmaps, const void *alm, int32_t alm_dtype, int64_t alm_stride,
+               double *cl, int64_t cl_stride, void *stream) {
+    if (lmax < 0 || mmax < 0 || mmax > lmax) return set_error(HBX_EINVAL, "need 0 <= mmax <= lmax");
+    if (nmaps < 0 || nmaps > 65535) return set_error(HBX_EINVAL, "nmaps must be in [0, 65535]");
+    if (nmaps == 0) return HBX_OK;
+    if (!alm || !cl) return set_error(HBX_EINVAL, "data pointer is NULL");
+    if (alm_dtype != HBX_DT_F64 && alm_dtype != HBX_DT_F32) return set_error(HBX_EINVAL, "alm_dtype");
+    if (alm_stride < nalm(lmax, mmax) || cl_stride < lmax + 1) return set_error(HBX_EINVAL, "stride too small");
+    return launch_alm2cl(lmax, mmax, nmaps, alm, alm_dtype, alm_stride, cl, cl_stride, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,48 @@
+// Device-side helpers: a minimal complex type over double2/float2 and the
+// XCD-aware block-index mappings.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hbx {
+
+template <typename T> struct V2;
+template <> struct V2<double> { using type = double2; };
+template <> struct V2<float> { using type = float2; };
+template <typename T> using c2 = typename V2<T>::type;
+
+template <typename T> __device__ __forceinline__ c2<T> mk(T x, T y) { c2<T> r; r.x = x; r.y = y; return r; }
+template <typename T> __device__ __forceinline__ c2<T> cadd(c2<T> a, c2<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ __forceinline__ c2<T> csub(c2<T> a, c2<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T> __device__ __forceinline__ c2<T> cmul(c2<T> a, c2<T> b) {
+    return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T> __device__ __forceinline__ c2<T> cconj(c2<T> a) { return mk<T>(a.x, -a.y); }
+template <typename T> __device__ __forceinline__ c2<T> cscale(c2<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
+// a * (-i)
+template <typename T> __device__ __forceinline__ c2<T> cmul_mi(c2<T> a) { return mk<T>(a.y, -a.x); }
+// a * (+i)
+template <typename T> __device__ __forceinline__ c2<T> cmul_pi(c2<T> a) { return mk<T>(-a.y, a.x); }
+
+__device__ __forceinline__ void sincospi_t(double a, double *s, double *c) { sincospi(a, s, c); }
+__device__ __forceinline__ void sincospi_t(float a, float *s, float *c) { sincospif(a, s, c); }
+
+// exp(i*pi*a) for a given as a rational q/d (exact reduction done by caller)
+template <typename T> __device__ __forceinline__ c2<T> expipi(T a) {
+    T s, c; sincospi_t(a, &s, &c); return mk<T>(c, s);
+}
+
+// Wave-uniform copy of a value (lets the compiler keep it in an SGPR).
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// XCD-aware mapping of a 1-D block id onto (group, item) so that the `per`
+// consecutive items of one group run on the same XCD back to back.  MI355X
+// dispatches consecutive workgroups round-robin over its 8 XCDs.
+//   g = 8*k + xcd ;  group = 8*(k / per) + xcd ;  item = k % per
+__device__ __forceinline__ void xcd_map(int g, int per, int *group, int *item) {
+    int xcd = g & 7, k = g >> 3;
+    *group = 8 * (k / per) + xcd;
+    *item = k % per;
+}
+
+}  // namespace hbx

@@ -1,0 +1,104 @@
+// Internal declarations shared by the libhbx.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hbx.h"
+
+namespace hbx {
+
+// ---------------------------------------------------------------------------
+// error reporting: thread-local message + status code
+// ---------------------------------------------------------------------------
+int set_error(int code, const std::string &msg);
+int hip_error(hipError_t e, const char *what);
+
+#define HBX_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t _e = (call);                                              \
+        if (_e != hipSuccess) return ::hbx::hip_error(_e, #call);            \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// sizes
+// ---------------------------------------------------------------------------
+constexpr int MB = 8;          // m values per m-block of the phase buffer
+constexpr int CHUNK = 8;       // l steps per recursion chunk
+
+inline int64_t nalm(int lmax, int mmax) {
+    return (int64_t)(mmax + 1) * (lmax + 1) - (int64_t)mmax * (mmax + 1) / 2;
+}
+// healpy m-major index of (l, m)
+__host__ __device__ inline int64_t alm_index(int l, int m, int lmax) {
+    return (int64_t)m * (2 * lmax + 1 - m) / 2 + l;
+}
+inline int ilog2(int64_t v) { int r = 0; while ((int64_t(1) << r) < v) ++r; return r; }
+
+// One FFT size class: rings whose inner power-of-two FFT has size M.
+struct FftClass {
+    int M = 0, log2M = 0, threads = 0, nb = 0;
+    int nrings = 0;
+    int32_t *d_rings = nullptr;   // ring indices in this class
+    size_t lds_bytes64 = 0, lds_bytes32 = 0;
+};
+
+struct Plan {
+    int device = 0;
+    int nrings = 0, npairs = 0, lmax = 0, mmax = 0, nmb = 0, spin = 0, precision = 0;
+    int64_t npix = 0, nlm = 0, nrow = 0;   // nrow: padded recursion-row storage
+    uint64_t device_bytes = 0;
+    // geometry (device)
+    int64_t *d_ring_start = nullptr;
+    int32_t *d_nphi = nullptr;
+    int32_t *d_mcut = nullptr;
+    double *d_phi0 = nullptr;              // nullptr when every phi0 is 0
+    double *d_scale_ana = nullptr;         // weight_r / nphi_r
+    double *d_scale_adj = nullptr;         // weight_r / nphi_r (map2alm adjoint: omega_r)
+    int64_t *d_vofs = nullptr;             // per ring offset into chirp spectra (-1: pow2)
+    // ring pairs (device)
+    int32_t *d_pair_n = nullptr, *d_pair_s = nullptr;
+    double *d_px = nullptr;                // cos theta of the north ring of each pair
+    // recursion tables (device), rows padded to CHUNK, row m at d_rofs[m]
+    int64_t *d_rofs = nullptr;
+    double *d_alpha = nullptr, *d_sl = nullptr;
+    int32_t *d_lseed = nullptr;            // [mmax+1][npairs]
+    double2 *d_seed = nullptr;             // [mmax+1][npairs] (mu_{ls-2}, mu_{ls-1})
+    // FFT tables (device)
+    int twn = 0, twshift = 0;
+    double2 *d_tw64 = nullptr; float2 *d_tw32 = nullptr;
+    double2 *d_V64 = nullptr;  float2 *d_V32 = nullptr;
+    int64_t vtotal = 0;
+    std::vector<FftClass> classes;
+    std::vector<void *> allocations;
+};
+
+// launchers implemented in ring_fft.hip / legendre.hip
+int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi);
+int launch_ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dtype,
+                         int64_t map_stride, void *phase, const double *scale,
+                         hipStream_t s);
+int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *maps,
+                          int map_dtype, int64_t map_stride, const double *scale,
+                          hipStream_t s);
+int plan_build_recursion(Plan *p, const std::vector<double> &pair_sin, double tol_log2);
+int launch_leg_analysis(const Plan *p, int nmaps, const void *phase, void *alm,
+                        int64_t alm_stride, int adjoint, hipStream_t s);
+int launch_prescale(const Plan *p, int nmaps, const void *alm, int64_t alm_stride,
+                    void *scaled, int adjoint, hipStream_t s);
+int launch_leg_synthesis(const Plan *p, int nmaps, const void *scaled, void *phase,
+                         hipStream_t s);
+int launch_alm2cl(int lmax, int mmax, int nmaps, const void *alm, int dt,
+                  int64_t alm_stride, double *cl, int64_t cl_stride, hipStream_t s);
+
+int64_t phase_elems_per_map(const Plan *p);   // complex elements
+int64_t scaled_elems_per_map(const Plan *p);  // complex elements
+
+}  // namespace hbx
+
+struct hbx_plan {
+    hbx::Plan impl;
+};

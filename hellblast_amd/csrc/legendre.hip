@@ -1,0 +1,559 @@
+// Legendre stage: lambda_lm(cos theta) recursion fused with the ring
+// contraction (analysis) or the l-sum (synthesis).
+//
+// Reference: plm_recurrence (legendre.cu:2-35) writes the unnormalised P_lm
+// for every (ring, l) of one m to HBM, the host multiplies by _alm_norm
+// (sht_raw.py:26-28, overflows for l+m >= 171) and reduces with cp.sum over
+// rings (sht_raw.py:53) or over l (isht.py:21).
+//
+// Here lambda_lm = s_lm * mu_lm with the rescaled three-term recursion
+//     mu_l = alpha_lm * x * mu_{l-1} - mu_{l-2}            (2 VALU ops)
+// where  a_lm = sqrt((4l^2-1)/(l^2-m^2)),  s_lm = (a_lm/a_{l-1,m}) s_{l-2,m},
+// alpha_lm = a_lm s_{l-1,m}/s_lm  (s_mm = s_{m+1,m} = 1).  This equals
+// lambda_l = a_l (x lambda_{l-1} - lambda_{l-2}/a_{l-1}) in exact arithmetic.
+// lambda is never stored.  sin^m theta underflow is handled once per plan: a
+// seed kernel runs the recursion in mantissa*2^E form and records, per
+// (m, ring pair), the first chunk-aligned l at which |lambda| can exceed
+// 2^tol ("switch-on") together with (mu_{l-2}, mu_{l-1}) there.  Transforms
+// start each (m, ring pair) at its seed and never rescale.
+//
+// North/south mirror rings share x up to sign: lambda_lm(-x) =
+// (-1)^{l-m} lambda_lm(x), so one recursion serves both rings:
+//   analysis   E = F_n + F_s (l-m even), O = F_n - F_s (l-m odd)
+//   synthesis  G_n = G_even + G_odd,  G_s = G_even - G_odd.
+#include <algorithm>
+
+#include "hbx_device.h"
+#include "hbx_internal.h"
+
+namespace hbx {
+
+constexpr int NEVER = 0x7fffffff;
+static int allow_analysis_lds();
+
+// ---------------------------------------------------------------------------
+// plan-time kernels
+// ---------------------------------------------------------------------------
+// one thread per m: alpha_lm and s_lm for l = m .. padded row end
+__global__ void recursion_tables_kernel(const int64_t *__restrict__ rofs, double *alpha, double *sl,
+                                        int lmax, int mmax) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > mmax) return;
+    const int64_t o = rofs[m];
+    const int len = (int)(rofs[m + 1] - o);
+    double s2 = 1.0, s1 = 1.0, aprev = 0.0;   // s_{l-2}, s_{l-1}, a_{l-1}
+    for (int j = 0; j < len; ++j) {
+        const int l = m + j;
+        double al = 0.0, s = 0.0;
+        if (l > lmax) {
+            al = 0.0; s = 0.0;
+        } else if (j == 0) {
+            al = 0.0; s = 1.0;
+        } else {
+            const double a = sqrt((4.0 * (double)l * l - 1.0) / ((double)(l - m) * (double)(l + m)));
+            if (j == 1) {
+                s = 1.0;
+                al = a;
+            } else {
+                s = (a / aprev) * s2;
+                al = a * s1 / s;
+            }
+            aprev = a;
+            s2 = s1;
+            s1 = s;
+        }
+        alpha[o + j] = al;
+        sl[o + j] = s;
+    }
+}
+
+// one thread per ring pair: lambda_mm = mant * 2^ex for m = 0..mmax
+__global__ void lambda_mm_kernel(const double *__restrict__ sinp, double *mant, int *ex, int npairs,
+                                 int mmax) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    const double st = sinp[p];
+    int e = 0;
+    double v = frexp(0.28209479177387814347403972578 /* 1/sqrt(4 pi) */, &e);
+    mant[p] = v;
+    ex[p] = e;
+    for (int m = 1; m <= mmax; ++m) {
+        v = v * (-sqrt((2.0 * m + 1.0) / (2.0 * m))) * st;
+        int de = 0;
+        v = frexp(v, &de);
+        e += de;
+        mant[(int64_t)m * npairs + p] = v;
+        ex[(int64_t)m * npairs + p] = e;
+    }
+}
+
+// one thread per (m, ring pair): switch-on seed of the recursion
+__global__ void seed_kernel(const double *__restrict__ px, const double *__restrict__ mant,
+                            const int *__restrict__ ex, const int64_t *__restrict__ rofs,
+                            const double *__restrict__ alpha, const double *__restrict__ sl,
+                            int32_t *lseed, double2 *seed, int npairs, int lmax, int tol_log2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (p >= npairs) return;
+    const int64_t q = (int64_t)m * npairs + p;
+    const double x = px[p];
+    const double m0 = mant[q];
+    const int e0 = ex[q];
+    const double thr = ldexp(1.0, tol_log2);
+    const double big = ldexp(1.0, 200), small = ldexp(1.0, -200);
+    const double *al = alpha + rofs[m];
+    const double *sv = sl + rofs[m];
+    int lon = NEVER;
+    if (m0 != 0.0) {
+        if (fabs(ldexp(m0, e0)) >= thr) {
+            lon = m;
+        } else {
+            double u2 = 0.0, u1 = m0;
+            int e = e0;
+            for (int l = m + 1; l <= lmax; ++l) {
+                const double u = al[l - m] * x * u1 - u2;
+                u2 = u1;
+                u1 = u;
+                if (fabs(u1) > big) { u1 *= small; u2 *= small; e += 200; }
+                if (e > tol_log2 - 1100 && fabs(ldexp(sv[l - m] * u1, e)) >= thr) { lon = l; break; }
+            }
+        }
+    }
+    if (lon == NEVER) {
+        lseed[q] = NEVER;
+        seed[q] = make_double2(0.0, 0.0);
+        return;
+    }
+    const int ls = m + ((lon - m) / CHUNK) * CHUNK;
+    lseed[q] = ls;
+    if (ls == m) {
+        seed[q] = make_double2(-ldexp(m0, e0), 0.0);
+        return;
+    }
+    double u2 = 0.0, u1 = m0;
+    int e = e0;
+    for (int l = m + 1; l <= ls - 1; ++l) {
+        const double u = al[l - m] * x * u1 - u2;
+        u2 = u1;
+        u1 = u;
+        if (fabs(u1) > big) { u1 *= small; u2 *= small; e += 200; }
+    }
+    seed[q] = make_double2(ldexp(u2, e), ldexp(u1, e));
+}
+
+int plan_build_recursion(Plan *p, const std::vector<double> &pair_sin, double tol_log2) {
+    const int mmax = p->mmax, lmax = p->lmax, np = p->npairs;
+    std::vector<int64_t> rofs(mmax + 2);
+    rofs[0] = 0;
+    for (int m = 0; m <= mmax; ++m) {
+        const int len = ((lmax - m + 1) + CHUNK - 1) / CHUNK * CHUNK;
+        rofs[m + 1] = rofs[m] + len;
+    }
+    p->nrow = rofs[mmax + 1];
+    auto dalloc = [&](void **ptr, size_t bytes) -> int {
+        hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
+        if (e != hipSuccess) return set_error(HBX_ENOMEM, "hipMalloc failed for recursion tables");
+        p->allocations.push_back(*ptr);
+        p->device_bytes += bytes;
+        return 0;
+    };
+    int rc;
+    if ((rc = dalloc((void **)&p->d_rofs, rofs.size() * sizeof(int64_t)))) return rc;
+    if ((rc = dalloc((void **)&p->d_alpha, p->nrow * sizeof(double)))) return rc;
+    if ((rc = dalloc((void **)&p->d_sl, p->nrow * sizeof(double)))) return rc;
+    const int64_t nseed = (int64_t)(mmax + 1) * np;
+    if ((rc = dalloc((void **)&p->d_lseed, nseed * sizeof(int32_t)))) return rc;
+    if ((rc = dalloc((void **)&p->d_seed, nseed * sizeof(double2)))) return rc;
+    HBX_HIP(hipMemcpy(p->d_rofs, rofs.data(), rofs.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(recursion_tables_kernel, dim3((mmax + 1 + 127) / 128), dim3(128), 0, 0, p->d_rofs,
+                       p->d_alpha, p->d_sl, lmax, mmax);
+    HBX_HIP(hipGetLastError());
+    // temporaries
+    double *d_sin = nullptr, *d_mant = nullptr;
+    int *d_ex = nullptr;
+    HBX_HIP(hipMalloc(&d_sin, np * sizeof(double)));
+    hipError_t e1 = hipMalloc(&d_mant, nseed * sizeof(double));
+    hipError_t e2 = hipMalloc(&d_ex, nseed * sizeof(int));
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        (void)(void)hipFree(d_sin); if (d_mant) (void)hipFree(d_mant); if (d_ex) (void)hipFree(d_ex);
+        return set_error(HBX_ENOMEM, "hipMalloc failed for seed temporaries");
+    }
+    hipError_t e = hipMemcpy(d_sin, pair_sin.data(), np * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(lambda_mm_kernel, dim3((np + 127) / 128), dim3(128), 0, 0, d_sin, d_mant, d_ex, np, mmax);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(seed_kernel, dim3((np + 127) / 128, mmax + 1), dim3(128), 0, 0, p->d_px, d_mant, d_ex,
+                           p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, np, lmax, (int)tol_log2);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(d_sin);
+    if (e == hipSuccess) {
+        const int rc2 = allow_analysis_lds();
+        if (rc2) { (void)hipFree(d_mant); (void)hipFree(d_ex); return rc2; }
+    }
+    (void)hipFree(d_mant);
+    (void)hipFree(d_ex);
+    if (e != hipSuccess) return hip_error(e, "recursion seed kernels");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// wave helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return uniform(v);
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return uniform(v);
+}
+
+// ---------------------------------------------------------------------------
+// analysis: phase -> alm.  One workgroup per (m, map); lane slots cover all
+// ring pairs (R per lane); per l-chunk the 16 partial sums are reduced over
+// the workgroup through LDS.
+// ---------------------------------------------------------------------------
+template <typename T, int R>
+__global__ void __launch_bounds__(512)
+leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__restrict__ alm,
+                    int64_t astride, const int32_t *__restrict__ pair_n, const int32_t *__restrict__ pair_s,
+                    const double *__restrict__ px, const int64_t *__restrict__ rofs,
+                    const double *__restrict__ alpha, const double *__restrict__ sl,
+                    const int32_t *__restrict__ lseed, const double2 *__restrict__ seed, int npairs, int nr,
+                    int lmax, int mmax, int adjoint) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *red = reinterpret_cast<T *>(smem_raw);          // [16][nt]
+    int &wg_start = *reinterpret_cast<int *>(smem_raw + 2 * CHUNK * blockDim.x * sizeof(T));
+    int m, item;
+    xcd_map(blockIdx.x, MB, &m, &item);
+    m = m * MB + item;
+    if (m > mmax) return;
+    const int b = blockIdx.y;
+    const int nt = blockDim.x, tid = threadIdx.x, lane = tid & 63;
+    const int w = uniform(tid >> 6), nw = nt >> 6;
+    const c2<T> *ph = phase + (int64_t)b * pstride + (int64_t)(m >> 3) * nr * MB + (m & 7);
+
+    double x[R], u1[R], u2[R];
+    c2<T> E[R], O[R];
+    int ls[R], pp[R], sst[R], sinj[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int p = (k * nw + w) * 64 + lane;
+        pp[k] = p;
+        u1[k] = 0.0;
+        u2[k] = 0.0;
+        if (p < npairs) {
+            x[k] = px[p];
+            ls[k] = lseed[(int64_t)m * npairs + p];
+            const int rn = pair_n[p], rs = pair_s[p];
+            const c2<T> fn = ph[(int64_t)rn * MB];
+            const c2<T> fs = (rs != rn) ? ph[(int64_t)rs * MB] : mk<T>(0, 0);
+            E[k] = cadd<T>(fn, fs);
+            O[k] = csub<T>(fn, fs);
+        } else {
+            x[k] = 0.0;
+            ls[k] = NEVER;
+            E[k] = mk<T>(0, 0);
+            O[k] = mk<T>(0, 0);
+        }
+        sst[k] = wave_min(ls[k]);
+        sinj[k] = wave_max(ls[k] <= lmax ? ls[k] : -1);
+    }
+    int mine = NEVER;
+#pragma unroll
+    for (int k = 0; k < R; ++k) mine = min(mine, sst[k]);
+    if (tid == 0) wg_start = NEVER;
+    __syncthreads();
+    if (lane == 0) atomicMin(&wg_start, mine);
+    __syncthreads();
+    const int lstart = wg_start;
+
+    c2<T> *out = alm + (int64_t)b * astride;
+    const int64_t ro = rofs[m];
+    const int per = nt / 16;          // threads per reduced value
+    const int v_id = tid / per, sub = tid % per;
+    const int64_t i0 = alm_index(m, m, lmax);
+    // coefficients below the switch-on of every ring pair are exactly zero
+    for (int l = m + tid; l < (lstart > lmax ? lmax + 1 : lstart); l += nt) out[i0 + (l - m)] = mk<T>(0, 0);
+
+    for (int l0 = lstart; l0 <= lmax; l0 += CHUNK) {
+        double a[CHUNK];
+#pragma unroll
+        for (int i = 0; i < CHUNK; ++i) a[i] = alpha[ro + (l0 - m) + i];
+        T acc[2 * CHUNK];
+#pragma unroll
+        for (int i = 0; i < 2 * CHUNK; ++i) acc[i] = (T)0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (l0 < sst[k]) continue;
+            if (l0 <= sinj[k] && ls[k] == l0) {
+                const double2 sd = seed[(int64_t)m * npairs + pp[k]];
+                u2[k] = sd.x;
+                u1[k] = sd.y;
+            }
+            double v2 = u2[k], v1 = u1[k];
+            const double xk = x[k];
+#pragma unroll
+            for (int i = 0; i < CHUNK; ++i) {
+                const double u = fma(a[i] * xk, v1, -v2);
+                v2 = v1;
+                v1 = u;
+                const T uu = (T)u;
+                const c2<T> d = (i & 1) ? O[k] : E[k];
+                acc[2 * i] = fma(uu, d.x, acc[2 * i]);
+                acc[2 * i + 1] = fma(uu, d.y, acc[2 * i + 1]);
+            }
+            u2[k] = v2;
+            u1[k] = v1;
+        }
+        // workgroup reduction of 16 values
+#pragma unroll
+        for (int v = 0; v < 2 * CHUNK; ++v) red[v * nt + tid] = acc[v];
+        __syncthreads();
+        T s = (T)0;
+        for (int q = sub; q < nt; q += per) s += red[v_id * nt + q];
+        for (int o = per >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (sub == 0) {
+            const int i = v_id >> 1, c = v_id & 1;
+            const int l = l0 + i;
+            if (l <= lmax) {
+                double val = (double)s * sl[ro + (l0 - m) + i];
+                if (adjoint) {
+                    if (m > 0) val *= 2.0;
+                    else if (c == 1) val = 0.0;
+                }
+                T *dst = reinterpret_cast<T *>(out + i0 + (l - m));
+                dst[c] = (T)val;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// synthesis pre-pass: scaled[row m][l-m] = s_lm * a_lm (halved for m > 0 in
+// the map2alm adjoint), zero-padded rows so chunks never read past a row.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void prescale_kernel(const c2<T> *__restrict__ alm, int64_t astride, c2<T> *__restrict__ scaled,
+                                int64_t nrow, const int64_t *__restrict__ rofs, const double *__restrict__ sl,
+                                int lmax, int adjoint) {
+    const int m = blockIdx.x, b = blockIdx.y;
+    const int64_t o = rofs[m];
+    const int len = (int)(rofs[m + 1] - o);
+    const int64_t i0 = alm_index(m, m, lmax);
+    const c2<T> *a = alm + (int64_t)b * astride + i0;
+    c2<T> *d = scaled + (int64_t)b * nrow + o;
+    const double f = (adjoint && m > 0) ? 0.5 : 1.0;
+    for (int j = threadIdx.x; j < len; j += blockDim.x) {
+        c2<T> v = mk<T>(0, 0);
+        if (m + j <= lmax) v = cscale<T>(a[j], (T)(sl[o + j] * f));
+        d[j] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// synthesis: scaled alm -> phase.  One wave per (m, 64 ring pairs); a
+// workgroup holds the 8 m of one m-block so its writes fill whole
+// [ring][m%8] lines of the phase buffer.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(512)
+leg_synthesis_kernel(const c2<T> *__restrict__ scaled, int64_t nrow, c2<T> *__restrict__ phase, int64_t pstride,
+                     const int32_t *__restrict__ pair_n, const int32_t *__restrict__ pair_s,
+                     const double *__restrict__ px, const int64_t *__restrict__ rofs,
+                     const double *__restrict__ alpha, const int32_t *__restrict__ lseed,
+                     const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int nmb, int npb) {
+    int mb, pb;
+    xcd_map(blockIdx.x, npb, &mb, &pb);
+    if (mb >= nmb) return;
+    const int b = blockIdx.y;
+    const int w = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int m = mb * MB + w;
+    if (m > mmax) return;
+    const int p = pb * 64 + lane;
+    const bool valid = p < npairs;
+    const double x = valid ? px[p] : 0.0;
+    const int ls = valid ? lseed[(int64_t)m * npairs + p] : NEVER;
+    const int lstart = wave_min(ls);
+    const int linj = wave_max(ls <= lmax ? ls : -1);
+    c2<T> ge = mk<T>(0, 0), go = mk<T>(0, 0);
+    if (lstart <= lmax) {
+        const int64_t ro = rofs[m];
+        const c2<T> *ar = scaled + (int64_t)b * nrow + ro - m;
+        const double *al = alpha + ro - m;
+        double u2 = 0.0, u1 = 0.0;
+        for (int l0 = lstart; l0 <= lmax; l0 += CHUNK) {
+            if (l0 <= linj && ls == l0) {
+                const double2 sd = seed[(int64_t)m * npairs + p];
+                u2 = sd.x;
+                u1 = sd.y;
+            }
+#pragma unroll
+            for (int i = 0; i < CHUNK; ++i) {
+                const double u = fma(al[l0 + i] * x, u1, -u2);
+                u2 = u1;
+                u1 = u;
+                const T uu = (T)u;
+                const c2<T> av = ar[l0 + i];
+                if (i & 1) {
+                    go.x = fma(uu, av.x, go.x);
+                    go.y = fma(uu, av.y, go.y);
+                } else {
+                    ge.x = fma(uu, av.x, ge.x);
+                    ge.y = fma(uu, av.y, ge.y);
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    c2<T> *out = phase + (int64_t)b * pstride + (int64_t)mb * nr * MB + w;
+    const int rn = pair_n[p], rs = pair_s[p];
+    out[(int64_t)rn * MB] = cadd<T>(ge, go);
+    if (rs != rn) out[(int64_t)rs * MB] = csub<T>(ge, go);
+}
+
+// ---------------------------------------------------------------------------
+// alm2cl (alm_ops.py:19-32) on m-major alm
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void alm2cl_kernel(const c2<T> *__restrict__ alm, int64_t astride, double *cl, int64_t cstride,
+                              int lmax, int mmax) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (l > lmax) return;
+    const c2<T> *a = alm + (int64_t)b * astride;
+    double acc = 0.0;
+    const int top = min(l, mmax);
+    for (int m = 0; m <= top; ++m) {
+        const c2<T> v = a[alm_index(l, m, lmax)];
+        const double p = (double)v.x * v.x + (double)v.y * v.y;
+        acc += (m == 0) ? p : 2.0 * p;
+    }
+    cl[(int64_t)b * cstride + l] = acc / (2.0 * l + 1.0);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+// Threads per workgroup (a power of two in [64, 512], so that the 16-way
+// LDS reduction splits into aligned shuffle groups) and ring-pair slots per
+// lane (a power of two <= 8).
+static void analysis_shape(const Plan *p, int *R, int *nt) {
+    int t = 64;
+    while (t < p->npairs && t < 512) t *= 2;
+    int r = 1;
+    while ((int64_t)r * t < p->npairs && r < 8) r *= 2;
+    *R = r;
+    *nt = t;
+}
+
+template <typename T> static size_t analysis_lds_bytes(int nt) {
+    return (size_t)2 * CHUNK * nt * sizeof(T) + 16;
+}
+
+template <typename T, typename K> static int raise_lds(K kernel, size_t bytes) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(leg_analysis_kernel)");
+    return 0;
+}
+
+// the 512-thread FP64 analysis reduction buffer is just above 64 KiB
+static int allow_analysis_lds() {
+    int rc;
+    if ((rc = raise_lds<double>(leg_analysis_kernel<double, 1>, analysis_lds_bytes<double>(512)))) return rc;
+    if ((rc = raise_lds<double>(leg_analysis_kernel<double, 2>, analysis_lds_bytes<double>(512)))) return rc;
+    if ((rc = raise_lds<double>(leg_analysis_kernel<double, 4>, analysis_lds_bytes<double>(512)))) return rc;
+    if ((rc = raise_lds<double>(leg_analysis_kernel<double, 8>, analysis_lds_bytes<double>(512)))) return rc;
+    return 0;
+}
+
+template <typename T>
+static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, c2<T> *alm, int64_t astride,
+                          int adjoint, hipStream_t s) {
+    int R, nt;
+    analysis_shape(p, &R, &nt);
+    if ((int64_t)R * nt < p->npairs)
+        return set_error(HBX_EUNSUPPORTED, "too many ring pairs for the analysis kernel (max 4096)");
+    const int nmpad = (p->mmax + 1 + 63) / 64 * 64;   // xcd_map covers whole groups of 64
+    dim3 grid(nmpad, nmaps);
+    const size_t lds = analysis_lds_bytes<T>(nt);
+    const int64_t pstride = phase_elems_per_map(p);
+#define HBX_LA(RV)                                                                                      \
+    hipLaunchKernelGGL((leg_analysis_kernel<T, RV>), grid, dim3(nt), lds, s, phase, pstride, alm, astride, \
+                       p->d_pair_n, p->d_pair_s, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed,  \
+                       p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, adjoint)
+    switch (R) {
+    case 1: HBX_LA(1); break;
+    case 2: HBX_LA(2); break;
+    case 4: HBX_LA(4); break;
+    case 8: HBX_LA(8); break;
+    default: return set_error(HBX_EUNSUPPORTED, "analysis slot count");
+    }
+#undef HBX_LA
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "leg_analysis_kernel launch");
+    return 0;
+}
+
+int launch_leg_analysis(const Plan *p, int nmaps, const void *phase, void *alm, int64_t astride,
+                        int adjoint, hipStream_t s) {
+    if (p->precision == HBX_F64)
+        return leg_analysis_t<double>(p, nmaps, (const double2 *)phase, (double2 *)alm, astride, adjoint, s);
+    return leg_analysis_t<float>(p, nmaps, (const float2 *)phase, (float2 *)alm, astride, adjoint, s);
+}
+
+int launch_prescale(const Plan *p, int nmaps, const void *alm, int64_t astride, void *scaled, int adjoint,
+                    hipStream_t s) {
+    dim3 grid(p->mmax + 1, nmaps);
+    if (p->precision == HBX_F64)
+        hipLaunchKernelGGL(prescale_kernel<double>, grid, dim3(256), 0, s, (const double2 *)alm, astride,
+                           (double2 *)scaled, p->nrow, p->d_rofs, p->d_sl, p->lmax, adjoint);
+    else
+        hipLaunchKernelGGL(prescale_kernel<float>, grid, dim3(256), 0, s, (const float2 *)alm, astride,
+                           (float2 *)scaled, p->nrow, p->d_rofs, p->d_sl, p->lmax, adjoint);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "prescale_kernel launch");
+    return 0;
+}
+
+int launch_leg_synthesis(const Plan *p, int nmaps, const void *scaled, void *phase, hipStream_t s) {
+    const int npb = (p->npairs + 63) / 64;
+    const int nmb8 = (p->nmb + 7) / 8 * 8;
+    dim3 grid(nmb8 * npb, nmaps);
+    const int64_t pstride = phase_elems_per_map(p);
+    if (p->precision == HBX_F64)
+        hipLaunchKernelGGL(leg_synthesis_kernel<double>, grid, dim3(512), 0, s, (const double2 *)scaled, p->nrow,
+                           (double2 *)phase, pstride, p->d_pair_n, p->d_pair_s, p->d_px, p->d_rofs, p->d_alpha,
+                           p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, p->nmb, npb);
+    else
+        hipLaunchKernelGGL(leg_synthesis_kernel<float>, grid, dim3(512), 0, s, (const float2 *)scaled, p->nrow,
+                           (float2 *)phase, pstride, p->d_pair_n, p->d_pair_s, p->d_px, p->d_rofs, p->d_alpha,
+                           p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, p->nmb, npb);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "leg_synthesis_kernel launch");
+    return 0;
+}
+
+int launch_alm2cl(int lmax, int mmax, int nmaps, const void *alm, int dt, int64_t astride, double *cl,
+                  int64_t cstride, hipStream_t s) {
+    dim3 grid((lmax + 1 + 255) / 256, nmaps);
+    if (dt == HBX_DT_F64)
+        hipLaunchKernelGGL(alm2cl_kernel<double>, grid, dim3(256), 0, s, (const double2 *)alm, astride, cl, cstride,
+                           lmax, mmax);
+    else
+        hipLaunchKernelGGL(alm2cl_kernel<float>, grid, dim3(256), 0, s, (const float2 *)alm, astride, cl, cstride,
+                           lmax, mmax);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "alm2cl_kernel launch");
+    return 0;
+}
+
+}  // namespace hbx

@@ -1,0 +1,102 @@
+"""healpy-style flat-map wrappers for the SHT path (reference: ``hp_compat.py:24-102``).
+
+Maps are flat RING-ordered arrays ([npix] or batched [nmaps, npix]); alm are
+m-major.  Defaults follow the reference: lmax = 3*nside (``hp_compat.py:45``),
+midpoint-Delta-z weights (``ring_tables.py:39-40``), phi0 = 0 and the
+reference aliasing rule.  ``nest=True`` is rejected instead of being ignored
+silently as the reference does (``hp_compat.py:40-54``).
+"""
+import math
+from functools import lru_cache
+
+import numpy as np
+import torch
+
+from . import _util
+from .plan import Geometry, get_plan, lmax_from_size, nalm
+from .ring_tables import healpix_geometry
+
+
+def _nside_from_npix(npix: int) -> int:
+    """``hp_compat._nside_from_npix`` (hp_compat.py:24-29)."""
+    nside = int(math.isqrt(int(npix) // 12))
+    if 12 * nside * nside != npix:
+        raise ValueError("npix is not 12*nside^2")
+    return nside
+
+
+@lru_cache(maxsize=64)
+def healpix_geom(nside, mmax, weights="dz", phi0="reference", alias="reference"):
+    """Geometry of a HEALPix RING map for the plan cache."""
+    g = healpix_geometry(nside)
+    nphi = g["nphi"]
+    if weights == "dz":
+        w = g["z_edges"][:-1] - g["z_edges"][1:]
+    elif weights == "area":
+        w = 2.0 * nphi.astype(np.float64) / g["npix"]
+    else:
+        raise ValueError("weights must be 'dz' or 'area'")
+    if phi0 in (None, "reference", "zero"):
+        ph = None
+    elif phi0 == "healpix":
+        ph = g["phi0_healpix"]
+    else:
+        raise ValueError("phi0 must be 'reference' or 'healpix'")
+    return Geometry(nphi, theta=g["theta_centers"], weight=2.0 * math.pi * w, phi0=ph,
+                    mcut=_util.alias_mcut(alias, nphi, mmax), ring_start=g["ring_start"])
+
+
+def _check_nest(nest):
+    if nest:
+        raise NotImplementedError("NESTED maps are not supported by the SHT path; reorder to RING first")
+
+
+def _plan(nside, lmax, mmax, precision, weights, phi0, alias):
+    return get_plan(healpix_geom(int(nside), int(mmax), weights, phi0, alias), int(lmax), int(mmax), precision)
+
+
+def map2alm(m, lmax=None, mmax=None, nest=False, iter=0, *, precision="f64", weights="dz",
+            phi0="reference", alias="reference"):
+    """Analysis of a flat RING map (``hp_compat.py:40-48``), m-major output.
+
+    ``iter > 0`` applies Jacobi refinement a <- a + A(f - S a).
+    """
+    _check_nest(nest)
+    t = _util.flat_maps(m, m.shape[-1] if hasattr(m, "shape") else len(m))
+    nside = _nside_from_npix(t.shape[-1])
+    lmax = 3 * nside if lmax is None else int(lmax)
+    mmax = lmax if mmax is None else int(mmax)
+    plan = _plan(nside, lmax, mmax, precision, weights, phi0, alias)
+    alm = plan.map2alm(t)
+    for _ in range(int(iter)):
+        resid = t - plan.alm2map(alm, dtype=t.dtype)
+        alm = alm + plan.map2alm(resid)
+    return alm
+
+
+def alm2map(alm, nside, lmax=None, nest=False, mmax=None, *, precision="f64", dtype=torch.float64,
+            phi0="reference", alias="reference"):
+    """Synthesis of m-major alm to a flat RING map (``hp_compat.py:50-54``)."""
+    _check_nest(nest)
+    a = torch.as_tensor(alm)
+    if lmax is None:
+        lmax = lmax_from_size(a.shape[-1])
+    mmax = lmax if mmax is None else int(mmax)
+    if a.shape[-1] != nalm(int(lmax), mmax):
+        raise ValueError("alm size does not match lmax/mmax (m-major layout expected)")
+    plan = _plan(nside, lmax, mmax, precision, "dz", phi0, alias)
+    return plan.alm2map(a, dtype=dtype)
+
+
+def anafast(m, lmax=None, mmax=None, nest=False, iter=0, **kw):
+    """C_l of map2alm(m) (``anafast.py:5-7`` on a flat map)."""
+    from .alm_ops import alm2cl
+    alm = map2alm(m, lmax=lmax, mmax=mmax, nest=nest, iter=iter, **kw)
+    nside = _nside_from_npix(m.shape[-1])
+    lmax = 3 * nside if lmax is None else int(lmax)
+    return alm2cl(alm, lmax=lmax, mmax=lmax if mmax is None else mmax)
+
+
+def alm2cl(alm, lmax=None, mmax=None):
+    from .alm_ops import alm2cl as _a
+    return _a(alm, lmax=lmax, mmax=mmax)

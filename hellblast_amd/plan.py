@@ -1,0 +1,276 @@
+"""Transform plans: ring geometry + lmax/mmax/precision bound to one device.
+
+A plan owns the device tables of one geometry (ring offsets, ring-pair
+cos theta, FFT chirp spectra, lambda recursion coefficients and switch-on
+seeds).  Plans are cached per (device, geometry, lmax, mmax, precision).
+Execute methods take and return torch tensors on the plan's device and
+enqueue on torch's current stream; they allocate only the output and a
+workspace through torch's caching allocator.
+"""
+import ctypes
+import hashlib
+import math
+from collections import OrderedDict
+
+import numpy as np
+import torch
+
+from . import _abi
+
+PRECISIONS = {"f64": _abi.HBX_F64, "float64": _abi.HBX_F64, "double": _abi.HBX_F64,
+              "f32": _abi.HBX_F32, "float32": _abi.HBX_F32, "single": _abi.HBX_F32}
+
+
+def precision_code(precision):
+    if isinstance(precision, int):
+        if precision in (_abi.HBX_F64, _abi.HBX_F32):
+            return precision
+    key = str(precision).lower().replace("torch.", "")
+    if key not in PRECISIONS:
+        raise ValueError(f"precision must be 'f64' or 'f32', got {precision!r}")
+    return PRECISIONS[key]
+
+
+def nalm(lmax, mmax=None):
+    mmax = lmax if mmax is None else mmax
+    return (mmax + 1) * (lmax + 1) - mmax * (mmax + 1) // 2
+
+
+class Geometry:
+    """Host description of a ring set (hbx_geom); arrays are NumPy."""
+
+    def __init__(self, nphi, theta=None, cos_theta=None, sin_theta=None, weight=None, phi0=None,
+                 mcut=None, ring_start=None):
+        self.nphi = np.ascontiguousarray(np.asarray(nphi, dtype=np.int32).ravel())
+        nr = self.nphi.size
+        if theta is not None:
+            th = np.asarray(theta, dtype=np.float64).ravel()
+            if th.size != nr:
+                raise ValueError("theta and nphi sizes differ")
+            cos_theta = np.cos(th) if cos_theta is None else cos_theta
+            sin_theta = np.sin(th) if sin_theta is None else sin_theta
+        if cos_theta is None or sin_theta is None:
+            raise ValueError("theta (or cos_theta and sin_theta) required")
+        self.cos_theta = np.ascontiguousarray(np.asarray(cos_theta, dtype=np.float64).ravel())
+        self.sin_theta = np.ascontiguousarray(np.asarray(sin_theta, dtype=np.float64).ravel())
+        if self.cos_theta.size != nr or self.sin_theta.size != nr:
+            raise ValueError("cos/sin theta and nphi sizes differ")
+        if ring_start is None:
+            ring_start = np.zeros(nr + 1, dtype=np.int64)
+            ring_start[1:] = np.cumsum(self.nphi.astype(np.int64))
+        self.ring_start = np.ascontiguousarray(np.asarray(ring_start, dtype=np.int64).ravel())
+        if self.ring_start.size != nr + 1:
+            raise ValueError("ring_start must have nrings+1 entries")
+        self.weight = None if weight is None else np.ascontiguousarray(np.asarray(weight, dtype=np.float64).ravel())
+        self.phi0 = None if phi0 is None else np.ascontiguousarray(np.asarray(phi0, dtype=np.float64).ravel())
+        self.mcut = None if mcut is None else np.ascontiguousarray(np.asarray(mcut, dtype=np.int32).ravel())
+        for name in ("weight", "phi0", "mcut"):
+            a = getattr(self, name)
+            if a is not None and a.size != nr:
+                raise ValueError(f"{name} must have one entry per ring")
+
+    @property
+    def nrings(self):
+        return int(self.nphi.size)
+
+    @property
+    def npix(self):
+        return int(self.ring_start[-1])
+
+    def key(self):
+        h = hashlib.sha1()
+        for a in (self.ring_start, self.nphi, self.cos_theta, self.sin_theta, self.weight, self.phi0, self.mcut):
+            h.update(b"|" if a is None else a.tobytes())
+        return h.hexdigest()
+
+    def as_struct(self):
+        """hbx_geom pointing into this object's arrays (keep self alive)."""
+        def p(a, ct):
+            return None if a is None else a.ctypes.data_as(ctypes.POINTER(ct))
+        g = _abi.hbx_geom()
+        g.nrings = self.nrings
+        g.ring_start = p(self.ring_start, ctypes.c_int64)
+        g.nphi = p(self.nphi, ctypes.c_int32)
+        g.cos_theta = p(self.cos_theta, ctypes.c_double)
+        g.sin_theta = p(self.sin_theta, ctypes.c_double)
+        g.phi0 = p(self.phi0, ctypes.c_double)
+        g.weight = p(self.weight, ctypes.c_double)
+        g.mcut = p(self.mcut, ctypes.c_int32)
+        return g
+
+    def validate(self, lmax, mmax):
+        _abi.check(_abi.lib().hbx_validate_geom(ctypes.byref(self.as_struct()), int(lmax), int(mmax)))
+
+
+def _stream_ptr(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _map_dtype_code(dtype):
+    if dtype == torch.float64:
+        return _abi.HBX_DT_F64
+    if dtype == torch.float32:
+        return _abi.HBX_DT_F32
+    raise TypeError(f"maps must be float64 or float32, got {dtype}")
+
+
+class Plan:
+    """A libhbx plan for one geometry, lmax, mmax and precision on one GPU."""
+
+    def __init__(self, geom, lmax, mmax=None, precision="f64", device=None):
+        mmax = lmax if mmax is None else int(mmax)
+        self.geom = geom
+        self.lmax, self.mmax = int(lmax), int(mmax)
+        self.precision = precision_code(precision)
+        if not torch.cuda.is_available():
+            geom.validate(self.lmax, self.mmax)
+            raise RuntimeError("hellblast_amd transforms need a ROCm GPU (torch.cuda.is_available() is False)")
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None
+                                   else torch.device(device).index or 0)
+        L = _abi.lib()
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _abi.check(L.hbx_plan_create(ctypes.byref(geom.as_struct()), self.lmax, self.mmax, 0,
+                                         self.precision, 0, ctypes.byref(handle)))
+        self._h = handle
+        info = _abi.hbx_plan_info()
+        _abi.check(L.hbx_plan_get_info(self._h, ctypes.byref(info)))
+        self.npix = int(info.npix)
+        self.nalm = int(info.nalm)
+        self.device_bytes = int(info.device_bytes)
+        self.alm_dtype = torch.complex128 if self.precision == _abi.HBX_F64 else torch.complex64
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                with torch.cuda.device(self.device):
+                    _abi.lib().hbx_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # ------------------------------------------------------------------
+    def workspace_bytes(self, nmaps):
+        b = ctypes.c_uint64()
+        _abi.check(_abi.lib().hbx_workspace_bytes(self._h, int(nmaps), ctypes.byref(b)))
+        return int(b.value)
+
+    def workspace(self, nmaps):
+        return torch.empty(max(self.workspace_bytes(nmaps), 1), dtype=torch.uint8, device=self.device)
+
+    def _as_maps(self, maps):
+        t = torch.as_tensor(maps)
+        if t.device != self.device:
+            t = t.to(self.device)
+        if t.dtype not in (torch.float64, torch.float32):
+            t = t.to(torch.float64)
+        squeeze = t.dim() == 1
+        t = t.reshape(-1, t.shape[-1]) if not squeeze else t.reshape(1, -1)
+        if t.shape[-1] != self.npix:
+            raise ValueError(f"map has {t.shape[-1]} pixels, plan expects {self.npix}")
+        return t.contiguous(), squeeze
+
+    def _as_alm(self, alm):
+        t = torch.as_tensor(alm)
+        if t.device != self.device:
+            t = t.to(self.device)
+        if t.dtype != self.alm_dtype:
+            t = t.to(self.alm_dtype)
+        squeeze = t.dim() == 1
+        t = t.reshape(1, -1) if squeeze else t.reshape(-1, t.shape[-1])
+        if t.shape[-1] != self.nalm:
+            raise ValueError(f"alm has {t.shape[-1]} coefficients, plan expects {self.nalm}")
+        return t.contiguous(), squeeze
+
+    def _analysis(self, fn, maps, out=None):
+        m, squeeze = self._as_maps(maps)
+        nb = m.shape[0]
+        alm = torch.empty((nb, self.nalm), dtype=self.alm_dtype, device=self.device) if out is None else out
+        ws = self.workspace(nb)
+        with torch.cuda.device(self.device):
+            _abi.check(fn(self._h, nb, ctypes.c_void_p(m.data_ptr()), _map_dtype_code(m.dtype), m.shape[1],
+                          ctypes.c_void_p(alm.data_ptr()), alm.shape[-1], ctypes.c_void_p(ws.data_ptr()),
+                          ws.numel(), _stream_ptr(self.device)))
+        return alm[0] if squeeze else alm
+
+    def _synthesis(self, fn, alm, dtype=torch.float64, out=None):
+        a, squeeze = self._as_alm(alm)
+        nb = a.shape[0]
+        maps = torch.empty((nb, self.npix), dtype=dtype, device=self.device) if out is None else out
+        ws = self.workspace(nb)
+        with torch.cuda.device(self.device):
+            _abi.check(fn(self._h, nb, ctypes.c_void_p(a.data_ptr()), a.shape[1], ctypes.c_void_p(maps.data_ptr()),
+                          _map_dtype_code(maps.dtype), maps.shape[-1], ctypes.c_void_p(ws.data_ptr()),
+                          ws.numel(), _stream_ptr(self.device)))
+        return maps[0] if squeeze else maps
+
+    def map2alm(self, maps, out=None):
+        """Analysis of [npix] or [nmaps, npix] maps -> m-major alm."""
+        return self._analysis(_abi.lib().hbx_map2alm, maps, out)
+
+    def alm2map(self, alm, dtype=torch.float64, out=None):
+        """Synthesis of [nalm] or [nmaps, nalm] alm -> maps."""
+        return self._synthesis(_abi.lib().hbx_alm2map, alm, dtype, out)
+
+    def alm2map_adjoint(self, maps, out=None):
+        """Transpose of alm2map (complex-gradient convention)."""
+        return self._analysis(_abi.lib().hbx_alm2map_adjoint, maps, out)
+
+    def map2alm_adjoint(self, alm, dtype=torch.float64, out=None):
+        """Transpose of map2alm (complex-gradient convention)."""
+        return self._synthesis(_abi.lib().hbx_map2alm_adjoint, alm, dtype, out)
+
+
+_CACHE = OrderedDict()
+_CACHE_MAX = 8
+
+
+def get_plan(geom, lmax, mmax=None, precision="f64", device=None):
+    """Cached Plan for (device, geometry, lmax, mmax, precision)."""
+    mmax = lmax if mmax is None else int(mmax)
+    dev = torch.cuda.current_device() if (device is None and torch.cuda.is_available()) else device
+    key = (str(dev), geom.key(), int(lmax), int(mmax), precision_code(precision))
+    p = _CACHE.get(key)
+    if p is not None:
+        _CACHE.move_to_end(key)
+        return p
+    p = Plan(geom, lmax, mmax, precision, device)
+    _CACHE[key] = p
+    while len(_CACHE) > _CACHE_MAX:
+        _CACHE.popitem(last=False)
+    return p
+
+
+def clear_cache():
+    _CACHE.clear()
+
+
+def alm2cl(alm, lmax, mmax=None):
+    """C_l of [nalm] or [nmaps, nalm] m-major alm on the GPU (FP64 result)."""
+    mmax = lmax if mmax is None else int(mmax)
+    a = torch.as_tensor(alm)
+    if not a.is_cuda:
+        raise ValueError("alm2cl expects a device tensor")
+    if a.dtype not in (torch.complex128, torch.complex64):
+        a = a.to(torch.complex128)
+    squeeze = a.dim() == 1
+    a = (a.reshape(1, -1) if squeeze else a.reshape(-1, a.shape[-1])).contiguous()
+    if a.shape[-1] != nalm(lmax, mmax):
+        raise ValueError("alm size does not match lmax/mmax")
+    nb = a.shape[0]
+    cl = torch.empty((nb, lmax + 1), dtype=torch.float64, device=a.device)
+    dt = _abi.HBX_DT_F64 if a.dtype == torch.complex128 else _abi.HBX_DT_F32
+    with torch.cuda.device(a.device):
+        _abi.check(_abi.lib().hbx_alm2cl(int(lmax), int(mmax), nb, ctypes.c_void_p(a.data_ptr()), dt,
+                                         a.shape[-1], ctypes.c_void_p(cl.data_ptr()), lmax + 1,
+                                         _stream_ptr(a.device)))
+    return cl[0] if squeeze else cl
+
+
+def lmax_from_size(size, mmax=None):
+    """lmax of an m-major alm array with mmax = lmax (healpy Alm.getlmax)."""
+    lmax = (math.isqrt(1 + 8 * int(size)) - 3) // 2
+    if nalm(lmax) != size:
+        raise ValueError(f"{size} is not the size of a full-triangle m-major alm array")
+    return lmax

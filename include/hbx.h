@@ -1,0 +1,135 @@
+/*
+ * hbx.h - C ABI of libhbx.so, the MI355X (gfx950) spherical-harmonic transform
+ * library behind hellblast_amd.
+ *
+ * Every entry point returns 0 (HBX_OK) or a negative hbx_status; the message
+ * of the last failure on the calling thread is returned by hbx_last_error().
+ * Data pointers passed to execute functions are caller-owned DEVICE pointers
+ * (contiguous, complex arrays interleaved re/im like torch complex64/128).
+ * Execute functions never allocate and never synchronise: they enqueue on the
+ * given hipStream_t (NULL = default stream) and are safe to capture in a HIP
+ * graph.  Plans are immutable after creation and bound to the device that was
+ * current when they were created.
+ *
+ * Reference interfaces replaced (file:line in Luce-Morningstar/hellblast):
+ *   hbx_plan_create      - per-call setup in sht_raw.py:30-45 (ring FFT prep,
+ *                          x = cos(theta)) and _compute_legendre_block
+ *                          sht_raw.py:16-24 / legendre.cu:2-35 (moved to plan
+ *                          time as recursion tables and switch-on seeds)
+ *   hbx_map2alm          - sht_raw.map2alm_gpu        sht_raw.py:30-57
+ *                          sht_fallback.map2alm_gpu   sht_fallback.py:36-65
+ *   hbx_alm2map          - isht.alm2map_gpu           isht.py:5-26
+ *   hbx_alm2cl           - alm_ops.alm2cl             alm_ops.py:19-32
+ *   hbx_map2alm_spin     - spin.map2alm_spin_gpu      spin.py:32-71
+ *                          spin1.map2alm_spin1_gpu    spin1.py:5-38
+ *   hbx_alm2map_spin     - spin.qu_from_eb_gpu        spin.py:80-115
+ *                          spin1.alm2map_spin1_gpu    spin1.py:40-69
+ *   hbx_alm2map_adjoint, hbx_map2alm_adjoint, hbx_alm2map_spin_adjoint,
+ *   hbx_map2alm_spin_adjoint - no reference counterpart (the reference has no
+ *                          gradients); they are the exact transposes used by
+ *                          hellblast_amd.autograd.
+ */
+#ifndef HBX_H
+#define HBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HBX_ABI_VERSION 1
+
+typedef enum hbx_status {
+    HBX_OK = 0,
+    HBX_EINVAL = -1,       /* bad argument (shape, range, null pointer)      */
+    HBX_ENOMEM = -2,       /* device allocation failed at plan creation      */
+    HBX_EDEVICE = -3,      /* HIP runtime error / no device                  */
+    HBX_EWORKSPACE = -4,   /* workspace smaller than hbx_workspace_bytes()   */
+    HBX_EUNSUPPORTED = -5, /* geometry or size outside what the kernels do   */
+    HBX_EPLAN = -6         /* plan used on another device or wrong kind      */
+} hbx_status;
+
+typedef enum hbx_precision {
+    HBX_F64 = 0,           /* FP64 data, FP64 recursion, FP64 contraction    */
+    HBX_F32 = 1            /* FP32 data + contraction, FP64 recursion/seeds  */
+} hbx_precision;
+
+typedef enum hbx_dtype {
+    HBX_DT_F64 = 0,
+    HBX_DT_F32 = 1
+} hbx_dtype;
+
+/* Ring geometry, all arrays in HOST memory; the plan copies what it needs.
+ * Rings r and nrings-1-r must be mirror images (equal nphi, cos theta of
+ * opposite sign); an odd middle ring must sit on the equator.  Every nphi
+ * must be even.  This holds for HEALPix RING ordering.                      */
+typedef struct hbx_geom {
+    int32_t nrings;
+    const int64_t *ring_start; /* nrings+1 offsets into the flat map          */
+    const int32_t *nphi;       /* pixels per ring                             */
+    const double *cos_theta;   /* cos theta per ring                          */
+    const double *sin_theta;   /* sin theta per ring (>= 0)                   */
+    const double *phi0;        /* longitude of pixel 0 per ring, NULL = 0     */
+    const double *weight;      /* analysis weight per ring (multiplies
+                                  DFT/nphi, e.g. 2*pi*dz), NULL = 1          */
+    const int32_t *mcut;       /* ring r contributes to m < mcut[r] with bin
+                                  m mod nphi; NULL = nphi (reference rule)   */
+} hbx_geom;
+
+typedef struct hbx_plan hbx_plan;
+
+typedef struct hbx_plan_info {
+    int32_t nrings, npairs, lmax, mmax, precision, spin;
+    int64_t npix, nalm;
+    uint64_t device_bytes;     /* device memory held by the plan             */
+} hbx_plan_info;
+
+int hbx_abi_version(void);
+const char *hbx_last_error(void);
+
+/* Host-only validation of a geometry (no device needed). */
+int hbx_validate_geom(const hbx_geom *g, int32_t lmax, int32_t mmax);
+
+/* spin: 0 (scalar) or 1, 2 (spin-weighted).  flags: reserved, pass 0. */
+int hbx_plan_create(const hbx_geom *g, int32_t lmax, int32_t mmax, int32_t spin,
+                    int32_t precision, uint32_t flags, hbx_plan **out);
+int hbx_plan_destroy(hbx_plan *plan);
+int hbx_plan_get_info(const hbx_plan *plan, hbx_plan_info *info);
+int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
+
+/* Scalar transforms.  maps: nmaps maps of plan npix values (dtype
+ * map_dtype), map_stride elements apart.  alm: nmaps m-major alm arrays
+ * (idx = m(2 lmax+1-m)/2 + l) of complex128 (HBX_F64 plans) or complex64
+ * (HBX_F32 plans), alm_stride complex elements apart.                       */
+int hbx_map2alm(const hbx_plan *plan, int32_t nmaps, const void *maps,
+                int32_t map_dtype, int64_t map_stride, void *alm,
+                int64_t alm_stride, void *ws, uint64_t ws_bytes, void *stream);
+int hbx_alm2map(const hbx_plan *plan, int32_t nmaps, const void *alm,
+                int64_t alm_stride, void *maps, int32_t map_dtype,
+                int64_t map_stride, void *ws, uint64_t ws_bytes, void *stream);
+/* Transposes: adjoint of alm2map (c_m-weighted unweighted analysis) and of
+ * map2alm (weighted synthesis of g/c_m), PyTorch complex-gradient
+ * convention grad = dL/dRe + i dL/dIm.                                      */
+int hbx_alm2map_adjoint(const hbx_plan *plan, int32_t nmaps, const void *maps,
+                        int32_t map_dtype, int64_t map_stride, void *alm,
+                        int64_t alm_stride, void *ws, uint64_t ws_bytes,
+                        void *stream);
+int hbx_map2alm_adjoint(const hbx_plan *plan, int32_t nmaps, const void *alm,
+                        int64_t alm_stride, void *maps, int32_t map_dtype,
+                        int64_t map_stride, void *ws, uint64_t ws_bytes,
+                        void *stream);
+
+/* C_l = (|a_l0|^2 + 2 sum_{m>=1} |a_lm|^2)/(2l+1) for nmaps m-major alm of
+ * complex128 (alm_dtype HBX_DT_F64) or complex64 (HBX_DT_F32); cl is FP64
+ * [nmaps][lmax+1] with cl_stride elements between maps.                     */
+int hbx_alm2cl(int32_t lmax, int32_t mmax, int32_t nmaps, const void *alm,
+               int32_t alm_dtype, int64_t alm_stride, double *cl,
+               int64_t cl_stride, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HBX_H */

@@ -1,0 +1,194 @@
+"""GPU parity of the scalar SHT path (libhbx.so via the C ABI) against the oracle.
+
+FP64 bar: relative max error <= 1e-11 against the FP64 CPU oracle (which is
+itself pinned to the reference formulas in test_oracle.py).  FP32 bar
+(the BASELINE gate): eps_max <= 1e-5 against the FP64 oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import geometry as OG
+from oracle import literal as OL
+from oracle import sht as OS
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-11
+
+
+def _rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _plan(nside, lmax, mmax=None, precision="f64", phi0="reference", alias="reference", weights="dz"):
+    from hellblast_amd.hp_compat import _plan
+    mmax = lmax if mmax is None else mmax
+    return _plan(nside, lmax, mmax, precision, weights, phi0, alias)
+
+
+def _oracle_kw(tabs, phi0, alias, mmax):
+    ph = tabs["phi0_healpix"] if phi0 == "healpix" else None
+    mc = np.full(tabs["Nr"], mmax + 1) if alias == "fold" else None
+    return ph, mc
+
+
+CASES = [(1, 2), (2, 5), (4, 8), (8, 16), (8, 24), (16, 40), (32, 64), (64, 128)]
+
+
+@pytest.mark.parametrize("nside,lmax", CASES)
+def test_map2alm_f64_matches_oracle(gpu, nside, lmax):
+    print(f"[case] map2alm f64 nside={nside} lmax={lmax}", flush=True)
+    tabs = OG.ring_tables(nside)
+    rng = np.random.default_rng(100 + nside)
+    f = rng.standard_normal(tabs["npix"])
+    ref = OS.map2alm(f, tabs, lmax)
+    got = _plan(nside, lmax).map2alm(torch.as_tensor(f, device="cuda")).cpu().numpy()
+    assert _rel(got, ref) <= F64_TOL
+
+
+@pytest.mark.parametrize("nside,lmax", CASES)
+def test_alm2map_f64_matches_oracle(gpu, nside, lmax):
+    print(f"[case] alm2map f64 nside={nside} lmax={lmax}", flush=True)
+    tabs = OG.ring_tables(nside)
+    alm = OS.synalm(1.0 / (np.arange(lmax + 1) + 1.0) ** 2, lmax, seed=200 + nside)
+    ref = OS.alm2map(alm, tabs, lmax)
+    got = _plan(nside, lmax).alm2map(torch.as_tensor(alm, device="cuda")).cpu().numpy()
+    assert _rel(got, ref) <= F64_TOL
+
+
+@pytest.mark.parametrize("phi0,alias", [("healpix", "reference"), ("reference", "fold"), ("healpix", "fold")])
+@pytest.mark.parametrize("nside,lmax,mmax", [(8, 24, 24), (16, 47, 30), (32, 95, 95)])
+def test_phase_options_match_oracle(gpu, nside, lmax, mmax, phi0, alias):
+    print(f"[case] options nside={nside} lmax={lmax} mmax={mmax} phi0={phi0} alias={alias}", flush=True)
+    tabs = OG.ring_tables(nside)
+    ph, mc = _oracle_kw(tabs, phi0, alias, mmax)
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(tabs["npix"])
+    plan = _plan(nside, lmax, mmax, phi0=phi0, alias=alias)
+    ref = OS.map2alm(f, tabs, lmax, mmax, phi0=ph, mcut=mc)
+    got = plan.map2alm(torch.as_tensor(f, device="cuda")).cpu().numpy()
+    assert _rel(got, ref) <= F64_TOL
+    alm = OS.synalm(np.ones(lmax + 1), lmax, mmax, seed=8)
+    refm = OS.alm2map(alm, tabs, lmax, mmax, phi0=ph, mcut=mc)
+    gotm = plan.alm2map(torch.as_tensor(alm, device="cuda")).cpu().numpy()
+    assert _rel(gotm, refm) <= F64_TOL
+
+
+def test_batched_equals_single(gpu):
+    nside, lmax = 16, 40
+    tabs = OG.ring_tables(nside)
+    rng = np.random.default_rng(3)
+    f = torch.as_tensor(rng.standard_normal((5, tabs["npix"])), device="cuda")
+    plan = _plan(nside, lmax)
+    batch = plan.map2alm(f)
+    for i in range(5):
+        assert torch.equal(batch[i], plan.map2alm(f[i]))
+    maps = plan.alm2map(batch)
+    for i in range(5):
+        assert torch.equal(maps[i], plan.alm2map(batch[i]))
+
+
+def test_f64_deterministic(gpu):
+    nside, lmax = 32, 80
+    tabs = OG.ring_tables(nside)
+    f = torch.as_tensor(np.random.default_rng(4).standard_normal(tabs["npix"]), device="cuda")
+    plan = _plan(nside, lmax)
+    a1 = plan.map2alm(f)
+    a2 = plan.map2alm(f)
+    assert torch.equal(a1, a2)
+    assert torch.equal(plan.alm2map(a1), plan.alm2map(a2))
+
+
+def test_adjoints_dot_product(gpu):
+    nside, lmax = 16, 40
+    tabs = OG.ring_tables(nside)
+    rng = np.random.default_rng(5)
+    plan = _plan(nside, lmax, phi0="healpix")
+    a = torch.as_tensor(OS.synalm(np.ones(lmax + 1), lmax, seed=9), device="cuda")
+    g = torch.as_tensor(rng.standard_normal(tabs["npix"]), device="cuda")
+    lhs = torch.dot(plan.alm2map(a), g).item()
+    adj = plan.alm2map_adjoint(g)
+    rhs = torch.sum((adj.conj() * a).real).item()
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+    ga = torch.as_tensor(OS.synalm(np.ones(lmax + 1), lmax, seed=10), device="cuda")
+    lhs2 = torch.sum((ga.conj() * plan.map2alm(g)).real).item()
+    rhs2 = torch.dot(plan.map2alm_adjoint(ga), g).item()
+    assert abs(lhs2 - rhs2) <= 1e-11 * abs(lhs2)
+    # and against the oracle's transposes
+    ref = OS.alm2map_adjoint(g.cpu().numpy(), tabs, lmax, phi0=tabs["phi0_healpix"])
+    assert _rel(adj.cpu().numpy(), ref) <= F64_TOL
+    ref2 = OS.map2alm_adjoint(ga.cpu().numpy(), tabs, lmax, phi0=tabs["phi0_healpix"])
+    assert _rel(plan.map2alm_adjoint(ga).cpu().numpy(), ref2) <= F64_TOL
+
+
+def test_alm2cl_matches_oracle(gpu):
+    from hellblast_amd import alm2cl
+    lmax, mmax = 60, 45
+    alm = OS.synalm(np.ones(lmax + 1), lmax, mmax, seed=11)
+    got = alm2cl(torch.as_tensor(alm, device="cuda"), lmax=lmax, mmax=mmax).cpu().numpy()
+    assert _rel(got, OS.alm2cl(alm, lmax, mmax)) <= 1e-14
+
+
+def test_reference_signature_matches_literal_fallback(gpu):
+    """map2alm_gpu(rings, thetas, nphi, weights_dz, lmax) == sht_fallback restated literally."""
+    from hellblast_amd import alm_ops, map2alm_gpu
+    nside, lmax = 16, 30
+    tabs = OG.ring_tables(nside)
+    f = np.random.default_rng(12).standard_normal(tabs["npix"])
+    rings = OG.flat_to_rings(f, tabs)
+    ref = OL.sht_fallback_map2alm(rings, tabs["theta_centers"], tabs["nphi"], tabs["weights_dz"], lmax)
+    got = map2alm_gpu([torch.as_tensor(r) for r in rings], tabs["theta_centers"], tabs["nphi"],
+                      tabs["weights_dz"], lmax)
+    got_ref = alm_ops.to_reference_layout(got, lmax).cpu().numpy()
+    assert _rel(got_ref, ref) <= F64_TOL
+    # sht_raw (the reference's dispatched path) differs by 2 pi and truncates m at max nphi//2
+    raw = OL.sht_raw_map2alm(rings, tabs["theta_centers"], tabs["nphi"], tabs["weights_dz"], lmax)
+    assert _rel(got_ref, 2.0 * math.pi * raw) <= F64_TOL
+
+
+def test_reference_synthesis_matches_literal_isht(gpu):
+    """alm2map_gpu equals isht restated literally on rings with nphi > 2 mmax."""
+    from hellblast_amd import alm2map_gpu, alm_ops
+    nside, lmax = 16, 30
+    tabs = OG.ring_tables(nside)
+    alm = OS.synalm(1.0 / (np.arange(lmax + 1) + 1.0) ** 2, lmax, seed=13)
+    ref = OL.isht_alm2map(OS.mmajor_to_ref(alm, lmax), tabs["theta_centers"], tabs["nphi"], lmax)
+    got = alm2map_gpu(torch.as_tensor(alm, device="cuda"), tabs["theta_centers"], tabs["nphi"], lmax)
+    for r in range(tabs["Nr"]):
+        if tabs["nphi"][r] > 2 * lmax:
+            assert _rel(got[r].cpu().numpy(), ref[r]) <= F64_TOL
+
+
+def test_config1_roundtrip(gpu):
+    """C1: nside 64, lmax 128, alm2map then map2alm (iter=0) vs the oracle."""
+    from hellblast_amd import hp_compat
+    nside, lmax = 64, 128
+    cl = 1.0 / (np.arange(lmax + 1) + 1.0) ** 2
+    alm = OS.synalm(cl, lmax, seed=1)
+    alm[0] = 0.0
+    tabs = OG.ring_tables(nside)
+    m = hp_compat.alm2map(torch.as_tensor(alm, device="cuda"), nside, lmax=lmax)
+    assert _rel(m.cpu().numpy(), OS.alm2map(alm, tabs, lmax)) <= F64_TOL
+    a2 = hp_compat.map2alm(m, lmax=lmax)
+    ref2 = OS.map2alm(m.cpu().numpy(), tabs, lmax)
+    assert _rel(a2.cpu().numpy(), ref2) <= F64_TOL
+
+
+@pytest.mark.parametrize("nside,lmax", [(32, 64), (64, 128), (128, 256)])
+def test_f32_within_gate(gpu, nside, lmax):
+    tabs = OG.ring_tables(nside)
+    alm = OS.synalm((np.arange(lmax + 1) + 10.0) ** -2, lmax, seed=14)
+    ref_map = OS.alm2map(alm, tabs, lmax)
+    ref_alm = OS.map2alm(ref_map, tabs, lmax)
+    p32 = _plan(nside, lmax, precision="f32")
+    m32 = p32.alm2map(torch.as_tensor(alm, device="cuda"), dtype=torch.float32)
+    emap = _rel(m32.double().cpu().numpy(), ref_map)
+    a32 = p32.map2alm(torch.as_tensor(ref_map, device="cuda"))
+    emax, erms = OS.alm_errors(a32.cpu().numpy().astype(np.complex128), ref_alm, lmax)
+    print(f"f32 nside={nside}: map rel {emap:.2e} alm eps_max {emax:.2e} eps_rms {erms:.2e}")
+    assert emap <= 1e-5 and emax <= 1e-5
