@@ -280,6 +280,67 @@ int hbx_map2alm_adjoint(const hbx_plan *plan, int32_t nmaps, const void *alm, in
     return synth(plan, nmaps, alm, alm_stride, maps, map_dtype, map_stride, ws, ws_bytes_, stream, 1);
 }
 
+static uint64_t phase_bytes_of(const Plan *p, int nmaps) {
+    const uint64_t csz = p->precision == HBX_F64 ? 16 : 8;
+    return (uint64_t)nmaps * phase_elems_per_map(p) * csz;
+}
+
+int hbx_phase_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes) {
+    if (!plan || !bytes) return set_error(HBX_EINVAL, "NULL argument");
+    if (nmaps < 0) return set_error(HBX_EINVAL, "nmaps must be >= 0");
+    *bytes = phase_bytes_of(&plan->impl, nmaps);
+    return HBX_OK;
+}
+
+static int check_phase(const Plan *p, int nmaps, const void *phase, uint64_t phase_bytes) {
+    if (!phase || phase_bytes < phase_bytes_of(p, nmaps))
+        return set_error(HBX_EWORKSPACE, "phase buffer too small: need " + std::to_string(phase_bytes_of(p, nmaps)) +
+                                             " bytes");
+    return HBX_OK;
+}
+
+int hbx_ring_analysis(const hbx_plan *plan, int32_t nmaps, const void *maps, int32_t map_dtype, int64_t map_stride,
+                      void *phase, uint64_t phase_bytes, void *stream) {
+    int rc = check_exec(plan, nmaps, maps, phase, nullptr, 0, 0);
+    if (rc || nmaps == 0) return rc;
+    const Plan *p = &plan->impl;
+    if ((rc = check_strides(p, map_dtype, map_stride, p->nlm))) return rc;
+    if ((rc = check_phase(p, nmaps, phase, phase_bytes))) return rc;
+    return launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, phase, p->d_scale_ana, (hipStream_t)stream);
+}
+
+int hbx_legendre_analysis(const hbx_plan *plan, int32_t nmaps, const void *phase, uint64_t phase_bytes, void *alm,
+                          int64_t alm_stride, void *stream) {
+    int rc = check_exec(plan, nmaps, phase, alm, nullptr, 0, 0);
+    if (rc || nmaps == 0) return rc;
+    const Plan *p = &plan->impl;
+    if ((rc = check_strides(p, HBX_DT_F64, p->npix, alm_stride))) return rc;
+    if ((rc = check_phase(p, nmaps, phase, phase_bytes))) return rc;
+    return launch_leg_analysis(p, nmaps, phase, alm, alm_stride, 0, (hipStream_t)stream);
+}
+
+int hbx_legendre_synthesis(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t alm_stride, void *phase,
+                           uint64_t phase_bytes, void *ws, uint64_t ws_bytes_, void *stream) {
+    int rc = check_exec(plan, nmaps, alm, phase, ws, ws_bytes_, 1);
+    if (rc || nmaps == 0) return rc;
+    const Plan *p = &plan->impl;
+    if ((rc = check_strides(p, HBX_DT_F64, p->npix, alm_stride))) return rc;
+    if ((rc = check_phase(p, nmaps, phase, phase_bytes))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = launch_prescale(p, nmaps, alm, alm_stride, ws, 0, s))) return rc;
+    return launch_leg_synthesis(p, nmaps, ws, phase, s);
+}
+
+int hbx_ring_synthesis(const hbx_plan *plan, int32_t nmaps, const void *phase, uint64_t phase_bytes, void *maps,
+                       int32_t map_dtype, int64_t map_stride, void *stream) {
+    int rc = check_exec(plan, nmaps, phase, maps, nullptr, 0, 0);
+    if (rc || nmaps == 0) return rc;
+    const Plan *p = &plan->impl;
+    if ((rc = check_strides(p, map_dtype, map_stride, p->nlm))) return rc;
+    if ((rc = check_phase(p, nmaps, phase, phase_bytes))) return rc;
+    return launch_ring_synthesis(p, nmaps, phase, maps, map_dtype, map_stride, nullptr, (hipStream_t)stream);
+}
+
 int hbx_alm2cl(int32_t lmax, int32_t mmax, int32_t nmaps, const void *alm, int32_t alm_dtype, int64_t alm_stride,
                double *cl, int64_t cl_stride, void *stream) {
     if (lmax < 0 || mmax < 0 || mmax > lmax) return set_error(HBX_EINVAL, "need 0 <= mmax <= lmax");

@@ -221,6 +221,48 @@ class Plan:
         """Transpose of map2alm (complex-gradient convention)."""
         return self._synthesis(_abi.lib().hbx_map2alm_adjoint, alm, dtype, out)
 
+    # ------------------------------------------------------------------
+    # stage entry points: maps <-> phase buffer <-> alm
+    # ------------------------------------------------------------------
+    def phase_buffer(self, nmaps):
+        b = ctypes.c_uint64()
+        _abi.check(_abi.lib().hbx_phase_bytes(self._h, int(nmaps), ctypes.byref(b)))
+        return torch.empty(max(int(b.value), 1), dtype=torch.uint8, device=self.device)
+
+    def ring_analysis(self, maps, phase):
+        m, _ = self._as_maps(maps)
+        with torch.cuda.device(self.device):
+            _abi.check(_abi.lib().hbx_ring_analysis(
+                self._h, m.shape[0], ctypes.c_void_p(m.data_ptr()), _map_dtype_code(m.dtype), m.shape[1],
+                ctypes.c_void_p(phase.data_ptr()), phase.numel(), _stream_ptr(self.device)))
+        return phase
+
+    def legendre_analysis(self, phase, nmaps, out=None):
+        alm = torch.empty((nmaps, self.nalm), dtype=self.alm_dtype, device=self.device) if out is None else out
+        with torch.cuda.device(self.device):
+            _abi.check(_abi.lib().hbx_legendre_analysis(
+                self._h, int(nmaps), ctypes.c_void_p(phase.data_ptr()), phase.numel(),
+                ctypes.c_void_p(alm.data_ptr()), alm.shape[-1], _stream_ptr(self.device)))
+        return alm
+
+    def legendre_synthesis(self, alm, phase, ws=None):
+        a, _ = self._as_alm(alm)
+        ws = self.workspace(a.shape[0]) if ws is None else ws
+        with torch.cuda.device(self.device):
+            _abi.check(_abi.lib().hbx_legendre_synthesis(
+                self._h, a.shape[0], ctypes.c_void_p(a.data_ptr()), a.shape[1], ctypes.c_void_p(phase.data_ptr()),
+                phase.numel(), ctypes.c_void_p(ws.data_ptr()), ws.numel(), _stream_ptr(self.device)))
+        return phase
+
+    def ring_synthesis(self, phase, nmaps, dtype=torch.float64, out=None):
+        maps = torch.empty((nmaps, self.npix), dtype=dtype, device=self.device) if out is None else out
+        with torch.cuda.device(self.device):
+            _abi.check(_abi.lib().hbx_ring_synthesis(
+                self._h, int(nmaps), ctypes.c_void_p(phase.data_ptr()), phase.numel(),
+                ctypes.c_void_p(maps.data_ptr()), _map_dtype_code(maps.dtype), maps.shape[-1],
+                _stream_ptr(self.device)))
+        return maps
+
 
 _CACHE = OrderedDict()
 _CACHE_MAX = 8

@@ -121,6 +121,32 @@ int hbx_map2alm_adjoint(const hbx_plan *plan, int32_t nmaps, const void *alm,
                         int64_t map_stride, void *ws, uint64_t ws_bytes,
                         void *stream);
 
+/* Stage entry points (the two halves of every transform).  The phase
+ * buffer holds F_r(m) / G_r(m) for every ring and m <= mmax as complex
+ * (precision of the plan) in the m-blocked ring-major layout
+ * [map][m/8][ring][m%8]; hbx_phase_bytes() gives its size.  They let a
+ * caller time or overlap the stages, or exchange the phase buffer between
+ * devices (ring-parallel FFT stage, m-parallel Legendre stage).
+ *   hbx_ring_analysis      - per-ring FFT loop      sht_raw.py:38-43
+ *   hbx_legendre_analysis  - per-m Legendre loop    sht_raw.py:46-56
+ *   hbx_legendre_synthesis - per-m l-sum            isht.py:11-23
+ *   hbx_ring_synthesis     - per-ring inverse FFT   isht.py:16-25
+ * hbx_legendre_synthesis needs ws of hbx_workspace_bytes() bytes.          */
+int hbx_phase_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
+int hbx_ring_analysis(const hbx_plan *plan, int32_t nmaps, const void *maps,
+                      int32_t map_dtype, int64_t map_stride, void *phase,
+                      uint64_t phase_bytes, void *stream);
+int hbx_legendre_analysis(const hbx_plan *plan, int32_t nmaps,
+                          const void *phase, uint64_t phase_bytes, void *alm,
+                          int64_t alm_stride, void *stream);
+int hbx_legendre_synthesis(const hbx_plan *plan, int32_t nmaps,
+                           const void *alm, int64_t alm_stride, void *phase,
+                           uint64_t phase_bytes, void *ws, uint64_t ws_bytes,
+                           void *stream);
+int hbx_ring_synthesis(const hbx_plan *plan, int32_t nmaps, const void *phase,
+                       uint64_t phase_bytes, void *maps, int32_t map_dtype,
+                       int64_t map_stride, void *stream);
+
 /* C_l = (|a_l0|^2 + 2 sum_{m>=1} |a_lm|^2)/(2l+1) for nmaps m-major alm of
  * complex128 (alm_dtype HBX_DT_F64) or complex64 (HBX_DT_F32); cl is FP64
  * [nmaps][lmax+1] with cl_stride elements between maps.                     */
