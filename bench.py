@@ -243,7 +243,7 @@ def main():
     ph = p32.phase_buffer(B)
     ws = p32.workspace(B)
     t_ra = time_stage(torch, lambda: p32.ring_analysis(maps32, ph))
-    t_la = time_stage(torch, lambda: p32.legendre_analysis(ph, B, out=out_alm))
+    t_la = time_stage(torch, lambda: p32.legendre_analysis(ph, B, out=out_alm, ws=ws))
     t_ls = time_stage(torch, lambda: p32.legendre_synthesis(alm32b, ph, ws))
     t_rs = time_stage(torch, lambda: p32.ring_synthesis(ph, B, dtype=torch.float32, out=out_map))
     stages = {"ring_analysis_ms": t_ra, "legendre_analysis_ms": t_la, "legendre_synthesis_ms": t_ls,
@@ -256,7 +256,8 @@ def main():
                 "flops_per_launch": w_dir * B, "launch_ms": round(dom_ms, 4)}
 
     ph64 = p64.phase_buffer(1)
-    t64_la = time_stage(torch, lambda: p64.legendre_analysis(p64.ring_analysis(map64, ph64), 1, out=out64a), 2)
+    ws64 = p64.workspace(1)
+    t64_la = time_stage(torch, lambda: p64.legendre_analysis(p64.ring_analysis(map64, ph64), 1, out=out64a, ws=ws64), 2)
     f64_ach = w_dir / (t64_la * 1e-3) / 1e12
 
     cpu = None

@@ -82,7 +82,7 @@ SIGNATURES = {
     "hbx_alm2cl": (ctypes.c_int, [_i32, _i32, _i32, _vp, _i32, _i64, _vp, _i64, _vp]),
     "hbx_phase_bytes": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_u64)]),
     "hbx_ring_analysis": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _u64, _vp]),
-    "hbx_legendre_analysis": (ctypes.c_int, [_vp, _i32, _vp, _u64, _vp, _i64, _vp]),
+    "hbx_legendre_analysis": (ctypes.c_int, [_vp, _i32, _vp, _u64, _vp, _i64, _vp, _u64, _vp]),
     "hbx_legendre_synthesis": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _u64, _vp, _u64, _vp]),
     "hbx_ring_synthesis": (ctypes.c_int, [_vp, _i32, _vp, _u64, _vp, _i32, _i64, _vp]),
 }

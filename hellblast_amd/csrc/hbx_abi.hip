@@ -27,10 +27,18 @@ int64_t scaled_elems_per_map(const Plan *p) { return p->nrow; }
 
 static uint64_t align256(uint64_t v) { return (v + 255) & ~uint64_t(255); }
 
+// workspace = phase buffer | max(analysis partial sums, synthesis scaled alm)
+static uint64_t ws_tail_offset(const Plan *p, int nmaps) {
+    const uint64_t csz = p->precision == HBX_F64 ? 16 : 8;
+    return align256((uint64_t)nmaps * phase_elems_per_map(p) * csz);
+}
+
 static uint64_t ws_bytes(const Plan *p, int nmaps) {
     const uint64_t csz = p->precision == HBX_F64 ? 16 : 8;
-    return align256((uint64_t)nmaps * phase_elems_per_map(p) * csz) +
-           align256((uint64_t)nmaps * scaled_elems_per_map(p) * csz);
+    const int g = analysis_groups(p);
+    const uint64_t partial = g > 1 ? (uint64_t)nmaps * g * p->nlm * csz : 0;
+    const uint64_t scaled = (uint64_t)nmaps * scaled_elems_per_map(p) * csz;
+    return ws_tail_offset(p, nmaps) + align256(partial > scaled ? partial : scaled);
 }
 
 static int validate_geom(const hbx_geom *g, int32_t lmax, int32_t mmax) {
@@ -238,8 +246,9 @@ int hbx_map2alm(const hbx_plan *plan, int32_t nmaps, const void *maps, int32_t m
     const Plan *p = &plan->impl;
     if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
     hipStream_t s = (hipStream_t)stream;
+    void *tail = static_cast<char *>(ws) + ws_tail_offset(p, nmaps);
     if ((rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, p->d_scale_ana, s))) return rc;
-    return launch_leg_analysis(p, nmaps, ws, alm, alm_stride, 0, s);
+    return launch_leg_analysis(p, nmaps, ws, alm, alm_stride, tail, 0, s);
 }
 
 int hbx_alm2map_adjoint(const hbx_plan *plan, int32_t nmaps, const void *maps, int32_t map_dtype,
@@ -250,8 +259,9 @@ int hbx_alm2map_adjoint(const hbx_plan *plan, int32_t nmaps, const void *maps, i
     const Plan *p = &plan->impl;
     if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
     hipStream_t s = (hipStream_t)stream;
+    void *tail = static_cast<char *>(ws) + ws_tail_offset(p, nmaps);
     if ((rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, nullptr, s))) return rc;
-    return launch_leg_analysis(p, nmaps, ws, alm, alm_stride, 1, s);
+    return launch_leg_analysis(p, nmaps, ws, alm, alm_stride, tail, 1, s);
 }
 
 static int synth(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t alm_stride, void *maps,
@@ -262,9 +272,8 @@ static int synth(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t a
     const Plan *p = &plan->impl;
     if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    const uint64_t csz = p->precision == HBX_F64 ? 16 : 8;
     void *phase = ws;
-    void *scaled = static_cast<char *>(ws) + align256((uint64_t)nmaps * phase_elems_per_map(p) * csz);
+    void *scaled = static_cast<char *>(ws) + ws_tail_offset(p, nmaps);
     if ((rc = launch_prescale(p, nmaps, alm, alm_stride, scaled, adjoint, s))) return rc;
     if ((rc = launch_leg_synthesis(p, nmaps, scaled, phase, s))) return rc;
     return launch_ring_synthesis(p, nmaps, phase, maps, map_dtype, map_stride, adjoint ? p->d_scale_adj : nullptr, s);
@@ -310,13 +319,13 @@ int hbx_ring_analysis(const hbx_plan *plan, int32_t nmaps, const void *maps, int
 }
 
 int hbx_legendre_analysis(const hbx_plan *plan, int32_t nmaps, const void *phase, uint64_t phase_bytes, void *alm,
-                          int64_t alm_stride, void *stream) {
-    int rc = check_exec(plan, nmaps, phase, alm, nullptr, 0, 0);
+                          int64_t alm_stride, void *ws, uint64_t ws_bytes_, void *stream) {
+    int rc = check_exec(plan, nmaps, phase, alm, ws, ws_bytes_, 1);
     if (rc || nmaps == 0) return rc;
     const Plan *p = &plan->impl;
     if ((rc = check_strides(p, HBX_DT_F64, p->npix, alm_stride))) return rc;
     if ((rc = check_phase(p, nmaps, phase, phase_bytes))) return rc;
-    return launch_leg_analysis(p, nmaps, phase, alm, alm_stride, 0, (hipStream_t)stream);
+    return launch_leg_analysis(p, nmaps, phase, alm, alm_stride, ws, 0, (hipStream_t)stream);
 }
 
 int hbx_legendre_synthesis(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t alm_stride, void *phase,

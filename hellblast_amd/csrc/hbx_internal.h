@@ -86,7 +86,8 @@ int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *map
                           hipStream_t s);
 int plan_build_recursion(Plan *p, const std::vector<double> &pair_sin, double tol_log2);
 int launch_leg_analysis(const Plan *p, int nmaps, const void *phase, void *alm,
-                        int64_t alm_stride, int adjoint, hipStream_t s);
+                        int64_t alm_stride, void *partial, int adjoint, hipStream_t s);
+int analysis_groups(const Plan *p);   // ring-pair groups (partial sums) per m
 int launch_prescale(const Plan *p, int nmaps, const void *alm, int64_t alm_stride,
                     void *scaled, int adjoint, hipStream_t s);
 int launch_leg_synthesis(const Plan *p, int nmaps, const void *scaled, void *phase,

@@ -29,7 +29,6 @@
 namespace hbx {
 
 constexpr int NEVER = 0x7fffffff;
-static int allow_analysis_lds();
 
 // ---------------------------------------------------------------------------
 // plan-time kernels
@@ -190,10 +189,7 @@ int plan_build_recursion(Plan *p, const std::vector<double> &pair_sin, double to
     }
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(d_sin);
-    if (e == hipSuccess) {
-        const int rc2 = allow_analysis_lds();
-        if (rc2) { (void)hipFree(d_mant); (void)hipFree(d_ex); return rc2; }
-    }
+
     (void)hipFree(d_mant);
     (void)hipFree(d_ex);
     if (e != hipSuccess) return hip_error(e, "recursion seed kernels");
@@ -215,37 +211,41 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // ---------------------------------------------------------------------------
-// analysis: phase -> alm.  One workgroup per (m, map); lane slots cover all
-// ring pairs (R per lane); per l-chunk the 16 partial sums are reduced over
-// the workgroup through LDS.
+// analysis: phase -> alm.  One wave per (m, ring-pair group, map); each lane
+// holds R ring pairs ("slots", 64 consecutive pairs per slot), so a wave
+// covers 64*R pairs and runs with no workgroup barrier.  Per l-chunk the 16
+// partial sums (8 l x re/im) are reduced over the wave by an LDS transpose:
+// every lane writes its 16 values, lane j then sums value j/4 over 16
+// source lanes and two xor-shuffles finish the sum.  With a single group the
+// wave writes s_lm * sum straight into alm; otherwise it writes raw partial
+// sums per group and group_reduce_kernel adds them in a fixed order.
 // ---------------------------------------------------------------------------
 template <typename T, int R>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(64)
 leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__restrict__ alm,
-                    int64_t astride, const int32_t *__restrict__ pair_n, const int32_t *__restrict__ pair_s,
-                    const double *__restrict__ px, const int64_t *__restrict__ rofs,
-                    const double *__restrict__ alpha, const double *__restrict__ sl,
-                    const int32_t *__restrict__ lseed, const double2 *__restrict__ seed, int npairs, int nr,
-                    int lmax, int mmax, int adjoint) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *red = reinterpret_cast<T *>(smem_raw);          // [16][nt]
-    int &wg_start = *reinterpret_cast<int *>(smem_raw + 2 * CHUNK * blockDim.x * sizeof(T));
-    int m, item;
-    xcd_map(blockIdx.x, MB, &m, &item);
-    m = m * MB + item;
+                    int64_t astride, c2<T> *__restrict__ partial, const int32_t *__restrict__ pair_n,
+                    const int32_t *__restrict__ pair_s, const double *__restrict__ px,
+                    const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
+                    const double *__restrict__ sl, const int32_t *__restrict__ lseed,
+                    const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
+                    int64_t nlm, int adjoint) {
+    __shared__ T red[2 * CHUNK * 64];
+    int mb, item;
+    xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
+    const int m = mb * MB + item / ngroups;
+    const int grp = item % ngroups;
     if (m > mmax) return;
     const int b = blockIdx.y;
-    const int nt = blockDim.x, tid = threadIdx.x, lane = tid & 63;
-    const int w = uniform(tid >> 6), nw = nt >> 6;
+    const int lane = threadIdx.x;
     const c2<T> *ph = phase + (int64_t)b * pstride + (int64_t)(m >> 3) * nr * MB + (m & 7);
 
     double x[R], u1[R], u2[R];
     c2<T> E[R], O[R];
-    int ls[R], pp[R], sst[R], sinj[R];
+    int ls[R], sst[R], sinj[R];
+    const int p0 = grp * 64 * R + lane;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const int p = (k * nw + w) * 64 + lane;
-        pp[k] = p;
+        const int p = p0 + 64 * k;
         u1[k] = 0.0;
         u2[k] = 0.0;
         if (p < npairs) {
@@ -265,23 +265,18 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
         sst[k] = wave_min(ls[k]);
         sinj[k] = wave_max(ls[k] <= lmax ? ls[k] : -1);
     }
-    int mine = NEVER;
+    int lstart = NEVER;
 #pragma unroll
-    for (int k = 0; k < R; ++k) mine = min(mine, sst[k]);
-    if (tid == 0) wg_start = NEVER;
-    __syncthreads();
-    if (lane == 0) atomicMin(&wg_start, mine);
-    __syncthreads();
-    const int lstart = wg_start;
+    for (int k = 0; k < R; ++k) lstart = min(lstart, sst[k]);
 
-    c2<T> *out = alm + (int64_t)b * astride;
-    const int64_t ro = rofs[m];
-    const int per = nt / 16;          // threads per reduced value
-    const int v_id = tid / per, sub = tid % per;
+    const bool direct = (ngroups == 1);
+    c2<T> *dst = direct ? alm + (int64_t)b * astride : partial + ((int64_t)b * ngroups + grp) * nlm;
     const int64_t i0 = alm_index(m, m, lmax);
-    // coefficients below the switch-on of every ring pair are exactly zero
-    for (int l = m + tid; l < (lstart > lmax ? lmax + 1 : lstart); l += nt) out[i0 + (l - m)] = mk<T>(0, 0);
+    const int64_t ro = rofs[m];
+    // rows below every slot's switch-on are zero
+    for (int l = m + lane; l < (lstart > lmax ? lmax + 1 : lstart); l += 64) dst[i0 + (l - m)] = mk<T>(0, 0);
 
+    const int v_id = lane >> 2, q0 = (lane & 3) * 16;
     for (int l0 = lstart; l0 <= lmax; l0 += CHUNK) {
         double a[CHUNK];
 #pragma unroll
@@ -293,7 +288,7 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
         for (int k = 0; k < R; ++k) {
             if (l0 < sst[k]) continue;
             if (l0 <= sinj[k] && ls[k] == l0) {
-                const double2 sd = seed[(int64_t)m * npairs + pp[k]];
+                const double2 sd = seed[(int64_t)m * npairs + p0 + 64 * k];
                 u2[k] = sd.x;
                 u1[k] = sd.y;
             }
@@ -312,27 +307,62 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
             u2[k] = v2;
             u1[k] = v1;
         }
-        // workgroup reduction of 16 values
+        // wave-level transpose-reduce of the 16 values
 #pragma unroll
-        for (int v = 0; v < 2 * CHUNK; ++v) red[v * nt + tid] = acc[v];
-        __syncthreads();
+        for (int v = 0; v < 2 * CHUNK; ++v) red[v * 64 + lane] = acc[v];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         T s = (T)0;
-        for (int q = sub; q < nt; q += per) s += red[v_id * nt + q];
-        for (int o = per >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (sub == 0) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) s += red[v_id * 64 + q0 + q];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if ((lane & 3) == 0) {
             const int i = v_id >> 1, c = v_id & 1;
             const int l = l0 + i;
             if (l <= lmax) {
-                double val = (double)s * sl[ro + (l0 - m) + i];
-                if (adjoint) {
-                    if (m > 0) val *= 2.0;
-                    else if (c == 1) val = 0.0;
+                T val = s;
+                if (direct) {
+                    double dv = (double)s * sl[ro + (l0 - m) + i];
+                    if (adjoint) {
+                        if (m > 0) dv *= 2.0;
+                        else if (c == 1) dv = 0.0;
+                    }
+                    val = (T)dv;
                 }
-                T *dst = reinterpret_cast<T *>(out + i0 + (l - m));
-                dst[c] = (T)val;
+                reinterpret_cast<T *>(dst + i0 + (l - m))[c] = val;
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// alm[idx(l,m)] = s_lm * sum_g partial[g][idx(l,m)] (fixed summation order)
+template <typename T>
+__global__ void group_reduce_kernel(const c2<T> *__restrict__ partial, c2<T> *__restrict__ alm, int64_t astride,
+                                    const int64_t *__restrict__ rofs, const double *__restrict__ sl, int lmax,
+                                    int ngroups, int64_t nlm, int adjoint) {
+    const int m = blockIdx.x, b = blockIdx.y;
+    const int64_t i0 = alm_index(m, m, lmax);
+    const int64_t ro = rofs[m];
+    const c2<T> *src = partial + (int64_t)b * ngroups * nlm + i0;
+    c2<T> *dst = alm + (int64_t)b * astride + i0;
+    for (int j = threadIdx.x; j <= lmax - m; j += blockDim.x) {
+        double re = 0.0, im = 0.0;
+        for (int g = 0; g < ngroups; ++g) {
+            const c2<T> v = src[(int64_t)g * nlm + j];
+            re += (double)v.x;
+            im += (double)v.y;
+        }
+        double s = sl[ro + j];
+        if (adjoint) {
+            if (m > 0) s *= 2.0;
+            else im = 0.0;
+        }
+        dst[j] = mk<T>((T)(re * s), (T)(im * s));
     }
 }
 
@@ -442,72 +472,61 @@ __global__ void alm2cl_kernel(const c2<T> *__restrict__ alm, int64_t astride, do
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// Threads per workgroup (a power of two in [64, 512], so that the 16-way
-// LDS reduction splits into aligned shuffle groups) and ring-pair slots per
-// lane (a power of two <= 8).
-static void analysis_shape(const Plan *p, int *R, int *nt) {
-    int t = 64;
-    while (t < p->npairs && t < 512) t *= 2;
+// Ring-pair slots per lane: FP64 holds 8 (E/O in registers cost 8 VGPRs
+// each), FP32 holds 16; fewer when the geometry has fewer pairs.
+template <typename T> static int analysis_slots(const Plan *p) {
+    const int rmax = sizeof(T) == 8 ? 8 : 16;
     int r = 1;
-    while ((int64_t)r * t < p->npairs && r < 8) r *= 2;
-    *R = r;
-    *nt = t;
+    while (r < rmax && (int64_t)r * 64 < p->npairs) r *= 2;
+    return r;
 }
 
-template <typename T> static size_t analysis_lds_bytes(int nt) {
-    return (size_t)2 * CHUNK * nt * sizeof(T) + 16;
-}
-
-template <typename T, typename K> static int raise_lds(K kernel, size_t bytes) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(leg_analysis_kernel)");
-    return 0;
-}
-
-// the 512-thread FP64 analysis reduction buffer is just above 64 KiB
-static int allow_analysis_lds() {
-    int rc;
-    if ((rc = raise_lds<double>(leg_analysis_kernel<double, 1>, analysis_lds_bytes<double>(512)))) return rc;
-    if ((rc = raise_lds<double>(leg_analysis_kernel<double, 2>, analysis_lds_bytes<double>(512)))) return rc;
-    if ((rc = raise_lds<double>(leg_analysis_kernel<double, 4>, analysis_lds_bytes<double>(512)))) return rc;
-    if ((rc = raise_lds<double>(leg_analysis_kernel<double, 8>, analysis_lds_bytes<double>(512)))) return rc;
-    return 0;
+int analysis_groups(const Plan *p) {
+    const int r = p->precision == HBX_F64 ? analysis_slots<double>(p) : analysis_slots<float>(p);
+    return (int)((p->npairs + 64 * r - 1) / (64 * r));
 }
 
 template <typename T>
 static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, c2<T> *alm, int64_t astride,
-                          int adjoint, hipStream_t s) {
-    int R, nt;
-    analysis_shape(p, &R, &nt);
-    if ((int64_t)R * nt < p->npairs)
-        return set_error(HBX_EUNSUPPORTED, "too many ring pairs for the analysis kernel (max 4096)");
-    const int nmpad = (p->mmax + 1 + 63) / 64 * 64;   // xcd_map covers whole groups of 64
-    dim3 grid(nmpad, nmaps);
-    const size_t lds = analysis_lds_bytes<T>(nt);
+                          c2<T> *partial, int adjoint, hipStream_t s) {
+    const int R = analysis_slots<T>(p);
+    const int G = analysis_groups(p);
+    const int nmb8 = (p->nmb + 7) / 8 * 8;
+    dim3 grid(nmb8 * MB * G, nmaps);
     const int64_t pstride = phase_elems_per_map(p);
-#define HBX_LA(RV)                                                                                      \
-    hipLaunchKernelGGL((leg_analysis_kernel<T, RV>), grid, dim3(nt), lds, s, phase, pstride, alm, astride, \
-                       p->d_pair_n, p->d_pair_s, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed,  \
-                       p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, adjoint)
+#define HBX_LA(RV)                                                                                          \
+    hipLaunchKernelGGL((leg_analysis_kernel<T, RV>), grid, dim3(64), 0, s, phase, pstride, alm, astride,    \
+                       partial, p->d_pair_n, p->d_pair_s, p->d_px, p->d_rofs, p->d_alpha, p->d_sl,          \
+                       p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, G, p->nlm, adjoint)
     switch (R) {
     case 1: HBX_LA(1); break;
     case 2: HBX_LA(2); break;
     case 4: HBX_LA(4); break;
     case 8: HBX_LA(8); break;
+    case 16:
+        if constexpr (sizeof(T) == 4) { HBX_LA(16); break; }
+        return set_error(HBX_EUNSUPPORTED, "analysis slot count");
     default: return set_error(HBX_EUNSUPPORTED, "analysis slot count");
     }
 #undef HBX_LA
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "leg_analysis_kernel launch");
+    if (G > 1) {
+        hipLaunchKernelGGL(group_reduce_kernel<T>, dim3(p->mmax + 1, nmaps), dim3(256), 0, s, partial, alm, astride,
+                           p->d_rofs, p->d_sl, p->lmax, G, p->nlm, adjoint);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_error(e, "group_reduce_kernel launch");
+    }
     return 0;
 }
 
-int launch_leg_analysis(const Plan *p, int nmaps, const void *phase, void *alm, int64_t astride,
+int launch_leg_analysis(const Plan *p, int nmaps, const void *phase, void *alm, int64_t astride, void *partial,
                         int adjoint, hipStream_t s) {
     if (p->precision == HBX_F64)
-        return leg_analysis_t<double>(p, nmaps, (const double2 *)phase, (double2 *)alm, astride, adjoint, s);
-    return leg_analysis_t<float>(p, nmaps, (const float2 *)phase, (float2 *)alm, astride, adjoint, s);
+        return leg_analysis_t<double>(p, nmaps, (const double2 *)phase, (double2 *)alm, astride, (double2 *)partial,
+                                      adjoint, s);
+    return leg_analysis_t<float>(p, nmaps, (const float2 *)phase, (float2 *)alm, astride, (float2 *)partial,
+                                 adjoint, s);
 }
 
 int launch_prescale(const Plan *p, int nmaps, const void *alm, int64_t astride, void *scaled, int adjoint,

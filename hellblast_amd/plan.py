@@ -237,12 +237,14 @@ class Plan:
                 ctypes.c_void_p(phase.data_ptr()), phase.numel(), _stream_ptr(self.device)))
         return phase
 
-    def legendre_analysis(self, phase, nmaps, out=None):
+    def legendre_analysis(self, phase, nmaps, out=None, ws=None):
         alm = torch.empty((nmaps, self.nalm), dtype=self.alm_dtype, device=self.device) if out is None else out
+        ws = self.workspace(nmaps) if ws is None else ws
         with torch.cuda.device(self.device):
             _abi.check(_abi.lib().hbx_legendre_analysis(
                 self._h, int(nmaps), ctypes.c_void_p(phase.data_ptr()), phase.numel(),
-                ctypes.c_void_p(alm.data_ptr()), alm.shape[-1], _stream_ptr(self.device)))
+                ctypes.c_void_p(alm.data_ptr()), alm.shape[-1], ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                _stream_ptr(self.device)))
         return alm
 
     def legendre_synthesis(self, alm, phase, ws=None):

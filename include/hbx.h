@@ -131,14 +131,15 @@ int hbx_map2alm_adjoint(const hbx_plan *plan, int32_t nmaps, const void *alm,
  *   hbx_legendre_analysis  - per-m Legendre loop    sht_raw.py:46-56
  *   hbx_legendre_synthesis - per-m l-sum            isht.py:11-23
  *   hbx_ring_synthesis     - per-ring inverse FFT   isht.py:16-25
- * hbx_legendre_synthesis needs ws of hbx_workspace_bytes() bytes.          */
+ * The Legendre stages need ws of hbx_workspace_bytes() bytes.             */
 int hbx_phase_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
 int hbx_ring_analysis(const hbx_plan *plan, int32_t nmaps, const void *maps,
                       int32_t map_dtype, int64_t map_stride, void *phase,
                       uint64_t phase_bytes, void *stream);
 int hbx_legendre_analysis(const hbx_plan *plan, int32_t nmaps,
                           const void *phase, uint64_t phase_bytes, void *alm,
-                          int64_t alm_stride, void *stream);
+                          int64_t alm_stride, void *ws, uint64_t ws_bytes,
+                          void *stream);
 int hbx_legendre_synthesis(const hbx_plan *plan, int32_t nmaps,
                            const void *alm, int64_t alm_stride, void *phase,
                            uint64_t phase_bytes, void *ws, uint64_t ws_bytes,
