@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libhbx.so")
+# HBX_LIBRARY selects another build of the library (e.g. a debug build)
+LIB_PATH = os.environ.get("HBX_LIBRARY") or os.path.join(_HERE, "libhbx.so")
 
 HBX_OK = 0
 HBX_EINVAL = -1

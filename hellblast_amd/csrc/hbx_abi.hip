@@ -1,6 +1,7 @@
 // C ABI of libhbx.so: argument validation, plan construction and the
 // execute entry points.  Validation is host-only so it can be exercised
 // without a GPU; execute functions only enqueue kernels on the given stream.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -35,7 +36,8 @@ static uint64_t ws_tail_offset(const Plan *p, int nmaps) {
 
 static uint64_t ws_bytes(const Plan *p, int nmaps) {
     const uint64_t csz = p->precision == HBX_F64 ? 16 : 8;
-    const int g = analysis_groups(p);
+    int g = analysis_groups(p);
+    if (p->precision == HBX_F32) g = std::max(g, mfma_analysis_groups(p));
     const uint64_t partial = g > 1 ? (uint64_t)nmaps * g * p->nlm * csz : 0;
     const uint64_t scaled = (uint64_t)nmaps * scaled_elems_per_map(p) * csz;
     return ws_tail_offset(p, nmaps) + align256(partial > scaled ? partial : scaled);
@@ -132,6 +134,7 @@ static int build_plan(Plan *p, const hbx_geom *g) {
     // contributions far below each format's rounding error)
     const double tol = p->precision == HBX_F64 ? -64.0 : -40.0;
     if ((rc = plan_build_recursion(p, psin, tol))) return rc;
+    if (p->precision == HBX_F32 && (rc = prepare_mfma_kernels())) return rc;
     return 0;
 }
 
@@ -152,6 +155,24 @@ static int check_exec(const hbx_plan *plan, int32_t nmaps, const void *a, const 
         return set_error(HBX_EPLAN, "plan was created on device " + std::to_string(plan->impl.device) +
                                         " but device " + std::to_string(dev) + " is current");
     return HBX_OK;
+}
+
+// Legendre analysis: matrix cores for FP32 batches, VALU otherwise
+static int legendre_analysis(const Plan *p, int nmaps, const void *phase, void *alm, int64_t alm_stride,
+                             void *partial, int adjoint, hipStream_t s) {
+    if (p->precision == HBX_F32 && nmaps >= MFMA_MIN_MAPS)
+        return launch_leg_analysis_mfma(p, nmaps, phase, alm, alm_stride, partial, adjoint, s);
+    return launch_leg_analysis(p, nmaps, phase, alm, alm_stride, partial, adjoint, s);
+}
+
+// Legendre synthesis: matrix cores for FP32 batches, VALU otherwise
+static int legendre_synthesis(const Plan *p, int nmaps, const void *alm, int64_t alm_stride, void *phase,
+                              void *scaled, int adjoint, hipStream_t s) {
+    if (p->precision == HBX_F32 && nmaps >= MFMA_MIN_MAPS)
+        return launch_leg_synthesis_mfma(p, nmaps, alm, alm_stride, phase, adjoint, s);
+    int rc = launch_prescale(p, nmaps, alm, alm_stride, scaled, adjoint, s);
+    if (rc) return rc;
+    return launch_leg_synthesis(p, nmaps, scaled, phase, s);
 }
 
 static int check_strides(const Plan *p, int32_t map_dtype, int64_t map_stride, int64_t alm_stride) {
@@ -248,7 +269,7 @@ int hbx_map2alm(const hbx_plan *plan, int32_t nmaps, const void *maps, int32_t m
     hipStream_t s = (hipStream_t)stream;
     void *tail = static_cast<char *>(ws) + ws_tail_offset(p, nmaps);
     if ((rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, p->d_scale_ana, s))) return rc;
-    return launch_leg_analysis(p, nmaps, ws, alm, alm_stride, tail, 0, s);
+    return legendre_analysis(p, nmaps, ws, alm, alm_stride, tail, 0, s);
 }
 
 int hbx_alm2map_adjoint(const hbx_plan *plan, int32_t nmaps, const void *maps, int32_t map_dtype,
@@ -261,7 +282,7 @@ int hbx_alm2map_adjoint(const hbx_plan *plan, int32_t nmaps, const void *maps, i
     hipStream_t s = (hipStream_t)stream;
     void *tail = static_cast<char *>(ws) + ws_tail_offset(p, nmaps);
     if ((rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, nullptr, s))) return rc;
-    return launch_leg_analysis(p, nmaps, ws, alm, alm_stride, tail, 1, s);
+    return legendre_analysis(p, nmaps, ws, alm, alm_stride, tail, 1, s);
 }
 
 static int synth(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t alm_stride, void *maps,
@@ -274,8 +295,7 @@ static int synth(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t a
     hipStream_t s = (hipStream_t)stream;
     void *phase = ws;
     void *scaled = static_cast<char *>(ws) + ws_tail_offset(p, nmaps);
-    if ((rc = launch_prescale(p, nmaps, alm, alm_stride, scaled, adjoint, s))) return rc;
-    if ((rc = launch_leg_synthesis(p, nmaps, scaled, phase, s))) return rc;
+    if ((rc = legendre_synthesis(p, nmaps, alm, alm_stride, phase, scaled, adjoint, s))) return rc;
     return launch_ring_synthesis(p, nmaps, phase, maps, map_dtype, map_stride, adjoint ? p->d_scale_adj : nullptr, s);
 }
 
@@ -325,7 +345,7 @@ int hbx_legendre_analysis(const hbx_plan *plan, int32_t nmaps, const void *phase
     const Plan *p = &plan->impl;
     if ((rc = check_strides(p, HBX_DT_F64, p->npix, alm_stride))) return rc;
     if ((rc = check_phase(p, nmaps, phase, phase_bytes))) return rc;
-    return launch_leg_analysis(p, nmaps, phase, alm, alm_stride, ws, 0, (hipStream_t)stream);
+    return legendre_analysis(p, nmaps, phase, alm, alm_stride, ws, 0, (hipStream_t)stream);
 }
 
 int hbx_legendre_synthesis(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t alm_stride, void *phase,
@@ -335,9 +355,7 @@ int hbx_legendre_synthesis(const hbx_plan *plan, int32_t nmaps, const void *alm,
     const Plan *p = &plan->impl;
     if ((rc = check_strides(p, HBX_DT_F64, p->npix, alm_stride))) return rc;
     if ((rc = check_phase(p, nmaps, phase, phase_bytes))) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    if ((rc = launch_prescale(p, nmaps, alm, alm_stride, ws, 0, s))) return rc;
-    return launch_leg_synthesis(p, nmaps, ws, phase, s);
+    return legendre_synthesis(p, nmaps, alm, alm_stride, phase, ws, 0, (hipStream_t)stream);
 }
 
 int hbx_ring_synthesis(const hbx_plan *plan, int32_t nmaps, const void *phase, uint64_t phase_bytes, void *maps,

@@ -28,6 +28,7 @@ int hip_error(hipError_t e, const char *what);
 // ---------------------------------------------------------------------------
 constexpr int MB = 8;          // m values per m-block of the phase buffer
 constexpr int CHUNK = 8;       // l steps per recursion chunk
+constexpr int TAILPAD = 64;    // zero coefficients after the last recursion row
 
 inline int64_t nalm(int lmax, int mmax) {
     return (int64_t)(mmax + 1) * (lmax + 1) - (int64_t)mmax * (mmax + 1) / 2;
@@ -92,6 +93,16 @@ int launch_prescale(const Plan *p, int nmaps, const void *alm, int64_t alm_strid
                     void *scaled, int adjoint, hipStream_t s);
 int launch_leg_synthesis(const Plan *p, int nmaps, const void *scaled, void *phase,
                          hipStream_t s);
+int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t alm_stride,
+                              void *phase, int adjoint, hipStream_t s);
+int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *alm, int64_t alm_stride,
+                             void *partial, int adjoint, hipStream_t s);
+int launch_group_reduce(const Plan *p, int nmaps, void *partial, void *alm, int64_t alm_stride,
+                        int ngroups, int adjoint, hipStream_t s);
+int mfma_analysis_groups(const Plan *p);
+int prepare_mfma_kernels();
+// batches at least this large use the matrix-core Legendre kernels (FP32 plans)
+constexpr int MFMA_MIN_MAPS = 2;
 int launch_alm2cl(int lmax, int mmax, int nmaps, const void *alm, int dt,
                   int64_t alm_stride, double *cl, int64_t cl_stride, hipStream_t s);
 

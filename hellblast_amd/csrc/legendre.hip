@@ -158,8 +158,12 @@ int plan_build_recursion(Plan *p, const std::vector<double> &pair_sin, double to
     };
     int rc;
     if ((rc = dalloc((void **)&p->d_rofs, rofs.size() * sizeof(int64_t)))) return rc;
-    if ((rc = dalloc((void **)&p->d_alpha, p->nrow * sizeof(double)))) return rc;
-    if ((rc = dalloc((void **)&p->d_sl, p->nrow * sizeof(double)))) return rc;
+    // TAILPAD zero entries after the last row: chunked kernels may read a
+    // few coefficients past l = lmax of the last m (they multiply zeros)
+    if ((rc = dalloc((void **)&p->d_alpha, (p->nrow + TAILPAD) * sizeof(double)))) return rc;
+    if ((rc = dalloc((void **)&p->d_sl, (p->nrow + TAILPAD) * sizeof(double)))) return rc;
+    HBX_HIP(hipMemset(p->d_alpha + p->nrow, 0, TAILPAD * sizeof(double)));
+    HBX_HIP(hipMemset(p->d_sl + p->nrow, 0, TAILPAD * sizeof(double)));
     const int64_t nseed = (int64_t)(mmax + 1) * np;
     if ((rc = dalloc((void **)&p->d_lseed, nseed * sizeof(int32_t)))) return rc;
     if ((rc = dalloc((void **)&p->d_seed, nseed * sizeof(double2)))) return rc;
@@ -265,9 +269,17 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
         sst[k] = wave_min(ls[k]);
         sinj[k] = wave_max(ls[k] <= lmax ? ls[k] : -1);
     }
+    constexpr int GS = R < 4 ? R : 4;
+    int gst[R / GS];
     int lstart = NEVER;
 #pragma unroll
-    for (int k = 0; k < R; ++k) lstart = min(lstart, sst[k]);
+    for (int k0 = 0; k0 < R; k0 += GS) {
+        int v = NEVER;
+#pragma unroll
+        for (int k = k0; k < k0 + GS; ++k) v = min(v, sst[k]);
+        gst[k0 / GS] = v;
+        lstart = min(lstart, v);
+    }
 
     const bool direct = (ngroups == 1);
     c2<T> *dst = direct ? alm + (int64_t)b * astride : partial + ((int64_t)b * ngroups + grp) * nlm;
@@ -284,28 +296,32 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
         T acc[2 * CHUNK];
 #pragma unroll
         for (int i = 0; i < 2 * CHUNK; ++i) acc[i] = (T)0;
+        // slots advance in groups of GS with their steps interleaved, so
+        // GS independent recursion chains hide the FP64 latency
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            if (l0 < sst[k]) continue;
-            if (l0 <= sinj[k] && ls[k] == l0) {
-                const double2 sd = seed[(int64_t)m * npairs + p0 + 64 * k];
-                u2[k] = sd.x;
-                u1[k] = sd.y;
+        for (int k0 = 0; k0 < R; k0 += GS) {
+            if (l0 < gst[k0 / GS]) continue;
+#pragma unroll
+            for (int k = k0; k < k0 + GS; ++k) {
+                if (l0 <= sinj[k] && ls[k] == l0) {
+                    const double2 sd = seed[(int64_t)m * npairs + p0 + 64 * k];
+                    u2[k] = sd.x;
+                    u1[k] = sd.y;
+                }
             }
-            double v2 = u2[k], v1 = u1[k];
-            const double xk = x[k];
 #pragma unroll
             for (int i = 0; i < CHUNK; ++i) {
-                const double u = fma(a[i] * xk, v1, -v2);
-                v2 = v1;
-                v1 = u;
-                const T uu = (T)u;
-                const c2<T> d = (i & 1) ? O[k] : E[k];
-                acc[2 * i] = fma(uu, d.x, acc[2 * i]);
-                acc[2 * i + 1] = fma(uu, d.y, acc[2 * i + 1]);
+#pragma unroll
+                for (int k = k0; k < k0 + GS; ++k) {
+                    const double u = fma(a[i] * x[k], u1[k], -u2[k]);
+                    u2[k] = u1[k];
+                    u1[k] = u;
+                    const T uu = (T)u;
+                    const c2<T> d = (i & 1) ? O[k] : E[k];
+                    acc[2 * i] = fma(uu, d.x, acc[2 * i]);
+                    acc[2 * i + 1] = fma(uu, d.y, acc[2 * i + 1]);
+                }
             }
-            u2[k] = v2;
-            u1[k] = v1;
         }
         // wave-level transpose-reduce of the 16 values
 #pragma unroll
@@ -511,12 +527,22 @@ static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, c2<T> *a
 #undef HBX_LA
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "leg_analysis_kernel launch");
-    if (G > 1) {
-        hipLaunchKernelGGL(group_reduce_kernel<T>, dim3(p->mmax + 1, nmaps), dim3(256), 0, s, partial, alm, astride,
-                           p->d_rofs, p->d_sl, p->lmax, G, p->nlm, adjoint);
-        e = hipGetLastError();
-        if (e != hipSuccess) return hip_error(e, "group_reduce_kernel launch");
-    }
+    if (G > 1) return launch_group_reduce(p, nmaps, partial, alm, astride, G, adjoint, s);
+    return 0;
+}
+
+int launch_group_reduce(const Plan *p, int nmaps, void *partial, void *alm, int64_t astride, int ngroups,
+                        int adjoint, hipStream_t s) {
+    if (p->precision == HBX_F64)
+        hipLaunchKernelGGL(group_reduce_kernel<double>, dim3(p->mmax + 1, nmaps), dim3(256), 0, s,
+                           (const double2 *)partial, (double2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups,
+                           p->nlm, adjoint);
+    else
+        hipLaunchKernelGGL(group_reduce_kernel<float>, dim3(p->mmax + 1, nmaps), dim3(256), 0, s,
+                           (const float2 *)partial, (float2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups,
+                           p->nlm, adjoint);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "group_reduce_kernel launch");
     return 0;
 }
 

@@ -1,0 +1,410 @@
+// Legendre synthesis on the matrix cores (FP32 plans, batches of maps).
+//
+// Per m the synthesis is a product  G[ring pair][n] = sum_l lambda_l(x_p) at[l][n]
+// with n = (map, re/im) over a group of 16 maps and at = s_lm * a_lm.  It maps
+// onto v_mfma_f32_32x32x2f32 (D[32x32] += A[32x2] B[2x32]):
+//   A[row = ring][k] = lambda_{l0+2k}(x_ring)   (FP64 recursion, cvt to FP32)
+//   B[k][col = n]    = at_{l0+2k}[n]            (staged in LDS per 64-l chunk)
+// one MFMA for the even l of a 4-l step and one for the odd l, so the two
+// accumulators are G_even and G_odd and the mirror rings follow as
+// G_n = G_even + G_odd, G_s = G_even - G_odd.
+// Lane layout of the 32x32x2 f32 instruction (checked on gfx950):
+//   A: lane = row + 32 k     B: lane = col + 32 k
+//   D[row][col]: lane = col + 32 ((row/4)%2), reg = (row%4) + 4 (row/8)
+// Reference counterpart: the per-(ring, m) l-sum of isht.py:15-21.
+#include "hbx_device.h"
+#include "hbx_internal.h"
+
+namespace hbx {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int SYN_LCHUNK = 64;     // l values staged per chunk
+constexpr int SYN_WAVES = 4;       // waves per workgroup (same m)
+constexpr int SYN_TILES = 1;       // 32-ring tiles per wave
+constexpr int SYN_PAIRS_WG = SYN_WAVES * SYN_TILES * 32;
+constexpr int NEVER_L = 0x7fffffff;
+
+__device__ __forceinline__ int wmin64(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int wmax64(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__global__ void __launch_bounds__(SYN_WAVES * 64)
+leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float *__restrict__ phase,
+                          const double *__restrict__ px, const int64_t *__restrict__ rofs,
+                          const double *__restrict__ alpha, const double *__restrict__ sl,
+                          const int32_t *__restrict__ lseed, const double2 *__restrict__ seed, int npairs, int nr,
+                          int lmax, int mmax, int nmb, int nsb, int nmaps, int adjoint) {
+    // per-chunk operands: at[l][n] and alpha_l
+    __shared__ float stage[1][SYN_LCHUNK][33];
+    __shared__ double alds[1][SYN_LCHUNK];
+    __shared__ int wg_start;
+    int mb, item;
+    xcd_map(blockIdx.x, MB * nsb, &mb, &item);
+    const int m = mb * MB + item / nsb;
+    const int sb = item % nsb;
+    if (m > mmax) return;
+    const int g = blockIdx.y;                       // group of 16 maps
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int r = lane & 31, h = lane >> 5;
+
+    double x[SYN_TILES], u1[SYN_TILES], u2[SYN_TILES];
+    double2 sd[SYN_TILES];
+    int ls[SYN_TILES];
+    int wstart = NEVER_L, winj = -1;
+#pragma unroll
+    for (int j = 0; j < SYN_TILES; ++j) {
+        const int p = (sb * SYN_WAVES * SYN_TILES + w * SYN_TILES + j) * 32 + r;
+        u1[j] = 0.0;
+        u2[j] = 0.0;
+        x[j] = 0.0;
+        ls[j] = NEVER_L;
+        sd[j] = make_double2(0.0, 0.0);
+        if (p < npairs) {
+            x[j] = px[p];
+            ls[j] = lseed[(int64_t)m * npairs + p];
+            if (ls[j] <= lmax) sd[j] = seed[(int64_t)m * npairs + p];
+        }
+        wstart = min(wstart, ls[j]);
+        if (ls[j] <= lmax) winj = max(winj, ls[j]);
+    }
+    wstart = wmin64(wstart);
+    winj = wmax64(winj);
+    if (tid == 0) wg_start = NEVER_L;
+    __syncthreads();
+    if (lane == 0) atomicMin(&wg_start, wstart);
+    __syncthreads();
+    const int lstart = wg_start;
+
+    const int64_t ro = rofs[m];
+    const double *al = alpha + ro - m;      // al[l] = alpha_lm (zero-padded past lmax)
+    const double *sv = sl + ro - m;
+    const double fadj = (adjoint && m > 0) ? 0.5 : 1.0;   // map2alm transpose: g/c_m
+    const int64_t i0 = alm_index(m, m, lmax) - m;   // alm[b][i0 + l]
+    // stage slots of this thread: l = c0 + (tid & 63), maps b = tid/64 + 4 q
+    constexpr int PER = 16 / SYN_WAVES;
+    const int sll = tid & 63, sb0 = tid >> 6;
+    const float2 *abase = alm + i0 + sll;
+    auto fetch = [&](int c0, float2 *v, double *a) {
+        const int l = c0 + sll;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int b = g * 16 + sb0 + SYN_WAVES * q;
+            float2 val = make_float2(0.f, 0.f);
+            if (l <= lmax && b < nmaps) {
+                const float2 av = abase[(int64_t)b * astride + c0];
+                const double f = sv[l] * fadj;
+                val = make_float2((float)(av.x * f), (float)(av.y * f));
+            }
+            v[q] = val;
+        }
+        *a = (tid < SYN_LCHUNK) ? al[c0 + tid] : 0.0;
+    };
+    auto put = [&](int buf, const float2 *v, double a) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int n = 2 * (sb0 + SYN_WAVES * q);
+            stage[buf][sll][n] = v[q].x;
+            stage[buf][sll][n + 1] = v[q].y;
+        }
+        if (tid < SYN_LCHUNK) alds[buf][tid] = a;
+    };
+
+    floatx16 De[SYN_TILES], Do[SYN_TILES];
+#pragma unroll
+    for (int j = 0; j < SYN_TILES; ++j) {
+        De[j] = floatx16{0};
+        Do[j] = floatx16{0};
+    }
+    // one buffer: the chunk's operands arrive in a single round trip and
+    // co-resident workgroups hide it (occupancy 3)
+    for (int c0 = lstart; c0 <= lmax; c0 += SYN_LCHUNK) {
+        {
+            float2 pv[PER];
+            double pa;
+            fetch(c0, pv, &pa);
+            put(0, pv, pa);
+        }
+        __syncthreads();
+        if (c0 + SYN_LCHUNK > wstart) {
+#pragma unroll 2
+            for (int kk = 0; kk < SYN_LCHUNK / 4; ++kk) {
+                const int l0 = c0 + 4 * kk;
+                if (l0 > lmax) break;
+                const double a0 = alds[0][4 * kk], a1 = alds[0][4 * kk + 1];
+                const double a2 = alds[0][4 * kk + 2], a3 = alds[0][4 * kk + 3];
+                float ae[SYN_TILES], ao[SYN_TILES];
+#pragma unroll
+                for (int j = 0; j < SYN_TILES; ++j) {
+                    if (l0 <= winj && ls[j] == l0) {
+                        u2[j] = sd[j].x;
+                        u1[j] = sd[j].y;
+                    }
+                    const double xj = x[j];
+                    const double v0 = fma(a0 * xj, u1[j], -u2[j]);
+                    const double v1 = fma(a1 * xj, v0, -u1[j]);
+                    const double v2 = fma(a2 * xj, v1, -v0);
+                    const double v3 = fma(a3 * xj, v2, -v1);
+                    u2[j] = v2;
+                    u1[j] = v3;
+                    ae[j] = (float)(h ? v2 : v0);
+                    ao[j] = (float)(h ? v3 : v1);
+                }
+                const float be = stage[0][4 * kk + 2 * h][r];
+                const float bo = stage[0][4 * kk + 2 * h + 1][r];
+#pragma unroll
+                for (int j = 0; j < SYN_TILES; ++j) {
+                    De[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(ae[j], be, De[j], 0, 0, 0);
+                    Do[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(ao[j], bo, Do[j], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // D[row = ring][col = n]: this lane holds col = r, rows (q%4) + 8 (q/4) + 4 h.
+    // Ring pair p couples ring p (north) and ring nr-1-p (south).
+    const int b = g * 16 + (r >> 1), c = r & 1;
+    if (b >= nmaps) return;
+    float *out = phase + (((int64_t)b * nmb + mb) * nr) * (2 * MB) + (m & 7) * 2 + c;
+#pragma unroll
+    for (int j = 0; j < SYN_TILES; ++j) {
+        const int base = (sb * SYN_WAVES * SYN_TILES + w * SYN_TILES + j) * 32;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int row = (q & 3) + 8 * (q >> 2) + 4 * h;
+            const int p = base + row;
+            if (p >= npairs) continue;
+            const float ge = De[j][q], go = Do[j][q];
+            const int rn = p, rs = nr - 1 - p;
+            out[(int64_t)rn * (2 * MB)] = ge + go;
+            if (rs != rn) out[(int64_t)rs * (2 * MB)] = ge - go;
+        }
+    }
+}
+
+int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t astride, void *phase,
+                              int adjoint, hipStream_t s) {
+    const int nsb = (p->npairs + SYN_PAIRS_WG - 1) / SYN_PAIRS_WG;
+    const int nmb8 = (p->nmb + 7) / 8 * 8;
+    dim3 grid(nmb8 * MB * nsb, (nmaps + 15) / 16);
+    hipLaunchKernelGGL(leg_synthesis_mfma_kernel, grid, dim3(SYN_WAVES * 64), 0, s, (const float2 *)alm, astride,
+                       (float *)phase, p->d_px, p->d_rofs, p->d_alpha, p->d_sl,
+                       p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, p->nmb, nsb, nmaps, adjoint);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "leg_synthesis_mfma_kernel launch");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Legendre analysis on the matrix cores (FP32 plans, batches of maps).
+//
+// Per m:  a[n][l] = sum_p S_par[p][n] lambda_l(x_p), n = (map, re/im) over 16
+// maps, par = parity of l-m (S_even = F_n + F_s, S_odd = F_n - F_s).  With
+// v_mfma_f32_16x16x4f32 (D[16x16] += A[16x4] B[4x16]) the ring pairs are the
+// contraction index:
+//   A[row = n][k] = S_par[ring k][n]             (held in registers all kernel)
+//   B[k][col]     = lambda_{c0+2col+par}(ring k) (FP64 recursion -> FP32 ->
+//                                                 wave-private LDS transpose)
+// A workgroup is 16 waves x 64 ring pairs of one m.  After its MFMAs each
+// wave parks its D tiles in its own (consumed) lambda tile; every thread then
+// sums one of the chunk's 32 l x 32 n outputs over the 16 waves in a fixed
+// order and writes it.  (LDS float atomics for this sum measured 4x slower.)
+// Lane layout of the 16x16x4 f32 instruction (checked on gfx950):
+//   A: lane = row + 16 k   B: lane = col + 16 k   D: lane = col + 16 (row/4), reg = row%4
+// Reference counterpart: the per-(m, l) ring sum of sht_raw.py:53 /
+// sht_fallback.py:63.
+// ---------------------------------------------------------------------------
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int ANA_WAVES = 16;
+constexpr int ANA_PAIRS_WG = ANA_WAVES * 64;
+constexpr int ANA_L = 32;                 // l per chunk: 16 even + 16 odd columns
+constexpr int ANA_STR = 17;               // padded LDS row (16 columns + 1)
+constexpr int ANA_LAM = 2 * 64 * ANA_STR; // floats of one wave's lambda tile
+constexpr size_t ANA_LDS = (size_t)(ANA_WAVES * ANA_LAM + 4 * ANA_L + 4) * sizeof(float);
+
+__global__ void __launch_bounds__(ANA_WAVES * 64, 1)
+leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, float2 *__restrict__ alm,
+                         int64_t astride, float2 *__restrict__ partial, const double *__restrict__ px,
+                         const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
+                         const double *__restrict__ sl, const int32_t *__restrict__ lseed,
+                         const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
+                         int64_t nlm, int nmaps, int adjoint) {
+    extern __shared__ __align__(16) float smem[];
+    int mb, item;
+    xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
+    const int m = mb * MB + item / ngroups;
+    const int grp = item % ngroups;
+    if (m > mmax) return;
+    const int g = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    float *lam = smem + w * ANA_LAM;
+    double *alds = reinterpret_cast<double *>(smem + ANA_WAVES * ANA_LAM);   // [2][32]
+    int &wg_start = *reinterpret_cast<int *>(smem + ANA_WAVES * ANA_LAM + 4 * ANA_L);
+
+    const int pw = grp * ANA_PAIRS_WG + w * 64;         // first ring pair of this wave
+    const int p = pw + lane;
+    double x = 0.0, u1 = 0.0, u2 = 0.0;
+    double2 sd = make_double2(0.0, 0.0);
+    int ls = NEVER_L;
+    if (p < npairs) {
+        x = px[p];
+        ls = lseed[(int64_t)m * npairs + p];
+        if (ls <= lmax) sd = seed[(int64_t)m * npairs + p];
+    }
+    const int wstart = wmin64(ls);
+    const int winj = wmax64(ls <= lmax ? ls : -1);
+
+    // A operands: S_par[ring pw + 4 s + lane/16][n = 16 tile + lane%16];
+    // ring pair q couples ring q (north) and ring nr-1-q (south)
+    float Sv[16][2][2];
+    {
+        const int c = lane & 1;
+        const float2 *ph = phase + (int64_t)mb * nr * MB + (m & 7);
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int pq = pw + 4 * s + (lane >> 4);
+            const int pp = min(pq, npairs - 1);
+            const bool self = (nr - 1 - pp) == pp;
+#pragma unroll
+            for (int tile = 0; tile < 2; ++tile) {
+                const int bq = g * 16 + tile * 8 + ((lane & 15) >> 1);
+                const float2 *pb = ph + (int64_t)min(bq, nmaps - 1) * pstride;
+                const float2 a = pb[(int64_t)pp * MB];
+                const float2 z = self ? make_float2(0.f, 0.f) : pb[(int64_t)(nr - 1 - pp) * MB];
+                const bool ok = pq < npairs && bq < nmaps;
+                Sv[s][tile][0] = ok ? (c ? a.y + z.y : a.x + z.x) : 0.0f;
+                Sv[s][tile][1] = ok ? (c ? a.y - z.y : a.x - z.x) : 0.0f;
+            }
+        }
+    }
+    if (tid == 0) wg_start = NEVER_L;
+    __syncthreads();
+    if (lane == 0) atomicMin(&wg_start, wstart);
+    __syncthreads();
+    const int lstart = wg_start;
+
+    const bool direct = (ngroups == 1);
+    const int64_t i0 = alm_index(m, m, lmax) - m;       // + l
+    const int64_t ro = rofs[m];
+    const double *al = alpha + ro - m;                  // zero-padded past lmax
+    const double *sv = sl + ro - m;
+    auto store = [&](int l, int n, float v) {
+        const int b = g * 16 + (n >> 1), c = n & 1;
+        if (b >= nmaps) return;
+        if (direct) {
+            double d = (double)v * sv[l];
+            if (adjoint) {
+                if (m > 0) d *= 2.0;
+                else if (c == 1) d = 0.0;
+            }
+            reinterpret_cast<float *>(alm + (int64_t)b * astride + i0 + l)[c] = (float)d;
+        } else {
+            reinterpret_cast<float *>(partial + ((int64_t)b * ngroups + grp) * nlm + i0 + l)[c] = v;
+        }
+    };
+    // coefficients below every ring pair's switch-on are zero
+    const int lz = lstart > lmax ? lmax + 1 : lstart;
+    for (int t = tid; t < (lz - m) * 32; t += ANA_WAVES * 64) store(m + (t >> 5), t & 31, 0.0f);
+    if (lstart > lmax) return;
+
+    // this thread's output element of a chunk and where each wave parks it
+    const int on = tid >> 5, olo = tid & 31;
+    const int ridx = (((on >> 4) * 2 + (olo & 1)) * 4 + (on & 3)) * 64 + (olo >> 1) + 16 * ((on & 15) >> 2);
+
+    if (tid < ANA_L) alds[tid] = al[lstart + tid];
+    __syncthreads();
+    int buf = 0;
+    for (int c0 = lstart; c0 <= lmax; c0 += ANA_L, buf ^= 1) {
+        const double *ab = alds + buf * ANA_L;
+        const double anext = (tid < ANA_L) ? al[c0 + ANA_L + tid] : 0.0;   // in flight this chunk
+        floatx4 D[2][2];
+#pragma unroll
+        for (int tile = 0; tile < 2; ++tile)
+#pragma unroll
+            for (int par = 0; par < 2; ++par) D[tile][par] = floatx4{0.f, 0.f, 0.f, 0.f};
+        if (c0 + ANA_L > wstart) {
+            // lambda for this lane's ring over l = c0 .. c0+31 into the wave tile
+#pragma unroll
+            for (int t = 0; t < ANA_L; ++t) {
+                if ((t & 7) == 0 && c0 <= winj && ls == c0 + t) {
+                    u2 = sd.x;
+                    u1 = sd.y;
+                }
+                const double u = fma(ab[t] * x, u1, -u2);
+                u2 = u1;
+                u1 = u;
+                lam[(t & 1) * 64 * ANA_STR + lane * ANA_STR + (t >> 1)] = (float)u;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+#pragma unroll
+                for (int par = 0; par < 2; ++par) {
+                    const float bop = lam[par * 64 * ANA_STR + (4 * s + (lane >> 4)) * ANA_STR + (lane & 15)];
+#pragma unroll
+                    for (int tile = 0; tile < 2; ++tile)
+                        D[tile][par] = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[s][tile][par], bop, D[tile][par], 0, 0, 0);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        // park D in this wave's (now consumed) lambda tile:
+        // [(tile*2 + par)*4 + reg][lane], n = 16 tile + 4 (lane/16) + reg, l = c0 + 2 (lane%16) + par
+#pragma unroll
+        for (int tile = 0; tile < 2; ++tile)
+#pragma unroll
+            for (int par = 0; par < 2; ++par)
+#pragma unroll
+                for (int reg = 0; reg < 4; ++reg) lam[((tile * 2 + par) * 4 + reg) * 64 + lane] = D[tile][par][reg];
+        if (tid < ANA_L) alds[(buf ^ 1) * ANA_L + tid] = anext;
+        __syncthreads();
+        float v = 0.0f;
+#pragma unroll
+        for (int q = 0; q < ANA_WAVES; ++q) v += smem[q * ANA_LAM + ridx];
+        const int l = c0 + olo;
+        if (l <= lmax) store(l, on, v);
+        __syncthreads();
+    }
+}
+
+int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *alm, int64_t astride, void *partial,
+                             int adjoint, hipStream_t s) {
+    const int G = (p->npairs + ANA_PAIRS_WG - 1) / ANA_PAIRS_WG;
+    const int nmb8 = (p->nmb + 7) / 8 * 8;
+    dim3 grid(nmb8 * MB * G, (nmaps + 15) / 16);
+    const int64_t pstride = phase_elems_per_map(p);
+    hipLaunchKernelGGL(leg_analysis_mfma_kernel, grid, dim3(ANA_WAVES * 64), ANA_LDS, s, (const float2 *)phase,
+                       pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,
+                       p->mmax, G, p->nlm, nmaps, adjoint);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "leg_analysis_mfma_kernel launch");
+    if (G > 1) return launch_group_reduce(p, nmaps, partial, alm, astride, G, adjoint, s);
+    return 0;
+}
+
+// raise the dynamic-LDS limit of the analysis kernel (called at plan creation)
+int prepare_mfma_kernels() {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_mfma_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)ANA_LDS);
+    if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(leg_analysis_mfma_kernel)");
+    return 0;
+}
+
+int mfma_analysis_groups(const Plan *p) { return (p->npairs + ANA_PAIRS_WG - 1) / ANA_PAIRS_WG; }
+
+}  // namespace hbx
