@@ -28,19 +28,54 @@ int64_t scaled_elems_per_map(const Plan *p) { return p->nrow; }
 
 static uint64_t align256(uint64_t v) { return (v + 255) & ~uint64_t(255); }
 
-// workspace = phase buffer | max(analysis partial sums, synthesis scaled alm)
-static uint64_t ws_tail_offset(const Plan *p, int nmaps) {
-    const uint64_t csz = p->precision == HBX_F64 ? 16 : 8;
-    return align256((uint64_t)nmaps * phase_elems_per_map(p) * csz);
+// Phase buffer: FP32/FP64 ring coefficients for every m, plus (FP32 plans)
+// the FP64 band m < M_LO.  Workspace: phase buffer | tail, where the tail
+// holds either the synthesis scaled alm or the analysis partial sums (and,
+// for FP32 plans, the FP64 band alm and its partial sums).
+static uint64_t csize(const Plan *p) { return p->precision == HBX_F64 ? 16 : 8; }
+
+static uint64_t phase_main_bytes(const Plan *p, int nmaps) {
+    return align256((uint64_t)nmaps * phase_elems_per_map(p) * csize(p));
 }
 
-static uint64_t ws_bytes(const Plan *p, int nmaps) {
-    const uint64_t csz = p->precision == HBX_F64 ? 16 : 8;
-    int g = analysis_groups(p);
+static uint64_t phase_total_bytes(const Plan *p, int nmaps) {
+    uint64_t b = phase_main_bytes(p, nmaps);
+    if (p->precision == HBX_F32) b += align256((uint64_t)nmaps * band_phase_elems(p) * 16);
+    return b;
+}
+
+static void *band_phase_ptr(const Plan *p, int nmaps, const void *phase) {
+    return static_cast<char *>(const_cast<void *>(phase)) + phase_main_bytes(p, nmaps);
+}
+
+struct AnalysisWs {
+    uint64_t partial = 0, band_alm = 0, band_partial = 0, total = 0;
+};
+
+static AnalysisWs analysis_ws(const Plan *p, int nmaps) {
+    AnalysisWs w;
+    const uint64_t cs = csize(p);
+    int g = analysis_groups(p, p->precision == HBX_F64);
     if (p->precision == HBX_F32) g = std::max(g, mfma_analysis_groups(p));
-    const uint64_t partial = g > 1 ? (uint64_t)nmaps * g * p->nlm * csz : 0;
-    const uint64_t scaled = (uint64_t)nmaps * scaled_elems_per_map(p) * csz;
-    return ws_tail_offset(p, nmaps) + align256(partial > scaled ? partial : scaled);
+    w.partial = 0;
+    uint64_t off = align256(g > 1 ? (uint64_t)nmaps * g * p->nlm * cs : 0);
+    if (p->precision == HBX_F32) {
+        const int g64 = analysis_groups(p, true);
+        w.band_alm = off;
+        off += align256((uint64_t)nmaps * band_nlm(p) * 16);
+        w.band_partial = off;
+        off += align256(g64 > 1 ? (uint64_t)nmaps * g64 * band_nlm(p) * 16 : 0);
+    }
+    w.total = off;
+    return w;
+}
+
+static uint64_t ws_tail_offset(const Plan *p, int nmaps) { return phase_total_bytes(p, nmaps); }
+
+static uint64_t ws_bytes(const Plan *p, int nmaps) {
+    const uint64_t scaled = align256((uint64_t)nmaps * scaled_elems_per_map(p) * csize(p));
+    const uint64_t ana = analysis_ws(p, nmaps).total;
+    return ws_tail_offset(p, nmaps) + std::max(scaled, ana);
 }
 
 static int validate_geom(const hbx_geom *g, int32_t lmax, int32_t mmax) {
@@ -157,12 +192,28 @@ static int check_exec(const hbx_plan *plan, int32_t nmaps, const void *a, const 
     return HBX_OK;
 }
 
-// Legendre analysis: matrix cores for FP32 batches, VALU otherwise
+// Ring stage of an analysis: FFTs for every m, plus the FP64 band of FP32 plans
+static int ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride, void *phase,
+                         const double *scale, hipStream_t s) {
+    int rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, phase, scale, s);
+    if (rc || p->precision != HBX_F32) return rc;
+    return launch_ring_band(p, nmaps, maps, map_dtype, map_stride, band_phase_ptr(p, nmaps, phase), scale, s);
+}
+
+// Legendre analysis: matrix cores for FP32 batches, VALU otherwise; FP32
+// plans take m < M_LO from the FP64 band.  tail: workspace after the phase.
 static int legendre_analysis(const Plan *p, int nmaps, const void *phase, void *alm, int64_t alm_stride,
-                             void *partial, int adjoint, hipStream_t s) {
+                             void *tail, int adjoint, hipStream_t s) {
+    const AnalysisWs w = analysis_ws(p, nmaps);
+    char *t = static_cast<char *>(tail);
+    int rc;
     if (p->precision == HBX_F32 && nmaps >= MFMA_MIN_MAPS)
-        return launch_leg_analysis_mfma(p, nmaps, phase, alm, alm_stride, partial, adjoint, s);
-    return launch_leg_analysis(p, nmaps, phase, alm, alm_stride, partial, adjoint, s);
+        rc = launch_leg_analysis_mfma(p, nmaps, phase, alm, alm_stride, t + w.partial, adjoint, s);
+    else
+        rc = launch_leg_analysis(p, nmaps, phase, alm, alm_stride, t + w.partial, adjoint, s);
+    if (rc || p->precision != HBX_F32) return rc;
+    return launch_leg_analysis_band(p, nmaps, band_phase_ptr(p, nmaps, phase), alm, alm_stride, t + w.band_alm,
+                                    t + w.band_partial, adjoint, s);
 }
 
 // Legendre synthesis: matrix cores for FP32 batches, VALU otherwise
@@ -268,7 +319,7 @@ int hbx_map2alm(const hbx_plan *plan, int32_t nmaps, const void *maps, int32_t m
     if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
     hipStream_t s = (hipStream_t)stream;
     void *tail = static_cast<char *>(ws) + ws_tail_offset(p, nmaps);
-    if ((rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, p->d_scale_ana, s))) return rc;
+    if ((rc = ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, p->d_scale_ana, s))) return rc;
     return legendre_analysis(p, nmaps, ws, alm, alm_stride, tail, 0, s);
 }
 
@@ -281,7 +332,7 @@ int hbx_alm2map_adjoint(const hbx_plan *plan, int32_t nmaps, const void *maps, i
     if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
     hipStream_t s = (hipStream_t)stream;
     void *tail = static_cast<char *>(ws) + ws_tail_offset(p, nmaps);
-    if ((rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, nullptr, s))) return rc;
+    if ((rc = ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, nullptr, s))) return rc;
     return legendre_analysis(p, nmaps, ws, alm, alm_stride, tail, 1, s);
 }
 
@@ -309,10 +360,7 @@ int hbx_map2alm_adjoint(const hbx_plan *plan, int32_t nmaps, const void *alm, in
     return synth(plan, nmaps, alm, alm_stride, maps, map_dtype, map_stride, ws, ws_bytes_, stream, 1);
 }
 
-static uint64_t phase_bytes_of(const Plan *p, int nmaps) {
-    const uint64_t csz = p->precision == HBX_F64 ? 16 : 8;
-    return (uint64_t)nmaps * phase_elems_per_map(p) * csz;
-}
+static uint64_t phase_bytes_of(const Plan *p, int nmaps) { return phase_total_bytes(p, nmaps); }
 
 int hbx_phase_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes) {
     if (!plan || !bytes) return set_error(HBX_EINVAL, "NULL argument");
@@ -335,7 +383,7 @@ int hbx_ring_analysis(const hbx_plan *plan, int32_t nmaps, const void *maps, int
     const Plan *p = &plan->impl;
     if ((rc = check_strides(p, map_dtype, map_stride, p->nlm))) return rc;
     if ((rc = check_phase(p, nmaps, phase, phase_bytes))) return rc;
-    return launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, phase, p->d_scale_ana, (hipStream_t)stream);
+    return ring_analysis(p, nmaps, maps, map_dtype, map_stride, phase, p->d_scale_ana, (hipStream_t)stream);
 }
 
 int hbx_legendre_analysis(const hbx_plan *plan, int32_t nmaps, const void *phase, uint64_t phase_bytes, void *alm,

@@ -29,6 +29,11 @@ int hip_error(hipError_t e, const char *what);
 constexpr int MB = 8;          // m values per m-block of the phase buffer
 constexpr int CHUNK = 8;       // l steps per recursion chunk
 constexpr int TAILPAD = 64;    // zero coefficients after the last recursion row
+// FP32 plans analyse m < M_LO in FP64: the ring coefficients of low m carry
+// the large-scale power, and rounding them to FP32 costs ~1e-5 of sqrt(C_l)
+// at high l for red spectra.
+constexpr int M_LO = 16;
+constexpr int NMB_LO = M_LO / MB;
 
 inline int64_t nalm(int lmax, int mmax) {
     return (int64_t)(mmax + 1) * (lmax + 1) - (int64_t)mmax * (mmax + 1) / 2;
@@ -88,7 +93,7 @@ int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *map
 int plan_build_recursion(Plan *p, const std::vector<double> &pair_sin, double tol_log2);
 int launch_leg_analysis(const Plan *p, int nmaps, const void *phase, void *alm,
                         int64_t alm_stride, void *partial, int adjoint, hipStream_t s);
-int analysis_groups(const Plan *p);   // ring-pair groups (partial sums) per m
+int analysis_groups(const Plan *p, bool f64);   // ring-pair groups (partial sums) per m
 int launch_prescale(const Plan *p, int nmaps, const void *alm, int64_t alm_stride,
                     void *scaled, int adjoint, hipStream_t s);
 int launch_leg_synthesis(const Plan *p, int nmaps, const void *scaled, void *phase,
@@ -98,7 +103,13 @@ int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t
 int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *alm, int64_t alm_stride,
                              void *partial, int adjoint, hipStream_t s);
 int launch_group_reduce(const Plan *p, int nmaps, void *partial, void *alm, int64_t alm_stride,
-                        int ngroups, int adjoint, hipStream_t s);
+                        int ngroups, int64_t nlm_stride, int mbeg, int mend, bool f64, int adjoint,
+                        hipStream_t s);
+int launch_leg_analysis_band(const Plan *p, int nmaps, const void *band_phase, void *alm,
+                             int64_t alm_stride, void *band_alm, void *band_partial, int adjoint,
+                             hipStream_t s);
+int launch_ring_band(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride,
+                     void *band_phase, const double *scale, hipStream_t s);
 int mfma_analysis_groups(const Plan *p);
 int prepare_mfma_kernels();
 // batches at least this large use the matrix-core Legendre kernels (FP32 plans)
@@ -107,6 +118,13 @@ int launch_alm2cl(int lmax, int mmax, int nmaps, const void *alm, int dt,
                   int64_t alm_stride, double *cl, int64_t cl_stride, hipStream_t s);
 
 int64_t phase_elems_per_map(const Plan *p);   // complex elements
+// FP32 plans: FP64 band phase [map][m/8 < NMB_LO][ring][m%8] and band alm (m < M_LO)
+inline int64_t band_phase_elems(const Plan *p) { return (int64_t)NMB_LO * p->nrings * MB; }
+// coefficients in rows m < min(M_LO, mmax+1) (rows are contiguous in m-major order)
+inline int64_t band_nlm(const Plan *p) {
+    const int mt = p->mmax + 1 < M_LO ? p->mmax + 1 : M_LO;
+    return alm_index(mt, mt, p->lmax);
+}
 int64_t scaled_elems_per_map(const Plan *p);  // complex elements
 
 }  // namespace hbx

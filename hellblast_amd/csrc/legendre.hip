@@ -232,13 +232,13 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
                     const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
                     const double *__restrict__ sl, const int32_t *__restrict__ lseed,
                     const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
-                    int64_t nlm, int adjoint) {
+                    int64_t nlm, int adjoint, int mbeg) {
     __shared__ T red[2 * CHUNK * 64];
     int mb, item;
     xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
     const int m = mb * MB + item / ngroups;
     const int grp = item % ngroups;
-    if (m > mmax) return;
+    if (m > mmax || m < mbeg) return;
     const int b = blockIdx.y;
     const int lane = threadIdx.x;
     const c2<T> *ph = phase + (int64_t)b * pstride + (int64_t)(m >> 3) * nr * MB + (m & 7);
@@ -360,8 +360,8 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
 template <typename T>
 __global__ void group_reduce_kernel(const c2<T> *__restrict__ partial, c2<T> *__restrict__ alm, int64_t astride,
                                     const int64_t *__restrict__ rofs, const double *__restrict__ sl, int lmax,
-                                    int ngroups, int64_t nlm, int adjoint) {
-    const int m = blockIdx.x, b = blockIdx.y;
+                                    int ngroups, int64_t nlm, int adjoint, int mbeg) {
+    const int m = mbeg + blockIdx.x, b = blockIdx.y;
     const int64_t i0 = alm_index(m, m, lmax);
     const int64_t ro = rofs[m];
     const c2<T> *src = partial + (int64_t)b * ngroups * nlm + i0;
@@ -497,23 +497,24 @@ template <typename T> static int analysis_slots(const Plan *p) {
     return r;
 }
 
-int analysis_groups(const Plan *p) {
-    const int r = p->precision == HBX_F64 ? analysis_slots<double>(p) : analysis_slots<float>(p);
+int analysis_groups(const Plan *p, bool f64) {
+    const int r = f64 ? analysis_slots<double>(p) : analysis_slots<float>(p);
     return (int)((p->npairs + 64 * r - 1) / (64 * r));
 }
 
 template <typename T>
-static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, c2<T> *alm, int64_t astride,
-                          c2<T> *partial, int adjoint, hipStream_t s) {
+static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, int64_t pstride, c2<T> *alm,
+                          int64_t astride, c2<T> *partial, int64_t nlm_stride, int mbeg, int mend, int adjoint,
+                          hipStream_t s) {
+    if (mend < mbeg) return 0;
     const int R = analysis_slots<T>(p);
-    const int G = analysis_groups(p);
-    const int nmb8 = (p->nmb + 7) / 8 * 8;
+    const int G = (int)((p->npairs + 64 * R - 1) / (64 * R));
+    const int nmb8 = (mend / MB + 1 + 7) / 8 * 8;
     dim3 grid(nmb8 * MB * G, nmaps);
-    const int64_t pstride = phase_elems_per_map(p);
 #define HBX_LA(RV)                                                                                          \
     hipLaunchKernelGGL((leg_analysis_kernel<T, RV>), grid, dim3(64), 0, s, phase, pstride, alm, astride,    \
                        partial, p->d_pair_n, p->d_pair_s, p->d_px, p->d_rofs, p->d_alpha, p->d_sl,          \
-                       p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, G, p->nlm, adjoint)
+                       p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, mend, G, nlm_stride, adjoint, mbeg)
     switch (R) {
     case 1: HBX_LA(1); break;
     case 2: HBX_LA(2); break;
@@ -527,32 +528,59 @@ static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, c2<T> *a
 #undef HBX_LA
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "leg_analysis_kernel launch");
-    if (G > 1) return launch_group_reduce(p, nmaps, partial, alm, astride, G, adjoint, s);
+    if (G > 1)
+        return launch_group_reduce(p, nmaps, partial, alm, astride, G, nlm_stride, mbeg, mend, sizeof(T) == 8,
+                                   adjoint, s);
     return 0;
 }
 
 int launch_group_reduce(const Plan *p, int nmaps, void *partial, void *alm, int64_t astride, int ngroups,
-                        int adjoint, hipStream_t s) {
-    if (p->precision == HBX_F64)
-        hipLaunchKernelGGL(group_reduce_kernel<double>, dim3(p->mmax + 1, nmaps), dim3(256), 0, s,
-                           (const double2 *)partial, (double2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups,
-                           p->nlm, adjoint);
+                        int64_t nlm_stride, int mbeg, int mend, bool f64, int adjoint, hipStream_t s) {
+    dim3 grid(mend - mbeg + 1, nmaps);
+    if (f64)
+        hipLaunchKernelGGL(group_reduce_kernel<double>, grid, dim3(256), 0, s, (const double2 *)partial,
+                           (double2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups, nlm_stride, adjoint, mbeg);
     else
-        hipLaunchKernelGGL(group_reduce_kernel<float>, dim3(p->mmax + 1, nmaps), dim3(256), 0, s,
-                           (const float2 *)partial, (float2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups,
-                           p->nlm, adjoint);
+        hipLaunchKernelGGL(group_reduce_kernel<float>, grid, dim3(256), 0, s, (const float2 *)partial,
+                           (float2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups, nlm_stride, adjoint, mbeg);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "group_reduce_kernel launch");
     return 0;
 }
 
+// FP64 band of an FP32 plan: alm[b][i] = (float) band[b][i] for i < n
+__global__ void band_to_f32_kernel(const double2 *__restrict__ band, int64_t n, float2 *__restrict__ alm,
+                                   int64_t astride) {
+    const int b = blockIdx.y;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = band[(int64_t)b * n + i];
+        alm[(int64_t)b * astride + i] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
 int launch_leg_analysis(const Plan *p, int nmaps, const void *phase, void *alm, int64_t astride, void *partial,
                         int adjoint, hipStream_t s) {
+    const int64_t pstride = phase_elems_per_map(p);
     if (p->precision == HBX_F64)
-        return leg_analysis_t<double>(p, nmaps, (const double2 *)phase, (double2 *)alm, astride, (double2 *)partial,
-                                      adjoint, s);
-    return leg_analysis_t<float>(p, nmaps, (const float2 *)phase, (float2 *)alm, astride, (float2 *)partial,
-                                 adjoint, s);
+        return leg_analysis_t<double>(p, nmaps, (const double2 *)phase, pstride, (double2 *)alm, astride,
+                                      (double2 *)partial, p->nlm, 0, p->mmax, adjoint, s);
+    return leg_analysis_t<float>(p, nmaps, (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial,
+                                 p->nlm, std::min(M_LO, p->mmax + 1), p->mmax, adjoint, s);
+}
+
+// m < M_LO of an FP32 plan in FP64: band phase -> band alm (FP64) -> alm
+int launch_leg_analysis_band(const Plan *p, int nmaps, const void *band_phase, void *alm, int64_t astride,
+                             void *band_alm, void *band_partial, int adjoint, hipStream_t s) {
+    const int mend = std::min(M_LO - 1, p->mmax);
+    const int64_t nb = band_nlm(p);
+    int rc = leg_analysis_t<double>(p, nmaps, (const double2 *)band_phase, band_phase_elems(p), (double2 *)band_alm,
+                                    nb, (double2 *)band_partial, nb, 0, mend, adjoint, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(band_to_f32_kernel, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024), nmaps),
+                       dim3(256), 0, s, (const double2 *)band_alm, nb, (float2 *)alm, astride);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "band_to_f32_kernel launch");
+    return 0;
 }
 
 int launch_prescale(const Plan *p, int nmaps, const void *alm, int64_t astride, void *scaled, int adjoint,

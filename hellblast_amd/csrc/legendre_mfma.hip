@@ -238,13 +238,13 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
                          const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
                          const double *__restrict__ sl, const int32_t *__restrict__ lseed,
                          const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
-                         int64_t nlm, int nmaps, int adjoint) {
+                         int64_t nlm, int nmaps, int adjoint, int mbeg) {
     extern __shared__ __align__(16) float smem[];
     int mb, item;
     xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
     const int m = mb * MB + item / ngroups;
     const int grp = item % ngroups;
-    if (m > mmax) return;
+    if (m > mmax || m < mbeg) return;
     const int g = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -384,16 +384,18 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
 
 int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *alm, int64_t astride, void *partial,
                              int adjoint, hipStream_t s) {
+    const int mbeg = std::min(M_LO, p->mmax + 1);    // m < M_LO run in the FP64 band
+    if (mbeg > p->mmax) return 0;
     const int G = (p->npairs + ANA_PAIRS_WG - 1) / ANA_PAIRS_WG;
     const int nmb8 = (p->nmb + 7) / 8 * 8;
     dim3 grid(nmb8 * MB * G, (nmaps + 15) / 16);
     const int64_t pstride = phase_elems_per_map(p);
     hipLaunchKernelGGL(leg_analysis_mfma_kernel, grid, dim3(ANA_WAVES * 64), ANA_LDS, s, (const float2 *)phase,
                        pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,
-                       p->mmax, G, p->nlm, nmaps, adjoint);
+                       p->mmax, G, p->nlm, nmaps, adjoint, mbeg);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "leg_analysis_mfma_kernel launch");
-    if (G > 1) return launch_group_reduce(p, nmaps, partial, alm, astride, G, adjoint, s);
+    if (G > 1) return launch_group_reduce(p, nmaps, partial, alm, astride, G, p->nlm, mbeg, p->mmax, false, adjoint, s);
     return 0;
 }
 

@@ -256,6 +256,93 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
 }
 
 // ---------------------------------------------------------------------------
+// FP64 low-m band of FP32 plans: direct DFT in FP64 for m < M_LO,
+//   F_r(m) = [m < mcut_r] scale_r e^{-i m phi0_r} sum_j f_j e^{-2 pi i m j / n}
+// (same conventions as ring_analysis_kernel), one workgroup per (ring, map).
+// e^{-2 pi i j/n} comes from sincospi per pixel, its powers by recurrence.
+// ---------------------------------------------------------------------------
+template <typename TIn>
+__global__ void __launch_bounds__(256)
+ring_band_kernel(const TIn *__restrict__ maps, int64_t map_stride, double2 *__restrict__ band, int64_t band_stride,
+                 const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
+                 const int32_t *__restrict__ mcut, const double *__restrict__ phi0, const double *__restrict__ scale,
+                 int mtop, int nr) {
+    __shared__ double part[4][2 * M_LO];
+    const int r = blockIdx.x, b = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int n = nphi[r];
+    const TIn *f = maps + (int64_t)b * map_stride + ring_start[r];
+    double re[M_LO], im[M_LO];
+#pragma unroll
+    for (int m = 0; m < M_LO; ++m) { re[m] = 0.0; im[m] = 0.0; }
+    const double inv = 2.0 / (double)n;
+    for (int j = tid; j < n; j += blockDim.x) {
+        const double v = (double)f[j];
+        double sn, cs;
+        sincospi(-(double)j * inv, &sn, &cs);
+        double wr = 1.0, wi = 0.0;
+#pragma unroll
+        for (int m = 0; m < M_LO; ++m) {
+            re[m] = fma(v, wr, re[m]);
+            im[m] = fma(v, wi, im[m]);
+            const double t = wr * cs - wi * sn;
+            wi = wr * sn + wi * cs;
+            wr = t;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < M_LO; ++m) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            re[m] += __shfl_xor(re[m], o, 64);
+            im[m] += __shfl_xor(im[m], o, 64);
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int m = 0; m < M_LO; ++m) { part[w][2 * m] = re[m]; part[w][2 * m + 1] = im[m]; }
+    }
+    __syncthreads();
+    if (tid <= mtop) {
+        const int m = tid;
+        double x = 0.0, y = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) { x += part[q][2 * m]; y += part[q][2 * m + 1]; }
+        double2 F = make_double2(0.0, 0.0);
+        if (m < mcut[r]) {
+            const double sc = scale ? scale[r] : 1.0;
+            x *= sc;
+            y *= sc;
+            const double ph = phi0 ? phi0[r] : 0.0;
+            if (ph != 0.0) {
+                double sn, cs;
+                sincos(-(double)m * ph, &sn, &cs);
+                F = make_double2(x * cs - y * sn, x * sn + y * cs);
+            } else {
+                F = make_double2(x, y);
+            }
+        }
+        band[(int64_t)b * band_stride + ((int64_t)(m >> 3) * nr + r) * MB + (m & 7)] = F;
+    }
+}
+
+int launch_ring_band(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride, void *band,
+                     const double *scale, hipStream_t s) {
+    const int mtop = std::min(M_LO - 1, p->mmax);
+    dim3 grid(p->nrings, nmaps);
+    if (map_dtype == HBX_DT_F64)
+        hipLaunchKernelGGL(ring_band_kernel<double>, grid, dim3(256), 0, s, (const double *)maps, map_stride,
+                           (double2 *)band, band_phase_elems(p), p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0,
+                           scale, mtop, p->nrings);
+    else
+        hipLaunchKernelGGL(ring_band_kernel<float>, grid, dim3(256), 0, s, (const float *)maps, map_stride,
+                           (double2 *)band, band_phase_elems(p), p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0,
+                           scale, mtop, p->nrings);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "ring_band_kernel launch");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // plan-time tables
 // ---------------------------------------------------------------------------
 template <typename T>
