@@ -2,7 +2,8 @@
 
 Drop-in for the SHT hot path of hellblast: ``map2alm_gpu`` (sht.py),
 ``alm2map_gpu`` (isht.py), ``anafast_ring`` (anafast.py), the ``hp_compat``
-SHT wrappers and ``HealpixRingTables``, running hand-written gfx950 HIP
+SHT and spin wrappers, ``spin`` / ``spin1`` (Q/U <-> E/B) and
+``HealpixRingTables``, running hand-written gfx950 HIP
 kernels (libhbx.so, C ABI in include/hbx.h) on PyTorch-ROCm tensors.
 
 Package aliases as in the reference (``__init__.py:125-127``):
@@ -13,6 +14,8 @@ from .sht import map2alm_gpu
 from .isht import alm2map_gpu
 from .anafast import anafast_ring
 from .alm_ops import alm2cl, almxfl, synalm, alm_index, getidx, getsize, to_reference_layout, from_reference_layout
+from .spin import map2alm_spin_gpu, eb_from_qu_gpu, qu_from_eb_gpu
+from .spin1 import map2alm_spin1_gpu, alm2map_spin1_gpu
 from . import hp_compat
 
 map2alm = map2alm_gpu
@@ -22,5 +25,6 @@ anafast = anafast_ring
 __all__ = [
     "HealpixRingTables", "map2alm_gpu", "alm2map_gpu", "anafast_ring", "alm2cl", "almxfl", "synalm",
     "alm_index", "getidx", "getsize", "to_reference_layout", "from_reference_layout", "hp_compat",
-    "map2alm", "alm2map", "anafast",
+    "map2alm", "alm2map", "anafast", "map2alm_spin_gpu", "eb_from_qu_gpu", "qu_from_eb_gpu",
+    "map2alm_spin1_gpu", "alm2map_spin1_gpu",
 ]

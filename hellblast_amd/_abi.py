@@ -17,6 +17,7 @@ HBX_EDEVICE = -3
 HBX_EWORKSPACE = -4
 HBX_EUNSUPPORTED = -5
 HBX_EPLAN = -6
+HBX_MAX_SPIN = 8
 
 HBX_F64 = 0
 HBX_F32 = 1
@@ -80,6 +81,10 @@ SIGNATURES = {
     "hbx_alm2map": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i32, _i64, _vp, _u64, _vp]),
     "hbx_alm2map_adjoint": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _i64, _vp, _u64, _vp]),
     "hbx_map2alm_adjoint": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i32, _i64, _vp, _u64, _vp]),
+    "hbx_map2alm_spin": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _i64, _vp, _u64, _vp]),
+    "hbx_alm2map_spin": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i32, _i64, _vp, _u64, _vp]),
+    "hbx_alm2map_spin_adjoint": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _i64, _vp, _u64, _vp]),
+    "hbx_map2alm_spin_adjoint": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i32, _i64, _vp, _u64, _vp]),
     "hbx_alm2cl": (ctypes.c_int, [_i32, _i32, _i32, _vp, _i32, _i64, _vp, _i64, _vp]),
     "hbx_phase_bytes": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_u64)]),
     "hbx_ring_analysis": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _u64, _vp]),

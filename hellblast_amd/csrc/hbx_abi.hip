@@ -72,7 +72,19 @@ static AnalysisWs analysis_ws(const Plan *p, int nmaps) {
 
 static uint64_t ws_tail_offset(const Plan *p, int nmaps) { return phase_total_bytes(p, nmaps); }
 
+// Spin plans, per polarisation pair: phase of Q and U | tail = scaled (E, B)
+// or the partial sums of both.
+static uint64_t spin_ws_bytes(const Plan *p, int npol) {
+    const uint64_t cs = csize(p), n2 = 2 * (uint64_t)npol;
+    const uint64_t phase = align256(n2 * phase_elems_per_map(p) * cs);
+    const uint64_t scaled = align256(n2 * scaled_elems_per_map(p) * cs);
+    const int g = spin_groups(p);
+    const uint64_t part = align256(g > 1 ? n2 * g * p->nlm * cs : 0);
+    return phase + std::max(scaled, part);
+}
+
 static uint64_t ws_bytes(const Plan *p, int nmaps) {
+    if (p->spin != 0) return spin_ws_bytes(p, nmaps);
     const uint64_t scaled = align256((uint64_t)nmaps * scaled_elems_per_map(p) * csize(p));
     const uint64_t ana = analysis_ws(p, nmaps).total;
     return ws_tail_offset(p, nmaps) + std::max(scaled, ana);
@@ -168,15 +180,21 @@ static int build_plan(Plan *p, const hbx_geom *g) {
     // switch-on tolerance: FP64 plans 2^-64, FP32 plans 2^-40 (relative
     // contributions far below each format's rounding error)
     const double tol = p->precision == HBX_F64 ? -64.0 : -40.0;
-    if ((rc = plan_build_recursion(p, psin, tol))) return rc;
-    if (p->precision == HBX_F32 && (rc = prepare_mfma_kernels())) return rc;
+    if ((rc = plan_build_recursion(p, px, psin, tol))) return rc;
+    if (p->precision == HBX_F32 && p->spin == 0 && (rc = prepare_mfma_kernels())) return rc;
     return 0;
 }
 
 static int check_exec(const hbx_plan *plan, int32_t nmaps, const void *a, const void *b, void *ws,
-                      uint64_t wsb, int need_ws) {
+                      uint64_t wsb, int need_ws, bool spin_entry = false) {
     if (!plan) return set_error(HBX_EINVAL, "plan is NULL");
-    if (nmaps < 0 || nmaps > 65535) return set_error(HBX_EINVAL, "nmaps must be in [0, 65535]");
+    if (spin_entry && plan->impl.spin == 0)
+        return set_error(HBX_EPLAN, "spin entry point called with a spin-0 plan");
+    if (!spin_entry && plan->impl.spin != 0)
+        return set_error(HBX_EPLAN, "spin-weighted plan: use the hbx_*_spin entry points");
+    const int nmax = spin_entry ? 32767 : 65535;
+    if (nmaps < 0 || nmaps > nmax)
+        return set_error(HBX_EINVAL, "nmaps must be in [0, " + std::to_string(nmax) + "]");
     if (nmaps == 0) return HBX_OK;
     if (!a || !b) return set_error(HBX_EINVAL, "data pointer is NULL");
     if (need_ws) {
@@ -252,7 +270,9 @@ int hbx_plan_create(const hbx_geom *g, int32_t lmax, int32_t mmax, int32_t spin,
     int rc = validate_geom(g, lmax, mmax);
     if (rc) return rc;
     if (precision != HBX_F64 && precision != HBX_F32) return set_error(HBX_EINVAL, "precision must be 0 or 1");
-    if (spin != 0) return set_error(HBX_EUNSUPPORTED, "spin-weighted plans are not available");
+    if (spin < 0 || spin > HBX_MAX_SPIN)
+        return set_error(HBX_EINVAL, "spin must be in [0, " + std::to_string(HBX_MAX_SPIN) + "]");
+    if (spin > lmax) return set_error(HBX_EINVAL, "spin must not exceed lmax");
     if (flags != 0) return set_error(HBX_EINVAL, "flags must be 0");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_error(HBX_EDEVICE, "no HIP device");
@@ -358,6 +378,61 @@ int hbx_alm2map(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t al
 int hbx_map2alm_adjoint(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t alm_stride, void *maps,
                         int32_t map_dtype, int64_t map_stride, void *ws, uint64_t ws_bytes_, void *stream) {
     return synth(plan, nmaps, alm, alm_stride, maps, map_dtype, map_stride, ws, ws_bytes_, stream, 1);
+}
+
+// ---------------------------------------------------------------------------
+// spin entry points: maps 2b / 2b+1 are Q / U and alm 2b / 2b+1 are E / B of
+// polarisation pair b, so npol pairs take 2*npol maps and 2*npol alm rows
+// ---------------------------------------------------------------------------
+static int spin_analysis(const hbx_plan *plan, int32_t npol, const void *maps, int32_t map_dtype, int64_t map_stride,
+                         void *alm, int64_t alm_stride, void *ws, uint64_t ws_bytes_, void *stream, int adjoint) {
+    int rc = check_exec(plan, npol, maps, alm, ws, ws_bytes_, 1, true);
+    if (rc || npol == 0) return rc;
+    const Plan *p = &plan->impl;
+    if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t phase = align256(2 * (uint64_t)npol * phase_elems_per_map(p) * csize(p));
+    void *tail = static_cast<char *>(ws) + phase;
+    if ((rc = launch_ring_analysis(p, 2 * npol, maps, map_dtype, map_stride, ws, adjoint ? nullptr : p->d_scale_ana,
+                                   s)))
+        return rc;
+    return launch_spin_analysis(p, npol, ws, alm, alm_stride, tail, adjoint, s);
+}
+
+static int spin_synthesis(const hbx_plan *plan, int32_t npol, const void *alm, int64_t alm_stride, void *maps,
+                          int32_t map_dtype, int64_t map_stride, void *ws, uint64_t ws_bytes_, void *stream,
+                          int adjoint) {
+    int rc = check_exec(plan, npol, alm, maps, ws, ws_bytes_, 1, true);
+    if (rc || npol == 0) return rc;
+    const Plan *p = &plan->impl;
+    if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t phase = align256(2 * (uint64_t)npol * phase_elems_per_map(p) * csize(p));
+    void *scaled = static_cast<char *>(ws) + phase;
+    if ((rc = launch_spin_prescale(p, npol, alm, alm_stride, scaled, adjoint, s))) return rc;
+    if ((rc = launch_spin_synthesis(p, npol, scaled, ws, s))) return rc;
+    return launch_ring_synthesis(p, 2 * npol, ws, maps, map_dtype, map_stride, adjoint ? p->d_scale_adj : nullptr, s);
+}
+
+int hbx_map2alm_spin(const hbx_plan *plan, int32_t npol, const void *maps, int32_t map_dtype, int64_t map_stride,
+                     void *alm, int64_t alm_stride, void *ws, uint64_t ws_bytes_, void *stream) {
+    return spin_analysis(plan, npol, maps, map_dtype, map_stride, alm, alm_stride, ws, ws_bytes_, stream, 0);
+}
+
+int hbx_alm2map_spin_adjoint(const hbx_plan *plan, int32_t npol, const void *maps, int32_t map_dtype,
+                             int64_t map_stride, void *alm, int64_t alm_stride, void *ws, uint64_t ws_bytes_,
+                             void *stream) {
+    return spin_analysis(plan, npol, maps, map_dtype, map_stride, alm, alm_stride, ws, ws_bytes_, stream, 1);
+}
+
+int hbx_alm2map_spin(const hbx_plan *plan, int32_t npol, const void *alm, int64_t alm_stride, void *maps,
+                     int32_t map_dtype, int64_t map_stride, void *ws, uint64_t ws_bytes_, void *stream) {
+    return spin_synthesis(plan, npol, alm, alm_stride, maps, map_dtype, map_stride, ws, ws_bytes_, stream, 0);
+}
+
+int hbx_map2alm_spin_adjoint(const hbx_plan *plan, int32_t npol, const void *alm, int64_t alm_stride, void *maps,
+                             int32_t map_dtype, int64_t map_stride, void *ws, uint64_t ws_bytes_, void *stream) {
+    return spin_synthesis(plan, npol, alm, alm_stride, maps, map_dtype, map_stride, ws, ws_bytes_, stream, 1);
 }
 
 static uint64_t phase_bytes_of(const Plan *p, int nmaps) { return phase_total_bytes(p, nmaps); }

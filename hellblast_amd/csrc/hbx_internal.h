@@ -73,6 +73,10 @@ struct Plan {
     double *d_alpha = nullptr, *d_sl = nullptr;
     int32_t *d_lseed = nullptr;            // [mmax+1][npairs]
     double2 *d_seed = nullptr;             // [mmax+1][npairs] (mu_{ls-2}, mu_{ls-1})
+    // spin plans: coupling coefficients of the lambda_+- recursion and
+    // seeds (mu+_{ls-2}, mu+_{ls-1}, mu-_{ls-2}, mu-_{ls-1})
+    double *d_beta = nullptr;
+    double4 *d_seed4 = nullptr;
     // FFT tables (device)
     int twn = 0, twshift = 0;
     double2 *d_tw64 = nullptr; float2 *d_tw32 = nullptr;
@@ -90,7 +94,19 @@ int launch_ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dty
 int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *maps,
                           int map_dtype, int64_t map_stride, const double *scale,
                           hipStream_t s);
-int plan_build_recursion(Plan *p, const std::vector<double> &pair_sin, double tol_log2);
+// recursion rows: row m holds l = m .. m + len - 1 (len a multiple of CHUNK);
+// spin plans start their chunks at l0 = max(m, |s|) and get longer rows
+int plan_build_recursion(Plan *p, const std::vector<double> &pair_cos, const std::vector<double> &pair_sin,
+                         double tol_log2);
+// spin.hip
+int plan_build_spin(Plan *p, const std::vector<double> &pair_cos, const std::vector<double> &pair_sin,
+                    double tol_log2);
+int spin_groups(const Plan *p);
+int launch_spin_analysis(const Plan *p, int npol, const void *phase, void *alm, int64_t alm_stride, void *partial,
+                         int adjoint, hipStream_t s);
+int launch_spin_prescale(const Plan *p, int npol, const void *alm, int64_t alm_stride, void *scaled, int adjoint,
+                         hipStream_t s);
+int launch_spin_synthesis(const Plan *p, int npol, const void *scaled, void *phase, hipStream_t s);
 int launch_leg_analysis(const Plan *p, int nmaps, const void *phase, void *alm,
                         int64_t alm_stride, void *partial, int adjoint, hipStream_t s);
 int analysis_groups(const Plan *p, bool f64);   // ring-pair groups (partial sums) per m

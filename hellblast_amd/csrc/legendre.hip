@@ -22,6 +22,7 @@
 //   analysis   E = F_n + F_s (l-m even), O = F_n - F_s (l-m odd)
 //   synthesis  G_n = G_even + G_odd,  G_s = G_even - G_odd.
 #include <algorithm>
+#include <cstdlib>
 
 #include "hbx_device.h"
 #include "hbx_internal.h"
@@ -140,12 +141,16 @@ __global__ void seed_kernel(const double *__restrict__ px, const double *__restr
     seed[q] = make_double2(ldexp(u2, e), ldexp(u1, e));
 }
 
-int plan_build_recursion(Plan *p, const std::vector<double> &pair_sin, double tol_log2) {
+int plan_build_recursion(Plan *p, const std::vector<double> &pair_cos, const std::vector<double> &pair_sin,
+                         double tol_log2) {
     const int mmax = p->mmax, lmax = p->lmax, np = p->npairs;
     std::vector<int64_t> rofs(mmax + 2);
     rofs[0] = 0;
+    auto rup = [](int v) { return (v + CHUNK - 1) / CHUNK * CHUNK; };
     for (int m = 0; m <= mmax; ++m) {
-        const int len = ((lmax - m + 1) + CHUNK - 1) / CHUNK * CHUNK;
+        // chunks run from l0 = max(m, |s|); the row must cover their last entry
+        const int l0 = std::max(m, std::abs(p->spin));
+        const int len = rup((l0 - m) + rup(lmax - l0 + 1));
         rofs[m + 1] = rofs[m] + len;
     }
     p->nrow = rofs[mmax + 1];
@@ -166,8 +171,9 @@ int plan_build_recursion(Plan *p, const std::vector<double> &pair_sin, double to
     HBX_HIP(hipMemset(p->d_sl + p->nrow, 0, TAILPAD * sizeof(double)));
     const int64_t nseed = (int64_t)(mmax + 1) * np;
     if ((rc = dalloc((void **)&p->d_lseed, nseed * sizeof(int32_t)))) return rc;
-    if ((rc = dalloc((void **)&p->d_seed, nseed * sizeof(double2)))) return rc;
     HBX_HIP(hipMemcpy(p->d_rofs, rofs.data(), rofs.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (p->spin != 0) return plan_build_spin(p, pair_cos, pair_sin, tol_log2);
+    if ((rc = dalloc((void **)&p->d_seed, nseed * sizeof(double2)))) return rc;
     hipLaunchKernelGGL(recursion_tables_kernel, dim3((mmax + 1 + 127) / 128), dim3(128), 0, 0, p->d_rofs,
                        p->d_alpha, p->d_sl, lmax, mmax);
     HBX_HIP(hipGetLastError());

@@ -1,0 +1,584 @@
+// Spin-weighted Legendre stage (spin s >= 1): Q/U <-> E/B.
+//
+// Reference: spin.py:32-115 (map2alm_spin_gpu, eb_from_qu_gpu, qu_from_eb_gpu)
+// and spin1.py:5-69, whose Wigner kernel (wigner_d_ms.cu:20-50) costs O(l) per
+// element and uses wrong exponents.  Here sY_lm = a_lm(theta) e^{i m phi}
+// (Goldberg et al. 1967), a = s_lambda, b = (-s)_lambda, sigma = (-1)^s, and
+// with P = F_Q + i F_U, M = F_Q - i F_U (ring coefficients, weights included)
+//   S_a = sum_r a P,  S_b = sigma sum_r b M,
+//   E = -(S_a + S_b)/2,  B = i (S_a - S_b)/2            (spin.py:73-78)
+// and for synthesis, with u = E + iB, v = E - iB,
+//   P(m) = -sum_l a u,  M(m) = -sigma sum_l b v,
+//   G_Q = (P + M)/2,    G_U = -i (P - M)/2.
+// a and b come from their own Wigner-d l-recursions,
+//   a_{l+1} = A_l (x + B_l) a_l - C_l a_{l-1},  B_l = m s/(l(l+1)),  b: B_l -> -B_l,
+// rescaled (lambda = s_l mu, s_{l+1} = C_l s_{l-1}) to
+//   mu_a(l) = (alpha_l x + beta_l) mu_a(l-1) - mu_a(l-2),  mu_b: beta_l -> -beta_l.
+// Two independent recursions keep each function's relative accuracy near the
+// poles, where a and b differ by tan^{2s}(theta/2) (a recursion for the
+// combinations (a +- sigma b)/2 loses it).  Mirror rings: a(pi - theta) =
+// (-1)^{l+m} b(theta), so one recursion serves a ring pair.
+#include <algorithm>
+#include <cmath>
+
+#include "hbx_device.h"
+#include "hbx_internal.h"
+
+namespace hbx {
+
+constexpr int SNEVER = 0x7fffffff;
+
+// ---------------------------------------------------------------------------
+// plan-time tables
+// ---------------------------------------------------------------------------
+__global__ void spin_tables_kernel(const int64_t *__restrict__ rofs, double *alpha, double *beta, double *sl,
+                                   int lmax, int mmax, int s) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > mmax) return;
+    const int64_t o = rofs[m];
+    const int len = (int)(rofs[m + 1] - o);
+    const int l0 = max(m, abs(s));
+    const double dm = m, ds = s;
+    double s2 = 1.0, s1 = 1.0;   // s_{l-2}, s_{l-1}
+    for (int j = 0; j < len; ++j) {
+        const int l = m + j;                  // target degree of this entry
+        double a = 0.0, b = 0.0, sv = 0.0;
+        if (l >= l0 && l <= lmax) {
+            if (l == l0) {
+                sv = 1.0;
+            } else {
+                const int k = l - 1;          // recursion from k to k+1 = l
+                const double dk = k;
+                const double den = sqrt(((dk + 1) * (dk + 1) - dm * dm) * ((dk + 1) * (dk + 1) - ds * ds));
+                const double A = (dk + 1) * sqrt((2 * dk + 1) * (2 * dk + 3)) / den;
+                const double Bk = dm * ds / (dk * (dk + 1));
+                if (l == l0 + 1) {
+                    sv = 1.0;
+                } else {
+                    const double C = (dk + 1) * sqrt((dk * dk - dm * dm) * (dk * dk - ds * ds)) *
+                                     sqrt((2 * dk + 3) / (2 * dk - 1)) / (dk * den);
+                    sv = C * s2;
+                }
+                a = A * s1 / sv;
+                b = A * Bk * s1 / sv;
+            }
+            s2 = s1;
+            s1 = sv;
+        }
+        alpha[o + j] = a;
+        beta[o + j] = b;
+        sl[o + j] = sv;
+    }
+}
+
+// s_lambda and (-s)_lambda at l0 = max(m, |s|) as (mant_a, mant_b) * 2^ex, one
+// thread per ring pair
+__global__ void spin_l0_kernel(const double *__restrict__ pcos, const double *__restrict__ psin, double *mp,
+                               double *mm, int *ex, int npairs, int mmax, int s) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    const double x = pcos[p], st = psin[p];
+    // half angles from x and sin theta without cancellation
+    const double sh = (x >= 0.0) ? st / sqrt(2.0 * (1.0 + x)) : sqrt(0.5 * (1.0 - x));
+    const double ch = (x >= 0.0) ? sqrt(0.5 * (1.0 + x)) : st / sqrt(2.0 * (1.0 - x));
+    const int a = abs(s);
+    auto fact = [](int n) { double f = 1.0; for (int i = 2; i <= n; ++i) f *= i; return f; };
+    auto comb = [&](int n, int k) { return fact(n) / (fact(k) * fact(n - k)); };
+    auto store = [&](int m, double vp, int ep, double vm, int em) {
+        // s_lambda and (-s)_lambda with a common exponent
+        const int e = max(ep, em);
+        const int64_t q = (int64_t)m * npairs + p;
+        mp[q] = ldexp(vp, ep - e);
+        mm[q] = ldexp(vm, em - e);
+        ex[q] = e;
+    };
+    // m < |s|: closed forms at l0 = |s|
+    const double base0 = 1.0 / sqrt(4.0 * 3.14159265358979323846);
+    for (int m = 0; m < a && m <= mmax; ++m) {
+        const double base = base0 * sqrt(fact(a + m) * fact(a - m) * (2 * a + 1) / fact(2 * a));
+        const double vs = ((m & 1) ? -1.0 : 1.0) * base * comb(2 * a, a - m) * pow(sh, a + m) * pow(ch, a - m);
+        const double vms = ((a & 1) ? -1.0 : 1.0) * base * comb(2 * a, a + m) * pow(sh, a - m) * pow(ch, a + m);
+        int e1 = 0, e2 = 0;
+        const double f1 = frexp(vs, &e1), f2 = frexp(vms, &e2);
+        store(m, f1, e1, f2, e2);
+    }
+    if (mmax < a) return;
+    // m >= |s|: s_lambda_mm chains for +s and -s, renormalised with frexp
+    int ep = 0, em = 0;
+    double vp = frexp(((a & 1) ? -1.0 : 1.0) * sqrt((2 * a + 1) * base0 * base0), &ep);
+    double vm = vp;
+    em = ep;
+    for (int i = 0; i < 2 * a; ++i) {
+        int d1, d2;
+        vp = frexp(vp * sh, &d1);
+        ep += d1;
+        vm = frexp(vm * ch, &d2);
+        em += d2;
+    }
+    store(a, vp, ep, vm, em);
+    for (int m = a; m < mmax; ++m) {
+        const double r1 = -sqrt((2.0 * m + 3) * (2.0 * m + 2) / ((double)(m + 1 + s) * (double)(m + 1 - s))) * 0.5 * st;
+        const double r2 = -sqrt((2.0 * m + 3) * (2.0 * m + 2) / ((double)(m + 1 - s) * (double)(m + 1 + s))) * 0.5 * st;
+        int d1, d2;
+        vp = frexp(vp * r1, &d1);
+        ep += d1;
+        vm = frexp(vm * r2, &d2);
+        em += d2;
+        store(m + 1, vp, ep, vm, em);
+    }
+}
+
+// switch-on seeds of the coupled recursion, one thread per (m, ring pair)
+__global__ void spin_seed_kernel(const double *__restrict__ px, const double *__restrict__ mp0,
+                                 const double *__restrict__ mm0, const int *__restrict__ ex0,
+                                 const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
+                                 const double *__restrict__ beta, const double *__restrict__ sl, int32_t *lseed,
+                                 double4 *seed, int npairs, int lmax, int s, int tol_log2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (p >= npairs) return;
+    const int64_t q = (int64_t)m * npairs + p;
+    const int l0 = max(m, abs(s));
+    const double x = px[p];
+    const double a0 = mp0[q], b0 = mm0[q];
+    const int e0 = ex0[q];
+    const double thr = ldexp(1.0, tol_log2);
+    const double big = ldexp(1.0, 200), small = ldexp(1.0, -200);
+    const double *al = alpha + rofs[m] - m, *be = beta + rofs[m] - m, *sv = sl + rofs[m] - m;
+    int lon = SNEVER;
+    if (l0 <= lmax && (a0 != 0.0 || b0 != 0.0)) {
+        double p2 = 0.0, p1 = a0, m2 = 0.0, m1 = b0;
+        int e = e0;
+        if (fmax(fabs(ldexp(p1, e)), fabs(ldexp(m1, e))) >= thr) {
+            lon = l0;
+        } else {
+            for (int l = l0 + 1; l <= lmax; ++l) {
+                const double t = al[l] * x;
+                const double np = fma(t + be[l], p1, -p2);
+                const double nm = fma(t - be[l], m1, -m2);
+                p2 = p1; p1 = np; m2 = m1; m1 = nm;
+                if (fmax(fabs(p1), fabs(m1)) > big) { p1 *= small; p2 *= small; m1 *= small; m2 *= small; e += 200; }
+                if (e > tol_log2 - 1100 && fmax(fabs(ldexp(sv[l] * p1, e)), fabs(ldexp(sv[l] * m1, e))) >= thr) {
+                    lon = l;
+                    break;
+                }
+            }
+        }
+    }
+    if (lon == SNEVER) {
+        lseed[q] = SNEVER;
+        seed[q] = make_double4(0, 0, 0, 0);
+        return;
+    }
+    const int ls = l0 + ((lon - l0) / CHUNK) * CHUNK;
+    lseed[q] = ls;
+    if (ls == l0) {
+        seed[q] = make_double4(-ldexp(a0, e0), 0.0, -ldexp(b0, e0), 0.0);
+        return;
+    }
+    double p2 = 0.0, p1 = a0, m2 = 0.0, m1 = b0;
+    int e = e0;
+    for (int l = l0 + 1; l <= ls - 1; ++l) {
+        const double t = al[l] * x;
+        const double np = fma(t + be[l], p1, -p2);
+        const double nm = fma(t - be[l], m1, -m2);
+        p2 = p1; p1 = np; m2 = m1; m1 = nm;
+        if (fmax(fabs(p1), fabs(m1)) > big) { p1 *= small; p2 *= small; m1 *= small; m2 *= small; e += 200; }
+    }
+    seed[q] = make_double4(ldexp(p2, e), ldexp(p1, e), ldexp(m2, e), ldexp(m1, e));
+}
+
+int plan_build_spin(Plan *p, const std::vector<double> &pair_cos, const std::vector<double> &pair_sin,
+                    double tol_log2) {
+    const int mmax = p->mmax, lmax = p->lmax, np = p->npairs, s = p->spin;
+    auto dalloc = [&](void **ptr, size_t bytes) -> int {
+        hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
+        if (e != hipSuccess) return set_error(HBX_ENOMEM, "hipMalloc failed for spin tables");
+        p->allocations.push_back(*ptr);
+        p->device_bytes += bytes;
+        return 0;
+    };
+    int rc;
+    if ((rc = dalloc((void **)&p->d_beta, (p->nrow + TAILPAD) * sizeof(double)))) return rc;
+    HBX_HIP(hipMemset(p->d_beta + p->nrow, 0, TAILPAD * sizeof(double)));
+    const int64_t nseed = (int64_t)(mmax + 1) * np;
+    if ((rc = dalloc((void **)&p->d_seed4, nseed * sizeof(double4)))) return rc;
+    hipLaunchKernelGGL(spin_tables_kernel, dim3((mmax + 1 + 127) / 128), dim3(128), 0, 0, p->d_rofs, p->d_alpha,
+                       p->d_beta, p->d_sl, lmax, mmax, s);
+    HBX_HIP(hipGetLastError());
+    double *d_cos = nullptr, *d_sin = nullptr, *d_mp = nullptr, *d_mm = nullptr;
+    int *d_ex = nullptr;
+    hipError_t e = hipMalloc(&d_cos, np * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&d_sin, np * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&d_mp, nseed * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&d_mm, nseed * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&d_ex, nseed * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(d_cos, pair_cos.data(), np * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_sin, pair_sin.data(), np * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(spin_l0_kernel, dim3((np + 127) / 128), dim3(128), 0, 0, d_cos, d_sin, d_mp, d_mm, d_ex, np,
+                           mmax, s);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(spin_seed_kernel, dim3((np + 127) / 128, mmax + 1), dim3(128), 0, 0, p->d_px, d_mp, d_mm,
+                           d_ex, p->d_rofs, p->d_alpha, p->d_beta, p->d_sl, p->d_lseed, p->d_seed4, np, lmax, s,
+                           (int)tol_log2);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    for (void *ptr : {(void *)d_cos, (void *)d_sin, (void *)d_mp, (void *)d_mm, (void *)d_ex})
+        if (ptr) (void)hipFree(ptr);
+    if (e != hipSuccess) return hip_error(e, "spin seed kernels");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// analysis: phase of (Q, U) -> (E, B).  One wave per (m, ring-pair group,
+// pair of maps); R slots of 64 ring pairs per lane; 32 partial sums per
+// 8-l chunk reduced over the wave through LDS.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int swmin(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int swmax(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64)
+spin_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__restrict__ alm, int64_t astride,
+                     c2<T> *__restrict__ partial, const double *__restrict__ px, const int64_t *__restrict__ rofs,
+                     const double *__restrict__ alpha, const double *__restrict__ beta,
+                     const double *__restrict__ sl, const int32_t *__restrict__ lseed,
+                     const double4 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
+                     int64_t nlm, int s, int adjoint) {
+    __shared__ T red[4 * CHUNK * 64];
+    int mb, item;
+    xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
+    const int m = mb * MB + item / ngroups;
+    const int grp = item % ngroups;
+    if (m > mmax) return;
+    const int b = blockIdx.y;                 // Q/U pair index
+    const int lane = threadIdx.x;
+    const int l0 = max(m, abs(s));
+    const int par0 = (l0 + m) & 1;            // parity of l + m at the first l of every chunk
+    const c2<T> *phq = phase + (int64_t)(2 * b) * pstride + (int64_t)(m >> 3) * nr * MB + (m & 7);
+    const c2<T> *phu = phq + pstride;
+
+    // per slot: P_n = Q_n + i U_n, M_n = Q_n - i U_n; south P_s, M_s carry the
+    // sign (-1)^{l+m} of the even-i steps of a chunk
+    double x[R], p1[R], p2[R], q1[R], q2[R];
+    c2<T> Pn[R], Mn[R], Ps[R], Ms[R];
+    int ls[R], sst[R], sinj[R];
+    const int pb = grp * 64 * R + lane;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int p = pb + 64 * k;
+        p1[k] = p2[k] = q1[k] = q2[k] = 0.0;
+        if (p < npairs) {
+            x[k] = px[p];
+            ls[k] = lseed[(int64_t)m * npairs + p];
+            const int rs = nr - 1 - p;
+            const c2<T> qn = phq[(int64_t)p * MB], un = phu[(int64_t)p * MB];
+            const c2<T> qs = (rs != p) ? phq[(int64_t)rs * MB] : mk<T>(0, 0);
+            const c2<T> us = (rs != p) ? phu[(int64_t)rs * MB] : mk<T>(0, 0);
+            const T sg = par0 ? (T)-1 : (T)1;
+            Pn[k] = mk<T>(qn.x - un.y, qn.y + un.x);
+            Mn[k] = mk<T>(qn.x + un.y, qn.y - un.x);
+            Ps[k] = mk<T>(sg * (qs.x - us.y), sg * (qs.y + us.x));
+            Ms[k] = mk<T>(sg * (qs.x + us.y), sg * (qs.y - us.x));
+        } else {
+            x[k] = 0.0;
+            ls[k] = SNEVER;
+            Pn[k] = Mn[k] = Ps[k] = Ms[k] = mk<T>(0, 0);
+        }
+        sst[k] = swmin(ls[k]);
+        sinj[k] = swmax(ls[k] <= lmax ? ls[k] : -1);
+    }
+    int lstart = SNEVER;
+#pragma unroll
+    for (int k = 0; k < R; ++k) lstart = min(lstart, sst[k]);
+
+    const bool direct = (ngroups == 1);
+    c2<T> *dE = direct ? alm + (int64_t)(2 * b) * astride : partial + ((int64_t)(2 * b) * ngroups + grp) * nlm;
+    c2<T> *dB = direct ? alm + (int64_t)(2 * b + 1) * astride : partial + ((int64_t)(2 * b + 1) * ngroups + grp) * nlm;
+    const int64_t i0 = alm_index(m, m, lmax) - m;    // + l
+    const int64_t ro = rofs[m] - m;                   // table index of l
+    for (int l = m + lane; l < (lstart > lmax ? lmax + 1 : lstart); l += 64) {
+        dE[i0 + l] = mk<T>(0, 0);
+        dB[i0 + l] = mk<T>(0, 0);
+    }
+    const double sigma = (s & 1) ? -1.0 : 1.0;
+    const int vid = lane >> 1, half = lane & 1;
+    for (int c0 = lstart; c0 <= lmax; c0 += CHUNK) {
+        double al[CHUNK], be[CHUNK];
+#pragma unroll
+        for (int i = 0; i < CHUNK; ++i) { al[i] = alpha[ro + c0 + i]; be[i] = beta[ro + c0 + i]; }
+        T acc[4 * CHUNK];     // per l: S_a.re, S_a.im, S_b.re, S_b.im (S_b without sigma)
+#pragma unroll
+        for (int i = 0; i < 4 * CHUNK; ++i) acc[i] = (T)0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (c0 < sst[k]) continue;
+            if (c0 <= sinj[k] && ls[k] == c0) {
+                const double4 sd = seed[(int64_t)m * npairs + pb + 64 * k];
+                p2[k] = sd.x; p1[k] = sd.y; q2[k] = sd.z; q1[k] = sd.w;
+            }
+#pragma unroll
+            for (int i = 0; i < CHUNK; ++i) {
+                const double t = al[i] * x[k];
+                const double na = fma(t + be[i], p1[k], -p2[k]);
+                const double nb = fma(t - be[i], q1[k], -q2[k]);
+                p2[k] = p1[k]; p1[k] = na; q2[k] = q1[k]; q1[k] = nb;
+                const T a = (T)na;
+                const T bs = (i & 1) ? -(T)nb : (T)nb;    // (-1)^{l+m} relative to the chunk start
+                const T as = (i & 1) ? -a : a;
+                acc[4 * i + 0] = fma(a, Pn[k].x, fma(bs, Ps[k].x, acc[4 * i + 0]));
+                acc[4 * i + 1] = fma(a, Pn[k].y, fma(bs, Ps[k].y, acc[4 * i + 1]));
+                acc[4 * i + 2] = fma((T)nb, Mn[k].x, fma(as, Ms[k].x, acc[4 * i + 2]));
+                acc[4 * i + 3] = fma((T)nb, Mn[k].y, fma(as, Ms[k].y, acc[4 * i + 3]));
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4 * CHUNK; ++v) red[v * 64 + lane] = acc[v];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        T sum = (T)0;
+#pragma unroll
+        for (int q = 0; q < 32; ++q) sum += red[vid * 64 + half * 32 + ((q + vid) & 31)];
+        sum += __shfl_xor(sum, 1, 64);
+        // partner: the same l's S_b component for an S_a lane and vice versa
+        const T partner = __shfl_xor(sum, 4, 64);
+        if (half == 0) {
+            const int i = vid >> 2, comp = vid & 3;
+            const int l = c0 + i;
+            if (l <= lmax) {
+                // comp 0/1: E.re/E.im = -(S_a + sigma S_b)/2;
+                // comp 2: B.im = (S_a.re - sigma S_b.re)/2; comp 3: B.re = -(S_a.im - sigma S_b.im)/2
+                double v;
+                if (comp < 2) v = -0.5 * ((double)sum + sigma * (double)partner);
+                else if (comp == 2) v = 0.5 * ((double)partner - sigma * (double)sum);
+                else v = -0.5 * ((double)partner - sigma * (double)sum);
+                const bool imag = (comp == 1 || comp == 2);
+                if (direct) {
+                    v *= sl[ro + l];
+                    if (adjoint) {
+                        if (m > 0) v *= 2.0;
+                        else if (imag) v = 0.0;
+                    }
+                }
+                T *dst = reinterpret_cast<T *>((comp < 2 ? dE : dB) + i0 + l);
+                dst[imag ? 1 : 0] = (T)v;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// synthesis pre-pass: row 2b = s_l (E + iB), row 2b+1 = s_l (E - iB) (halved
+// for m > 0 in the map2alm adjoint), zero-padded to the row length
+template <typename T>
+__global__ void spin_prescale_kernel(const c2<T> *__restrict__ alm, int64_t astride, c2<T> *__restrict__ scaled,
+                                     int64_t nrow, const int64_t *__restrict__ rofs, const double *__restrict__ sl,
+                                     int lmax, int adjoint) {
+    const int m = blockIdx.x, b = blockIdx.y;
+    const int64_t o = rofs[m];
+    const int len = (int)(rofs[m + 1] - o);
+    const int64_t i0 = alm_index(m, m, lmax);
+    const c2<T> *E = alm + (int64_t)(2 * b) * astride + i0;
+    const c2<T> *B = alm + (int64_t)(2 * b + 1) * astride + i0;
+    c2<T> *u = scaled + (int64_t)(2 * b) * nrow + o;
+    c2<T> *v = scaled + (int64_t)(2 * b + 1) * nrow + o;
+    const double f = (adjoint && m > 0) ? 0.5 : 1.0;
+    for (int j = threadIdx.x; j < len; j += blockDim.x) {
+        c2<T> pu = mk<T>(0, 0), pv = pu;
+        if (m + j <= lmax) {
+            const T g = (T)(sl[o + j] * f);
+            const c2<T> e = E[j], bb = B[j];
+            pu = mk<T>(g * (e.x - bb.y), g * (e.y + bb.x));
+            pv = mk<T>(g * (e.x + bb.y), g * (e.y - bb.x));
+        }
+        u[j] = pu;
+        v[j] = pv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// synthesis: u, v rows -> phase of (Q, U).  One wave per (m, 64 ring pairs);
+// the 8 m of an m-block share a workgroup.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(512)
+spin_synthesis_kernel(const c2<T> *__restrict__ scaled, int64_t nrow, c2<T> *__restrict__ phase, int64_t pstride,
+                      const double *__restrict__ px, const int64_t *__restrict__ rofs,
+                      const double *__restrict__ alpha, const double *__restrict__ beta,
+                      const int32_t *__restrict__ lseed, const double4 *__restrict__ seed, int npairs, int nr,
+                      int lmax, int mmax, int nmb, int npb, int s) {
+    int mb, pb;
+    xcd_map(blockIdx.x, npb, &mb, &pb);
+    if (mb >= nmb) return;
+    const int b = blockIdx.y;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int m = mb * MB + w;
+    if (m > mmax) return;
+    const int p = pb * 64 + lane;
+    const bool valid = p < npairs;
+    const int l0 = max(m, abs(s));
+    const int par0 = (l0 + m) & 1;
+    const double x = valid ? px[p] : 0.0;
+    const int ls = valid ? lseed[(int64_t)m * npairs + p] : SNEVER;
+    const int lstart = swmin(ls);
+    const int linj = swmax(ls <= lmax ? ls : -1);
+    // sums by class (A: even step of a chunk, B: odd): a u, b v (north); b u, a v (south)
+    c2<T> auA = mk<T>(0, 0), auB = auA, bvA = auA, bvB = auA, buA = auA, buB = auA, avA = auA, avB = auA;
+    if (lstart <= lmax) {
+        const int64_t ro = rofs[m] - m;
+        const c2<T> *ru = scaled + (int64_t)(2 * b) * nrow + ro;
+        const c2<T> *rv = scaled + (int64_t)(2 * b + 1) * nrow + ro;
+        double p2 = 0.0, p1 = 0.0, q2 = 0.0, q1 = 0.0;
+        for (int c0 = lstart; c0 <= lmax; c0 += CHUNK) {
+            if (c0 <= linj && ls == c0) {
+                const double4 sd = seed[(int64_t)m * npairs + p];
+                p2 = sd.x; p1 = sd.y; q2 = sd.z; q1 = sd.w;
+            }
+#pragma unroll
+            for (int i = 0; i < CHUNK; ++i) {
+                const int l = c0 + i;
+                const double t = alpha[ro + l] * x;
+                const double bl = beta[ro + l];
+                const double na = fma(t + bl, p1, -p2);
+                const double nb = fma(t - bl, q1, -q2);
+                p2 = p1; p1 = na; q2 = q1; q1 = nb;
+                const T a = (T)na, bt = (T)nb;
+                const c2<T> u = ru[l], v = rv[l];
+                if (i & 1) {
+                    auB.x = fma(a, u.x, auB.x);  auB.y = fma(a, u.y, auB.y);
+                    bvB.x = fma(bt, v.x, bvB.x); bvB.y = fma(bt, v.y, bvB.y);
+                    buB.x = fma(bt, u.x, buB.x); buB.y = fma(bt, u.y, buB.y);
+                    avB.x = fma(a, v.x, avB.x);  avB.y = fma(a, v.y, avB.y);
+                } else {
+                    auA.x = fma(a, u.x, auA.x);  auA.y = fma(a, u.y, auA.y);
+                    bvA.x = fma(bt, v.x, bvA.x); bvA.y = fma(bt, v.y, bvA.y);
+                    buA.x = fma(bt, u.x, buA.x); buA.y = fma(bt, u.y, buA.y);
+                    avA.x = fma(a, v.x, avA.x);  avA.y = fma(a, v.y, avA.y);
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    const T sigma = (s & 1) ? (T)-1 : (T)1;
+    // (-1)^{l+m} = +1 on class A when par0 = 0
+    const T sg = par0 ? (T)-1 : (T)1;
+    // north: P = -sum a u, M = -sigma sum b v; south: P = -sum (-1)^{l+m} b u, M = -sigma sum (-1)^{l+m} a v
+    const c2<T> Pn = mk<T>(-(auA.x + auB.x), -(auA.y + auB.y));
+    const c2<T> Mn = mk<T>(-sigma * (bvA.x + bvB.x), -sigma * (bvA.y + bvB.y));
+    const c2<T> Ps = mk<T>(-sg * (buA.x - buB.x), -sg * (buA.y - buB.y));
+    const c2<T> Ms = mk<T>(-sigma * sg * (avA.x - avB.x), -sigma * sg * (avA.y - avB.y));
+    // G_Q = (P + M)/2, G_U = -i (P - M)/2
+    const c2<T> gqn = mk<T>((T)0.5 * (Pn.x + Mn.x), (T)0.5 * (Pn.y + Mn.y));
+    const c2<T> gun = mk<T>((T)0.5 * (Pn.y - Mn.y), (T)-0.5 * (Pn.x - Mn.x));
+    const c2<T> gqs = mk<T>((T)0.5 * (Ps.x + Ms.x), (T)0.5 * (Ps.y + Ms.y));
+    const c2<T> gus = mk<T>((T)0.5 * (Ps.y - Ms.y), (T)-0.5 * (Ps.x - Ms.x));
+    c2<T> *oq = phase + (int64_t)(2 * b) * pstride + (int64_t)mb * nr * MB + w;
+    c2<T> *ou = oq + pstride;
+    const int rs = nr - 1 - p;
+    oq[(int64_t)p * MB] = gqn;
+    ou[(int64_t)p * MB] = gun;
+    if (rs != p) {
+        oq[(int64_t)rs * MB] = gqs;
+        ou[(int64_t)rs * MB] = gus;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+// ring-pair slots per lane: up to 4 (more costs occupancy: ~56 VGPRs a slot)
+template <typename T> static int spin_slots(const Plan *p) {
+    const int rmax = 4;
+    int r = 1;
+    while (r < rmax && (int64_t)r * 64 < p->npairs) r *= 2;
+    return r;
+}
+
+int spin_groups(const Plan *p) {
+    const int r = p->precision == HBX_F64 ? spin_slots<double>(p) : spin_slots<float>(p);
+    return (int)((p->npairs + 64 * r - 1) / (64 * r));
+}
+
+template <typename T>
+static int spin_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, c2<T> *alm, int64_t astride, c2<T> *partial,
+                           int adjoint, hipStream_t s) {
+    const int R = spin_slots<T>(p);
+    const int G = spin_groups(p);
+    const int nmb8 = (p->nmb + 7) / 8 * 8;
+    dim3 grid(nmb8 * MB * G, nmaps);
+    const int64_t pstride = phase_elems_per_map(p);
+#define HBX_SA(RV)                                                                                            \
+    hipLaunchKernelGGL((spin_analysis_kernel<T, RV>), grid, dim3(64), 0, s, phase, pstride, alm, astride, partial, \
+                       p->d_px, p->d_rofs, p->d_alpha, p->d_beta, p->d_sl, p->d_lseed, p->d_seed4, p->npairs,      \
+                       p->nrings, p->lmax, p->mmax, G, p->nlm, p->spin, adjoint)
+    switch (R) {
+    case 1: HBX_SA(1); break;
+    case 2: HBX_SA(2); break;
+    case 4: HBX_SA(4); break;
+    default: return set_error(HBX_EUNSUPPORTED, "spin slot count");
+    }
+#undef HBX_SA
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "spin_analysis_kernel launch");
+    if (G > 1) return launch_group_reduce(p, 2 * nmaps, partial, alm, astride, G, p->nlm, 0, p->mmax,
+                                          sizeof(T) == 8, adjoint, s);
+    return 0;
+}
+
+int launch_spin_analysis(const Plan *p, int nmaps, const void *phase, void *alm, int64_t astride, void *partial,
+                         int adjoint, hipStream_t s) {
+    if (p->precision == HBX_F64)
+        return spin_analysis_t<double>(p, nmaps, (const double2 *)phase, (double2 *)alm, astride, (double2 *)partial,
+                                       adjoint, s);
+    return spin_analysis_t<float>(p, nmaps, (const float2 *)phase, (float2 *)alm, astride, (float2 *)partial,
+                                  adjoint, s);
+}
+
+int launch_spin_prescale(const Plan *p, int npol, const void *alm, int64_t astride, void *scaled, int adjoint,
+                         hipStream_t s) {
+    dim3 grid(p->mmax + 1, npol);
+    if (p->precision == HBX_F64)
+        hipLaunchKernelGGL(spin_prescale_kernel<double>, grid, dim3(256), 0, s, (const double2 *)alm, astride,
+                           (double2 *)scaled, p->nrow, p->d_rofs, p->d_sl, p->lmax, adjoint);
+    else
+        hipLaunchKernelGGL(spin_prescale_kernel<float>, grid, dim3(256), 0, s, (const float2 *)alm, astride,
+                           (float2 *)scaled, p->nrow, p->d_rofs, p->d_sl, p->lmax, adjoint);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "spin_prescale_kernel launch");
+    return 0;
+}
+
+int launch_spin_synthesis(const Plan *p, int nmaps, const void *scaled, void *phase, hipStream_t s) {
+    const int npb = (p->npairs + 63) / 64;
+    const int nmb8 = (p->nmb + 7) / 8 * 8;
+    dim3 grid(nmb8 * npb, nmaps);
+    const int64_t pstride = phase_elems_per_map(p);
+    if (p->precision == HBX_F64)
+        hipLaunchKernelGGL(spin_synthesis_kernel<double>, grid, dim3(512), 0, s, (const double2 *)scaled, p->nrow,
+                           (double2 *)phase, pstride, p->d_px, p->d_rofs, p->d_alpha, p->d_beta, p->d_lseed,
+                           p->d_seed4, p->npairs, p->nrings, p->lmax, p->mmax, p->nmb, npb, p->spin);
+    else
+        hipLaunchKernelGGL(spin_synthesis_kernel<float>, grid, dim3(512), 0, s, (const float2 *)scaled, p->nrow,
+                           (float2 *)phase, pstride, p->d_px, p->d_rofs, p->d_alpha, p->d_beta, p->d_lseed,
+                           p->d_seed4, p->npairs, p->nrings, p->lmax, p->mmax, p->nmb, npb, p->spin);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "spin_synthesis_kernel launch");
+    return 0;
+}
+
+}  // namespace hbx

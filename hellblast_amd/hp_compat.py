@@ -51,8 +51,9 @@ def _check_nest(nest):
         raise NotImplementedError("NESTED maps are not supported by the SHT path; reorder to RING first")
 
 
-def _plan(nside, lmax, mmax, precision, weights, phi0, alias):
-    return get_plan(healpix_geom(int(nside), int(mmax), weights, phi0, alias), int(lmax), int(mmax), precision)
+def _plan(nside, lmax, mmax, precision, weights, phi0, alias, spin=0):
+    return get_plan(healpix_geom(int(nside), int(mmax), weights, phi0, alias), int(lmax), int(mmax), precision,
+                    spin=spin)
 
 
 def map2alm(m, lmax=None, mmax=None, nest=False, iter=0, *, precision="f64", weights="dz",
@@ -100,3 +101,64 @@ def anafast(m, lmax=None, mmax=None, nest=False, iter=0, **kw):
 def alm2cl(alm, lmax=None, mmax=None):
     from .alm_ops import alm2cl as _a
     return _a(alm, lmax=lmax, mmax=mmax)
+
+
+# ---------- spin transforms ----------
+def _check_spin(spin):
+    spin = int(spin)
+    if spin < 1 or spin > 8:
+        raise ValueError("spin must be in 1..8 (the reference accepts 1 or 2)")
+    return spin
+
+
+def map2alm_spin(Q, U, spin=2, lmax=None, mmax=None, nest=False, *, precision="f64", weights="dz",
+                 phi0="reference", alias="reference"):
+    """(E, B) m-major from flat RING Q, U maps (``hp_compat.py:57-69``).
+
+    Unlike the reference, whose spin=1 branch returns (1_a, -1_a)
+    (``hp_compat.py:66-67``), every spin returns (E, B); the pair is
+    ``spin.eb_to_spin_pair(E, B, spin)``.  Batched [nmaps, npix] inputs give
+    [nmaps, nalm] outputs.
+    """
+    _check_nest(nest)
+    spin = _check_spin(spin)
+    q = _util.flat_maps(Q, Q.shape[-1] if hasattr(Q, "shape") else len(Q))
+    u = _util.flat_maps(U, q.shape[-1], dtype=q.dtype)
+    nside = _nside_from_npix(q.shape[-1])
+    lmax = 3 * nside if lmax is None else int(lmax)
+    mmax = lmax if mmax is None else int(mmax)
+    plan = _plan(nside, lmax, mmax, precision, weights, phi0, alias, spin)
+    eb = plan.map2alm_spin(torch.stack([q, u], dim=-2))
+    return eb[..., 0, :], eb[..., 1, :]
+
+
+def alm2map_spin(E, B, spin=2, nside=None, lmax=None, nest=False, mmax=None, *, precision="f64",
+                 dtype=torch.float64, phi0="reference", alias="reference"):
+    """Flat RING (Q, U) from m-major (E, B) (``hp_compat.py:71-84``).
+
+    nside defaults to ceil(lmax/3) as in the reference (``hp_compat.py:72-75``).
+    """
+    _check_nest(nest)
+    spin = _check_spin(spin)
+    e = torch.as_tensor(E)
+    b = torch.as_tensor(B)
+    if lmax is None:
+        lmax = lmax_from_size(e.shape[-1])
+    mmax = lmax if mmax is None else int(mmax)
+    if e.shape != b.shape or e.shape[-1] != nalm(int(lmax), mmax):
+        raise ValueError("E and B must both be m-major alm of lmax/mmax")
+    nside = max(1, -(-int(lmax) // 3)) if nside is None else int(nside)
+    plan = _plan(nside, lmax, mmax, precision, "dz", phi0, alias, spin)
+    dev = plan.device
+    qu = plan.alm2map_spin(torch.stack([e.to(dev), b.to(dev)], dim=-2), dtype=dtype)
+    return qu[..., 0, :], qu[..., 1, :]
+
+
+def iqu2eb(Q, U, lmax=None, nest=False, **kw):
+    """(E, B) of a spin-2 field (``hp_compat.py:87-94``)."""
+    return map2alm_spin(Q, U, 2, lmax=lmax, nest=nest, **kw)
+
+
+def eb2iqu(E, B, nside=None, lmax=None, nest=False, **kw):
+    """(Q, U) of spin-2 (E, B) (``hp_compat.py:96-102``)."""
+    return alm2map_spin(E, B, 2, nside=nside, lmax=lmax, nest=nest, **kw)

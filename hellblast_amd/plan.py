@@ -115,12 +115,18 @@ def _map_dtype_code(dtype):
 
 
 class Plan:
-    """A libhbx plan for one geometry, lmax, mmax and precision on one GPU."""
+    """A libhbx plan for one geometry, lmax, mmax, spin and precision on one GPU.
 
-    def __init__(self, geom, lmax, mmax=None, precision="f64", device=None):
+    spin = 0 plans serve map2alm/alm2map (and adjoints); spin >= 1 plans
+    serve the *_spin methods, which take (Q, U) map pairs and (E, B) alm
+    pairs.
+    """
+
+    def __init__(self, geom, lmax, mmax=None, precision="f64", device=None, spin=0):
         mmax = lmax if mmax is None else int(mmax)
         self.geom = geom
         self.lmax, self.mmax = int(lmax), int(mmax)
+        self.spin = int(spin)
         self.precision = precision_code(precision)
         if not torch.cuda.is_available():
             geom.validate(self.lmax, self.mmax)
@@ -130,7 +136,7 @@ class Plan:
         L = _abi.lib()
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            _abi.check(L.hbx_plan_create(ctypes.byref(geom.as_struct()), self.lmax, self.mmax, 0,
+            _abi.check(L.hbx_plan_create(ctypes.byref(geom.as_struct()), self.lmax, self.mmax, self.spin,
                                          self.precision, 0, ctypes.byref(handle)))
         self._h = handle
         info = _abi.hbx_plan_info()
@@ -222,6 +228,60 @@ class Plan:
         return self._synthesis(_abi.lib().hbx_map2alm_adjoint, alm, dtype, out)
 
     # ------------------------------------------------------------------
+    # spin transforms: [..., 2, npix] (Q, U) <-> [..., 2, nalm] (E, B)
+    # ------------------------------------------------------------------
+    def _pairs(self, t, n):
+        squeeze = t.dim() == 2
+        if t.dim() < 2 or t.shape[-2] != 2 or t.shape[-1] != n:
+            raise ValueError(f"expected [..., 2, {n}] (a pair per polarisation field), got {tuple(t.shape)}")
+        return t.reshape(-1, 2, n).contiguous(), squeeze
+
+    def _spin_analysis(self, fn, maps, out=None):
+        t = torch.as_tensor(maps)
+        t = t.to(self.device) if t.device != self.device else t
+        if t.dtype not in (torch.float64, torch.float32):
+            t = t.to(torch.float64)
+        m, squeeze = self._pairs(t, self.npix)
+        npol = m.shape[0]
+        alm = torch.empty((npol, 2, self.nalm), dtype=self.alm_dtype, device=self.device) if out is None else out
+        ws = self.workspace(npol)
+        with torch.cuda.device(self.device):
+            _abi.check(fn(self._h, npol, ctypes.c_void_p(m.data_ptr()), _map_dtype_code(m.dtype), self.npix,
+                          ctypes.c_void_p(alm.data_ptr()), self.nalm, ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                          _stream_ptr(self.device)))
+        return alm[0] if squeeze else alm
+
+    def _spin_synthesis(self, fn, alm, dtype=torch.float64, out=None):
+        t = torch.as_tensor(alm)
+        t = t.to(self.device) if t.device != self.device else t
+        t = t.to(self.alm_dtype) if t.dtype != self.alm_dtype else t
+        a, squeeze = self._pairs(t, self.nalm)
+        npol = a.shape[0]
+        maps = torch.empty((npol, 2, self.npix), dtype=dtype, device=self.device) if out is None else out
+        ws = self.workspace(npol)
+        with torch.cuda.device(self.device):
+            _abi.check(fn(self._h, npol, ctypes.c_void_p(a.data_ptr()), self.nalm, ctypes.c_void_p(maps.data_ptr()),
+                          _map_dtype_code(maps.dtype), self.npix, ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                          _stream_ptr(self.device)))
+        return maps[0] if squeeze else maps
+
+    def map2alm_spin(self, maps, out=None):
+        """(Q, U) [..., 2, npix] -> (E, B) [..., 2, nalm], healpy convention."""
+        return self._spin_analysis(_abi.lib().hbx_map2alm_spin, maps, out)
+
+    def alm2map_spin(self, alm, dtype=torch.float64, out=None):
+        """(E, B) [..., 2, nalm] -> (Q, U) [..., 2, npix]."""
+        return self._spin_synthesis(_abi.lib().hbx_alm2map_spin, alm, dtype, out)
+
+    def alm2map_spin_adjoint(self, maps, out=None):
+        """Transpose of alm2map_spin (complex-gradient convention)."""
+        return self._spin_analysis(_abi.lib().hbx_alm2map_spin_adjoint, maps, out)
+
+    def map2alm_spin_adjoint(self, alm, dtype=torch.float64, out=None):
+        """Transpose of map2alm_spin (complex-gradient convention)."""
+        return self._spin_synthesis(_abi.lib().hbx_map2alm_spin_adjoint, alm, dtype, out)
+
+    # ------------------------------------------------------------------
     # stage entry points: maps <-> phase buffer <-> alm
     # ------------------------------------------------------------------
     def phase_buffer(self, nmaps):
@@ -270,16 +330,16 @@ _CACHE = OrderedDict()
 _CACHE_MAX = 8
 
 
-def get_plan(geom, lmax, mmax=None, precision="f64", device=None):
-    """Cached Plan for (device, geometry, lmax, mmax, precision)."""
+def get_plan(geom, lmax, mmax=None, precision="f64", device=None, spin=0):
+    """Cached Plan for (device, geometry, lmax, mmax, precision, spin)."""
     mmax = lmax if mmax is None else int(mmax)
     dev = torch.cuda.current_device() if (device is None and torch.cuda.is_available()) else device
-    key = (str(dev), geom.key(), int(lmax), int(mmax), precision_code(precision))
+    key = (str(dev), geom.key(), int(lmax), int(mmax), precision_code(precision), int(spin))
     p = _CACHE.get(key)
     if p is not None:
         _CACHE.move_to_end(key)
         return p
-    p = Plan(geom, lmax, mmax, precision, device)
+    p = Plan(geom, lmax, mmax, precision, device, spin)
     _CACHE[key] = p
     while len(_CACHE) > _CACHE_MAX:
         _CACHE.popitem(last=False)

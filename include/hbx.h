@@ -92,7 +92,11 @@ const char *hbx_last_error(void);
 /* Host-only validation of a geometry (no device needed). */
 int hbx_validate_geom(const hbx_geom *g, int32_t lmax, int32_t mmax);
 
-/* spin: 0 (scalar) or 1, 2 (spin-weighted).  flags: reserved, pass 0. */
+/* spin: 0 (scalar) or 1 .. HBX_MAX_SPIN (spin-weighted, spin <= lmax; the
+ * reference's spin = 2 and spin = 1 transforms).  A spin plan serves only the
+ * hbx_*_spin entry points and a spin-0 plan only the others (HBX_EPLAN).
+ * flags: reserved, pass 0.                                                  */
+#define HBX_MAX_SPIN 8
 int hbx_plan_create(const hbx_geom *g, int32_t lmax, int32_t mmax, int32_t spin,
                     int32_t precision, uint32_t flags, hbx_plan **out);
 int hbx_plan_destroy(hbx_plan *plan);
@@ -120,6 +124,37 @@ int hbx_map2alm_adjoint(const hbx_plan *plan, int32_t nmaps, const void *alm,
                         int64_t alm_stride, void *maps, int32_t map_dtype,
                         int64_t map_stride, void *ws, uint64_t ws_bytes,
                         void *stream);
+
+/* Spin-weighted transforms, healpy E/B convention:
+ *   Q + iU = sum_lm  s_a_lm  sY_lm,  s_a_lm = -(E_lm + i B_lm),
+ *   Q - iU = sum_lm -s_a_lm -sY_lm, -s_a_lm = -(-1)^s (E_lm - i B_lm),
+ * sY_lm as in Goldberg et al. (1967) (2Y_20 = 1/4 sqrt(15/2pi) sin^2 theta).
+ * npol polarisation pairs: maps 2b and 2b+1 are Q and U of pair b (map_stride
+ * apart), alm rows 2b and 2b+1 are E and B (alm_stride apart, m-major).
+ * Entries l < max(m, spin) are written as 0.  Workspace: hbx_workspace_bytes
+ * with nmaps = npol.  npol <= 32767.
+ *   hbx_map2alm_spin  (Q, U) -> (E, B)  eb_from_qu_gpu spin.py:73-78 (with
+ *                     map2alm_spin_gpu spin.py:32-71); spin1.py:5-38
+ *   hbx_alm2map_spin  (E, B) -> (Q, U)  qu_from_eb_gpu spin.py:80-115;
+ *                     spin1.py:40-69
+ *   hbx_alm2map_spin_adjoint, hbx_map2alm_spin_adjoint: transposes, as for
+ *                     the scalar pair.                                      */
+int hbx_map2alm_spin(const hbx_plan *plan, int32_t npol, const void *maps,
+                     int32_t map_dtype, int64_t map_stride, void *alm,
+                     int64_t alm_stride, void *ws, uint64_t ws_bytes,
+                     void *stream);
+int hbx_alm2map_spin(const hbx_plan *plan, int32_t npol, const void *alm,
+                     int64_t alm_stride, void *maps, int32_t map_dtype,
+                     int64_t map_stride, void *ws, uint64_t ws_bytes,
+                     void *stream);
+int hbx_alm2map_spin_adjoint(const hbx_plan *plan, int32_t npol,
+                             const void *maps, int32_t map_dtype,
+                             int64_t map_stride, void *alm, int64_t alm_stride,
+                             void *ws, uint64_t ws_bytes, void *stream);
+int hbx_map2alm_spin_adjoint(const hbx_plan *plan, int32_t npol,
+                             const void *alm, int64_t alm_stride, void *maps,
+                             int32_t map_dtype, int64_t map_stride, void *ws,
+                             uint64_t ws_bytes, void *stream);
 
 /* Stage entry points (the two halves of every transform).  The phase
  * buffer holds F_r(m) / G_r(m) for every ring and m <= mmax as complex
