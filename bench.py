@@ -5,8 +5,10 @@ One step = map2alm + alm2map of a batch of B synthetic maps (one "pair"
 per map) through the HIP path, inputs resident in HBM.  The headline value
 is FP32-batched pairs/s; the plain FP64 drop-in (B = 1) is timed in the same
 run and reported beside it, together with the FP32-vs-FP64 alm error (the
-1e-5 gate), the Legendre-kernel roofline and the CPU oracle timed on a
-bounded sample.  Prints ONE JSON line.
+1e-5 gate), the roofline of the dominant kernel (its mean launch time from
+HIP events recorded around it on its own stream over the timed steps), the
+ring stage's HBM rate and the CPU oracle timed on a bounded sample.  Prints
+ONE JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -32,12 +34,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--nside", type=int, default=2048)
     ap.add_argument("--lmax", type=int, default=None)
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB -> 1 GiB elementwise copy kernel (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -81,32 +85,42 @@ def eps_metrics(torch, a, ref, l_t, lmax):
     return emax, erms
 
 
-def time_steps(torch, fn, steps, warmup):
+def time_steps(torch, fn, steps, warmup, timer=None):
+    """ms per step over exactly ``steps`` steps (after ``warmup``), bracketed by
+    device synchronisation; ``timer`` (a KernelTimer) is armed over the same
+    timed region."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
+    if timer is not None:
+        timer.__enter__()
     t0.record()
     for _ in range(steps):
         fn()
     t1.record()
     torch.cuda.synchronize()
+    if timer is not None:
+        timer.__exit__(None, None, None)
     return t0.elapsed_time(t1) / steps
 
 
-def time_stage(torch, fn, reps=3):
-    """Average duration of one stage launch, HIP events on the current stream."""
-    fn()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        fn()
-    e1.record()
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps
+PROFILE_DIR = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of ``kernel`` on ``workload`` from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when no summary covers this workload."""
+    path = os.path.join(PROFILE_DIR, f"{kernel}_{workload}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(nside, lmax, seconds):
@@ -142,7 +156,9 @@ def cpu_baseline(nside, lmax, seconds):
         budget = max(seconds - t_ring * sel_pix / npix, 2.0)
         done_units, spent = 0.0, 0.0
         samples = 0
-        targets = list(range(0, lmax + 1, max(1, (lmax + 1) // 16)))
+        # every 8th m, in increasing m, until the budget is spent; the cost is
+        # extrapolated per (l, ring) unit, so stopping early does not bias it
+        targets = set(range(0, lmax + 1, 8))
         gen = OLg.lambda_mm_scaled(lmax, st)
         for m, mant, ex in gen:
             if m not in targets:
@@ -219,6 +235,7 @@ def main():
     log(f"FP32 vs FP64: eps_max {emax:.2e} (single-map path {emax_single:.2e}) eps_rms {erms:.2e} map {map_err:.2e}")
 
     # ---------------- FP32 batched (headline)
+    from hellblast_amd.plan import KernelTimer
     out_alm = torch.empty((B, nlm), dtype=torch.complex64, device=dev)
     out_map = torch.empty((B, p32.npix), dtype=torch.float32, device=dev)
 
@@ -227,8 +244,12 @@ def main():
         p32.alm2map(alm32b, dtype=torch.float32, out=out_map)
 
     log(f"timing FP32 batch={B} steps={args.steps}")
-    ms32 = time_steps(torch, step32, args.steps, args.warmup)
+    kt = KernelTimer("legendre_analysis", "legendre_synthesis", "ring_analysis", "ring_synthesis")
+    ms32 = time_steps(torch, step32, args.steps, args.warmup, kt)
     pairs32 = B * 1000.0 / ms32
+    kms = {k: kt.mean_ms(k) for k in kt.kernels}
+    log("kernel means over the timed region (ms):", json.dumps({k: round(v, 3) for k, v in kms.items()}),
+        "launches:", json.dumps({k: v[0] for k, v in kt.result.items()}))
 
     # ---------------- FP64 plain drop-in
     out64a = torch.empty((1, nlm), dtype=torch.complex128, device=dev)
@@ -239,29 +260,42 @@ def main():
         p64.alm2map(alm64, out=out64m)
 
     log(f"timing FP64 steps={args.fp64_steps}")
-    ms64 = time_steps(torch, step64, args.fp64_steps, 1)
+    kt64 = KernelTimer("legendre_analysis", "legendre_synthesis", "ring_analysis", "ring_synthesis")
+    ms64 = time_steps(torch, step64, args.fp64_steps, 1, kt64)
     pairs64 = 1000.0 / ms64
+    kms64 = {k: kt64.mean_ms(k) for k in kt64.kernels}
+    log("FP64 kernel means (ms):", json.dumps({k: round(v, 3) for k, v in kms64.items()}))
 
-    # ---------------- per-stage kernel timing (FP32 batched)
-    ph = p32.phase_buffer(B)
-    ws = p32.workspace(B)
-    t_ra = time_stage(torch, lambda: p32.ring_analysis(maps32, ph))
-    t_la = time_stage(torch, lambda: p32.legendre_analysis(ph, B, out=out_alm, ws=ws))
-    t_ls = time_stage(torch, lambda: p32.legendre_synthesis(alm32b, ph, ws))
-    t_rs = time_stage(torch, lambda: p32.ring_synthesis(ph, B, dtype=torch.float32, out=out_map))
-    stages = {"ring_analysis_ms": t_ra, "legendre_analysis_ms": t_la, "legendre_synthesis_ms": t_ls,
-              "ring_synthesis_ms": t_rs}
-    log("stages (ms):", json.dumps({k: round(v, 3) for k, v in stages.items()}))
-    dom_name, dom_ms = max((("legendre_analysis", t_la), ("legendre_synthesis", t_ls)), key=lambda kv: kv[1])
-    achieved = w_dir * B / (dom_ms * 1e-3) / 1e12
+    # ---------------- roofline of the dominant kernel (FP32 batch, matrix cores)
+    # analysis covers m >= 16 (m < 16 run in the FP64 band); synthesis covers every m
+    band = 16 * (2 * lmax + 1 - 16) // 2 + 16 if lmax >= 16 else nlm
+    flops = {"legendre_analysis": 4.0 * npairs * (nlm - band) * B,
+             "legendre_synthesis": 4.0 * npairs * nlm * B}
+    names = {"legendre_analysis": "leg_analysis_mfma_kernel", "legendre_synthesis": "leg_synthesis_mfma_kernel"}
+    dom = max(flops, key=lambda k: kms[k])
+    achieved = flops[dom] / (kms[dom] * 1e-3) / 1e12
+    workload = f"nside{nside}_lmax{lmax}_b{B}"
+    traffic = measured_traffic(names[dom], workload)
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_F32, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_F32, 4), "traffic": None, "kernel": dom_name,
-                "flops_per_launch": w_dir * B, "launch_ms": round(dom_ms, 4)}
+                "frac": round(achieved / PEAK_F32, 4), "traffic": traffic, "kernel": names[dom],
+                "flops_per_launch": flops[dom], "launch_ms": round(kms[dom], 4)}
+    other = [k for k in flops if k != dom][0]
+    other_tf = flops[other] / (kms[other] * 1e-3) / 1e12
+    # ring stage: HBM bound; bytes per map per direction = npix*s + Nr*(mmax+1)*2*s (s = 4 for FP32)
+    nr = 4 * nside - 1
+    ring_bytes = (p32.npix * 4 + nr * (lmax + 1) * 8) * B
+    ring_gbs = {k: ring_bytes / (kms[k] * 1e-3) / 1e9 for k in ("ring_analysis", "ring_synthesis")}
+    f64_ach = 4.0 * npairs * nlm / (kms64["legendre_analysis"] * 1e-3) / 1e12
+    f64_syn = 4.0 * npairs * nlm / (kms64["legendre_synthesis"] * 1e-3) / 1e12
 
-    ph64 = p64.phase_buffer(1)
-    ws64 = p64.workspace(1)
-    t64_la = time_stage(torch, lambda: p64.legendre_analysis(p64.ring_analysis(map64, ph64), 1, out=out64a, ws=ws64), 2)
-    f64_ach = w_dir / (t64_la * 1e-3) / 1e12
+    if args.copy_calibration:
+        log("copy calibration: 1 GiB -> 1 GiB elementwise kernel, 2 launches")
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     cpu = None
     if not args.no_cpu:
@@ -278,9 +312,14 @@ def main():
                    "fp32_over_fp64": round(pairs32 / pairs64, 3),
                    "eps_max_f32_vs_f64": emax, "eps_rms_f32_vs_f64": erms, "eps_max_f32_single_map": emax_single,
                    "map_rel_err_f32": map_err, "gate_1e-5": bool(emax <= 1e-5),
-                   "stages_ms_f32_batch": {k: round(v, 3) for k, v in stages.items()},
-                   "fp64_ring+legendre_analysis_tflops": round(f64_ach, 3),
-                   "fp64_frac_of_78.6": round(f64_ach / PEAK_F64, 4),
+                   "kernel_ms_f32_batch": {k: round(v, 4) for k, v in kms.items()},
+                   "kernel_ms_f64": {k: round(v, 4) for k, v in kms64.items()},
+                   f"{other}_tflops_f32": round(other_tf, 3),
+                   "ring_stage_hbm_gbs_f32": {k: round(v, 1) for k, v in ring_gbs.items()},
+                   "ring_stage_hbm_frac_of_8000": {k: round(v / 8000.0, 4) for k, v in ring_gbs.items()},
+                   "fp64_legendre_analysis_tflops": round(f64_ach, 3),
+                   "fp64_legendre_synthesis_tflops": round(f64_syn, 3),
+                   "fp64_analysis_frac_of_78.6": round(f64_ach / PEAK_F64, 4),
                    "algorithmic_flops_per_map_per_direction": w_dir},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -18,6 +18,12 @@ HBX_EWORKSPACE = -4
 HBX_EUNSUPPORTED = -5
 HBX_EPLAN = -6
 HBX_MAX_SPIN = 8
+HBX_TIMER_LEGENDRE_ANALYSIS = 1
+HBX_TIMER_LEGENDRE_SYNTHESIS = 2
+HBX_TIMER_RING_ANALYSIS = 3
+HBX_TIMER_RING_SYNTHESIS = 4
+HBX_TIMER_SPIN_ANALYSIS = 5
+HBX_TIMER_SPIN_SYNTHESIS = 6
 
 HBX_F64 = 0
 HBX_F32 = 1
@@ -85,6 +91,8 @@ SIGNATURES = {
     "hbx_alm2map_spin": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i32, _i64, _vp, _u64, _vp]),
     "hbx_alm2map_spin_adjoint": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _i64, _vp, _u64, _vp]),
     "hbx_map2alm_spin_adjoint": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i32, _i64, _vp, _u64, _vp]),
+    "hbx_timer_arm": (ctypes.c_int, [ctypes.c_uint32]),
+    "hbx_timer_read": (ctypes.c_int, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(ctypes.c_double)]),
     "hbx_alm2cl": (ctypes.c_int, [_i32, _i32, _i32, _vp, _i32, _i64, _vp, _i64, _vp]),
     "hbx_phase_bytes": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_u64)]),
     "hbx_ring_analysis": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _u64, _vp]),

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,36 @@ int set_error(int code, const std::string &msg) {
 int hip_error(hipError_t e, const char *what) {
     g_err = std::string(what) + ": " + hipGetErrorString(e);
     return HBX_EDEVICE;
+}
+
+// ---------------------------------------------------------------------------
+// kernel timer
+// ---------------------------------------------------------------------------
+constexpr int TIMER_IDS = 7;                 // HBX_TIMER_* ids are 1..6
+constexpr int TIMER_MAX_LAUNCHES = 256;
+struct KernelTimer {
+    std::vector<hipEvent_t> ev;                 // pairs (begin, end)
+    int used = 0;
+    bool open = false;
+};
+static std::mutex g_tmu;
+static uint32_t g_tmask = 0;
+static int g_tdevice = -1;
+static KernelTimer g_timer[TIMER_IDS];
+
+void timer_mark(int kernel, bool begin, hipStream_t s) {
+    if (kernel <= 0 || kernel >= TIMER_IDS) return;
+    if (!(__atomic_load_n(&g_tmask, __ATOMIC_RELAXED) & (1u << kernel))) return;
+    std::lock_guard<std::mutex> lk(g_tmu);
+    if (!(g_tmask & (1u << kernel))) return;
+    KernelTimer &t = g_timer[kernel];
+    if (begin) {
+        if (t.used + 2 > (int)t.ev.size()) return;
+        t.open = hipEventRecord(t.ev[t.used], s) == hipSuccess;
+    } else if (t.open) {
+        if (hipEventRecord(t.ev[t.used + 1], s) == hipSuccess) t.used += 2;
+        t.open = false;
+    }
 }
 
 int64_t phase_elems_per_map(const Plan *p) { return (int64_t)p->nmb * p->nrings * MB; }
@@ -489,6 +520,54 @@ int hbx_ring_synthesis(const hbx_plan *plan, int32_t nmaps, const void *phase, u
     if ((rc = check_strides(p, map_dtype, map_stride, p->nlm))) return rc;
     if ((rc = check_phase(p, nmaps, phase, phase_bytes))) return rc;
     return launch_ring_synthesis(p, nmaps, phase, maps, map_dtype, map_stride, nullptr, (hipStream_t)stream);
+}
+
+int hbx_timer_arm(uint32_t mask) {
+    if (mask & ~(((1u << TIMER_IDS) - 1) & ~1u)) return set_error(HBX_EINVAL, "timer mask has unknown bits");
+    std::lock_guard<std::mutex> lk(g_tmu);
+    __atomic_store_n(&g_tmask, 0u, __ATOMIC_RELAXED);
+    if (mask == 0) return HBX_OK;
+    int dev = -1;
+    HBX_HIP(hipGetDevice(&dev));
+    if (dev != g_tdevice) {
+        for (KernelTimer &t : g_timer) {
+            for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
+            t.ev.clear();
+        }
+        g_tdevice = dev;
+    }
+    for (int k = 1; k < TIMER_IDS; ++k) {
+        if (!(mask & (1u << k))) continue;
+        KernelTimer &t = g_timer[k];
+        while ((int)t.ev.size() < 2 * TIMER_MAX_LAUNCHES) {
+            hipEvent_t e;
+            HBX_HIP(hipEventCreate(&e));
+            t.ev.push_back(e);
+        }
+        t.used = 0;
+        t.open = false;
+    }
+    __atomic_store_n(&g_tmask, mask, __ATOMIC_RELAXED);
+    return HBX_OK;
+}
+
+int hbx_timer_read(int32_t kernel, int32_t *launches, double *total_ms) {
+    if (!launches || !total_ms) return set_error(HBX_EINVAL, "NULL argument");
+    if (kernel <= 0 || kernel >= TIMER_IDS) return set_error(HBX_EINVAL, "timer kernel id");
+    std::lock_guard<std::mutex> lk(g_tmu);
+    KernelTimer &t = g_timer[kernel];
+    double sum = 0.0;
+    for (int i = 0; i < t.used; i += 2) {
+        HBX_HIP(hipEventSynchronize(t.ev[i + 1]));
+        float ms = 0.0f;
+        HBX_HIP(hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]));
+        sum += ms;
+    }
+    *launches = t.used / 2;
+    *total_ms = sum;
+    t.used = 0;
+    t.open = false;
+    return HBX_OK;
 }
 
 int hbx_alm2cl(int32_t lmax, int32_t mmax, int32_t nmaps, const void *alm, int32_t alm_dtype, int64_t alm_stride,

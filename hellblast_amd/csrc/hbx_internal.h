@@ -23,6 +23,10 @@ int hip_error(hipError_t e, const char *what);
         if (_e != hipSuccess) return ::hbx::hip_error(_e, #call);            \
     } while (0)
 
+// kernel timer: brackets the launches of the armed kernel id (HBX_TIMER_*)
+// with HIP events on their stream; a no-op unless that id is armed
+void timer_mark(int kernel, bool begin, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // sizes
 // ---------------------------------------------------------------------------

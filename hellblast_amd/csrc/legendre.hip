@@ -511,7 +511,7 @@ int analysis_groups(const Plan *p, bool f64) {
 template <typename T>
 static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, int64_t pstride, c2<T> *alm,
                           int64_t astride, c2<T> *partial, int64_t nlm_stride, int mbeg, int mend, int adjoint,
-                          hipStream_t s) {
+                          hipStream_t s, int timer_id) {
     if (mend < mbeg) return 0;
     const int R = analysis_slots<T>(p);
     const int G = (int)((p->npairs + 64 * R - 1) / (64 * R));
@@ -521,6 +521,7 @@ static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, int64_t 
     hipLaunchKernelGGL((leg_analysis_kernel<T, RV>), grid, dim3(64), 0, s, phase, pstride, alm, astride,    \
                        partial, p->d_pair_n, p->d_pair_s, p->d_px, p->d_rofs, p->d_alpha, p->d_sl,          \
                        p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, mend, G, nlm_stride, adjoint, mbeg)
+    timer_mark(timer_id, true, s);
     switch (R) {
     case 1: HBX_LA(1); break;
     case 2: HBX_LA(2); break;
@@ -532,6 +533,7 @@ static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, int64_t 
     default: return set_error(HBX_EUNSUPPORTED, "analysis slot count");
     }
 #undef HBX_LA
+    timer_mark(timer_id, false, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "leg_analysis_kernel launch");
     if (G > 1)
@@ -569,9 +571,11 @@ int launch_leg_analysis(const Plan *p, int nmaps, const void *phase, void *alm, 
     const int64_t pstride = phase_elems_per_map(p);
     if (p->precision == HBX_F64)
         return leg_analysis_t<double>(p, nmaps, (const double2 *)phase, pstride, (double2 *)alm, astride,
-                                      (double2 *)partial, p->nlm, 0, p->mmax, adjoint, s);
+                                      (double2 *)partial, p->nlm, 0, p->mmax, adjoint, s,
+                                      HBX_TIMER_LEGENDRE_ANALYSIS);
     return leg_analysis_t<float>(p, nmaps, (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial,
-                                 p->nlm, std::min(M_LO, p->mmax + 1), p->mmax, adjoint, s);
+                                 p->nlm, std::min(M_LO, p->mmax + 1), p->mmax, adjoint, s,
+                                 HBX_TIMER_LEGENDRE_ANALYSIS);
 }
 
 // m < M_LO of an FP32 plan in FP64: band phase -> band alm (FP64) -> alm
@@ -580,7 +584,7 @@ int launch_leg_analysis_band(const Plan *p, int nmaps, const void *band_phase, v
     const int mend = std::min(M_LO - 1, p->mmax);
     const int64_t nb = band_nlm(p);
     int rc = leg_analysis_t<double>(p, nmaps, (const double2 *)band_phase, band_phase_elems(p), (double2 *)band_alm,
-                                    nb, (double2 *)band_partial, nb, 0, mend, adjoint, s);
+                                    nb, (double2 *)band_partial, nb, 0, mend, adjoint, s, 0);
     if (rc) return rc;
     hipLaunchKernelGGL(band_to_f32_kernel, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024), nmaps),
                        dim3(256), 0, s, (const double2 *)band_alm, nb, (float2 *)alm, astride);
@@ -608,6 +612,7 @@ int launch_leg_synthesis(const Plan *p, int nmaps, const void *scaled, void *pha
     const int nmb8 = (p->nmb + 7) / 8 * 8;
     dim3 grid(nmb8 * npb, nmaps);
     const int64_t pstride = phase_elems_per_map(p);
+    timer_mark(HBX_TIMER_LEGENDRE_SYNTHESIS, true, s);
     if (p->precision == HBX_F64)
         hipLaunchKernelGGL(leg_synthesis_kernel<double>, grid, dim3(512), 0, s, (const double2 *)scaled, p->nrow,
                            (double2 *)phase, pstride, p->d_pair_n, p->d_pair_s, p->d_px, p->d_rofs, p->d_alpha,
@@ -616,6 +621,7 @@ int launch_leg_synthesis(const Plan *p, int nmaps, const void *scaled, void *pha
         hipLaunchKernelGGL(leg_synthesis_kernel<float>, grid, dim3(512), 0, s, (const float2 *)scaled, p->nrow,
                            (float2 *)phase, pstride, p->d_pair_n, p->d_pair_s, p->d_px, p->d_rofs, p->d_alpha,
                            p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, p->nmb, npb);
+    timer_mark(HBX_TIMER_LEGENDRE_SYNTHESIS, false, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "leg_synthesis_kernel launch");
     return 0;

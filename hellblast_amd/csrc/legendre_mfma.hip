@@ -196,9 +196,11 @@ int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t
     const int nsb = (p->npairs + SYN_PAIRS_WG - 1) / SYN_PAIRS_WG;
     const int nmb8 = (p->nmb + 7) / 8 * 8;
     dim3 grid(nmb8 * MB * nsb, (nmaps + 15) / 16);
+    timer_mark(HBX_TIMER_LEGENDRE_SYNTHESIS, true, s);
     hipLaunchKernelGGL(leg_synthesis_mfma_kernel, grid, dim3(SYN_WAVES * 64), 0, s, (const float2 *)alm, astride,
                        (float *)phase, p->d_px, p->d_rofs, p->d_alpha, p->d_sl,
                        p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, p->nmb, nsb, nmaps, adjoint);
+    timer_mark(HBX_TIMER_LEGENDRE_SYNTHESIS, false, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "leg_synthesis_mfma_kernel launch");
     return 0;
@@ -390,9 +392,11 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
     const int nmb8 = (p->nmb + 7) / 8 * 8;
     dim3 grid(nmb8 * MB * G, (nmaps + 15) / 16);
     const int64_t pstride = phase_elems_per_map(p);
+    timer_mark(HBX_TIMER_LEGENDRE_ANALYSIS, true, s);
     hipLaunchKernelGGL(leg_analysis_mfma_kernel, grid, dim3(ANA_WAVES * 64), ANA_LDS, s, (const float2 *)phase,
                        pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,
                        p->mmax, G, p->nlm, nmaps, adjoint, mbeg);
+    timer_mark(HBX_TIMER_LEGENDRE_ANALYSIS, false, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "leg_analysis_mfma_kernel launch");
     if (G > 1) return launch_group_reduce(p, nmaps, partial, alm, astride, G, p->nlm, mbeg, p->mmax, false, adjoint, s);

@@ -544,6 +544,7 @@ static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t ma
                            c2<T> *phase, const double *scale, const c2<T> *tw, const c2<T> *V,
                            hipStream_t s) {
     const int64_t pstride = phase_elems_per_map(p);
+    timer_mark(HBX_TIMER_RING_ANALYSIS, true, s);
     for (const auto &c : p->classes) {
         const size_t lds = (size_t)c.M * sizeof(c2<T>);
         dim3 grid((unsigned)c.nrings, (unsigned)nmaps);
@@ -556,6 +557,7 @@ static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t ma
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_error(e, "ring_analysis_kernel launch");
     }
+    timer_mark(HBX_TIMER_RING_ANALYSIS, false, s);
     return 0;
 }
 
@@ -564,6 +566,7 @@ static int ring_synthesis_t(const Plan *p, int nmaps, const c2<T> *phase, TOut *
                             int64_t map_stride, const double *scale, const c2<T> *tw,
                             const c2<T> *V, hipStream_t s) {
     const int64_t pstride = phase_elems_per_map(p);
+    timer_mark(HBX_TIMER_RING_SYNTHESIS, true, s);
     for (const auto &c : p->classes) {
         const size_t lds = (size_t)c.M * sizeof(c2<T>);
         dim3 grid((unsigned)c.nrings, (unsigned)nmaps);
@@ -576,6 +579,7 @@ static int ring_synthesis_t(const Plan *p, int nmaps, const c2<T> *phase, TOut *
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_error(e, "ring_synthesis_kernel launch");
     }
+    timer_mark(HBX_TIMER_RING_SYNTHESIS, false, s);
     return 0;
 }
 

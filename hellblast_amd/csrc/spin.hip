@@ -526,6 +526,7 @@ static int spin_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, c2<T> *
     hipLaunchKernelGGL((spin_analysis_kernel<T, RV>), grid, dim3(64), 0, s, phase, pstride, alm, astride, partial, \
                        p->d_px, p->d_rofs, p->d_alpha, p->d_beta, p->d_sl, p->d_lseed, p->d_seed4, p->npairs,      \
                        p->nrings, p->lmax, p->mmax, G, p->nlm, p->spin, adjoint)
+    timer_mark(HBX_TIMER_SPIN_ANALYSIS, true, s);
     switch (R) {
     case 1: HBX_SA(1); break;
     case 2: HBX_SA(2); break;
@@ -533,6 +534,7 @@ static int spin_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, c2<T> *
     default: return set_error(HBX_EUNSUPPORTED, "spin slot count");
     }
 #undef HBX_SA
+    timer_mark(HBX_TIMER_SPIN_ANALYSIS, false, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "spin_analysis_kernel launch");
     if (G > 1) return launch_group_reduce(p, 2 * nmaps, partial, alm, astride, G, p->nlm, 0, p->mmax,
@@ -568,6 +570,7 @@ int launch_spin_synthesis(const Plan *p, int nmaps, const void *scaled, void *ph
     const int nmb8 = (p->nmb + 7) / 8 * 8;
     dim3 grid(nmb8 * npb, nmaps);
     const int64_t pstride = phase_elems_per_map(p);
+    timer_mark(HBX_TIMER_SPIN_SYNTHESIS, true, s);
     if (p->precision == HBX_F64)
         hipLaunchKernelGGL(spin_synthesis_kernel<double>, grid, dim3(512), 0, s, (const double2 *)scaled, p->nrow,
                            (double2 *)phase, pstride, p->d_px, p->d_rofs, p->d_alpha, p->d_beta, p->d_lseed,
@@ -576,6 +579,7 @@ int launch_spin_synthesis(const Plan *p, int nmaps, const void *scaled, void *ph
         hipLaunchKernelGGL(spin_synthesis_kernel<float>, grid, dim3(512), 0, s, (const float2 *)scaled, p->nrow,
                            (float2 *)phase, pstride, p->d_px, p->d_rofs, p->d_alpha, p->d_beta, p->d_lseed,
                            p->d_seed4, p->npairs, p->nrings, p->lmax, p->mmax, p->nmb, npb, p->spin);
+    timer_mark(HBX_TIMER_SPIN_SYNTHESIS, false, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "spin_synthesis_kernel launch");
     return 0;

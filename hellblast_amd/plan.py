@@ -326,6 +326,47 @@ class Plan:
         return maps
 
 
+class KernelTimer:
+    """HIP-event timing of library kernels (hbx_timer_arm / hbx_timer_read).
+
+    ``with KernelTimer("legendre_analysis", "legendre_synthesis") as t: ...``
+    then ``t.result["legendre_analysis"] == (launches, total_ms)``.  The
+    events are recorded on the stream each kernel is launched on.
+    """
+
+    IDS = {"legendre_analysis": _abi.HBX_TIMER_LEGENDRE_ANALYSIS,
+           "legendre_synthesis": _abi.HBX_TIMER_LEGENDRE_SYNTHESIS,
+           "ring_analysis": _abi.HBX_TIMER_RING_ANALYSIS,
+           "ring_synthesis": _abi.HBX_TIMER_RING_SYNTHESIS,
+           "spin_analysis": _abi.HBX_TIMER_SPIN_ANALYSIS,
+           "spin_synthesis": _abi.HBX_TIMER_SPIN_SYNTHESIS}
+
+    def __init__(self, *kernels):
+        self.kernels = kernels
+        self.result = {}
+
+    def __enter__(self):
+        mask = 0
+        for k in self.kernels:
+            mask |= 1 << self.IDS[k]
+        _abi.check(_abi.lib().hbx_timer_arm(mask))
+        return self
+
+    def __exit__(self, *exc):
+        L = _abi.lib()
+        for k in self.kernels:
+            n = ctypes.c_int32()
+            ms = ctypes.c_double()
+            _abi.check(L.hbx_timer_read(self.IDS[k], ctypes.byref(n), ctypes.byref(ms)))
+            self.result[k] = (int(n.value), float(ms.value))
+        _abi.check(L.hbx_timer_arm(0))
+        return False
+
+    def mean_ms(self, kernel):
+        n, ms = self.result[kernel]
+        return ms / n if n else float("nan")
+
+
 _CACHE = OrderedDict()
 _CACHE_MAX = 8
 

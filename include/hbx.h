@@ -190,6 +190,30 @@ int hbx_alm2cl(int32_t lmax, int32_t mmax, int32_t nmaps, const void *alm,
                int32_t alm_dtype, int64_t alm_stride, double *cl,
                int64_t cl_stride, void *stream);
 
+/* Kernel timers (diagnostics; no reference counterpart).  hbx_timer_arm(mask)
+ * arms the kernels whose bits (1 << HBX_TIMER_*) are set and clears their
+ * records (mask 0 disarms all).  While armed, each launch of such a kernel is
+ * bracketed by HIP events recorded on the stream it is launched on (up to
+ * 256 launches per kernel; later launches are not recorded).
+ * hbx_timer_read() waits for one kernel's recorded events and returns the
+ * number of recorded launches and their summed duration in ms.  Do not arm
+ * timers while a stream is being captured into a HIP graph.
+ *   HBX_TIMER_LEGENDRE_ANALYSIS  - main Legendre analysis kernel (matrix-core
+ *                                  kernel for FP32 batches; excludes the FP64
+ *                                  low-m band and partial-sum reductions)
+ *   HBX_TIMER_LEGENDRE_SYNTHESIS - main Legendre synthesis kernel
+ *   HBX_TIMER_RING_ANALYSIS      - all ring-FFT launches of one ring analysis
+ *   HBX_TIMER_RING_SYNTHESIS     - all ring-FFT launches of one ring synthesis
+ *   HBX_TIMER_SPIN_ANALYSIS, HBX_TIMER_SPIN_SYNTHESIS - spin Legendre kernels */
+#define HBX_TIMER_LEGENDRE_ANALYSIS 1
+#define HBX_TIMER_LEGENDRE_SYNTHESIS 2
+#define HBX_TIMER_RING_ANALYSIS 3
+#define HBX_TIMER_RING_SYNTHESIS 4
+#define HBX_TIMER_SPIN_ANALYSIS 5
+#define HBX_TIMER_SPIN_SYNTHESIS 6
+int hbx_timer_arm(uint32_t mask);
+int hbx_timer_read(int32_t kernel, int32_t *launches, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif
