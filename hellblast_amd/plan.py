@@ -102,6 +102,10 @@ class Geometry:
         _abi.check(_abi.lib().hbx_validate_geom(ctypes.byref(self.as_struct()), int(lmax), int(mmax)))
 
 
+def _needs_grad(t):
+    return torch.is_grad_enabled() and isinstance(t, torch.Tensor) and t.requires_grad
+
+
 def _stream_ptr(device):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
@@ -211,20 +215,47 @@ class Plan:
                           ws.numel(), _stream_ptr(self.device)))
         return maps[0] if squeeze else maps
 
+    # Public transforms are differentiable: an input that requires grad (with
+    # grad mode on and no ``out=``) goes through hellblast_amd.autograd, whose
+    # backward runs the transpose kernels.
     def map2alm(self, maps, out=None):
         """Analysis of [npix] or [nmaps, npix] maps -> m-major alm."""
-        return self._analysis(_abi.lib().hbx_map2alm, maps, out)
+        if out is None and _needs_grad(maps):
+            from .autograd import _Map2Alm
+            return _Map2Alm.apply(maps, self)
+        return self._map2alm_raw(maps, out)
 
     def alm2map(self, alm, dtype=torch.float64, out=None):
         """Synthesis of [nalm] or [nmaps, nalm] alm -> maps."""
-        return self._synthesis(_abi.lib().hbx_alm2map, alm, dtype, out)
+        if out is None and _needs_grad(alm):
+            from .autograd import _Alm2Map
+            return _Alm2Map.apply(alm, self, dtype)
+        return self._alm2map_raw(alm, dtype, out)
 
     def alm2map_adjoint(self, maps, out=None):
         """Transpose of alm2map (complex-gradient convention)."""
-        return self._analysis(_abi.lib().hbx_alm2map_adjoint, maps, out)
+        if out is None and _needs_grad(maps):
+            from .autograd import _Alm2MapAdjoint
+            return _Alm2MapAdjoint.apply(maps, self)
+        return self._alm2map_adjoint_raw(maps, out)
 
     def map2alm_adjoint(self, alm, dtype=torch.float64, out=None):
         """Transpose of map2alm (complex-gradient convention)."""
+        if out is None and _needs_grad(alm):
+            from .autograd import _Map2AlmAdjoint
+            return _Map2AlmAdjoint.apply(alm, self, dtype)
+        return self._map2alm_adjoint_raw(alm, dtype, out)
+
+    def _map2alm_raw(self, maps, out=None):
+        return self._analysis(_abi.lib().hbx_map2alm, maps, out)
+
+    def _alm2map_raw(self, alm, dtype=torch.float64, out=None):
+        return self._synthesis(_abi.lib().hbx_alm2map, alm, dtype, out)
+
+    def _alm2map_adjoint_raw(self, maps, out=None):
+        return self._analysis(_abi.lib().hbx_alm2map_adjoint, maps, out)
+
+    def _map2alm_adjoint_raw(self, alm, dtype=torch.float64, out=None):
         return self._synthesis(_abi.lib().hbx_map2alm_adjoint, alm, dtype, out)
 
     # ------------------------------------------------------------------
@@ -267,18 +298,42 @@ class Plan:
 
     def map2alm_spin(self, maps, out=None):
         """(Q, U) [..., 2, npix] -> (E, B) [..., 2, nalm], healpy convention."""
-        return self._spin_analysis(_abi.lib().hbx_map2alm_spin, maps, out)
+        if out is None and _needs_grad(maps):
+            from .autograd import _Map2AlmSpin
+            return _Map2AlmSpin.apply(maps, self)
+        return self._map2alm_spin_raw(maps, out)
 
     def alm2map_spin(self, alm, dtype=torch.float64, out=None):
         """(E, B) [..., 2, nalm] -> (Q, U) [..., 2, npix]."""
-        return self._spin_synthesis(_abi.lib().hbx_alm2map_spin, alm, dtype, out)
+        if out is None and _needs_grad(alm):
+            from .autograd import _Alm2MapSpin
+            return _Alm2MapSpin.apply(alm, self, dtype)
+        return self._alm2map_spin_raw(alm, dtype, out)
 
     def alm2map_spin_adjoint(self, maps, out=None):
         """Transpose of alm2map_spin (complex-gradient convention)."""
-        return self._spin_analysis(_abi.lib().hbx_alm2map_spin_adjoint, maps, out)
+        if out is None and _needs_grad(maps):
+            from .autograd import _Alm2MapSpinAdjoint
+            return _Alm2MapSpinAdjoint.apply(maps, self)
+        return self._alm2map_spin_adjoint_raw(maps, out)
 
     def map2alm_spin_adjoint(self, alm, dtype=torch.float64, out=None):
         """Transpose of map2alm_spin (complex-gradient convention)."""
+        if out is None and _needs_grad(alm):
+            from .autograd import _Map2AlmSpinAdjoint
+            return _Map2AlmSpinAdjoint.apply(alm, self, dtype)
+        return self._map2alm_spin_adjoint_raw(alm, dtype, out)
+
+    def _map2alm_spin_raw(self, maps, out=None):
+        return self._spin_analysis(_abi.lib().hbx_map2alm_spin, maps, out)
+
+    def _alm2map_spin_raw(self, alm, dtype=torch.float64, out=None):
+        return self._spin_synthesis(_abi.lib().hbx_alm2map_spin, alm, dtype, out)
+
+    def _alm2map_spin_adjoint_raw(self, maps, out=None):
+        return self._spin_analysis(_abi.lib().hbx_alm2map_spin_adjoint, maps, out)
+
+    def _map2alm_spin_adjoint_raw(self, alm, dtype=torch.float64, out=None):
         return self._spin_synthesis(_abi.lib().hbx_map2alm_spin_adjoint, alm, dtype, out)
 
     # ------------------------------------------------------------------
