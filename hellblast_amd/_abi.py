@@ -68,6 +68,11 @@ class hbx_plan_info(ctypes.Structure):
     ]
 
 
+class hbx_tiles(ctypes.Structure):
+    _fields_ = [("valu_slots", ctypes.c_int32), ("mfma_waves", ctypes.c_int32),
+                ("mfma_min_maps", ctypes.c_int32), ("spin_slots", ctypes.c_int32)]
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -83,6 +88,8 @@ SIGNATURES = {
     "hbx_plan_destroy": (ctypes.c_int, [_vp]),
     "hbx_plan_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(hbx_plan_info)]),
     "hbx_workspace_bytes": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_u64)]),
+    "hbx_plan_set_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(hbx_tiles)]),
+    "hbx_plan_get_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(hbx_tiles)]),
     "hbx_map2alm": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _i64, _vp, _u64, _vp]),
     "hbx_alm2map": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i32, _i64, _vp, _u64, _vp]),
     "hbx_alm2map_adjoint": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _i64, _vp, _u64, _vp]),

@@ -1,0 +1,120 @@
+"""Tile autotuner for libhbx plans (no reference counterpart).
+
+For one plan and batch size it times map2alm + alm2map (spin plans: the
+spin pair) under each candidate tile set (``hbx_plan_set_tiles``) and keeps
+the fastest.  Winners are cached in a JSON file keyed by device name,
+geometry, lmax, mmax, precision, spin and batch size, so a tuned
+configuration is reused across processes:
+``$HBX_TUNE_CACHE`` or ``~/.cache/hellblast_amd/tiles.json``.
+
+Tiles only change how work is split (ring pairs per workgroup or lane, and
+the matrix-core / VALU switch), so every candidate computes the same
+transform; results agree to rounding.
+"""
+import json
+import os
+import tempfile
+
+import torch
+
+CACHE_ENV = "HBX_TUNE_CACHE"
+
+
+def cache_path():
+    return os.environ.get(CACHE_ENV) or os.path.join(os.path.expanduser("~"), ".cache", "hellblast_amd",
+                                                     "tiles.json")
+
+
+def _load():
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(db):
+    path = cache_path()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    fd, tmp = tempfile.mkstemp(dir=os.path.dirname(path), suffix=".tmp")
+    with os.fdopen(fd, "w") as f:
+        json.dump(db, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def key(plan, nmaps):
+    dev = torch.cuda.get_device_name(plan.device)
+    return (f"{dev}|{plan.geom.key()[:16]}|lmax{plan.lmax}|mmax{plan.mmax}|p{plan.precision}|"
+            f"s{plan.spin}|n{int(nmaps)}")
+
+
+def candidates(plan, nmaps):
+    """Tile sets worth timing for this plan and batch size."""
+    from . import _abi
+    npairs = (plan.geom.nrings + 1) // 2
+    # slots beyond what covers every ring pair with one wave only add idle lanes
+    cover = 1
+    while cover * 64 < npairs and cover < 16:
+        cover *= 2
+    slots = [s for s in (1, 2, 4, 8, 16) if s <= cover] or [1]
+    if plan.spin:
+        return [{"spin_slots": s} for s in (1, 2, 4) if s <= max(cover, 1)]
+    if plan.precision == _abi.HBX_F64:
+        return [{"valu_slots": s} for s in slots if s <= 8]
+    out = []
+    if nmaps >= 2:
+        waves = [w for w in (4, 8, 16) if w == 4 or (w // 2) * 64 < npairs]
+        out += [{"mfma_waves": w, "mfma_min_maps": 2} for w in waves]
+    # the VALU kernels (also for batches: mfma_min_maps above the batch size)
+    out += [{"valu_slots": s, "mfma_min_maps": nmaps + 1} for s in slots]
+    return out
+
+
+def _time(plan, nmaps, reps):
+    dev = plan.device
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1)
+    rdt = torch.float64 if plan.alm_dtype == torch.complex128 else torch.float32
+    if plan.spin:
+        maps = torch.randn((nmaps, 2, plan.npix), generator=gen, device=dev, dtype=rdt)
+        fwd = lambda: plan.alm2map_spin(plan.map2alm_spin(maps), dtype=rdt)   # noqa: E731
+    else:
+        maps = torch.randn((nmaps, plan.npix), generator=gen, device=dev, dtype=rdt)
+        fwd = lambda: plan.alm2map(plan.map2alm(maps), dtype=rdt)   # noqa: E731
+    fwd()
+    torch.cuda.synchronize(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fwd()
+    e1.record()
+    torch.cuda.synchronize(dev)
+    return e0.elapsed_time(e1) / reps
+
+
+def tune(plan, nmaps, reps=3, force=False, verbose=False):
+    """Apply the best tiles for (plan, nmaps), timing candidates unless cached."""
+    db = _load()
+    k = key(plan, nmaps)
+    if not force and k in db:
+        plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
+        plan.set_tiles(**db[k]["tiles"])
+        return db[k]
+    results = []
+    for cand in candidates(plan, nmaps):
+        plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
+        plan.set_tiles(**cand)
+        ms = _time(plan, nmaps, reps)
+        results.append((ms, cand))
+        if verbose:
+            print(f"[autotune] {k} {cand} {ms:.3f} ms", flush=True)
+    ms, best = min(results, key=lambda r: r[0])
+    plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
+    plan.set_tiles(**best)
+    entry = {"tiles": best, "ms": ms, "timed": [[m, c] for m, c in results]}
+    db = _load()
+    db[k] = entry
+    _save(db)
+    return entry
+

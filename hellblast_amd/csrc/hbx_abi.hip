@@ -256,7 +256,7 @@ static int legendre_analysis(const Plan *p, int nmaps, const void *phase, void *
     const AnalysisWs w = analysis_ws(p, nmaps);
     char *t = static_cast<char *>(tail);
     int rc;
-    if (p->precision == HBX_F32 && nmaps >= MFMA_MIN_MAPS)
+    if (p->precision == HBX_F32 && nmaps >= mfma_min_maps(p))
         rc = launch_leg_analysis_mfma(p, nmaps, phase, alm, alm_stride, t + w.partial, adjoint, s);
     else
         rc = launch_leg_analysis(p, nmaps, phase, alm, alm_stride, t + w.partial, adjoint, s);
@@ -268,7 +268,7 @@ static int legendre_analysis(const Plan *p, int nmaps, const void *phase, void *
 // Legendre synthesis: matrix cores for FP32 batches, VALU otherwise
 static int legendre_synthesis(const Plan *p, int nmaps, const void *alm, int64_t alm_stride, void *phase,
                               void *scaled, int adjoint, hipStream_t s) {
-    if (p->precision == HBX_F32 && nmaps >= MFMA_MIN_MAPS)
+    if (p->precision == HBX_F32 && nmaps >= mfma_min_maps(p))
         return launch_leg_synthesis_mfma(p, nmaps, alm, alm_stride, phase, adjoint, s);
     int rc = launch_prescale(p, nmaps, alm, alm_stride, scaled, adjoint, s);
     if (rc) return rc;
@@ -352,6 +352,34 @@ int hbx_plan_get_info(const hbx_plan *plan, hbx_plan_info *info) {
     info->npix = p.npix;
     info->nalm = p.nlm;
     info->device_bytes = p.device_bytes;
+    return HBX_OK;
+}
+
+int hbx_plan_set_tiles(hbx_plan *plan, const hbx_tiles *t) {
+    if (!plan || !t) return set_error(HBX_EINVAL, "NULL argument");
+    auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+    if (t->valu_slots != 0 && (!pow2(t->valu_slots) || t->valu_slots > 16))
+        return set_error(HBX_EINVAL, "valu_slots must be 0 or a power of two <= 16");
+    if (t->mfma_waves != 0 && t->mfma_waves != 4 && t->mfma_waves != 8 && t->mfma_waves != 16)
+        return set_error(HBX_EINVAL, "mfma_waves must be 0, 4, 8 or 16");
+    if (t->mfma_min_maps < 0) return set_error(HBX_EINVAL, "mfma_min_maps must be >= 0");
+    if (t->spin_slots != 0 && t->spin_slots != 1 && t->spin_slots != 2 && t->spin_slots != 4)
+        return set_error(HBX_EINVAL, "spin_slots must be 0, 1, 2 or 4");
+    Plan &p = plan->impl;
+    p.tile_valu_slots = t->valu_slots;
+    p.tile_mfma_waves = t->mfma_waves;
+    p.tile_mfma_min_maps = t->mfma_min_maps;
+    p.tile_spin_slots = t->spin_slots;
+    return HBX_OK;
+}
+
+int hbx_plan_get_tiles(const hbx_plan *plan, hbx_tiles *t) {
+    if (!plan || !t) return set_error(HBX_EINVAL, "NULL argument");
+    const Plan &p = plan->impl;
+    t->valu_slots = analysis_slots_of(&p, p.precision == HBX_F64);
+    t->mfma_waves = mfma_analysis_waves(&p);
+    t->mfma_min_maps = mfma_min_maps(&p);
+    t->spin_slots = spin_slots_of(&p);
     return HBX_OK;
 }
 

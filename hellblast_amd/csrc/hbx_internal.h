@@ -88,6 +88,11 @@ struct Plan {
     int64_t vtotal = 0;
     std::vector<FftClass> classes;
     std::vector<void *> allocations;
+    // tiles (hbx_plan_set_tiles); 0 = size-based default
+    int tile_valu_slots = 0;      // VALU analysis ring-pair slots per lane
+    int tile_mfma_waves = 0;      // waves per matrix-core analysis workgroup (4, 8, 16)
+    int tile_mfma_min_maps = 0;   // FP32 batches of at least this many maps use the matrix cores
+    int tile_spin_slots = 0;      // spin analysis ring-pair slots per lane (1, 2, 4)
 };
 
 // launchers implemented in ring_fft.hip / legendre.hip
@@ -106,6 +111,7 @@ int plan_build_recursion(Plan *p, const std::vector<double> &pair_cos, const std
 int plan_build_spin(Plan *p, const std::vector<double> &pair_cos, const std::vector<double> &pair_sin,
                     double tol_log2);
 int spin_groups(const Plan *p);
+int spin_slots_of(const Plan *p);
 int launch_spin_analysis(const Plan *p, int npol, const void *phase, void *alm, int64_t alm_stride, void *partial,
                          int adjoint, hipStream_t s);
 int launch_spin_prescale(const Plan *p, int npol, const void *alm, int64_t alm_stride, void *scaled, int adjoint,
@@ -131,9 +137,12 @@ int launch_leg_analysis_band(const Plan *p, int nmaps, const void *band_phase, v
 int launch_ring_band(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride,
                      void *band_phase, const double *scale, hipStream_t s);
 int mfma_analysis_groups(const Plan *p);
+int mfma_analysis_waves(const Plan *p);
 int prepare_mfma_kernels();
-// batches at least this large use the matrix-core Legendre kernels (FP32 plans)
+// by default, batches at least this large use the matrix-core Legendre kernels (FP32 plans)
 constexpr int MFMA_MIN_MAPS = 2;
+inline int mfma_min_maps(const Plan *p) { return p->tile_mfma_min_maps > 0 ? p->tile_mfma_min_maps : MFMA_MIN_MAPS; }
+int analysis_slots_of(const Plan *p, bool f64);   // effective VALU analysis slots
 int launch_alm2cl(int lmax, int mmax, int nmaps, const void *alm, int dt,
                   int64_t alm_stride, double *cl, int64_t cl_stride, hipStream_t s);
 

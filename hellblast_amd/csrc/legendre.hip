@@ -496,12 +496,18 @@ __global__ void alm2cl_kernel(const c2<T> *__restrict__ alm, int64_t astride, do
 // ---------------------------------------------------------------------------
 // Ring-pair slots per lane: FP64 holds 8 (E/O in registers cost 8 VGPRs
 // each), FP32 holds 16; fewer when the geometry has fewer pairs.
+// ring-pair slots (64 pairs each) per lane of the VALU analysis: the plan's
+// tile if set (clamped to what the type allows), else enough to cover every
+// pair with one wave up to 8 (FP64) / 16 (FP32)
 template <typename T> static int analysis_slots(const Plan *p) {
     const int rmax = sizeof(T) == 8 ? 8 : 16;
+    if (p->tile_valu_slots > 0) return std::min(p->tile_valu_slots, rmax);
     int r = 1;
     while (r < rmax && (int64_t)r * 64 < p->npairs) r *= 2;
     return r;
 }
+
+int analysis_slots_of(const Plan *p, bool f64) { return f64 ? analysis_slots<double>(p) : analysis_slots<float>(p); }
 
 int analysis_groups(const Plan *p, bool f64) {
     const int r = f64 ? analysis_slots<double>(p) : analysis_slots<float>(p);

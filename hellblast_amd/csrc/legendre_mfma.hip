@@ -216,10 +216,11 @@ int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t
 //   A[row = n][k] = S_par[ring k][n]             (held in registers all kernel)
 //   B[k][col]     = lambda_{c0+2col+par}(ring k) (FP64 recursion -> FP32 ->
 //                                                 wave-private LDS transpose)
-// A workgroup is 16 waves x 64 ring pairs of one m.  After its MFMAs each
-// wave parks its D tiles in its own (consumed) lambda tile; every thread then
-// sums one of the chunk's 32 l x 32 n outputs over the 16 waves in a fixed
-// order and writes it.  (LDS float atomics for this sum measured 4x slower.)
+// A workgroup is W waves x 64 ring pairs of one m (W = 4, 8 or 16, a plan
+// tile).  After its MFMAs each wave parks its D tiles in its own (consumed)
+// lambda tile; the threads then sum the chunk's 32 l x 32 n outputs over the
+// W waves in a fixed order and write them.  (LDS float atomics for this sum
+// measured 4x slower.)
 // Lane layout of the 16x16x4 f32 instruction (checked on gfx950):
 //   A: lane = row + 16 k   B: lane = col + 16 k   D: lane = col + 16 (row/4), reg = row%4
 // Reference counterpart: the per-(m, l) ring sum of sht_raw.py:53 /
@@ -227,14 +228,13 @@ int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int ANA_WAVES = 16;
-constexpr int ANA_PAIRS_WG = ANA_WAVES * 64;
 constexpr int ANA_L = 32;                 // l per chunk: 16 even + 16 odd columns
 constexpr int ANA_STR = 17;               // padded LDS row (16 columns + 1)
 constexpr int ANA_LAM = 2 * 64 * ANA_STR; // floats of one wave's lambda tile
-constexpr size_t ANA_LDS = (size_t)(ANA_WAVES * ANA_LAM + 4 * ANA_L + 4) * sizeof(float);
+constexpr size_t ana_lds(int waves) { return (size_t)(waves * ANA_LAM + 4 * ANA_L + 4) * sizeof(float); }
 
-__global__ void __launch_bounds__(ANA_WAVES * 64, 1)
+template <int ANA_WAVES>
+__global__ void __launch_bounds__(ANA_WAVES * 64)
 leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, float2 *__restrict__ alm,
                          int64_t astride, float2 *__restrict__ partial, const double *__restrict__ px,
                          const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
@@ -254,7 +254,7 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
     double *alds = reinterpret_cast<double *>(smem + ANA_WAVES * ANA_LAM);   // [2][32]
     int &wg_start = *reinterpret_cast<int *>(smem + ANA_WAVES * ANA_LAM + 4 * ANA_L);
 
-    const int pw = grp * ANA_PAIRS_WG + w * 64;         // first ring pair of this wave
+    const int pw = grp * ANA_WAVES * 64 + w * 64;       // first ring pair of this wave
     const int p = pw + lane;
     double x = 0.0, u1 = 0.0, u2 = 0.0;
     double2 sd = make_double2(0.0, 0.0);
@@ -320,9 +320,6 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
     for (int t = tid; t < (lz - m) * 32; t += ANA_WAVES * 64) store(m + (t >> 5), t & 31, 0.0f);
     if (lstart > lmax) return;
 
-    // this thread's output element of a chunk and where each wave parks it
-    const int on = tid >> 5, olo = tid & 31;
-    const int ridx = (((on >> 4) * 2 + (olo & 1)) * 4 + (on & 3)) * 64 + (olo >> 1) + 16 * ((on & 15) >> 2);
 
     if (tid < ANA_L) alds[tid] = al[lstart + tid];
     __syncthreads();
@@ -375,27 +372,50 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
                 for (int reg = 0; reg < 4; ++reg) lam[((tile * 2 + par) * 4 + reg) * 64 + lane] = D[tile][par][reg];
         if (tid < ANA_L) alds[(buf ^ 1) * ANA_L + tid] = anext;
         __syncthreads();
-        float v = 0.0f;
+        // output o = (n = o/32, l = c0 + o%32), parked by each wave at ridx
 #pragma unroll
-        for (int q = 0; q < ANA_WAVES; ++q) v += smem[q * ANA_LAM + ridx];
-        const int l = c0 + olo;
-        if (l <= lmax) store(l, on, v);
+        for (int o = tid; o < 32 * ANA_L; o += ANA_WAVES * 64) {
+            const int on = o >> 5, olo = o & 31;
+            const int ridx = (((on >> 4) * 2 + (olo & 1)) * 4 + (on & 3)) * 64 + (olo >> 1) + 16 * ((on & 15) >> 2);
+            float v = 0.0f;
+#pragma unroll
+            for (int q = 0; q < ANA_WAVES; ++q) v += smem[q * ANA_LAM + ridx];
+            const int l = c0 + olo;
+            if (l <= lmax) store(l, on, v);
+        }
         __syncthreads();
     }
+}
+
+int mfma_analysis_waves(const Plan *p) { return p->tile_mfma_waves > 0 ? p->tile_mfma_waves : 16; }
+
+int mfma_analysis_groups(const Plan *p) {
+    const int pairs_wg = mfma_analysis_waves(p) * 64;
+    return (p->npairs + pairs_wg - 1) / pairs_wg;
 }
 
 int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *alm, int64_t astride, void *partial,
                              int adjoint, hipStream_t s) {
     const int mbeg = std::min(M_LO, p->mmax + 1);    // m < M_LO run in the FP64 band
     if (mbeg > p->mmax) return 0;
-    const int G = (p->npairs + ANA_PAIRS_WG - 1) / ANA_PAIRS_WG;
+    const int W = mfma_analysis_waves(p);
+    const int G = mfma_analysis_groups(p);
     const int nmb8 = (p->nmb + 7) / 8 * 8;
     dim3 grid(nmb8 * MB * G, (nmaps + 15) / 16);
     const int64_t pstride = phase_elems_per_map(p);
     timer_mark(HBX_TIMER_LEGENDRE_ANALYSIS, true, s);
-    hipLaunchKernelGGL(leg_analysis_mfma_kernel, grid, dim3(ANA_WAVES * 64), ANA_LDS, s, (const float2 *)phase,
-                       pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,
-                       p->mmax, G, p->nlm, nmaps, adjoint, mbeg);
+#define HBX_AM(WV)                                                                                            \
+    hipLaunchKernelGGL(leg_analysis_mfma_kernel<WV>, grid, dim3(WV * 64), ana_lds(WV), s, (const float2 *)phase, \
+                       pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px, p->d_rofs, p->d_alpha,     \
+                       p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, G, p->nlm, nmaps, \
+                       adjoint, mbeg)
+    switch (W) {
+    case 4: HBX_AM(4); break;
+    case 8: HBX_AM(8); break;
+    case 16: HBX_AM(16); break;
+    default: return set_error(HBX_EUNSUPPORTED, "matrix-core analysis waves must be 4, 8 or 16");
+    }
+#undef HBX_AM
     timer_mark(HBX_TIMER_LEGENDRE_ANALYSIS, false, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "leg_analysis_mfma_kernel launch");
@@ -403,14 +423,17 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
     return 0;
 }
 
-// raise the dynamic-LDS limit of the analysis kernel (called at plan creation)
+// raise the dynamic-LDS limit of the analysis kernels (called at plan creation)
 int prepare_mfma_kernels() {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_mfma_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)ANA_LDS);
-    if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(leg_analysis_mfma_kernel)");
+    const void *k[3] = {reinterpret_cast<const void *>(leg_analysis_mfma_kernel<4>),
+                        reinterpret_cast<const void *>(leg_analysis_mfma_kernel<8>),
+                        reinterpret_cast<const void *>(leg_analysis_mfma_kernel<16>)};
+    const int wv[3] = {4, 8, 16};
+    for (int i = 0; i < 3; ++i) {
+        hipError_t e = hipFuncSetAttribute(k[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)ana_lds(wv[i]));
+        if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(leg_analysis_mfma_kernel)");
+    }
     return 0;
 }
-
-int mfma_analysis_groups(const Plan *p) { return (p->npairs + ANA_PAIRS_WG - 1) / ANA_PAIRS_WG; }
 
 }  // namespace hbx

@@ -501,23 +501,24 @@ spin_synthesis_kernel(const c2<T> *__restrict__ scaled, int64_t nrow, c2<T> *__r
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// ring-pair slots per lane: up to 4 (more costs occupancy: ~56 VGPRs a slot)
-template <typename T> static int spin_slots(const Plan *p) {
-    const int rmax = 4;
+// ring-pair slots per lane: the plan's tile, else up to 4 (more costs
+// occupancy: ~56 VGPRs a slot)
+int spin_slots_of(const Plan *p) {
+    if (p->tile_spin_slots > 0) return p->tile_spin_slots;
     int r = 1;
-    while (r < rmax && (int64_t)r * 64 < p->npairs) r *= 2;
+    while (r < 4 && (int64_t)r * 64 < p->npairs) r *= 2;
     return r;
 }
 
 int spin_groups(const Plan *p) {
-    const int r = p->precision == HBX_F64 ? spin_slots<double>(p) : spin_slots<float>(p);
+    const int r = spin_slots_of(p);
     return (int)((p->npairs + 64 * r - 1) / (64 * r));
 }
 
 template <typename T>
 static int spin_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, c2<T> *alm, int64_t astride, c2<T> *partial,
                            int adjoint, hipStream_t s) {
-    const int R = spin_slots<T>(p);
+    const int R = spin_slots_of(p);
     const int G = spin_groups(p);
     const int nmb8 = (p->nmb + 7) / 8 * 8;
     dim3 grid(nmb8 * MB * G, nmaps);

@@ -161,6 +161,28 @@ class Plan:
             self._h = None
 
     # ------------------------------------------------------------------
+    TILE_FIELDS = ("valu_slots", "mfma_waves", "mfma_min_maps", "spin_slots")
+
+    def set_tiles(self, **tiles):
+        """Set plan tiles (0 = default); unknown names raise.  Changes the
+        workspace size: do not call while work on this plan is in flight."""
+        cur = self.tiles(effective=False)
+        for k, v in tiles.items():
+            if k not in self.TILE_FIELDS:
+                raise ValueError(f"unknown tile {k!r}")
+            cur[k] = int(v)
+        t = _abi.hbx_tiles(**cur)
+        _abi.check(_abi.lib().hbx_plan_set_tiles(self._h, ctypes.byref(t)))
+        self._tiles = dict(cur)
+
+    def tiles(self, effective=True):
+        """Effective tiles (library defaults resolved), or the requested ones."""
+        if not effective:
+            return dict(getattr(self, "_tiles", {k: 0 for k in self.TILE_FIELDS}))
+        t = _abi.hbx_tiles()
+        _abi.check(_abi.lib().hbx_plan_get_tiles(self._h, ctypes.byref(t)))
+        return {k: int(getattr(t, k)) for k in self.TILE_FIELDS}
+
     def workspace_bytes(self, nmaps):
         b = ctypes.c_uint64()
         _abi.check(_abi.lib().hbx_workspace_bytes(self._h, int(nmaps), ctypes.byref(b)))

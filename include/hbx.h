@@ -103,6 +103,24 @@ int hbx_plan_destroy(hbx_plan *plan);
 int hbx_plan_get_info(const hbx_plan *plan, hbx_plan_info *info);
 int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
 
+/* Tiles (no reference counterpart; the autotuner's hook).  0 = the
+ * size-based default.  Tiles change the workspace size and the summation
+ * order (results agree to rounding), so set them before the plan is used and
+ * never while work using the plan is in flight.  get_tiles returns the
+ * effective values.
+ *   valu_slots     VALU Legendre analysis: ring-pair slots of 64 pairs per lane
+ *                  (power of two <= 16; FP64 kernels clamp to 8)
+ *   mfma_waves     matrix-core analysis: waves (64 ring pairs each) per
+ *                  workgroup, 4, 8 or 16
+ *   mfma_min_maps  FP32 batches of at least this many maps use the
+ *                  matrix-core Legendre kernels (default 2)
+ *   spin_slots     spin Legendre analysis: ring-pair slots per lane, 1, 2, 4 */
+typedef struct hbx_tiles {
+    int32_t valu_slots, mfma_waves, mfma_min_maps, spin_slots;
+} hbx_tiles;
+int hbx_plan_set_tiles(hbx_plan *plan, const hbx_tiles *tiles);
+int hbx_plan_get_tiles(const hbx_plan *plan, hbx_tiles *tiles);
+
 /* Scalar transforms.  maps: nmaps maps of plan npix values (dtype
  * map_dtype), map_stride elements apart.  alm: nmaps m-major alm arrays
  * (idx = m(2 lmax+1-m)/2 + l) of complex128 (HBX_F64 plans) or complex64

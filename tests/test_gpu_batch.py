@@ -1,0 +1,136 @@
+"""Tiles, the autotuner and mixed-geometry batches in one HIP graph.
+
+Bars: every tile set computes the same transform (FP64: 1e-13 relative,
+FP32: 2e-6 relative to the default tiles); graph replay equals the eager run
+bit for bit on every replay, with eager work between replays; every map of
+the mixed batch is within eps_max 1e-5 of its FP64 transform.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import sht as OS
+
+pytestmark = pytest.mark.gpu
+
+
+def _plan(nside, lmax, precision="f64", spin=0):
+    from hellblast_amd.hp_compat import _plan
+    return _plan(nside, lmax, lmax, precision, "dz", "reference", "reference", spin)
+
+
+def _eps_max(a, ref, lmax):
+    """max |a - ref| / sqrt(C_l(ref)) over (l, m) (m-major alm)."""
+    a = np.asarray(a, dtype=np.complex128)
+    ref = np.asarray(ref, dtype=np.complex128)
+    cl = OS.alm2cl(ref, lmax)
+    ls = np.concatenate([np.arange(m, lmax + 1) for m in range(lmax + 1)])
+    c = cl[ls]
+    ok = c > 0
+    return float(np.max(np.abs(a - ref)[ok] / np.sqrt(c[ok])))
+
+
+def test_tiles_set_get_and_validation(gpu):
+    from hellblast_amd._abi import HbxValueError
+    plan = _plan(32, 64, "f32")
+    base = plan.tiles()
+    assert base["mfma_waves"] == 16 and base["mfma_min_maps"] == 2
+    plan.set_tiles(mfma_waves=4, valu_slots=2)
+    t = plan.tiles()
+    assert t["mfma_waves"] == 4 and t["valu_slots"] == 2
+    with pytest.raises(HbxValueError):
+        plan.set_tiles(mfma_waves=5)
+    with pytest.raises(HbxValueError):
+        plan.set_tiles(valu_slots=3)
+    plan.set_tiles(mfma_waves=0, valu_slots=0)
+    assert plan.tiles() == base
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_every_candidate_tile_gives_the_same_transform(gpu, precision):
+    from hellblast_amd.autotune import candidates
+    nside, lmax, n = 64, 128, 4
+    print(f"[case] tile candidates {precision}", flush=True)
+    plan = _plan(nside, lmax, precision)
+    rdt = torch.float64 if precision == "f64" else torch.float32
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(3)
+    maps = torch.randn((n, plan.npix), generator=gen, device="cuda", dtype=rdt)
+    plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
+    ref_a = plan.map2alm(maps)
+    ref_m = plan.alm2map(ref_a, dtype=rdt)
+    tol = 1e-13 if precision == "f64" else 2e-6
+    for cand in candidates(plan, n):
+        plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
+        plan.set_tiles(**cand)
+        a = plan.map2alm(maps)
+        m = plan.alm2map(ref_a, dtype=rdt)
+        assert float((a - ref_a).abs().max()) <= tol * float(ref_a.abs().max()), cand
+        assert float((m - ref_m).abs().max()) <= tol * float(ref_m.abs().max()), cand
+    plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
+
+
+def test_autotune_caches_its_choice(gpu, tmp_path, monkeypatch):
+    from hellblast_amd import autotune
+    monkeypatch.setenv(autotune.CACHE_ENV, str(tmp_path / "tiles.json"))
+    plan = _plan(64, 128, "f32")
+    first = autotune.tune(plan, 4, reps=1)
+    assert (tmp_path / "tiles.json").exists()
+    assert len(first["timed"]) == len(autotune.candidates(plan, 4))
+    second = autotune.tune(plan, 4, reps=1)
+    assert second["tiles"] == first["tiles"]
+    eff = plan.tiles()
+    for k, v in first["tiles"].items():
+        assert eff[k] == v
+    plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
+
+
+def _c3_specs(rng):
+    """64 maps, nside allocation {2048:1, 1024:2, 512:5, 256:14, 128:42}, shuffled."""
+    alloc = {2048: 1, 1024: 2, 512: 5, 256: 14, 128: 42}
+    specs = [n for n, k in alloc.items() for _ in range(k)]
+    rng.shuffle(specs)
+    return [(n, 2 * n) for n in specs]
+
+
+def test_mixed_batch_graph_replay_equals_eager(gpu, tmp_path, monkeypatch):
+    """The 64-map mixed batch (lmax = 2 nside, C_l = (l+10)^-2), FP32, tuned
+    tiles, one graph launch: replays equal the eager run bit for bit and every
+    map's alm is within eps_max 1e-5 of its FP64 transform."""
+    from hellblast_amd import autotune
+    from hellblast_amd.batch import MixedBatch
+    monkeypatch.setenv(autotune.CACHE_ENV, str(tmp_path / "tiles.json"))
+    rng = np.random.default_rng(3)
+    specs = _c3_specs(rng)
+    print(f"[case] mixed batch of {len(specs)} maps", flush=True)
+    batch = MixedBatch(specs, precision="f32", tune=True)
+    # inputs: FP64 syntheses of C_l = (l+10)^-2 alm, stored as FP32 maps
+    maps64 = []
+    for i, (nside, lmax) in enumerate(specs):
+        cl = (np.arange(lmax + 1) + 10.0) ** -2
+        alm = torch.as_tensor(OS.synalm(cl, lmax, seed=1000 + i), device="cuda")
+        maps64.append(_plan(nside, lmax).alm2map(alm))
+    batch.set_maps([m.float() for m in maps64])
+    batch.run_eager()
+    torch.cuda.synchronize()
+    eager_alm = [batch.alm_of(i).clone() for i in range(batch.nmaps)]
+    eager_out = [batch.output(i).clone() for i in range(batch.nmaps)]
+    batch.capture()
+    for rep in range(3):
+        for key in batch.alm:
+            batch.alm[key].zero_()
+            batch.out[key].zero_()
+        batch.replay()
+        torch.cuda.synchronize()
+        for i in range(batch.nmaps):
+            assert torch.equal(batch.alm_of(i), eager_alm[i]), (rep, i)
+            assert torch.equal(batch.output(i), eager_out[i]), (rep, i)
+        # eager work between replays
+        _ = _plan(128, 256, "f32").map2alm(torch.randn((2, 12 * 128 * 128), device="cuda"))
+        torch.cuda.synchronize()
+    worst = 0.0
+    for i, (nside, lmax) in enumerate(specs):
+        ref = _plan(nside, lmax).map2alm(batch.map(i).double()).cpu().numpy()
+        worst = max(worst, _eps_max(eager_alm[i].cpu().numpy(), ref, lmax))
+    print(f"worst eps_max over the batch {worst:.2e}; tiles {batch.tiles}")
+    assert worst <= 1e-5
