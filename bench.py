@@ -267,8 +267,9 @@ def main():
     log("FP64 kernel means (ms):", json.dumps({k: round(v, 3) for k, v in kms64.items()}))
 
     # ---------------- roofline of the dominant kernel (FP32 batch, matrix cores)
-    # analysis covers m >= 16 (m < 16 run in the FP64 band); synthesis covers every m
-    band = 16 * (2 * lmax + 1 - 16) // 2 + 16 if lmax >= 16 else nlm
+    # analysis covers m >= mb (rows m < mb run in the FP64 band); synthesis covers every m
+    mb = p32.fp64_band_m
+    band = mb * (2 * lmax + 1 - mb) // 2 + mb
     flops = {"legendre_analysis": 4.0 * npairs * (nlm - band) * B,
              "legendre_synthesis": 4.0 * npairs * nlm * B}
     names = {"legendre_analysis": "leg_analysis_mfma_kernel", "legendre_synthesis": "leg_synthesis_mfma_kernel"}

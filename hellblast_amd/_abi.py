@@ -65,6 +65,7 @@ class hbx_plan_info(ctypes.Structure):
         ("nrings", ctypes.c_int32), ("npairs", ctypes.c_int32), ("lmax", ctypes.c_int32),
         ("mmax", ctypes.c_int32), ("precision", ctypes.c_int32), ("spin", ctypes.c_int32),
         ("npix", ctypes.c_int64), ("nalm", ctypes.c_int64), ("device_bytes", ctypes.c_uint64),
+        ("fp64_band_m", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 

@@ -91,7 +91,7 @@ static AnalysisWs analysis_ws(const Plan *p, int nmaps) {
     w.partial = 0;
     uint64_t off = align256(g > 1 ? (uint64_t)nmaps * g * p->nlm * cs : 0);
     if (p->precision == HBX_F32) {
-        const int g64 = analysis_groups(p, true);
+        const int g64 = band_groups(p);
         w.band_alm = off;
         off += align256((uint64_t)nmaps * band_nlm(p) * 16);
         w.band_partial = off;
@@ -352,6 +352,8 @@ int hbx_plan_get_info(const hbx_plan *plan, hbx_plan_info *info) {
     info->npix = p.npix;
     info->nalm = p.nlm;
     info->device_bytes = p.device_bytes;
+    info->fp64_band_m = (p.precision == HBX_F32 && p.spin == 0) ? std::min(M_LO, p.mmax + 1) : 0;
+    info->reserved = 0;
     return HBX_OK;
 }
 

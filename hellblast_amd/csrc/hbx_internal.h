@@ -35,8 +35,9 @@ constexpr int CHUNK = 8;       // l steps per recursion chunk
 constexpr int TAILPAD = 64;    // zero coefficients after the last recursion row
 // FP32 plans analyse m < M_LO in FP64: the ring coefficients of low m carry
 // the large-scale power, and rounding them to FP32 costs ~1e-5 of sqrt(C_l)
-// at high l for red spectra.
-constexpr int M_LO = 16;
+// at high l for red spectra (measured at nside 2048, lmax 4095, C_l =
+// 1/(l+1)^2: eps_max 1.45e-5 with no band, 3.5e-6 with m < 8 or m < 16).
+constexpr int M_LO = 8;
 constexpr int NMB_LO = M_LO / MB;
 
 inline int64_t nalm(int lmax, int mmax) {
@@ -120,6 +121,7 @@ int launch_spin_synthesis(const Plan *p, int npol, const void *scaled, void *pha
 int launch_leg_analysis(const Plan *p, int nmaps, const void *phase, void *alm,
                         int64_t alm_stride, void *partial, int adjoint, hipStream_t s);
 int analysis_groups(const Plan *p, bool f64);   // ring-pair groups (partial sums) per m
+int band_groups(const Plan *p);                 // the same for the FP64 band of FP32 plans
 int launch_prescale(const Plan *p, int nmaps, const void *alm, int64_t alm_stride,
                     void *scaled, int adjoint, hipStream_t s);
 int launch_leg_synthesis(const Plan *p, int nmaps, const void *scaled, void *phase,

@@ -514,12 +514,16 @@ int analysis_groups(const Plan *p, bool f64) {
     return (int)((p->npairs + 64 * r - 1) / (64 * r));
 }
 
+// FP64 band of FP32 plans: the FP64 slot count of the plan (the per-chunk
+// wave reduction is amortised over the slots, so fewer slots cost more)
+static int band_slots(const Plan *p) { return analysis_slots<double>(p); }
+int band_groups(const Plan *p) { return (p->npairs + 64 * band_slots(p) - 1) / (64 * band_slots(p)); }
+
 template <typename T>
 static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, int64_t pstride, c2<T> *alm,
                           int64_t astride, c2<T> *partial, int64_t nlm_stride, int mbeg, int mend, int adjoint,
-                          hipStream_t s, int timer_id) {
+                          hipStream_t s, int timer_id, int R) {
     if (mend < mbeg) return 0;
-    const int R = analysis_slots<T>(p);
     const int G = (int)((p->npairs + 64 * R - 1) / (64 * R));
     const int nmb8 = (mend / MB + 1 + 7) / 8 * 8;
     dim3 grid(nmb8 * MB * G, nmaps);
@@ -578,10 +582,10 @@ int launch_leg_analysis(const Plan *p, int nmaps, const void *phase, void *alm, 
     if (p->precision == HBX_F64)
         return leg_analysis_t<double>(p, nmaps, (const double2 *)phase, pstride, (double2 *)alm, astride,
                                       (double2 *)partial, p->nlm, 0, p->mmax, adjoint, s,
-                                      HBX_TIMER_LEGENDRE_ANALYSIS);
+                                      HBX_TIMER_LEGENDRE_ANALYSIS, analysis_slots<double>(p));
     return leg_analysis_t<float>(p, nmaps, (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial,
                                  p->nlm, std::min(M_LO, p->mmax + 1), p->mmax, adjoint, s,
-                                 HBX_TIMER_LEGENDRE_ANALYSIS);
+                                 HBX_TIMER_LEGENDRE_ANALYSIS, analysis_slots<float>(p));
 }
 
 // m < M_LO of an FP32 plan in FP64: band phase -> band alm (FP64) -> alm
@@ -590,7 +594,7 @@ int launch_leg_analysis_band(const Plan *p, int nmaps, const void *band_phase, v
     const int mend = std::min(M_LO - 1, p->mmax);
     const int64_t nb = band_nlm(p);
     int rc = leg_analysis_t<double>(p, nmaps, (const double2 *)band_phase, band_phase_elems(p), (double2 *)band_alm,
-                                    nb, (double2 *)band_partial, nb, 0, mend, adjoint, s, 0);
+                                    nb, (double2 *)band_partial, nb, 0, mend, adjoint, s, 0, band_slots(p));
     if (rc) return rc;
     hipLaunchKernelGGL(band_to_f32_kernel, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024), nmaps),
                        dim3(256), 0, s, (const double2 *)band_alm, nb, (float2 *)alm, astride);

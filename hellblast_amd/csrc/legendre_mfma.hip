@@ -387,7 +387,12 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
     }
 }
 
-int mfma_analysis_waves(const Plan *p) { return p->tile_mfma_waves > 0 ? p->tile_mfma_waves : 16; }
+// waves per workgroup: the plan's tile, else 8 (4 for small geometries, whose
+// ring pairs would leave most of a larger workgroup idle)
+int mfma_analysis_waves(const Plan *p) {
+    if (p->tile_mfma_waves > 0) return p->tile_mfma_waves;
+    return p->npairs <= 256 ? 4 : 8;
+}
 
 int mfma_analysis_groups(const Plan *p) {
     const int pairs_wg = mfma_analysis_waves(p) * 64;

@@ -148,6 +148,7 @@ class Plan:
         self.npix = int(info.npix)
         self.nalm = int(info.nalm)
         self.device_bytes = int(info.device_bytes)
+        self.fp64_band_m = int(info.fp64_band_m)
         self.alm_dtype = torch.complex128 if self.precision == _abi.HBX_F64 else torch.complex64
 
     def __del__(self):

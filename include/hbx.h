@@ -84,6 +84,9 @@ typedef struct hbx_plan_info {
     int32_t nrings, npairs, lmax, mmax, precision, spin;
     int64_t npix, nalm;
     uint64_t device_bytes;     /* device memory held by the plan             */
+    int32_t fp64_band_m;       /* FP32 scalar plans: rows m < fp64_band_m are
+                                  analysed in FP64 (0 otherwise)             */
+    int32_t reserved;
 } hbx_plan_info;
 
 int hbx_abi_version(void);

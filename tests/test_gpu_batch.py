@@ -34,7 +34,7 @@ def test_tiles_set_get_and_validation(gpu):
     from hellblast_amd._abi import HbxValueError
     plan = _plan(32, 64, "f32")
     base = plan.tiles()
-    assert base["mfma_waves"] == 16 and base["mfma_min_maps"] == 2
+    assert base["mfma_waves"] == 4 and base["mfma_min_maps"] == 2
     plan.set_tiles(mfma_waves=4, valu_slots=2)
     t = plan.tiles()
     assert t["mfma_waves"] == 4 and t["valu_slots"] == 2
