@@ -125,7 +125,8 @@ leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float
         Do[j] = floatx16{0};
     }
     // one buffer: the chunk's operands arrive in a single round trip and
-    // co-resident workgroups hide it (occupancy 3)
+    // co-resident workgroups hide it (a register prefetch of the next chunk
+    // measured no faster: it costs occupancy)
     for (int c0 = lstart; c0 <= lmax; c0 += SYN_LCHUNK) {
         {
             float2 pv[PER];
@@ -229,8 +230,14 @@ int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int ANA_L = 32;                 // l per chunk: 16 even + 16 odd columns
-constexpr int ANA_STR = 17;               // padded LDS row (16 columns + 1)
-constexpr int ANA_LAM = 2 * 64 * ANA_STR; // floats of one wave's lambda tile
+// A wave's lambda tile holds, per parity and column (16 l of that parity),
+// the 64 ring pairs q at (q%4)*16 + q/4: one recursion step writes 64
+// consecutive floats, and the lane that feeds B[k = q%4][col] reads its 16
+// values (q/4 = s = 0..15) as four 16-byte loads.  Rows are padded to 68
+// floats so the 16 lanes of a quarter wave hit disjoint banks.
+constexpr int ANA_COL = 68;
+constexpr int ANA_PAR = 16 * ANA_COL;
+constexpr int ANA_LAM = 2 * ANA_PAR;      // floats of one wave's lambda tile
 constexpr size_t ana_lds(int waves) { return (size_t)(waves * ANA_LAM + 4 * ANA_L + 4) * sizeof(float); }
 
 template <int ANA_WAVES>
@@ -343,20 +350,30 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
                 const double u = fma(ab[t] * x, u1, -u2);
                 u2 = u1;
                 u1 = u;
-                lam[(t & 1) * 64 * ANA_STR + lane * ANA_STR + (t >> 1)] = (float)u;
+                lam[(t & 1) * ANA_PAR + (t >> 1) * ANA_COL + (lane & 3) * 16 + (lane >> 2)] = (float)u;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-            for (int s = 0; s < 16; ++s) {
+            for (int par = 0; par < 2; ++par) {
+                // B[k = lane/16][col = lane%16] for s = 0..15: ring pair 4 s + k
+                const float4 *bp =
+                    reinterpret_cast<const float4 *>(lam + par * ANA_PAR + (lane & 15) * ANA_COL + (lane >> 4) * 16);
+                float bv[16];
 #pragma unroll
-                for (int par = 0; par < 2; ++par) {
-                    const float bop = lam[par * 64 * ANA_STR + (4 * s + (lane >> 4)) * ANA_STR + (lane & 15)];
+                for (int j = 0; j < 4; ++j) {
+                    const float4 q4 = bp[j];
+                    bv[4 * j] = q4.x;
+                    bv[4 * j + 1] = q4.y;
+                    bv[4 * j + 2] = q4.z;
+                    bv[4 * j + 3] = q4.w;
+                }
+#pragma unroll
+                for (int s = 0; s < 16; ++s)
 #pragma unroll
                     for (int tile = 0; tile < 2; ++tile)
-                        D[tile][par] = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[s][tile][par], bop, D[tile][par], 0, 0, 0);
-                }
+                        D[tile][par] = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[s][tile][par], bv[s], D[tile][par], 0, 0, 0);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();

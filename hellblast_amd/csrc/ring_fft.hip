@@ -24,15 +24,41 @@
 namespace hbx {
 
 // ---------------------------------------------------------------------------
+// Twiddles w^J = e^{-2 pi i J / 2^twshift} come from a two-level table in LDS
+// (staged from the plan's global table once per workgroup):
+//   w^J = coarse[J >> f] * fine[J & (2^f - 1)],  f = min(7, twshift),
+// 2^(twshift-f) + 2^f entries (192 for 8192-point rings), so the FFT passes
+// read LDS instead of waiting on global loads between barriers.
+// ---------------------------------------------------------------------------
+constexpr int TW_FINE_LOG = 7;
+inline int tw_fine_log(int twshift) { return twshift < TW_FINE_LOG ? twshift : TW_FINE_LOG; }
+__device__ __forceinline__ int tw_fine_log_d(int twshift) { return min(twshift, TW_FINE_LOG); }
+inline int tw_lds_entries(int twshift) {
+    const int f = tw_fine_log(twshift);
+    return (1 << (twshift - f)) + (1 << f);
+}
+
+template <typename T>
+__device__ void stage_twiddles(c2<T> *tab, const c2<T> *__restrict__ tw, int twshift) {
+    const int f = tw_fine_log_d(twshift), nc = 1 << (twshift - f), nf = 1 << f;
+    for (int i = threadIdx.x; i < nc + nf; i += blockDim.x) tab[i] = (i < nc) ? tw[i << f] : tw[i - nc];
+}
+
+template <typename T>
+__device__ __forceinline__ c2<T> tw_get(const c2<T> *tab, int J, int f, int nc) {
+    return cmul<T>(tab[J >> f], tab[nc + (J & ((1 << f) - 1))]);
+}
+
+// ---------------------------------------------------------------------------
 // in-LDS FFT of size M (power of two), forward (e^{-i}) or inverse (e^{+i},
 // unnormalised).  Stockham radix-4 passes plus one radix-2 pass when log2 M
 // is odd; each pass stages its inputs in registers so one buffer suffices.
-// tw[k] = e^{-2 pi i k / 2^twshift}.
+// tab: the LDS twiddle table (stage_twiddles, visible to every thread).
 // ---------------------------------------------------------------------------
 template <typename T, int NB>
-__device__ void lds_fft(c2<T> *buf, int M, int log2M, const c2<T> *__restrict__ tw,
-                        int twshift, bool inv) {
+__device__ void lds_fft(c2<T> *buf, int M, int log2M, const c2<T> *tab, int twshift, bool inv) {
     const int nt = blockDim.x, tid = threadIdx.x;
+    const int tf = tw_fine_log_d(twshift), tnc = 1 << (twshift - tf);
     const int Q = M >> 2;
     int p = 1, lp = 0;
     const int n4 = log2M >> 1;
@@ -53,7 +79,8 @@ __device__ void lds_fft(c2<T> *buf, int M, int log2M, const c2<T> *__restrict__ 
             const int i = tid + b * nt;
             if (i < Q) {
                 const int k = i & (p - 1);
-                c2<T> w1 = tw[k << sh], w2 = tw[(2 * k) << sh], w3 = tw[(3 * k) << sh];
+                c2<T> w1 = tw_get<T>(tab, k << sh, tf, tnc), w2 = tw_get<T>(tab, (2 * k) << sh, tf, tnc),
+                      w3 = tw_get<T>(tab, (3 * k) << sh, tf, tnc);
                 if (inv) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
                 const c2<T> x0 = v[b][0];
                 const c2<T> x1 = cmul<T>(v[b][1], w1);
@@ -80,7 +107,7 @@ __device__ void lds_fft(c2<T> *buf, int M, int log2M, const c2<T> *__restrict__ 
         for (int b = 0; b < 2 * NB; ++b) {
             const int i = tid + b * nt;
             if (i < H) {
-                c2<T> w = tw[i << sh];
+                c2<T> w = tw_get<T>(tab, i << sh, tf, tnc);
                 if (inv) w.y = -w.y;
                 const c2<T> x0 = buf[i];
                 const c2<T> x1 = cmul<T>(buf[i + H], w);
@@ -106,7 +133,7 @@ template <typename T> __device__ __forceinline__ T load_as(const float *p) { ret
 // analysis: maps -> phase
 // ---------------------------------------------------------------------------
 template <typename T, typename TIn, int NB>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
 ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__restrict__ phase,
                      int64_t phase_stride, const int32_t *__restrict__ rings,
                      const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
@@ -116,12 +143,14 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
                      int mmax, int nr) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
+    c2<T> *tab = buf + M;
     const int r = rings[blockIdx.x];
     const int b = blockIdx.y;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = nphi[r], h = n >> 1;
     const bool pow2 = (h == M);
     const TIn *f = maps + (int64_t)b * map_stride + ring_start[r];
+    stage_twiddles<T>(tab, tw, twshift);
 
     for (int j = tid; j < M; j += nt) {
         c2<T> z = mk<T>(0, 0);
@@ -132,12 +161,12 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
         buf[j] = z;
     }
     __syncthreads();
-    lds_fft<T, NB>(buf, M, log2M, tw, twshift, false);
+    lds_fft<T, NB>(buf, M, log2M, tab, twshift, false);
     if (!pow2) {
         const c2<T> *Vr = V + vofs[r];
         for (int k = tid; k < M; k += nt) buf[k] = cmul<T>(buf[k], Vr[k]);
         __syncthreads();
-        lds_fft<T, NB>(buf, M, log2M, tw, twshift, true);
+        lds_fft<T, NB>(buf, M, log2M, tab, twshift, true);
         const T invM = (T)1 / (T)M;
         for (int k = tid; k < h; k += nt)
             buf[k] = cscale<T>(cmul<T>(buf[k], cconj<T>(chirp<T>(k, h))), invM);
@@ -176,7 +205,7 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
 // synthesis: phase -> maps
 // ---------------------------------------------------------------------------
 template <typename T, typename TOut, int NB>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
 ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOut *__restrict__ maps,
                       int64_t map_stride, const int32_t *__restrict__ rings,
                       const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
@@ -186,12 +215,14 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
                       int mmax, int nr) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
+    c2<T> *tab = buf + M;
     const int r = rings[blockIdx.x];
     const int b = blockIdx.y;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = nphi[r], h = n >> 1;
     const bool pow2 = (h == M);
     const int top = min(mmax, mcut[r] - 1);
+    stage_twiddles<T>(tab, tw, twshift);
     const double ph0 = phi0 ? phi0[r] : 0.0;
     const c2<T> *G = phase + (int64_t)b * phase_stride;
 
@@ -234,13 +265,13 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
         for (int j = h + tid; j < M; j += nt) buf[j] = mk<T>(0, 0);
     __syncthreads();
     if (pow2) {
-        lds_fft<T, NB>(buf, M, log2M, tw, twshift, true);
+        lds_fft<T, NB>(buf, M, log2M, tab, twshift, true);
     } else {
-        lds_fft<T, NB>(buf, M, log2M, tw, twshift, false);
+        lds_fft<T, NB>(buf, M, log2M, tab, twshift, false);
         const c2<T> *Vr = V + vofs[r];
         for (int k = tid; k < M; k += nt) buf[k] = cmul<T>(buf[k], Vr[k]);
         __syncthreads();
-        lds_fft<T, NB>(buf, M, log2M, tw, twshift, true);
+        lds_fft<T, NB>(buf, M, log2M, tab, twshift, true);
         const T invM = (T)1 / (T)M;
         for (int j = tid; j < h; j += nt)
             buf[j] = cscale<T>(cmul<T>(chirp<T>(j, h), cconj<T>(buf[j])), invM);
@@ -357,12 +388,14 @@ __global__ void twiddle_kernel(c2<T> *tw, int twn) {
 
 // V_h = FFT_M(v), v_j = c_j (0 <= j < h), v_{M-j} = c_j (1 <= j < h), else 0
 template <typename T, int NB>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
 chirp_spectrum_kernel(const int32_t *__restrict__ hs, const int64_t *__restrict__ ofs, c2<T> *V,
                       const c2<T> *__restrict__ tw, int twshift, int M, int log2M) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
+    c2<T> *tab = buf + M;
     const int h = hs[blockIdx.x];
+    stage_twiddles<T>(tab, tw, twshift);
     for (int j = threadIdx.x; j < M; j += blockDim.x) {
         c2<T> v = mk<T>(0, 0);
         if (j < h) v = chirp<T>(j, h);
@@ -370,12 +403,15 @@ chirp_spectrum_kernel(const int32_t *__restrict__ hs, const int64_t *__restrict_
         buf[j] = v;
     }
     __syncthreads();
-    lds_fft<T, NB>(buf, M, log2M, tw, twshift, false);
+    lds_fft<T, NB>(buf, M, log2M, tab, twshift, false);
     c2<T> *out = V + ofs[blockIdx.x];
     for (int j = threadIdx.x; j < M; j += blockDim.x) out[j] = buf[j];
 }
 
-static int class_threads(int M) { return M >= 4096 ? 512 : 256; }
+// threads per ring workgroup: 2 radix-4 butterflies per thread for M >= 4096
+// (8192-point Bluestein rings need 1024 threads to stay at ~60 VGPRs, so
+// two workgroups share a CU)
+static int class_threads(int M) { return M >= 8192 ? 1024 : (M >= 4096 ? 512 : 256); }
 static int class_nb(int M) {
     const int nt = class_threads(M);
     return std::max(1, M / (4 * nt));
@@ -413,7 +449,7 @@ static int build_chirp_spectra(Plan *p, const std::vector<int32_t> &hs_host,
     HBX_HIP(hipMemcpy(d_hs, hs_host.data(), hs_host.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HBX_HIP(hipMemcpy(d_ofs, ofs_host.data(), ofs_host.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     const int nt = class_threads(M), nb = class_nb(M), lg = ilog2(M);
-    const size_t lds = (size_t)M * sizeof(c2<T>);
+    const size_t lds = (size_t)(M + tw_lds_entries(p->twshift)) * sizeof(c2<T>);
     int rc = 0;
     HBX_NB_SWITCH(nb, {
         auto k = chirp_spectrum_kernel<T, NBV>;
@@ -468,7 +504,7 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi) {
     int maxlds = 0;
     HBX_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeSharedMemPerBlockOptin, dev));
     if (maxlds <= 0) HBX_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    if ((size_t)maxM * csz > (size_t)maxlds)
+    if ((size_t)(maxM + tw_lds_entries(ilog2(maxM))) * csz > (size_t)maxlds)
         return set_error(HBX_EUNSUPPORTED, "ring FFT of size " + std::to_string(maxM) +
                                                " does not fit in LDS (" + std::to_string(maxlds) +
                                                " bytes) at this precision");
@@ -513,8 +549,8 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi) {
         c.nrings = (int)kv.second.size();
         if ((rc = dalloc((void **)&c.d_rings, kv.second.size() * sizeof(int32_t)))) return rc;
         HBX_HIP(hipMemcpy(c.d_rings, kv.second.data(), kv.second.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        c.lds_bytes64 = (size_t)c.M * sizeof(double2);
-        c.lds_bytes32 = (size_t)c.M * sizeof(float2);
+        c.lds_bytes64 = (size_t)(c.M + tw_lds_entries(p->twshift)) * sizeof(double2);
+        c.lds_bytes32 = (size_t)(c.M + tw_lds_entries(p->twshift)) * sizeof(float2);
         p->classes.push_back(c);
     }
     // raise the dynamic-LDS limit of every kernel instance we may launch
@@ -546,7 +582,7 @@ static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t ma
     const int64_t pstride = phase_elems_per_map(p);
     timer_mark(HBX_TIMER_RING_ANALYSIS, true, s);
     for (const auto &c : p->classes) {
-        const size_t lds = (size_t)c.M * sizeof(c2<T>);
+        const size_t lds = (size_t)(c.M + tw_lds_entries(p->twshift)) * sizeof(c2<T>);
         dim3 grid((unsigned)c.nrings, (unsigned)nmaps);
         HBX_NB_SWITCH(c.nb, {
             hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, NBV>), grid, dim3(c.threads), lds, s, maps,
@@ -568,7 +604,7 @@ static int ring_synthesis_t(const Plan *p, int nmaps, const c2<T> *phase, TOut *
     const int64_t pstride = phase_elems_per_map(p);
     timer_mark(HBX_TIMER_RING_SYNTHESIS, true, s);
     for (const auto &c : p->classes) {
-        const size_t lds = (size_t)c.M * sizeof(c2<T>);
+        const size_t lds = (size_t)(c.M + tw_lds_entries(p->twshift)) * sizeof(c2<T>);
         dim3 grid((unsigned)c.nrings, (unsigned)nmaps);
         HBX_NB_SWITCH(c.nb, {
             hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, NBV>), grid, dim3(c.threads), lds, s, phase,
