@@ -8,9 +8,17 @@
 // one MFMA for the even l of a 4-l step and one for the odd l, so the two
 // accumulators are G_even and G_odd and the mirror rings follow as
 // G_n = G_even + G_odd, G_s = G_even - G_odd.
+// A wave covers two 32-ring tiles: lane t runs the recursion of ring t of the
+// wave (tile t/32) for l0..l0+3, and v_permlane32_swap exchanges the halves
+// so each tile's A operand gets l0 (lanes 0-31) and l0+2 (lanes 32-63):
+//   swap(v0, v2) -> (tile 0: [v0 | v2 of lanes 0-31], tile 1: [v0 of lanes
+//   32-63 | v2]), and likewise (v1, v3) for the odd l.
+// Every ring's recursion runs once, and the staged B operands serve 64 rings.
 // Lane layout of the 32x32x2 f32 instruction (checked on gfx950):
 //   A: lane = row + 32 k     B: lane = col + 32 k
 //   D[row][col]: lane = col + 32 ((row/4)%2), reg = (row%4) + 4 (row/8)
+// and of v_permlane32_swap(a, b) (checked on gfx950): returns
+//   ([a lanes 0-31 | b lanes 0-31], [a lanes 32-63 | b lanes 32-63]).
 // Reference counterpart: the per-(ring, m) l-sum of isht.py:15-21.
 #include "hbx_device.h"
 #include "hbx_internal.h"
@@ -21,8 +29,7 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int SYN_LCHUNK = 64;     // l values staged per chunk
 constexpr int SYN_WAVES = 4;       // waves per workgroup (same m)
-constexpr int SYN_TILES = 1;       // 32-ring tiles per wave
-constexpr int SYN_PAIRS_WG = SYN_WAVES * SYN_TILES * 32;
+constexpr int SYN_PAIRS_WG = SYN_WAVES * 64;
 constexpr int NEVER_L = 0x7fffffff;
 
 __device__ __forceinline__ int wmin64(int v) {
@@ -36,15 +43,24 @@ __device__ __forceinline__ int wmax64(int v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-__global__ void __launch_bounds__(SYN_WAVES * 64)
+// (lo, hi) halves exchange: returns {[a.lo | b.lo], [a.hi | b.hi]}
+__device__ __forceinline__ void swap_halves(float a, float b, float *t0, float *t1) {
+    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    *t0 = __uint_as_float(r[0]);
+    *t1 = __uint_as_float(r[1]);
+}
+
+// (the waves-per-EU bound lets the four 32x32 accumulators live in VGPRs;
+// without it the compiler cycles them through one AGPR block with copies)
+__global__ void __launch_bounds__(SYN_WAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 4)))
 leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float *__restrict__ phase,
                           const double *__restrict__ px, const int64_t *__restrict__ rofs,
                           const double *__restrict__ alpha, const double *__restrict__ sl,
                           const int32_t *__restrict__ lseed, const double2 *__restrict__ seed, int npairs, int nr,
                           int lmax, int mmax, int nmb, int nsb, int nmaps, int adjoint) {
     // per-chunk operands: at[l][n] and alpha_l
-    __shared__ float stage[1][SYN_LCHUNK][33];
-    __shared__ double alds[1][SYN_LCHUNK];
+    __shared__ float stage[SYN_LCHUNK][33];
+    __shared__ double alds[SYN_LCHUNK];
     __shared__ int wg_start;
     int mb, item;
     xcd_map(blockIdx.x, MB * nsb, &mb, &item);
@@ -56,28 +72,19 @@ leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int r = lane & 31, h = lane >> 5;
 
-    double x[SYN_TILES], u1[SYN_TILES], u2[SYN_TILES];
-    double2 sd[SYN_TILES];
-    int ls[SYN_TILES];
-    int wstart = NEVER_L, winj = -1;
-#pragma unroll
-    for (int j = 0; j < SYN_TILES; ++j) {
-        const int p = (sb * SYN_WAVES * SYN_TILES + w * SYN_TILES + j) * 32 + r;
-        u1[j] = 0.0;
-        u2[j] = 0.0;
-        x[j] = 0.0;
-        ls[j] = NEVER_L;
-        sd[j] = make_double2(0.0, 0.0);
-        if (p < npairs) {
-            x[j] = px[p];
-            ls[j] = lseed[(int64_t)m * npairs + p];
-            if (ls[j] <= lmax) sd[j] = seed[(int64_t)m * npairs + p];
-        }
-        wstart = min(wstart, ls[j]);
-        if (ls[j] <= lmax) winj = max(winj, ls[j]);
+    // this lane's ring pair: ring lane of the wave's 64
+    const int pbase = (sb * SYN_WAVES + w) * 64;
+    const int p = pbase + lane;
+    double x = 0.0, u1 = 0.0, u2 = 0.0;
+    double2 sd = make_double2(0.0, 0.0);
+    int ls = NEVER_L;
+    if (p < npairs) {
+        x = px[p];
+        ls = lseed[(int64_t)m * npairs + p];
+        if (ls <= lmax) sd = seed[(int64_t)m * npairs + p];
     }
-    wstart = wmin64(wstart);
-    winj = wmax64(winj);
+    const int wstart = wmin64(ls);
+    const int winj = wmax64(ls <= lmax ? ls : -1);
     if (tid == 0) wg_start = NEVER_L;
     __syncthreads();
     if (lane == 0) atomicMin(&wg_start, wstart);
@@ -108,31 +115,30 @@ leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float
         }
         *a = (tid < SYN_LCHUNK) ? al[c0 + tid] : 0.0;
     };
-    auto put = [&](int buf, const float2 *v, double a) {
+    auto put = [&](const float2 *v, double a) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int n = 2 * (sb0 + SYN_WAVES * q);
-            stage[buf][sll][n] = v[q].x;
-            stage[buf][sll][n + 1] = v[q].y;
+            stage[sll][n] = v[q].x;
+            stage[sll][n + 1] = v[q].y;
         }
-        if (tid < SYN_LCHUNK) alds[buf][tid] = a;
+        if (tid < SYN_LCHUNK) alds[tid] = a;
     };
 
-    floatx16 De[SYN_TILES], Do[SYN_TILES];
+    floatx16 De[2], Do[2];
 #pragma unroll
-    for (int j = 0; j < SYN_TILES; ++j) {
+    for (int j = 0; j < 2; ++j) {
         De[j] = floatx16{0};
         Do[j] = floatx16{0};
     }
     // one buffer: the chunk's operands arrive in a single round trip and
-    // co-resident workgroups hide it (a register prefetch of the next chunk
-    // measured no faster: it costs occupancy)
+    // co-resident workgroups hide it
     for (int c0 = lstart; c0 <= lmax; c0 += SYN_LCHUNK) {
         {
             float2 pv[PER];
             double pa;
             fetch(c0, pv, &pa);
-            put(0, pv, pa);
+            put(pv, pa);
         }
         __syncthreads();
         if (c0 + SYN_LCHUNK > wstart) {
@@ -140,32 +146,27 @@ leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float
             for (int kk = 0; kk < SYN_LCHUNK / 4; ++kk) {
                 const int l0 = c0 + 4 * kk;
                 if (l0 > lmax) break;
-                const double a0 = alds[0][4 * kk], a1 = alds[0][4 * kk + 1];
-                const double a2 = alds[0][4 * kk + 2], a3 = alds[0][4 * kk + 3];
-                float ae[SYN_TILES], ao[SYN_TILES];
-#pragma unroll
-                for (int j = 0; j < SYN_TILES; ++j) {
-                    if (l0 <= winj && ls[j] == l0) {
-                        u2[j] = sd[j].x;
-                        u1[j] = sd[j].y;
-                    }
-                    const double xj = x[j];
-                    const double v0 = fma(a0 * xj, u1[j], -u2[j]);
-                    const double v1 = fma(a1 * xj, v0, -u1[j]);
-                    const double v2 = fma(a2 * xj, v1, -v0);
-                    const double v3 = fma(a3 * xj, v2, -v1);
-                    u2[j] = v2;
-                    u1[j] = v3;
-                    ae[j] = (float)(h ? v2 : v0);
-                    ao[j] = (float)(h ? v3 : v1);
+                const double a0 = alds[4 * kk], a1 = alds[4 * kk + 1];
+                const double a2 = alds[4 * kk + 2], a3 = alds[4 * kk + 3];
+                if (l0 <= winj && ls == l0) {
+                    u2 = sd.x;
+                    u1 = sd.y;
                 }
-                const float be = stage[0][4 * kk + 2 * h][r];
-                const float bo = stage[0][4 * kk + 2 * h + 1][r];
-#pragma unroll
-                for (int j = 0; j < SYN_TILES; ++j) {
-                    De[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(ae[j], be, De[j], 0, 0, 0);
-                    Do[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(ao[j], bo, Do[j], 0, 0, 0);
-                }
+                const double v0 = fma(a0 * x, u1, -u2);
+                const double v1 = fma(a1 * x, v0, -u1);
+                const double v2 = fma(a2 * x, v1, -v0);
+                const double v3 = fma(a3 * x, v2, -v1);
+                u2 = v2;
+                u1 = v3;
+                float ae0, ae1, ao0, ao1;
+                swap_halves((float)v0, (float)v2, &ae0, &ae1);
+                swap_halves((float)v1, (float)v3, &ao0, &ao1);
+                const float be = stage[4 * kk + 2 * h][r];
+                const float bo = stage[4 * kk + 2 * h + 1][r];
+                De[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(ae0, be, De[0], 0, 0, 0);
+                Do[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(ao0, bo, Do[0], 0, 0, 0);
+                De[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(ae1, be, De[1], 0, 0, 0);
+                Do[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(ao1, bo, Do[1], 0, 0, 0);
             }
         }
         __syncthreads();
@@ -177,15 +178,15 @@ leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float
     if (b >= nmaps) return;
     float *out = phase + (((int64_t)b * nmb + mb) * nr) * (2 * MB) + (m & 7) * 2 + c;
 #pragma unroll
-    for (int j = 0; j < SYN_TILES; ++j) {
-        const int base = (sb * SYN_WAVES * SYN_TILES + w * SYN_TILES + j) * 32;
+    for (int j = 0; j < 2; ++j) {
+        const int base = pbase + 32 * j;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int row = (q & 3) + 8 * (q >> 2) + 4 * h;
-            const int p = base + row;
-            if (p >= npairs) continue;
+            const int pp = base + row;
+            if (pp >= npairs) continue;
             const float ge = De[j][q], go = Do[j][q];
-            const int rn = p, rs = nr - 1 - p;
+            const int rn = pp, rs = nr - 1 - pp;
             out[(int64_t)rn * (2 * MB)] = ge + go;
             if (rs != rn) out[(int64_t)rs * (2 * MB)] = ge - go;
         }
