@@ -142,7 +142,6 @@ leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float
         }
         __syncthreads();
         if (c0 + SYN_LCHUNK > wstart) {
-#pragma unroll 2
             for (int kk = 0; kk < SYN_LCHUNK / 4; ++kk) {
                 const int l0 = c0 + 4 * kk;
                 if (l0 > lmax) break;

@@ -24,11 +24,13 @@
 namespace hbx {
 
 // ---------------------------------------------------------------------------
-// Twiddles w^J = e^{-2 pi i J / 2^twshift} come from a two-level table in LDS
-// (staged from the plan's global table once per workgroup):
+// Twiddles w^J = e^{-2 pi i J / 2^twshift} come from a two-level FP64 table in
+// LDS (staged from the plan's FP64 global table once per workgroup):
 //   w^J = coarse[J >> f] * fine[J & (2^f - 1)],  f = min(7, twshift),
-// 2^(twshift-f) + 2^f entries (192 for 8192-point rings), so the FFT passes
-// read LDS instead of waiting on global loads between barriers.
+// 2^(twshift-f) + 2^f entries (192 for 8192-point rings).  The product is
+// formed in FP64 and rounded once, so FP32 transforms get correctly rounded
+// twiddles (the FP32 ring stage's error budget is set by them), and the FFT
+// passes read LDS instead of waiting on global loads between barriers.
 // ---------------------------------------------------------------------------
 constexpr int TW_FINE_LOG = 7;
 inline int tw_fine_log(int twshift) { return twshift < TW_FINE_LOG ? twshift : TW_FINE_LOG; }
@@ -37,16 +39,20 @@ inline int tw_lds_entries(int twshift) {
     const int f = tw_fine_log(twshift);
     return (1 << (twshift - f)) + (1 << f);
 }
+// LDS bytes of a ring workgroup: M data elements + the FP64 twiddle table
+template <typename T> inline size_t ring_lds_bytes(int M, int twshift) {
+    return (size_t)M * sizeof(c2<T>) + (size_t)tw_lds_entries(twshift) * sizeof(double2);
+}
 
-template <typename T>
-__device__ void stage_twiddles(c2<T> *tab, const c2<T> *__restrict__ tw, int twshift) {
+__device__ void stage_twiddles(double2 *tab, const double2 *__restrict__ tw, int twshift) {
     const int f = tw_fine_log_d(twshift), nc = 1 << (twshift - f), nf = 1 << f;
     for (int i = threadIdx.x; i < nc + nf; i += blockDim.x) tab[i] = (i < nc) ? tw[i << f] : tw[i - nc];
 }
 
 template <typename T>
-__device__ __forceinline__ c2<T> tw_get(const c2<T> *tab, int J, int f, int nc) {
-    return cmul<T>(tab[J >> f], tab[nc + (J & ((1 << f) - 1))]);
+__device__ __forceinline__ c2<T> tw_get(const double2 *tab, int J, int f, int nc) {
+    const double2 a = tab[J >> f], b = tab[nc + (J & ((1 << f) - 1))];
+    return mk<T>((T)(a.x * b.x - a.y * b.y), (T)(a.x * b.y + a.y * b.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -56,7 +62,7 @@ __device__ __forceinline__ c2<T> tw_get(const c2<T> *tab, int J, int f, int nc) 
 // tab: the LDS twiddle table (stage_twiddles, visible to every thread).
 // ---------------------------------------------------------------------------
 template <typename T, int NB>
-__device__ void lds_fft(c2<T> *buf, int M, int log2M, const c2<T> *tab, int twshift, bool inv) {
+__device__ void lds_fft(c2<T> *buf, int M, int log2M, const double2 *tab, int twshift, bool inv) {
     const int nt = blockDim.x, tid = threadIdx.x;
     const int tf = tw_fine_log_d(twshift), tnc = 1 << (twshift - tf);
     const int Q = M >> 2;
@@ -138,19 +144,19 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
                      int64_t phase_stride, const int32_t *__restrict__ rings,
                      const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
                      const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
-                     const double *__restrict__ scale, const c2<T> *__restrict__ tw, int twshift,
+                     const double *__restrict__ scale, const double2 *__restrict__ tw, int twshift,
                      const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs, int M, int log2M,
                      int mmax, int nr) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
-    c2<T> *tab = buf + M;
+    double2 *tab = reinterpret_cast<double2 *>(buf + M);
     const int r = rings[blockIdx.x];
     const int b = blockIdx.y;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = nphi[r], h = n >> 1;
     const bool pow2 = (h == M);
     const TIn *f = maps + (int64_t)b * map_stride + ring_start[r];
-    stage_twiddles<T>(tab, tw, twshift);
+    stage_twiddles(tab, tw, twshift);
 
     for (int j = tid; j < M; j += nt) {
         c2<T> z = mk<T>(0, 0);
@@ -210,19 +216,19 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
                       int64_t map_stride, const int32_t *__restrict__ rings,
                       const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
                       const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
-                      const double *__restrict__ scale, const c2<T> *__restrict__ tw, int twshift,
+                      const double *__restrict__ scale, const double2 *__restrict__ tw, int twshift,
                       const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs, int M, int log2M,
                       int mmax, int nr) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
-    c2<T> *tab = buf + M;
+    double2 *tab = reinterpret_cast<double2 *>(buf + M);
     const int r = rings[blockIdx.x];
     const int b = blockIdx.y;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = nphi[r], h = n >> 1;
     const bool pow2 = (h == M);
     const int top = min(mmax, mcut[r] - 1);
-    stage_twiddles<T>(tab, tw, twshift);
+    stage_twiddles(tab, tw, twshift);
     const double ph0 = phi0 ? phi0[r] : 0.0;
     const c2<T> *G = phase + (int64_t)b * phase_stride;
 
@@ -390,12 +396,12 @@ __global__ void twiddle_kernel(c2<T> *tw, int twn) {
 template <typename T, int NB>
 __global__ void __launch_bounds__(1024)
 chirp_spectrum_kernel(const int32_t *__restrict__ hs, const int64_t *__restrict__ ofs, c2<T> *V,
-                      const c2<T> *__restrict__ tw, int twshift, int M, int log2M) {
+                      const double2 *__restrict__ tw, int twshift, int M, int log2M) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
-    c2<T> *tab = buf + M;
+    double2 *tab = reinterpret_cast<double2 *>(buf + M);
     const int h = hs[blockIdx.x];
-    stage_twiddles<T>(tab, tw, twshift);
+    stage_twiddles(tab, tw, twshift);
     for (int j = threadIdx.x; j < M; j += blockDim.x) {
         c2<T> v = mk<T>(0, 0);
         if (j < h) v = chirp<T>(j, h);
@@ -440,7 +446,7 @@ static int allow_lds(K kernel, size_t bytes) {
 template <typename T>
 static int build_chirp_spectra(Plan *p, const std::vector<int32_t> &hs_host,
                                const std::vector<int64_t> &ofs_host, int M, c2<T> *V,
-                               const c2<T> *tw) {
+                               const double2 *tw) {
     if (hs_host.empty()) return 0;
     int32_t *d_hs = nullptr;
     int64_t *d_ofs = nullptr;
@@ -449,7 +455,7 @@ static int build_chirp_spectra(Plan *p, const std::vector<int32_t> &hs_host,
     HBX_HIP(hipMemcpy(d_hs, hs_host.data(), hs_host.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HBX_HIP(hipMemcpy(d_ofs, ofs_host.data(), ofs_host.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     const int nt = class_threads(M), nb = class_nb(M), lg = ilog2(M);
-    const size_t lds = (size_t)(M + tw_lds_entries(p->twshift)) * sizeof(c2<T>);
+    const size_t lds = ring_lds_bytes<T>(M, p->twshift);
     int rc = 0;
     HBX_NB_SWITCH(nb, {
         auto k = chirp_spectrum_kernel<T, NBV>;
@@ -504,7 +510,7 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi) {
     int maxlds = 0;
     HBX_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeSharedMemPerBlockOptin, dev));
     if (maxlds <= 0) HBX_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    if ((size_t)(maxM + tw_lds_entries(ilog2(maxM))) * csz > (size_t)maxlds)
+    if ((size_t)maxM * csz + (size_t)tw_lds_entries(ilog2(maxM)) * sizeof(double2) > (size_t)maxlds)
         return set_error(HBX_EUNSUPPORTED, "ring FFT of size " + std::to_string(maxM) +
                                                " does not fit in LDS (" + std::to_string(maxlds) +
                                                " bytes) at this precision");
@@ -522,21 +528,19 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi) {
     int rc;
     if ((rc = dalloc((void **)&p->d_vofs, nr * sizeof(int64_t)))) return rc;
     HBX_HIP(hipMemcpy(p->d_vofs, vofs.data(), nr * sizeof(int64_t), hipMemcpyHostToDevice));
+    // FP64 twiddles for every plan (the LDS tables are FP64 at both precisions)
+    if ((rc = dalloc((void **)&p->d_tw64, maxM * sizeof(double2)))) return rc;
+    hipLaunchKernelGGL(twiddle_kernel<double>, dim3((maxM + 255) / 256), dim3(256), 0, 0, p->d_tw64, maxM);
+    HBX_HIP(hipGetLastError());
     if (p->precision == HBX_F64) {
-        if ((rc = dalloc((void **)&p->d_tw64, maxM * sizeof(double2)))) return rc;
-        hipLaunchKernelGGL(twiddle_kernel<double>, dim3((maxM + 255) / 256), dim3(256), 0, 0, p->d_tw64, maxM);
-        HBX_HIP(hipGetLastError());
         if ((rc = dalloc((void **)&p->d_V64, vtotal * sizeof(double2)))) return rc;
         for (auto &kv : hs_by_M)
             if ((rc = build_chirp_spectra<double>(p, kv.second, ofs_by_M[kv.first], kv.first, p->d_V64, p->d_tw64)))
                 return rc;
     } else {
-        if ((rc = dalloc((void **)&p->d_tw32, maxM * sizeof(float2)))) return rc;
-        hipLaunchKernelGGL(twiddle_kernel<float>, dim3((maxM + 255) / 256), dim3(256), 0, 0, p->d_tw32, maxM);
-        HBX_HIP(hipGetLastError());
         if ((rc = dalloc((void **)&p->d_V32, vtotal * sizeof(float2)))) return rc;
         for (auto &kv : hs_by_M)
-            if ((rc = build_chirp_spectra<float>(p, kv.second, ofs_by_M[kv.first], kv.first, p->d_V32, p->d_tw32)))
+            if ((rc = build_chirp_spectra<float>(p, kv.second, ofs_by_M[kv.first], kv.first, p->d_V32, p->d_tw64)))
                 return rc;
     }
     // size classes
@@ -549,8 +553,8 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi) {
         c.nrings = (int)kv.second.size();
         if ((rc = dalloc((void **)&c.d_rings, kv.second.size() * sizeof(int32_t)))) return rc;
         HBX_HIP(hipMemcpy(c.d_rings, kv.second.data(), kv.second.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        c.lds_bytes64 = (size_t)(c.M + tw_lds_entries(p->twshift)) * sizeof(double2);
-        c.lds_bytes32 = (size_t)(c.M + tw_lds_entries(p->twshift)) * sizeof(float2);
+        c.lds_bytes64 = ring_lds_bytes<double>(c.M, p->twshift);
+        c.lds_bytes32 = ring_lds_bytes<float>(c.M, p->twshift);
         p->classes.push_back(c);
     }
     // raise the dynamic-LDS limit of every kernel instance we may launch
@@ -577,12 +581,12 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi) {
 
 template <typename T, typename TIn>
 static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t map_stride,
-                           c2<T> *phase, const double *scale, const c2<T> *tw, const c2<T> *V,
+                           c2<T> *phase, const double *scale, const double2 *tw, const c2<T> *V,
                            hipStream_t s) {
     const int64_t pstride = phase_elems_per_map(p);
     timer_mark(HBX_TIMER_RING_ANALYSIS, true, s);
     for (const auto &c : p->classes) {
-        const size_t lds = (size_t)(c.M + tw_lds_entries(p->twshift)) * sizeof(c2<T>);
+        const size_t lds = ring_lds_bytes<T>(c.M, p->twshift);
         dim3 grid((unsigned)c.nrings, (unsigned)nmaps);
         HBX_NB_SWITCH(c.nb, {
             hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, NBV>), grid, dim3(c.threads), lds, s, maps,
@@ -599,12 +603,12 @@ static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t ma
 
 template <typename T, typename TOut>
 static int ring_synthesis_t(const Plan *p, int nmaps, const c2<T> *phase, TOut *maps,
-                            int64_t map_stride, const double *scale, const c2<T> *tw,
+                            int64_t map_stride, const double *scale, const double2 *tw,
                             const c2<T> *V, hipStream_t s) {
     const int64_t pstride = phase_elems_per_map(p);
     timer_mark(HBX_TIMER_RING_SYNTHESIS, true, s);
     for (const auto &c : p->classes) {
-        const size_t lds = (size_t)(c.M + tw_lds_entries(p->twshift)) * sizeof(c2<T>);
+        const size_t lds = ring_lds_bytes<T>(c.M, p->twshift);
         dim3 grid((unsigned)c.nrings, (unsigned)nmaps);
         HBX_NB_SWITCH(c.nb, {
             hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, NBV>), grid, dim3(c.threads), lds, s, phase,
@@ -629,8 +633,8 @@ int launch_ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dty
     }
     auto *ph = static_cast<float2 *>(phase);
     if (map_dtype == HBX_DT_F64)
-        return ring_analysis_t<float, double>(p, nmaps, (const double *)maps, map_stride, ph, scale, p->d_tw32, p->d_V32, s);
-    return ring_analysis_t<float, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_tw32, p->d_V32, s);
+        return ring_analysis_t<float, double>(p, nmaps, (const double *)maps, map_stride, ph, scale, p->d_tw64, p->d_V32, s);
+    return ring_analysis_t<float, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_tw64, p->d_V32, s);
 }
 
 int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *maps, int map_dtype,
@@ -643,8 +647,8 @@ int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *map
     }
     auto *ph = static_cast<const float2 *>(phase);
     if (map_dtype == HBX_DT_F64)
-        return ring_synthesis_t<float, double>(p, nmaps, ph, (double *)maps, map_stride, scale, p->d_tw32, p->d_V32, s);
-    return ring_synthesis_t<float, float>(p, nmaps, ph, (float *)maps, map_stride, scale, p->d_tw32, p->d_V32, s);
+        return ring_synthesis_t<float, double>(p, nmaps, ph, (double *)maps, map_stride, scale, p->d_tw64, p->d_V32, s);
+    return ring_synthesis_t<float, float>(p, nmaps, ph, (float *)maps, map_stride, scale, p->d_tw64, p->d_V32, s);
 }
 
 }  // namespace hbx
