@@ -63,7 +63,7 @@ def candidates(plan, nmaps):
         return [{"valu_slots": s} for s in slots if s <= 8]
     out = []
     if nmaps >= 2:
-        waves = [w for w in (4, 8, 16) if w == 4 or (w // 2) * 64 < npairs]
+        waves = [w for w in (4, 8) if w == 4 or (w // 2) * 64 < npairs]
         out += [{"mfma_waves": w, "mfma_min_maps": 2} for w in waves]
     # the VALU kernels (also for batches: mfma_min_maps above the batch size)
     out += [{"valu_slots": s, "mfma_min_maps": nmaps + 1} for s in slots]

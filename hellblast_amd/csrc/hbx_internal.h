@@ -32,7 +32,7 @@ void timer_mark(int kernel, bool begin, hipStream_t s);
 // ---------------------------------------------------------------------------
 constexpr int MB = 8;          // m values per m-block of the phase buffer
 constexpr int CHUNK = 8;       // l steps per recursion chunk
-constexpr int TAILPAD = 64;    // zero coefficients after the last recursion row
+constexpr int TAILPAD = 128;   // zero coefficients after the last recursion row
 // FP32 plans analyse m < M_LO in FP64: the ring coefficients of low m carry
 // the large-scale power, and rounding them to FP32 costs ~1e-5 of sqrt(C_l)
 // at high l for red spectra (measured at nside 2048, lmax 4095, C_l =

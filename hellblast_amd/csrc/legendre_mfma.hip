@@ -214,14 +214,17 @@ int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t
 // maps, par = parity of l-m (S_even = F_n + F_s, S_odd = F_n - F_s).  With
 // v_mfma_f32_16x16x4f32 (D[16x16] += A[16x4] B[4x16]) the ring pairs are the
 // contraction index:
-//   A[row = n][k] = S_par[ring k][n]             (held in registers all kernel)
-//   B[k][col]     = lambda_{c0+2col+par}(ring k) (FP64 recursion -> FP32 ->
-//                                                 wave-private LDS transpose)
-// A workgroup is W waves x 64 ring pairs of one m (W = 4, 8 or 16, a plan
-// tile).  After its MFMAs each wave parks its D tiles in its own (consumed)
-// lambda tile; the threads then sum the chunk's 32 l x 32 n outputs over the
-// W waves in a fixed order and write them.  (LDS float atomics for this sum
-// measured 4x slower.)
+//   A[row = n][k]  = S_par[pair 4 s + k][n]       (registers, whole kernel)
+//   B[k][col]      = lambda_{c0+2col+par}(pair)   (shared LDS tile per chunk)
+// A workgroup of 4 Q waves covers P = 256 Q ring pairs of one m.  Per 32-l
+// chunk every wave first runs the FP64 recursion of 64 of the P pairs into
+// the shared lambda tile; after one barrier each wave contracts one output
+// tile -- (n-tile t, parity par) -- over its K-part of 256 pairs (64 MFMAs),
+// so no wave waits on partial sums of another except the Q = 2 pair of
+// K-halves, which add through a small LDS exchange.
+// Lambda tile: [par][col][k][s] with pair q = 4 s + k, rows padded so that a
+// recursion step's 64 lane writes and a quarter wave's 16-byte B reads hit
+// disjoint banks.
 // Lane layout of the 16x16x4 f32 instruction (checked on gfx950):
 //   A: lane = row + 16 k   B: lane = col + 16 k   D: lane = col + 16 (row/4), reg = row%4
 // Reference counterpart: the per-(m, l) ring sum of sht_raw.py:53 /
@@ -230,182 +233,177 @@ int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int ANA_L = 32;                 // l per chunk: 16 even + 16 odd columns
-// A wave's lambda tile holds, per parity and column (16 l of that parity),
-// the 64 ring pairs q at (q%4)*16 + q/4: one recursion step writes 64
-// consecutive floats, and the lane that feeds B[k = q%4][col] reads its 16
-// values (q/4 = s = 0..15) as four 16-byte loads.  Rows are padded to 68
-// floats so the 16 lanes of a quarter wave hit disjoint banks.
-constexpr int ANA_COL = 68;
-constexpr int ANA_PAR = 16 * ANA_COL;
-constexpr int ANA_LAM = 2 * ANA_PAR;      // floats of one wave's lambda tile
-constexpr size_t ana_lds(int waves) { return (size_t)(waves * ANA_LAM + 4 * ANA_L + 4) * sizeof(float); }
+template <int Q> struct AnaGeom {
+    static constexpr int W = 4 * Q;                    // waves per workgroup
+    static constexpr int P = 256 * Q;                  // ring pairs per workgroup
+    static constexpr int RUN = P / 4 + 16;             // stride between the k runs of a column
+    static constexpr int COL = P + 68;                 // column stride (= 4 mod 64 banks)
+    static constexpr int PAR = 16 * COL;
+    static constexpr int TILE = 2 * PAR;               // floats
+    static constexpr int XCH = 4 * 256;                // Q = 2 exchange: 4 tiles x 64 lanes x 4
+    static constexpr size_t LDS = (size_t)(TILE + XCH) * sizeof(float) + 2 * ANA_L * sizeof(double) + 16;
+};
 
-template <int ANA_WAVES>
-__global__ void __launch_bounds__(ANA_WAVES * 64)
+template <int Q>
+__global__ void __launch_bounds__(AnaGeom<Q>::W * 64)
 leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, float2 *__restrict__ alm,
                          int64_t astride, float2 *__restrict__ partial, const double *__restrict__ px,
                          const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
                          const double *__restrict__ sl, const int32_t *__restrict__ lseed,
                          const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
                          int64_t nlm, int nmaps, int adjoint, int mbeg) {
+    using G = AnaGeom<Q>;
     extern __shared__ __align__(16) float smem[];
+    float *lam = smem;                                               // [TILE]
+    float *xch = smem + G::TILE;                                     // [XCH]
+    double *alds = reinterpret_cast<double *>(smem + G::TILE + G::XCH);   // [2][ANA_L]
+    int *starts = reinterpret_cast<int *>(alds + 2 * ANA_L);         // [0]: workgroup, [1 + kh]: K-part
+
     int mb, item;
     xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
     const int m = mb * MB + item / ngroups;
     const int grp = item % ngroups;
     if (m > mmax || m < mbeg) return;
-    const int g = blockIdx.y;
+    const int g = blockIdx.y;                         // group of 16 maps
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    float *lam = smem + w * ANA_LAM;
-    double *alds = reinterpret_cast<double *>(smem + ANA_WAVES * ANA_LAM);   // [2][32]
-    int &wg_start = *reinterpret_cast<int *>(smem + ANA_WAVES * ANA_LAM + 4 * ANA_L);
+    const int t = w & 1, par = (w >> 1) & 1, kh = w >> 2;    // output tile, parity, K-part
+    const int P0 = grp * G::P;                        // first ring pair of the workgroup
 
-    const int pw = grp * ANA_WAVES * 64 + w * 64;       // first ring pair of this wave
-    const int p = pw + lane;
+    // recursion: pair P0 + 64 w + lane
+    const int pr = P0 + 64 * w + lane;
     double x = 0.0, u1 = 0.0, u2 = 0.0;
     double2 sd = make_double2(0.0, 0.0);
     int ls = NEVER_L;
-    if (p < npairs) {
-        x = px[p];
-        ls = lseed[(int64_t)m * npairs + p];
-        if (ls <= lmax) sd = seed[(int64_t)m * npairs + p];
+    if (pr < npairs) {
+        x = px[pr];
+        ls = lseed[(int64_t)m * npairs + pr];
+        if (ls <= lmax) sd = seed[(int64_t)m * npairs + pr];
     }
     const int wstart = wmin64(ls);
     const int winj = wmax64(ls <= lmax ? ls : -1);
 
-    // A operands: S_par[ring pw + 4 s + lane/16][n = 16 tile + lane%16];
-    // ring pair q couples ring q (north) and ring nr-1-q (south)
-    float Sv[16][2][2];
+    // A operands: S_par[pair P0 + 256 kh + 4 s + k][n = 16 t + row], k = lane/16, row = lane%16
+    float Sv[64];
     {
-        const int c = lane & 1;
-        const float2 *ph = phase + (int64_t)mb * nr * MB + (m & 7);
+        const int row = lane & 15, k = lane >> 4;
+        const int n = 16 * t + row;
+        const int b = g * 16 + (n >> 1), c = n & 1;
+        const float *ph = reinterpret_cast<const float *>(phase + (int64_t)min(b, nmaps - 1) * pstride +
+                                                          (int64_t)mb * nr * MB + (m & 7)) + c;
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const int pq = pw + 4 * s + (lane >> 4);
-            const int pp = min(pq, npairs - 1);
-            const bool self = (nr - 1 - pp) == pp;
-#pragma unroll
-            for (int tile = 0; tile < 2; ++tile) {
-                const int bq = g * 16 + tile * 8 + ((lane & 15) >> 1);
-                const float2 *pb = ph + (int64_t)min(bq, nmaps - 1) * pstride;
-                const float2 a = pb[(int64_t)pp * MB];
-                const float2 z = self ? make_float2(0.f, 0.f) : pb[(int64_t)(nr - 1 - pp) * MB];
-                const bool ok = pq < npairs && bq < nmaps;
-                Sv[s][tile][0] = ok ? (c ? a.y + z.y : a.x + z.x) : 0.0f;
-                Sv[s][tile][1] = ok ? (c ? a.y - z.y : a.x - z.x) : 0.0f;
-            }
+        for (int s = 0; s < 64; ++s) {
+            const int q = P0 + 256 * kh + 4 * s + k;
+            const int qq = min(q, npairs - 1);
+            const float a = ph[(int64_t)qq * (2 * MB)];
+            const float z = (nr - 1 - qq != qq) ? ph[(int64_t)(nr - 1 - qq) * (2 * MB)] : 0.0f;
+            const bool ok = q < npairs && b < nmaps;
+            Sv[s] = ok ? (par ? a - z : a + z) : 0.0f;
         }
     }
-    if (tid == 0) wg_start = NEVER_L;
+    if (tid < 1 + Q) starts[tid] = NEVER_L;
     __syncthreads();
-    if (lane == 0) atomicMin(&wg_start, wstart);
+    if (lane == 0) {
+        atomicMin(&starts[0], wstart);
+        atomicMin(&starts[1 + (w >> 2)], wstart);     // waves 4j..4j+3 compute K-part j's pairs
+    }
     __syncthreads();
-    const int lstart = wg_start;
+    const int lstart = starts[0];
+    const int kstart = starts[1 + kh];
 
     const bool direct = (ngroups == 1);
     const int64_t i0 = alm_index(m, m, lmax) - m;       // + l
     const int64_t ro = rofs[m];
     const double *al = alpha + ro - m;                  // zero-padded past lmax
     const double *sv = sl + ro - m;
-    auto store = [&](int l, int n, float v) {
-        const int b = g * 16 + (n >> 1), c = n & 1;
+    // out(l, b): 2 maps x (re, im) per lane; b' = map, v = value pair
+    auto store = [&](int l, int b, float2 v) {
         if (b >= nmaps) return;
         if (direct) {
-            double d = (double)v * sv[l];
+            double f = sv[l];
+            double re = (double)v.x * f, im = (double)v.y * f;
             if (adjoint) {
-                if (m > 0) d *= 2.0;
-                else if (c == 1) d = 0.0;
+                if (m > 0) { re *= 2.0; im *= 2.0; }
+                else im = 0.0;
             }
-            reinterpret_cast<float *>(alm + (int64_t)b * astride + i0 + l)[c] = (float)d;
+            alm[(int64_t)b * astride + i0 + l] = make_float2((float)re, (float)im);
         } else {
-            reinterpret_cast<float *>(partial + ((int64_t)b * ngroups + grp) * nlm + i0 + l)[c] = v;
+            partial[((int64_t)b * ngroups + grp) * nlm + i0 + l] = v;
         }
     };
     // coefficients below every ring pair's switch-on are zero
     const int lz = lstart > lmax ? lmax + 1 : lstart;
-    for (int t = tid; t < (lz - m) * 32; t += ANA_WAVES * 64) store(m + (t >> 5), t & 31, 0.0f);
+    for (int e = tid; e < (lz - m) * 16; e += G::W * 64) store(m + (e >> 4), g * 16 + (e & 15), make_float2(0.f, 0.f));
     if (lstart > lmax) return;
 
-
     if (tid < ANA_L) alds[tid] = al[lstart + tid];
+    // alpha of the chunk after next is loaded a whole chunk before it is
+    // staged, so the staging never waits on memory right before a barrier
+    double anext = (tid < ANA_L) ? al[lstart + ANA_L + tid] : 0.0;
     __syncthreads();
+    // this lane's writes: pair q = 64 w + lane -> [.][.][k = q%4][s = q/4]
+    const int qw = 64 * w + lane;
+    float *lw = lam + (qw & 3) * G::RUN + (qw >> 2);
+    // this lane's B reads: col = lane%16, k = lane/16, s = 64 kh ..
+    const float *lr = lam + par * G::PAR + (lane & 15) * G::COL + (lane >> 4) * G::RUN + 64 * kh;
     int buf = 0;
     for (int c0 = lstart; c0 <= lmax; c0 += ANA_L, buf ^= 1) {
         const double *ab = alds + buf * ANA_L;
-        const double anext = (tid < ANA_L) ? al[c0 + ANA_L + tid] : 0.0;   // in flight this chunk
-        floatx4 D[2][2];
+        const double anext2 = (tid < ANA_L) ? al[c0 + 2 * ANA_L + tid] : 0.0;   // in flight a chunk
+        // 1. recursion of this lane's pair over l = c0 .. c0+31 into the tile
 #pragma unroll
-        for (int tile = 0; tile < 2; ++tile)
-#pragma unroll
-            for (int par = 0; par < 2; ++par) D[tile][par] = floatx4{0.f, 0.f, 0.f, 0.f};
-        if (c0 + ANA_L > wstart) {
-            // lambda for this lane's ring over l = c0 .. c0+31 into the wave tile
-#pragma unroll
-            for (int t = 0; t < ANA_L; ++t) {
-                if ((t & 7) == 0 && c0 <= winj && ls == c0 + t) {
-                    u2 = sd.x;
-                    u1 = sd.y;
-                }
-                const double u = fma(ab[t] * x, u1, -u2);
-                u2 = u1;
-                u1 = u;
-                lam[(t & 1) * ANA_PAR + (t >> 1) * ANA_COL + (lane & 3) * 16 + (lane >> 2)] = (float)u;
+        for (int tt = 0; tt < ANA_L; ++tt) {
+            if ((tt & 7) == 0 && c0 <= winj && ls == c0 + tt) {
+                u2 = sd.x;
+                u1 = sd.y;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-            for (int par = 0; par < 2; ++par) {
-                // B[k = lane/16][col = lane%16] for s = 0..15: ring pair 4 s + k
-                const float4 *bp =
-                    reinterpret_cast<const float4 *>(lam + par * ANA_PAR + (lane & 15) * ANA_COL + (lane >> 4) * 16);
-                float bv[16];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const float4 q4 = bp[j];
-                    bv[4 * j] = q4.x;
-                    bv[4 * j + 1] = q4.y;
-                    bv[4 * j + 2] = q4.z;
-                    bv[4 * j + 3] = q4.w;
-                }
-#pragma unroll
-                for (int s = 0; s < 16; ++s)
-#pragma unroll
-                    for (int tile = 0; tile < 2; ++tile)
-                        D[tile][par] = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[s][tile][par], bv[s], D[tile][par], 0, 0, 0);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const double u = fma(ab[tt] * x, u1, -u2);
+            u2 = u1;
+            u1 = u;
+            lw[(tt & 1) * G::PAR + (tt >> 1) * G::COL] = (float)u;
         }
-        // park D in this wave's (now consumed) lambda tile:
-        // [(tile*2 + par)*4 + reg][lane], n = 16 tile + 4 (lane/16) + reg, l = c0 + 2 (lane%16) + par
-#pragma unroll
-        for (int tile = 0; tile < 2; ++tile)
-#pragma unroll
-            for (int par = 0; par < 2; ++par)
-#pragma unroll
-                for (int reg = 0; reg < 4; ++reg) lam[((tile * 2 + par) * 4 + reg) * 64 + lane] = D[tile][par][reg];
         if (tid < ANA_L) alds[(buf ^ 1) * ANA_L + tid] = anext;
+        anext = anext2;
         __syncthreads();
-        // output o = (n = o/32, l = c0 + o%32), parked by each wave at ridx
+        // 2. this wave's output tile over its 256 pairs (skipped before any
+        // switches on); four chains of 64 pairs keep the FP32 sums short
+        floatx4 D0 = floatx4{0.f, 0.f, 0.f, 0.f}, D1 = D0, D2 = D0, D3 = D0;
+        if (c0 + ANA_L > kstart) {
 #pragma unroll
-        for (int o = tid; o < 32 * ANA_L; o += ANA_WAVES * 64) {
-            const int on = o >> 5, olo = o & 31;
-            const int ridx = (((on >> 4) * 2 + (olo & 1)) * 4 + (on & 3)) * 64 + (olo >> 1) + 16 * ((on & 15) >> 2);
-            float v = 0.0f;
+            for (int j = 0; j < 16; ++j) {
+                const float4 bq = *reinterpret_cast<const float4 *>(lr + 4 * j);
+                floatx4 &Dj = (j < 4) ? D0 : (j < 8) ? D1 : (j < 12) ? D2 : D3;
+                Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j], bq.x, Dj, 0, 0, 0);
+                Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j + 1], bq.y, Dj, 0, 0, 0);
+                Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j + 2], bq.z, Dj, 0, 0, 0);
+                Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j + 3], bq.w, Dj, 0, 0, 0);
+            }
+        }
+        floatx4 D = (D0 + D1) + (D2 + D3);
+        // 3. K-parts: part 1 hands its tile to part 0 through LDS
+        if (Q == 2 && kh == 1) {
 #pragma unroll
-            for (int q = 0; q < ANA_WAVES; ++q) v += smem[q * ANA_LAM + ridx];
-            const int l = c0 + olo;
-            if (l <= lmax) store(l, on, v);
+            for (int r = 0; r < 4; ++r) xch[((t * 2 + par) * 4 + r) * 64 + lane] = D[r];
         }
         __syncthreads();
+        if (kh == 0) {
+            if (Q == 2) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) D[r] += xch[((t * 2 + par) * 4 + r) * 64 + lane];
+            }
+            // D[row][col]: lane = col + 16 (row/4), reg = row%4; row = n - 16 t, col -> l
+            const int l = c0 + 2 * (lane & 15) + par;
+            if (l <= lmax) {
+                const int b0 = g * 16 + 8 * t + 2 * (lane >> 4);
+                store(l, b0, make_float2(D[0], D[1]));
+                store(l, b0 + 1, make_float2(D[2], D[3]));
+            }
+        }
     }
 }
 
-// waves per workgroup: the plan's tile, else 8 (4 for small geometries, whose
-// ring pairs would leave most of a larger workgroup idle)
+// waves per workgroup: the plan's tile (4 or 8), else 8 (4 for small
+// geometries, whose ring pairs would leave a larger workgroup idle)
 int mfma_analysis_waves(const Plan *p) {
     if (p->tile_mfma_waves > 0) return p->tile_mfma_waves;
     return p->npairs <= 256 ? 4 : 8;
@@ -426,16 +424,15 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
     dim3 grid(nmb8 * MB * G, (nmaps + 15) / 16);
     const int64_t pstride = phase_elems_per_map(p);
     timer_mark(HBX_TIMER_LEGENDRE_ANALYSIS, true, s);
-#define HBX_AM(WV)                                                                                            \
-    hipLaunchKernelGGL(leg_analysis_mfma_kernel<WV>, grid, dim3(WV * 64), ana_lds(WV), s, (const float2 *)phase, \
-                       pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px, p->d_rofs, p->d_alpha,     \
-                       p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, G, p->nlm, nmaps, \
-                       adjoint, mbeg)
+#define HBX_AM(QV)                                                                                              \
+    hipLaunchKernelGGL(leg_analysis_mfma_kernel<QV>, grid, dim3(AnaGeom<QV>::W * 64), AnaGeom<QV>::LDS, s,     \
+                       (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,       \
+                       p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,     \
+                       p->mmax, G, p->nlm, nmaps, adjoint, mbeg)
     switch (W) {
-    case 4: HBX_AM(4); break;
-    case 8: HBX_AM(8); break;
-    case 16: HBX_AM(16); break;
-    default: return set_error(HBX_EUNSUPPORTED, "matrix-core analysis waves must be 4, 8 or 16");
+    case 4: HBX_AM(1); break;
+    case 8: HBX_AM(2); break;
+    default: return set_error(HBX_EUNSUPPORTED, "matrix-core analysis waves must be 4 or 8");
     }
 #undef HBX_AM
     timer_mark(HBX_TIMER_LEGENDRE_ANALYSIS, false, s);
@@ -447,14 +444,12 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
 
 // raise the dynamic-LDS limit of the analysis kernels (called at plan creation)
 int prepare_mfma_kernels() {
-    const void *k[3] = {reinterpret_cast<const void *>(leg_analysis_mfma_kernel<4>),
-                        reinterpret_cast<const void *>(leg_analysis_mfma_kernel<8>),
-                        reinterpret_cast<const void *>(leg_analysis_mfma_kernel<16>)};
-    const int wv[3] = {4, 8, 16};
-    for (int i = 0; i < 3; ++i) {
-        hipError_t e = hipFuncSetAttribute(k[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)ana_lds(wv[i]));
-        if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(leg_analysis_mfma_kernel)");
-    }
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_mfma_kernel<1>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)AnaGeom<1>::LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_mfma_kernel<2>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)AnaGeom<2>::LDS);
+    if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(leg_analysis_mfma_kernel)");
     return 0;
 }
 

@@ -114,7 +114,7 @@ int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
  *   valu_slots     VALU Legendre analysis: ring-pair slots of 64 pairs per lane
  *                  (power of two <= 16; FP64 kernels clamp to 8)
  *   mfma_waves     matrix-core analysis: waves (64 ring pairs each) per
- *                  workgroup, 4, 8 or 16
+ *                  workgroup, 4 or 8
  *   mfma_min_maps  FP32 batches of at least this many maps use the
  *                  matrix-core Legendre kernels (default 2)
  *   spin_slots     spin Legendre analysis: ring-pair slots per lane, 1, 2, 4 */
