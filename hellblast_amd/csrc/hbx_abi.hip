@@ -91,7 +91,7 @@ static AnalysisWs analysis_ws(const Plan *p, int nmaps) {
     w.partial = 0;
     uint64_t off = align256(g > 1 ? (uint64_t)nmaps * g * p->nlm * cs : 0);
     if (p->precision == HBX_F32) {
-        const int g64 = band_groups(p);
+        const int g64 = std::max(band_groups(p), band_mfma_groups(p));
         w.band_alm = off;
         off += align256((uint64_t)nmaps * band_nlm(p) * 16);
         w.band_partial = off;

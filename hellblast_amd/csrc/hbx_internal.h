@@ -139,6 +139,10 @@ int launch_leg_analysis_band(const Plan *p, int nmaps, const void *band_phase, v
 int launch_ring_band(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride,
                      void *band_phase, const double *scale, hipStream_t s);
 int mfma_analysis_groups(const Plan *p);
+int band_mfma_groups(const Plan *p);
+// FP64 band of FP32 batches on the FP64 matrix cores: band phase -> band alm (FP64, scaled)
+int launch_band_analysis_mfma(const Plan *p, int nmaps, const void *band, void *band_alm, void *band_partial,
+                              int adjoint, hipStream_t s);
 int mfma_analysis_waves(const Plan *p);
 int prepare_mfma_kernels();
 // by default, batches at least this large use the matrix-core Legendre kernels (FP32 plans)

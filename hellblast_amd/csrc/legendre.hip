@@ -593,7 +593,11 @@ int launch_leg_analysis_band(const Plan *p, int nmaps, const void *band_phase, v
                              void *band_alm, void *band_partial, int adjoint, hipStream_t s) {
     const int mend = std::min(M_LO - 1, p->mmax);
     const int64_t nb = band_nlm(p);
-    int rc = leg_analysis_t<double>(p, nmaps, (const double2 *)band_phase, band_phase_elems(p), (double2 *)band_alm,
+    int rc;
+    if (nmaps >= mfma_min_maps(p))
+        rc = launch_band_analysis_mfma(p, nmaps, band_phase, band_alm, band_partial, adjoint, s);
+    else
+        rc = leg_analysis_t<double>(p, nmaps, (const double2 *)band_phase, band_phase_elems(p), (double2 *)band_alm,
                                     nb, (double2 *)band_partial, nb, 0, mend, adjoint, s, 0, band_slots(p));
     if (rc) return rc;
     hipLaunchKernelGGL(band_to_f32_kernel, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024), nmaps),

@@ -402,6 +402,185 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 band of FP32 batches (m < M_LO) on the FP64 matrix cores,
+// v_mfma_f64_16x16x4f64 (D[16x16] += A[16x4] B[4x16], FP64 throughout), with
+// the same decomposition as the FP32 kernel: a workgroup of 8 waves covers
+// 256 ring pairs of one m; each wave runs the recursion of 32 pairs into a
+// shared FP64 lambda tile and contracts one output tile (n-tile, parity) over
+// one K-half of 128 pairs; the K-halves add through LDS.
+// Lane layout of the 16x16x4 f64 instruction (checked on gfx950):
+//   A: lane = row + 16 k   B: lane = col + 16 k   D: lane = col + 16 (row%4), reg = row/4
+// Output: band alm (FP64, scaled) for one group, else FP64 partial sums.
+// ---------------------------------------------------------------------------
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+struct BandGeom {
+    static constexpr int W = 8, P = 256;
+    static constexpr int RUN = P / 4 + 8;          // = 8 mod 32 doubles: the 4 write runs hit disjoint banks
+    static constexpr int COL = 4 * RUN + 2;        // = 2 mod 32 doubles: quarter-wave 16-byte reads disjoint
+    static constexpr int PAR = 16 * COL;
+    static constexpr int TILE = 2 * PAR;           // doubles
+    static constexpr int XCH = 4 * 256;            // doubles
+    static constexpr size_t LDS = (size_t)(TILE + XCH + 2 * ANA_L) * sizeof(double) + 16;
+};
+
+__global__ void __launch_bounds__(BandGeom::W * 64)
+band_analysis_mfma_kernel(const double2 *__restrict__ band, int64_t bstride, double2 *__restrict__ out,
+                          double2 *__restrict__ partial, const double *__restrict__ px,
+                          const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
+                          const double *__restrict__ sl, const int32_t *__restrict__ lseed,
+                          const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mtop, int ngroups,
+                          int64_t nlmb, int nmaps, int adjoint) {
+    using G = BandGeom;
+    extern __shared__ __align__(16) double dsm[];
+    double *lam = dsm;
+    double *xch = dsm + G::TILE;
+    double *alds = dsm + G::TILE + G::XCH;                        // [2][ANA_L]
+    int *starts = reinterpret_cast<int *>(alds + 2 * ANA_L);      // [0]: workgroup, [1 + kh]
+
+    const int m = blockIdx.x / ngroups, grp = blockIdx.x % ngroups;
+    if (m > mtop) return;
+    const int g = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int t = w & 1, par = (w >> 1) & 1, kh = w >> 2;
+    const int P0 = grp * G::P;
+
+    // recursion: pair P0 + 32 w + lane % 32 (the upper half-wave repeats it)
+    const int qw = 32 * w + (lane & 31);
+    const int pr = P0 + qw;
+    double x = 0.0, u1 = 0.0, u2 = 0.0;
+    double2 sd = make_double2(0.0, 0.0);
+    int ls = NEVER_L;
+    if (pr < npairs) {
+        x = px[pr];
+        ls = lseed[(int64_t)m * npairs + pr];
+        if (ls <= lmax) sd = seed[(int64_t)m * npairs + pr];
+    }
+    const int wstart = wmin64(ls);
+    const int winj = wmax64(ls <= lmax ? ls : -1);
+
+    // A operands: S_par[pair P0 + 128 kh + 4 s + k][n = 16 t + row]
+    double Sv[32];
+    {
+        const int row = lane & 15, k = lane >> 4;
+        const int n = 16 * t + row;
+        const int b = g * 16 + (n >> 1), c = n & 1;
+        const double *ph = reinterpret_cast<const double *>(band + (int64_t)min(b, nmaps - 1) * bstride + m) + c;
+#pragma unroll
+        for (int s = 0; s < 32; ++s) {
+            const int q = P0 + 128 * kh + 4 * s + k;
+            const int qq = min(q, npairs - 1);
+            const double a = ph[(int64_t)qq * (2 * MB)];
+            const double z = (nr - 1 - qq != qq) ? ph[(int64_t)(nr - 1 - qq) * (2 * MB)] : 0.0;
+            const bool ok = q < npairs && b < nmaps;
+            Sv[s] = ok ? (par ? a - z : a + z) : 0.0;
+        }
+    }
+    if (tid < 3) starts[tid] = NEVER_L;
+    __syncthreads();
+    if (lane == 0) {
+        atomicMin(&starts[0], wstart);
+        atomicMin(&starts[1 + (w >> 2)], wstart);     // waves 4j..4j+3 run K-half j's pairs
+    }
+    __syncthreads();
+    const int lstart = starts[0];
+    const int kstart = starts[1 + kh];
+
+    const bool direct = (ngroups == 1);
+    const int64_t i0 = alm_index(m, m, lmax) - m;
+    const int64_t ro = rofs[m];
+    const double *al = alpha + ro - m;
+    const double *sv = sl + ro - m;
+    auto store = [&](int l, int b, int c, double v) {
+        if (b >= nmaps) return;
+        if (direct) {
+            v *= sv[l];
+            if (adjoint) {
+                if (m > 0) v *= 2.0;
+                else if (c == 1) v = 0.0;
+            }
+            reinterpret_cast<double *>(out + (int64_t)b * nlmb + i0 + l)[c] = v;
+        } else {
+            reinterpret_cast<double *>(partial + ((int64_t)b * ngroups + grp) * nlmb + i0 + l)[c] = v;
+        }
+    };
+    const int lz = lstart > lmax ? lmax + 1 : lstart;
+    for (int e = tid; e < (lz - m) * 32; e += G::W * 64) store(m + (e >> 5), g * 16 + ((e & 31) >> 1), e & 1, 0.0);
+    if (lstart > lmax) return;
+
+    if (tid < ANA_L) alds[tid] = al[lstart + tid];
+    double anext = (tid < ANA_L) ? al[lstart + ANA_L + tid] : 0.0;
+    __syncthreads();
+    double *lw = lam + (qw & 3) * G::RUN + (qw >> 2);
+    const double *lr = lam + par * G::PAR + (lane & 15) * G::COL + (lane >> 4) * G::RUN + 32 * kh;
+    int buf = 0;
+    for (int c0 = lstart; c0 <= lmax; c0 += ANA_L, buf ^= 1) {
+        const double *ab = alds + buf * ANA_L;
+        const double anext2 = (tid < ANA_L) ? al[c0 + 2 * ANA_L + tid] : 0.0;
+#pragma unroll
+        for (int tt = 0; tt < ANA_L; ++tt) {
+            if ((tt & 7) == 0 && c0 <= winj && ls == c0 + tt) {
+                u2 = sd.x;
+                u1 = sd.y;
+            }
+            const double u = fma(ab[tt] * x, u1, -u2);
+            u2 = u1;
+            u1 = u;
+            if (lane < 32) lw[(tt & 1) * G::PAR + (tt >> 1) * G::COL] = u;
+        }
+        if (tid < ANA_L) alds[(buf ^ 1) * ANA_L + tid] = anext;
+        anext = anext2;
+        __syncthreads();
+        doublex4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = D0;
+        if (c0 + ANA_L > kstart) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const double2 bq = *reinterpret_cast<const double2 *>(lr + 2 * j);
+                D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Sv[2 * j], bq.x, D0, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Sv[2 * j + 1], bq.y, D1, 0, 0, 0);
+            }
+        }
+        doublex4 D = D0 + D1;
+        if (kh == 1) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) xch[((t * 2 + par) * 4 + r) * 64 + lane] = D[r];
+        }
+        __syncthreads();
+        if (kh == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) D[r] += xch[((t * 2 + par) * 4 + r) * 64 + lane];
+            // D[row][col]: lane = col + 16 (row%4), reg = row/4; n = 16 t + row
+            const int l = c0 + 2 * (lane & 15) + par;
+            if (l <= lmax) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int n = 16 * t + (lane >> 4) + 4 * r;
+                    store(l, g * 16 + (n >> 1), n & 1, D[r]);
+                }
+            }
+        }
+    }
+}
+
+int band_mfma_groups(const Plan *p) { return (p->npairs + BandGeom::P - 1) / BandGeom::P; }
+
+int launch_band_analysis_mfma(const Plan *p, int nmaps, const void *band, void *band_alm, void *band_partial,
+                              int adjoint, hipStream_t s) {
+    const int mtop = std::min(M_LO - 1, p->mmax);
+    const int G = band_mfma_groups(p);
+    const int64_t nb = band_nlm(p);
+    dim3 grid((mtop + 1) * G, (nmaps + 15) / 16);
+    hipLaunchKernelGGL(band_analysis_mfma_kernel, grid, dim3(BandGeom::W * 64), BandGeom::LDS, s,
+                       (const double2 *)band, band_phase_elems(p), (double2 *)band_alm, (double2 *)band_partial,
+                       p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings,
+                       p->lmax, mtop, G, nb, nmaps, adjoint);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "band_analysis_mfma_kernel launch");
+    if (G > 1) return launch_group_reduce(p, nmaps, band_partial, band_alm, nb, G, nb, 0, mtop, true, adjoint, s);
+    return 0;
+}
+
 // waves per workgroup: the plan's tile (4 or 8), else 8 (4 for small
 // geometries, whose ring pairs would leave a larger workgroup idle)
 int mfma_analysis_waves(const Plan *p) {
@@ -449,7 +628,10 @@ int prepare_mfma_kernels() {
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_mfma_kernel<2>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)AnaGeom<2>::LDS);
-    if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(leg_analysis_mfma_kernel)");
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(band_analysis_mfma_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)BandGeom::LDS);
+    if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(matrix-core analysis kernels)");
     return 0;
 }
 
