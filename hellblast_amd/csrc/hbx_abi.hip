@@ -207,7 +207,7 @@ static int build_plan(Plan *p, const hbx_geom *g) {
     if ((rc = upload(p, &p->d_pair_n, pn.data(), np))) return rc;
     if ((rc = upload(p, &p->d_pair_s, ps.data(), np))) return rc;
     if ((rc = upload(p, &p->d_px, px.data(), np))) return rc;
-    if ((rc = plan_build_fft_tables(p, nphi))) return rc;
+    if ((rc = plan_build_fft_tables(p, nphi, any_phi0 ? g->phi0 : nullptr))) return rc;
     // switch-on tolerance: FP64 plans 2^-64, FP32 plans 2^-40 (relative
     // contributions far below each format's rounding error)
     const double tol = p->precision == HBX_F64 ? -64.0 : -40.0;

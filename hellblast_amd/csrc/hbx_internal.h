@@ -70,6 +70,7 @@ struct Plan {
     double *d_scale_ana = nullptr;         // weight_r / nphi_r
     double *d_scale_adj = nullptr;         // weight_r / nphi_r (map2alm adjoint: omega_r)
     int64_t *d_vofs = nullptr;             // per ring offset into chirp spectra (-1: pow2)
+    double2 *d_rot = nullptr;              // [4][nrings] FP64 rotation steps of the ring kernels
     // ring pairs (device)
     int32_t *d_pair_n = nullptr, *d_pair_s = nullptr;
     double *d_px = nullptr;                // cos theta of the north ring of each pair
@@ -97,7 +98,7 @@ struct Plan {
 };
 
 // launchers implemented in ring_fft.hip / legendre.hip
-int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi);
+int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const double *phi0);
 int launch_ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dtype,
                          int64_t map_stride, void *phase, const double *scale,
                          hipStream_t s);
