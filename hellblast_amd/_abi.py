@@ -18,6 +18,8 @@ HBX_EWORKSPACE = -4
 HBX_EUNSUPPORTED = -5
 HBX_EPLAN = -6
 HBX_MAX_SPIN = 8
+HBX_MFMA_F32 = 1
+HBX_MFMA_F16X2 = 2
 HBX_TIMER_LEGENDRE_ANALYSIS = 1
 HBX_TIMER_LEGENDRE_SYNTHESIS = 2
 HBX_TIMER_RING_ANALYSIS = 3
@@ -71,7 +73,8 @@ class hbx_plan_info(ctypes.Structure):
 
 class hbx_tiles(ctypes.Structure):
     _fields_ = [("valu_slots", ctypes.c_int32), ("mfma_waves", ctypes.c_int32),
-                ("mfma_min_maps", ctypes.c_int32), ("spin_slots", ctypes.c_int32)]
+                ("mfma_min_maps", ctypes.c_int32), ("spin_slots", ctypes.c_int32),
+                ("mfma_kind", ctypes.c_int32)]
 
 
 _vp = ctypes.c_void_p

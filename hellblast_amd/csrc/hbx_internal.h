@@ -92,9 +92,10 @@ struct Plan {
     std::vector<void *> allocations;
     // tiles (hbx_plan_set_tiles); 0 = size-based default
     int tile_valu_slots = 0;      // VALU analysis ring-pair slots per lane
-    int tile_mfma_waves = 0;      // waves per matrix-core analysis workgroup (4, 8, 16)
+    int tile_mfma_waves = 0;      // waves per matrix-core analysis workgroup (4, 8, 12)
     int tile_mfma_min_maps = 0;   // FP32 batches of at least this many maps use the matrix cores
     int tile_spin_slots = 0;      // spin analysis ring-pair slots per lane (1, 2, 4)
+    int tile_mfma_kind = 0;       // HBX_MFMA_F32 or HBX_MFMA_F16X2 (default)
 };
 
 // launchers implemented in ring_fft.hip / legendre.hip
@@ -129,6 +130,7 @@ int launch_leg_synthesis(const Plan *p, int nmaps, const void *scaled, void *pha
                          hipStream_t s);
 int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t alm_stride,
                               void *phase, int adjoint, hipStream_t s);
+int mfma_kind_of(const Plan *p);   // HBX_MFMA_F32 or HBX_MFMA_F16X2
 int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *alm, int64_t alm_stride,
                              void *partial, int adjoint, hipStream_t s);
 int launch_group_reduce(const Plan *p, int nmaps, void *partial, void *alm, int64_t alm_stride,

@@ -403,6 +403,471 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
 }
 
 // ---------------------------------------------------------------------------
+// Split-fp16 matrix-core analysis.  The same contraction as
+// leg_analysis_mfma_kernel, but on v_mfma_f32_16x16x32_f16 (16x the FP32
+// matrix rate) with every FP32 operand carried as a pair of fp16 values,
+// v = hi + lo (hi = fp16(v), lo = fp16(v - hi): 22 significant bits), and
+//   S lambda = S_hi lambda_hi + S_hi lambda_lo + S_lo lambda_hi
+// (the dropped S_lo lambda_lo is ~2^-22 relative); products of fp16 values
+// are exact in the FP32 accumulators.  Range: each wave scales its A rows
+// (one n = map re/im over its 256 ring pairs) by 2^k so the row maximum sits
+// in [2^13, 2^14), and unscales its output rows by 2^-k (exact); the
+// rescaled recursion values stay far below the fp16 maximum (|mu| < 100 up
+// to lmax 8191).
+//   A[row = n][k = pair]   = S_par(pair)[n]   (registers, hi/lo, whole kernel)
+//   B[k = pair][col]       = mu_{c0+2col+par}(pair)   (LDS tile, hi/lo)
+// Lane layout of the 16x16x32 f16 instruction (checked on gfx950):
+//   A: lane = row + 16 (k/8), halves k%8   B: lane = col + 16 (k/8), halves k%8
+//   D: lane = col + 16 (row/4), reg = row%4
+// Work split: 4 Q waves = (n-tile t, parity par, K-half kh); the first 2 Q
+// waves also run the FP64 recursion, two adjacent ring pairs per lane, so the
+// hi (and lo) halves of a lane's pairs pack into one dword LDS store.
+// Reference counterpart: the per-(m, l) ring sum of sht_raw.py:53 /
+// sht_fallback.py:63.
+// ---------------------------------------------------------------------------
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+
+template <int Q> struct H16Geom {
+    static constexpr int W = 4 * Q;               // waves per workgroup
+    static constexpr int P = 256 * Q;             // ring pairs per workgroup
+    static constexpr int RW = 2 * Q;              // recursion waves (128 ring pairs each)
+    static constexpr int COLB = 2 * P + 16;       // bytes per tile column: P halves + a 16-byte skew
+    static constexpr int PARB = 16 * COLB;
+    static constexpr int TILEB = 2 * PARB;        // bytes of the hi (or lo) tile
+    static constexpr int XCH = 4 * 256;           // floats: the K-halves' exchange (Q = 2)
+    static constexpr size_t LDS = 2 * (size_t)TILEB + XCH * sizeof(float) + 2 * ANA_L * sizeof(double) + 16;
+};
+
+__device__ __forceinline__ uint32_t pack_h2(half2v h) { return __builtin_bit_cast(uint32_t, h); }
+
+template <int Q>
+__global__ void __launch_bounds__(H16Geom<Q>::W * 64) __attribute__((amdgpu_waves_per_eu(4)))
+leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float2 *__restrict__ alm,
+                        int64_t astride, float2 *__restrict__ partial, const double *__restrict__ px,
+                        const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
+                        const double *__restrict__ sl, const int32_t *__restrict__ lseed,
+                        const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
+                        int64_t nlm, int nmaps, int adjoint, int mbeg) {
+    using G = H16Geom<Q>;
+    extern __shared__ __align__(16) unsigned char hsm[];
+    unsigned char *thi = hsm, *tlo = hsm + G::TILEB;
+    float *xch = reinterpret_cast<float *>(hsm + 2 * G::TILEB);
+    double *alds = reinterpret_cast<double *>(xch + G::XCH);          // [2][ANA_L]
+    int *starts = reinterpret_cast<int *>(alds + 2 * ANA_L);         // [0]: workgroup, [1 + kh]: K-half
+
+    int mb, item;
+    xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
+    const int m = mb * MB + item / ngroups;
+    const int grp = item % ngroups;
+    if (m > mmax || m < mbeg) return;
+    const int g = blockIdx.y;                         // group of 16 maps
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int t = w & 1, par = (w >> 1) & 1, kh = w >> 2;    // output tile, parity, K-half
+    const int P0 = grp * G::P;
+    const bool rec = (w < G::RW);
+
+    // recursion: pairs P0 + 128 w + 2 lane + h, h = 0, 1 (recursion waves only)
+    double x[2] = {0.0, 0.0}, u1[2] = {0.0, 0.0}, u2[2] = {0.0, 0.0};
+    double2 sd[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+    int ls[2] = {NEVER_L, NEVER_L};
+    if (rec) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int pr = P0 + 128 * w + 2 * lane + h;
+            if (pr < npairs) {
+                x[h] = px[pr];
+                ls[h] = lseed[(int64_t)m * npairs + pr];
+                if (ls[h] <= lmax) sd[h] = seed[(int64_t)m * npairs + pr];
+            }
+        }
+    }
+    const int wstart = wmin64(min(ls[0], ls[1]));
+    const int winj = wmax64(max(ls[0] <= lmax ? ls[0] : -1, ls[1] <= lmax ? ls[1] : -1));
+
+    // A operands: S_par[pair P0 + 256 kh + 32 s + 8 q + i][n = 16 t + row], q = lane/16, row = lane%16
+    half8 Ahi[8], Alo[8];
+    float usc[4];                                      // 2^-k of the output rows 4 q + j
+    {
+        const int row = lane & 15, q = lane >> 4;
+        const int n = 16 * t + row;
+        const int b = g * 16 + (n >> 1), c = n & 1;
+        const float *ph = reinterpret_cast<const float *>(phase + (int64_t)min(b, nmaps - 1) * pstride +
+                                                          (int64_t)mb * nr * MB + (m & 7)) + c;
+        auto sval = [&](int s, int i) -> float {
+            const int qp = P0 + 256 * kh + 32 * s + 8 * q + i;
+            const int qq = min(qp, npairs - 1);
+            const float a = ph[(int64_t)qq * (2 * MB)];
+            const float z = (nr - 1 - qq != qq) ? ph[(int64_t)(nr - 1 - qq) * (2 * MB)] : 0.0f;
+            const bool ok = qp < npairs && b < nmaps;
+            return ok ? (par ? a - z : a + z) : 0.0f;
+        };
+        // one load pass (16 loads in flight at a time), then the row maximum,
+        // then the split in place
+        float Sf[64];
+        float amax = 0.0f;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                Sf[8 * s + i] = sval(s, i);
+                amax = fmaxf(amax, fabsf(Sf[8 * s + i]));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        amax = fmaxf(amax, __shfl_xor(amax, 16, 64));
+        amax = fmaxf(amax, __shfl_xor(amax, 32, 64));
+        int e = 0;
+        (void)frexpf(amax, &e);
+        const int k = amax > 0.0f ? min(max(14 - e, -100), 100) : 0;
+        const float sc = ldexpf(1.0f, k);
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+#pragma unroll
+            for (int i = 0; i < 8; i += 2) {
+                const float2v v = {Sf[8 * s + i] * sc, Sf[8 * s + i + 1] * sc};
+                const half2v hv = __builtin_convertvector(v, half2v);
+                const half2v lv = __builtin_convertvector(v - __builtin_convertvector(hv, float2v), half2v);
+                Ahi[s][i] = hv[0];
+                Ahi[s][i + 1] = hv[1];
+                Alo[s][i] = lv[0];
+                Alo[s][i + 1] = lv[1];
+            }
+        const float inv = ldexpf(1.0f, -k);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) usc[j] = __shfl(inv, 4 * q + j, 64);
+    }
+    if (tid < 1 + Q) starts[tid] = NEVER_L;
+    __syncthreads();
+    if (rec && lane == 0) {
+        atomicMin(&starts[0], wstart);
+        atomicMin(&starts[1 + (w >> 1)], wstart);     // recursion waves 2j, 2j+1 cover K-half j
+    }
+    __syncthreads();
+    const int lstart = starts[0];
+    const int kstart = starts[1 + kh];
+
+    const bool direct = (ngroups == 1);
+    const int64_t i0 = alm_index(m, m, lmax) - m;       // + l
+    const int64_t ro = rofs[m];
+    const double *al = alpha + ro - m;                  // zero-padded past lmax
+    const double *sv = sl + ro - m;
+    auto store = [&](int l, int b, float2 v) {
+        if (b >= nmaps) return;
+        if (direct) {
+            double f = sv[l];
+            double re = (double)v.x * f, im = (double)v.y * f;
+            if (adjoint) {
+                if (m > 0) { re *= 2.0; im *= 2.0; }
+                else im = 0.0;
+            }
+            alm[(int64_t)b * astride + i0 + l] = make_float2((float)re, (float)im);
+        } else {
+            partial[((int64_t)b * ngroups + grp) * nlm + i0 + l] = v;
+        }
+    };
+    // coefficients below every ring pair's switch-on are zero
+    const int lz = lstart > lmax ? lmax + 1 : lstart;
+    for (int e = tid; e < (lz - m) * 16; e += G::W * 64) store(m + (e >> 4), g * 16 + (e & 15), make_float2(0.f, 0.f));
+    if (lstart > lmax) return;
+
+    if (tid < ANA_L) alds[tid] = al[lstart + tid];
+    double anext = (tid < ANA_L) ? al[lstart + ANA_L + tid] : 0.0;
+    __syncthreads();
+    // this lane's recursion stores: byte 2 (128 w + 2 lane) of a column
+    const int wofs = 256 * w + 4 * lane;
+    // this lane's B reads: column lane%16, halves 256 kh + 32 s + 8 q
+    const int rofs16 = par * G::PARB + (lane & 15) * G::COLB + 2 * (256 * kh + 8 * (lane >> 4));
+    int buf = 0;
+    for (int c0 = lstart; c0 <= lmax; c0 += ANA_L, buf ^= 1) {
+        const double *ab = alds + buf * ANA_L;
+        const double anext2 = (tid < ANA_L) ? al[c0 + 2 * ANA_L + tid] : 0.0;   // in flight a chunk
+        // 1. recursion of this lane's two pairs over l = c0 .. c0+31 into the hi/lo tiles
+        if (rec) {
+#pragma unroll
+            for (int tt = 0; tt < ANA_L; ++tt) {
+                if ((tt & 7) == 0 && c0 <= winj) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+                        if (ls[h] == c0 + tt) {
+                            u2[h] = sd[h].x;
+                            u1[h] = sd[h].y;
+                        }
+                }
+                const double a = ab[tt];
+                float2v f;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const double u = fma(a * x[h], u1[h], -u2[h]);
+                    u2[h] = u1[h];
+                    u1[h] = u;
+                    f[h] = (float)u;
+                }
+                // packed conversions (v_cvt_pk_f16_f32, round to nearest even)
+                const half2v hh = __builtin_convertvector(f, half2v);
+                const half2v hl = __builtin_convertvector(f - __builtin_convertvector(hh, float2v), half2v);
+                const int o = (tt & 1) * G::PARB + (tt >> 1) * G::COLB + wofs;
+                *reinterpret_cast<uint32_t *>(thi + o) = pack_h2(hh);
+                *reinterpret_cast<uint32_t *>(tlo + o) = pack_h2(hl);
+            }
+        }
+        if (tid < ANA_L) alds[(buf ^ 1) * ANA_L + tid] = anext;
+        anext = anext2;
+        __syncthreads();
+        // 2. this wave's output tile over its 256 pairs (skipped before any switches on)
+        floatx4 D0 = floatx4{0.f, 0.f, 0.f, 0.f}, D1 = D0;
+        if (c0 + ANA_L > kstart) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const half8 bh = *reinterpret_cast<const half8 *>(thi + rofs16 + 64 * s);
+                const half8 bl = *reinterpret_cast<const half8 *>(tlo + rofs16 + 64 * s);
+                D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[s], bh, D0, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[s], bl, D1, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[s], bh, D1, 0, 0, 0);
+            }
+        }
+        floatx4 D = D0 + D1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) D[r] *= usc[r];
+        // 3. K-halves: the second hands its tile to the first through LDS
+        if (Q == 2 && kh == 1) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) xch[((t * 2 + par) * 4 + r) * 64 + lane] = D[r];
+        }
+        __syncthreads();
+        if (kh == 0) {
+            if (Q == 2) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) D[r] += xch[((t * 2 + par) * 4 + r) * 64 + lane];
+            }
+            // D[row][col]: lane = col + 16 (row/4), reg = row%4; row = n - 16 t, col -> l
+            const int l = c0 + 2 * (lane & 15) + par;
+            if (l <= lmax) {
+                const int b0 = g * 16 + 8 * t + 2 * (lane >> 4);
+                store(l, b0, make_float2(D[0], D[1]));
+                store(l, b0 + 1, make_float2(D[2], D[3]));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined matrix-core analysis (12-wave workgroups): the contraction of
+// leg_analysis_mfma_kernel<2> -- 512 ring pairs of one m per workgroup, eight
+// MFMA waves = 2 n-tiles x 2 parities x 2 K-halves of 256 pairs -- plus four
+// recursion waves that run the FP64 lambda recursion of all 512 pairs (two
+// per lane) one chunk ahead, into the other half of a double-buffered LDS
+// tile.  One barrier per 32-l chunk: in step c the recursion waves fill chunk
+// c+1 while the MFMA waves contract chunk c, so the FP64 VALU work overlaps
+// the matrix cores instead of alternating with them.  The K-halves add
+// through a double-buffered exchange that the first half reads after the
+// barrier.  LDS: 2 lambda tiles (145 KB) + 2 exchange buffers + alpha.
+// ---------------------------------------------------------------------------
+struct PipeGeom {
+    static constexpr int CW = 8, PW = 4, W = CW + PW;   // MFMA waves, recursion waves
+    static constexpr int P = 512;                        // ring pairs per workgroup
+    static constexpr int RUN = AnaGeom<2>::RUN, COL = AnaGeom<2>::COL, PAR = AnaGeom<2>::PAR;
+    static constexpr int TILE = AnaGeom<2>::TILE;        // floats
+    static constexpr int XCH = 4 * 256;                  // floats per exchange buffer
+    static constexpr size_t LDS = (size_t)(2 * TILE + 2 * XCH) * sizeof(float) + 2 * ANA_L * sizeof(double) + 16;
+};
+static_assert(PipeGeom::P == AnaGeom<2>::P, "pipelined tile geometry follows the 8-wave kernel");
+
+__global__ void __launch_bounds__(PipeGeom::W * 64)
+leg_analysis_mfma_pipe_kernel(const float2 *__restrict__ phase, int64_t pstride, float2 *__restrict__ alm,
+                              int64_t astride, float2 *__restrict__ partial, const double *__restrict__ px,
+                              const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
+                              const double *__restrict__ sl, const int32_t *__restrict__ lseed,
+                              const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
+                              int64_t nlm, int nmaps, int adjoint, int mbeg) {
+    using G = PipeGeom;
+    extern __shared__ __align__(16) float smem[];
+    float *lam0 = smem;                                              // [2][TILE]
+    float *xch0 = smem + 2 * G::TILE;                                // [2][XCH]
+    double *alds = reinterpret_cast<double *>(smem + 2 * G::TILE + 2 * G::XCH);   // [2][ANA_L]
+    int *starts = reinterpret_cast<int *>(alds + 2 * ANA_L);         // [0]: workgroup, [1 + kh]: K-half
+
+    int mb, item;
+    xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
+    const int m = mb * MB + item / ngroups;
+    const int grp = item % ngroups;
+    if (m > mmax || m < mbeg) return;
+    const int g = blockIdx.y;                         // group of 16 maps
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool prod = (w >= G::CW);                   // recursion wave
+    const int t = w & 1, par = (w >> 1) & 1, kh = (w >> 2) & 1;   // MFMA waves: tile, parity, K-half
+    const int P0 = grp * G::P;
+    const int64_t ro = rofs[m];
+    const double *al = alpha + ro - m;                  // zero-padded past lmax
+    const double *sv = sl + ro - m;
+    const int64_t i0 = alm_index(m, m, lmax) - m;       // + l
+    const bool direct = (ngroups == 1);
+
+    auto store = [&](int l, int b, float2 v) {
+        if (b >= nmaps) return;
+        if (direct) {
+            double f = sv[l];
+            double re = (double)v.x * f, im = (double)v.y * f;
+            if (adjoint) {
+                if (m > 0) { re *= 2.0; im *= 2.0; }
+                else im = 0.0;
+            }
+            alm[(int64_t)b * astride + i0 + l] = make_float2((float)re, (float)im);
+        } else {
+            partial[((int64_t)b * ngroups + grp) * nlm + i0 + l] = v;
+        }
+    };
+
+    if (tid < 3) starts[tid] = NEVER_L;
+    __syncthreads();
+
+    if (prod) {
+        // ----- recursion waves: pairs P0 + q0 (K-half 0) and P0 + q0 + 256 (K-half 1)
+        const int q0 = 64 * (w - G::CW) + lane, ptid = tid - G::CW * 64;
+        double x[2], u1[2] = {0.0, 0.0}, u2[2] = {0.0, 0.0};
+        double2 sd[2];
+        int ls[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int pr = P0 + q0 + 256 * h;
+            x[h] = 0.0;
+            sd[h] = make_double2(0.0, 0.0);
+            ls[h] = NEVER_L;
+            if (pr < npairs) {
+                x[h] = px[pr];
+                ls[h] = lseed[(int64_t)m * npairs + pr];
+                if (ls[h] <= lmax) sd[h] = seed[(int64_t)m * npairs + pr];
+            }
+        }
+        const int ws0 = wmin64(ls[0]), ws1 = wmin64(ls[1]);
+        const int winj = wmax64(max(ls[0] <= lmax ? ls[0] : -1, ls[1] <= lmax ? ls[1] : -1));
+        if (lane == 0) {
+            atomicMin(&starts[0], min(ws0, ws1));
+            atomicMin(&starts[1], ws0);
+            atomicMin(&starts[2], ws1);
+        }
+        __syncthreads();
+        const int lstart = starts[0];
+        const int lz = lstart > lmax ? lmax + 1 : lstart;
+        for (int e = tid; e < (lz - m) * 16; e += G::W * 64)
+            store(m + (e >> 4), g * 16 + (e & 15), make_float2(0.f, 0.f));
+        if (lstart > lmax) return;
+        const int nch = (lmax - lstart) / ANA_L + 1;
+        // alpha: chunks 0 and 1 staged now, chunk c + 2 staged in step c
+        double anext = 0.0;
+        if (ptid < ANA_L) {
+            alds[ptid] = al[lstart + ptid];
+            alds[ANA_L + ptid] = (nch > 1) ? al[lstart + ANA_L + ptid] : 0.0;
+            anext = (nch > 2) ? al[lstart + 2 * ANA_L + ptid] : 0.0;
+        }
+        __syncthreads();
+        float *lwb = lam0 + (q0 & 3) * G::RUN + (q0 >> 2);   // pair q0 + 256 sits 64 floats later
+        auto produce = [&](int cc) {
+            const int c0 = lstart + ANA_L * cc;
+            const double *ab = alds + (cc & 1) * ANA_L;
+            float *lw = lwb + (cc & 1) * G::TILE;
+#pragma unroll
+            for (int tt = 0; tt < ANA_L; ++tt) {
+                if ((tt & 7) == 0 && c0 <= winj) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+                        if (ls[h] == c0 + tt) {
+                            u2[h] = sd[h].x;
+                            u1[h] = sd[h].y;
+                        }
+                }
+                const double a = ab[tt];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const double u = fma(a * x[h], u1[h], -u2[h]);
+                    u2[h] = u1[h];
+                    u1[h] = u;
+                    lw[(tt & 1) * G::PAR + (tt >> 1) * G::COL + 64 * h] = (float)u;
+                }
+            }
+        };
+        produce(0);
+        __syncthreads();
+        for (int c = 0; c < nch; ++c) {
+            if (c + 1 < nch) produce(c + 1);
+            if (ptid < ANA_L) {
+                alds[(c & 1) * ANA_L + ptid] = anext;
+                anext = (c + 3 < nch) ? al[lstart + (c + 3) * ANA_L + ptid] : 0.0;
+            }
+            __syncthreads();
+        }
+        return;
+    }
+
+    // ----- MFMA waves.  A operands: S_par[pair P0 + 256 kh + 4 s + k][n = 16 t + row]
+    float Sv[64];
+    {
+        const int row = lane & 15, k = lane >> 4;
+        const int n = 16 * t + row;
+        const int b = g * 16 + (n >> 1), c = n & 1;
+        const float *ph = reinterpret_cast<const float *>(phase + (int64_t)min(b, nmaps - 1) * pstride +
+                                                          (int64_t)mb * nr * MB + (m & 7)) + c;
+#pragma unroll
+        for (int s = 0; s < 64; ++s) {
+            const int q = P0 + 256 * kh + 4 * s + k;
+            const int qq = min(q, npairs - 1);
+            const float a = ph[(int64_t)qq * (2 * MB)];
+            const float z = (nr - 1 - qq != qq) ? ph[(int64_t)(nr - 1 - qq) * (2 * MB)] : 0.0f;
+            const bool ok = q < npairs && b < nmaps;
+            Sv[s] = ok ? (par ? a - z : a + z) : 0.0f;
+        }
+    }
+    __syncthreads();
+    const int lstart = starts[0];
+    const int kstart = starts[1 + kh];
+    const int lz = lstart > lmax ? lmax + 1 : lstart;
+    for (int e = tid; e < (lz - m) * 16; e += G::W * 64) store(m + (e >> 4), g * 16 + (e & 15), make_float2(0.f, 0.f));
+    if (lstart > lmax) return;
+    const int nch = (lmax - lstart) / ANA_L + 1;
+    __syncthreads();                                    // alpha staged
+    __syncthreads();                                    // chunk 0 in tile 0
+    const int lro = par * G::PAR + (lane & 15) * G::COL + (lane >> 4) * G::RUN + 64 * kh;
+    float *xw = xch0 + ((t * 2 + par) * 4) * 64 + lane;
+    for (int c = 0; c < nch; ++c) {
+        const int c0 = lstart + ANA_L * c;
+        const float *lr = lam0 + (c & 1) * G::TILE + lro;
+        floatx4 D0 = floatx4{0.f, 0.f, 0.f, 0.f}, D1 = D0, D2 = D0, D3 = D0;
+        if (c0 + ANA_L > kstart) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const float4 bq = *reinterpret_cast<const float4 *>(lr + 4 * j);
+                floatx4 &Dj = (j < 4) ? D0 : (j < 8) ? D1 : (j < 12) ? D2 : D3;
+                Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j], bq.x, Dj, 0, 0, 0);
+                Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j + 1], bq.y, Dj, 0, 0, 0);
+                Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j + 2], bq.z, Dj, 0, 0, 0);
+                Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j + 3], bq.w, Dj, 0, 0, 0);
+            }
+        }
+        floatx4 D = (D0 + D1) + (D2 + D3);
+        float *xb = xw + (c & 1) * G::XCH;
+        if (kh == 1) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) xb[r * 64] = D[r];
+        }
+        __syncthreads();
+        if (kh == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) D[r] += xb[r * 64];
+            // D[row][col]: lane = col + 16 (row/4), reg = row%4; row = n - 16 t, col -> l
+            const int l = c0 + 2 * (lane & 15) + par;
+            if (l <= lmax) {
+                const int b0 = g * 16 + 8 * t + 2 * (lane >> 4);
+                store(l, b0, make_float2(D[0], D[1]));
+                store(l, b0 + 1, make_float2(D[2], D[3]));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 band of FP32 batches (m < M_LO) on the FP64 matrix cores,
 // v_mfma_f64_16x16x4f64 (D[16x16] += A[16x4] B[4x16], FP64 throughout), with
 // the same decomposition as the FP32 kernel: a workgroup of 8 waves covers
@@ -583,13 +1048,20 @@ int launch_band_analysis_mfma(const Plan *p, int nmaps, const void *band, void *
 
 // waves per workgroup: the plan's tile (4 or 8), else 8 (4 for small
 // geometries, whose ring pairs would leave a larger workgroup idle)
+// operand format of the matrix-core kernels (split fp16 unless set)
+int mfma_kind_of(const Plan *p) { return p->tile_mfma_kind > 0 ? p->tile_mfma_kind : HBX_MFMA_F16X2; }
+
+// waves per matrix-core analysis workgroup: 4 or 8 (64 ring pairs per wave),
+// or 12 (FP32 only: the pipelined kernel, 512 ring pairs, 8 MFMA + 4
+// recursion waves)
 int mfma_analysis_waves(const Plan *p) {
     if (p->tile_mfma_waves > 0) return p->tile_mfma_waves;
     return p->npairs <= 256 ? 4 : 8;
 }
 
 int mfma_analysis_groups(const Plan *p) {
-    const int pairs_wg = mfma_analysis_waves(p) * 64;
+    const int W = mfma_analysis_waves(p);
+    const int pairs_wg = W == 12 ? PipeGeom::P : W * 64;
     return (p->npairs + pairs_wg - 1) / pairs_wg;
 }
 
@@ -608,10 +1080,28 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
                        (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,       \
                        p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,     \
                        p->mmax, G, p->nlm, nmaps, adjoint, mbeg)
-    switch (W) {
+    if (mfma_kind_of(p) == HBX_MFMA_F16X2) {
+#define HBX_AH(QV)                                                                                              \
+    hipLaunchKernelGGL(leg_analysis_h16_kernel<QV>, grid, dim3(H16Geom<QV>::W * 64), H16Geom<QV>::LDS, s,      \
+                       (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,       \
+                       p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,     \
+                       p->mmax, G, p->nlm, nmaps, adjoint, mbeg)
+        switch (W) {
+        case 4: HBX_AH(1); break;
+        case 8: HBX_AH(2); break;
+        default: return set_error(HBX_EUNSUPPORTED, "split-fp16 analysis waves must be 4 or 8");
+        }
+#undef HBX_AH
+    } else switch (W) {
     case 4: HBX_AM(1); break;
     case 8: HBX_AM(2); break;
-    default: return set_error(HBX_EUNSUPPORTED, "matrix-core analysis waves must be 4 or 8");
+    case 12:
+        hipLaunchKernelGGL(leg_analysis_mfma_pipe_kernel, grid, dim3(PipeGeom::W * 64), PipeGeom::LDS, s,
+                           (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,
+                           p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,
+                           p->mmax, G, p->nlm, nmaps, adjoint, mbeg);
+        break;
+    default: return set_error(HBX_EUNSUPPORTED, "matrix-core analysis waves must be 4, 8 or 12");
     }
 #undef HBX_AM
     timer_mark(HBX_TIMER_LEGENDRE_ANALYSIS, false, s);
@@ -628,6 +1118,15 @@ int prepare_mfma_kernels() {
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_mfma_kernel<2>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)AnaGeom<2>::LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_h16_kernel<1>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)H16Geom<1>::LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_h16_kernel<2>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)H16Geom<2>::LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_mfma_pipe_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)PipeGeom::LDS);
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void *>(band_analysis_mfma_kernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)BandGeom::LDS);

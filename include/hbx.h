@@ -114,12 +114,21 @@ int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
  *   valu_slots     VALU Legendre analysis: ring-pair slots of 64 pairs per lane
  *                  (power of two <= 16; FP64 kernels clamp to 8)
  *   mfma_waves     matrix-core analysis: waves (64 ring pairs each) per
- *                  workgroup, 4 or 8
+ *                  workgroup, 4 or 8; or 12: 512 ring pairs per workgroup,
+ *                  8 MFMA waves fed by 4 lambda-recursion waves one chunk
+ *                  ahead (double-buffered LDS tile)
  *   mfma_min_maps  FP32 batches of at least this many maps use the
  *                  matrix-core Legendre kernels (default 2)
- *   spin_slots     spin Legendre analysis: ring-pair slots per lane, 1, 2, 4 */
+ *   spin_slots     spin Legendre analysis: ring-pair slots per lane, 1, 2, 4
+ *   mfma_kind      matrix-core operand format of FP32 batches:
+ *                  HBX_MFMA_F32 (FP32 instructions) or HBX_MFMA_F16X2
+ *                  (each FP32 operand as an fp16 hi/lo pair, three fp16
+ *                  products per FP32 product, FP32 accumulation; the
+ *                  default); with HBX_MFMA_F16X2, mfma_waves is 4 or 8 */
+#define HBX_MFMA_F32 1
+#define HBX_MFMA_F16X2 2
 typedef struct hbx_tiles {
-    int32_t valu_slots, mfma_waves, mfma_min_maps, spin_slots;
+    int32_t valu_slots, mfma_waves, mfma_min_maps, spin_slots, mfma_kind;
 } hbx_tiles;
 int hbx_plan_set_tiles(hbx_plan *plan, const hbx_tiles *tiles);
 int hbx_plan_get_tiles(const hbx_plan *plan, hbx_tiles *tiles);

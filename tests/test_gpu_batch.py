@@ -31,10 +31,12 @@ def _eps_max(a, ref, lmax):
 
 
 def test_tiles_set_get_and_validation(gpu):
+    from hellblast_amd import _abi
     from hellblast_amd._abi import HbxValueError
     plan = _plan(32, 64, "f32")
     base = plan.tiles()
     assert base["mfma_waves"] == 4 and base["mfma_min_maps"] == 2
+    assert base["mfma_kind"] == _abi.HBX_MFMA_F16X2
     plan.set_tiles(mfma_waves=4, valu_slots=2)
     t = plan.tiles()
     assert t["mfma_waves"] == 4 and t["valu_slots"] == 2
@@ -42,6 +44,10 @@ def test_tiles_set_get_and_validation(gpu):
         plan.set_tiles(mfma_waves=5)
     with pytest.raises(HbxValueError):
         plan.set_tiles(valu_slots=3)
+    with pytest.raises(HbxValueError):
+        plan.set_tiles(mfma_kind=3)
+    with pytest.raises(HbxValueError):
+        plan.set_tiles(mfma_waves=12, mfma_kind=_abi.HBX_MFMA_F16X2)
     plan.set_tiles(mfma_waves=0, valu_slots=0)
     assert plan.tiles() == base
 
