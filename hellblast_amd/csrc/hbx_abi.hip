@@ -269,7 +269,7 @@ static int legendre_analysis(const Plan *p, int nmaps, const void *phase, void *
 static int legendre_synthesis(const Plan *p, int nmaps, const void *alm, int64_t alm_stride, void *phase,
                               void *scaled, int adjoint, hipStream_t s) {
     if (p->precision == HBX_F32 && nmaps >= mfma_min_maps(p))
-        return launch_leg_synthesis_mfma(p, nmaps, alm, alm_stride, phase, adjoint, s);
+        return launch_leg_synthesis_mfma(p, nmaps, alm, alm_stride, phase, scaled, adjoint, s);
     int rc = launch_prescale(p, nmaps, alm, alm_stride, scaled, adjoint, s);
     if (rc) return rc;
     return launch_leg_synthesis(p, nmaps, scaled, phase, s);

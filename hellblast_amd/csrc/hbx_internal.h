@@ -128,8 +128,9 @@ int launch_prescale(const Plan *p, int nmaps, const void *alm, int64_t alm_strid
                     void *scaled, int adjoint, hipStream_t s);
 int launch_leg_synthesis(const Plan *p, int nmaps, const void *scaled, void *phase,
                          hipStream_t s);
+// ws: nmaps * (mmax + 1) ints (the split-fp16 kernel's per-(map, m) scales)
 int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t alm_stride,
-                              void *phase, int adjoint, hipStream_t s);
+                              void *phase, void *ws, int adjoint, hipStream_t s);
 int mfma_kind_of(const Plan *p);   // HBX_MFMA_F32 or HBX_MFMA_F16X2
 int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *alm, int64_t alm_stride,
                              void *partial, int adjoint, hipStream_t s);

@@ -192,12 +192,260 @@ leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split-fp16 matrix-core synthesis: the product of leg_synthesis_mfma_kernel
+// on v_mfma_f32_32x32x16_f16 with both operands as fp16 hi/lo pairs (three
+// products, FP32 accumulation; see leg_analysis_h16_kernel):
+//   A[row = ring][k]  = mu_l(x_ring), l = l0 + 2 k + par (16 same-parity l
+//                       of a 32-l step; lane = row + 32 (k/8), halves k%8)
+//   B[k][col = n]     = 2^k_b s_lm a_lm  (LDS, per 64-l chunk)
+// Each lane runs the FP64 recursion of its ring pair over the 32 l of a step;
+// the first 16 l form block 0, the next 16 block 1, and v_permlane32_swap of
+// (block 0, block 1) yields the A operands of both 32-ring tiles, so every
+// recursion runs once.  Range: per (map, m) the alm row is scaled by 2^k_b
+// so its maximum lies in [2^13, 2^14) (syn_exp_kernel), and the output
+// columns are unscaled by 2^-k_b (exact).
+// Lane layout of the 32x32x16 f16 instruction (checked on gfx950):
+//   A: lane = row + 32 (k/8), halves k%8   B: lane = col + 32 (k/8), halves k%8
+//   D[row][col]: lane = col + 32 ((row/4)%2), reg = (row%4) + 4 (row/8)
+// Reference counterpart: the per-(ring, m) l-sum of isht.py:15-21.
+// ---------------------------------------------------------------------------
+typedef _Float16 half8s __attribute__((ext_vector_type(8)));
+typedef _Float16 half2s __attribute__((ext_vector_type(2)));
+typedef float float2s __attribute__((ext_vector_type(2)));
+
+constexpr int SH_WAVES = 4;                     // waves per workgroup (same m), 64 ring pairs each
+constexpr int SH_LCHUNK = 64;                   // l staged per chunk (two 32-l steps)
+constexpr int SH_BCOL = 40;                     // halves per staged column: 32 + an 8-half skew
+
+// k_b(m) such that max_l |s_lm a_lm fadj| 2^k lies in [2^13, 2^14); one block per (m, map)
+__global__ void __launch_bounds__(256)
+syn_exp_kernel(const float2 *__restrict__ alm, int64_t astride, const int64_t *__restrict__ rofs,
+               const double *__restrict__ sl, int lmax, int mmax, int adjoint, int *__restrict__ kexp) {
+    __shared__ float red[4];
+    const int m = blockIdx.x, b = blockIdx.y;
+    const int64_t i0 = alm_index(m, m, lmax) - m;
+    const double *sv = sl + rofs[m] - m;
+    const float fadj = (adjoint && m > 0) ? 0.5f : 1.0f;
+    float mx = 0.0f;
+    for (int l = m + (int)threadIdx.x; l <= lmax; l += blockDim.x) {
+        const float2 a = alm[(int64_t)b * astride + i0 + l];
+        mx = fmaxf(mx, fmaxf(fabsf(a.x), fabsf(a.y)) * (float)sv[l] * fadj);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) mx = fmaxf(mx, red[i]);
+        int e = 0;
+        (void)frexpf(mx, &e);
+        kexp[(int64_t)b * (mmax + 1) + m] = mx > 0.0f ? min(max(14 - e, -100), 100) : 0;
+    }
+}
+
+__device__ __forceinline__ uint32_t h2bits(half2s h) { return __builtin_bit_cast(uint32_t, h); }
+
+__global__ void __launch_bounds__(SH_WAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 4)))
+leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float *__restrict__ phase,
+                         const double *__restrict__ px, const int64_t *__restrict__ rofs,
+                         const double *__restrict__ alpha, const double *__restrict__ sl,
+                         const int32_t *__restrict__ lseed, const double2 *__restrict__ seed, int npairs, int nr,
+                         int lmax, int mmax, int nmb, int nsb, int nmaps, int adjoint, const int *__restrict__ kexp) {
+    // staged B operands: [hi/lo][parity][n][same-parity l of the chunk]
+    __shared__ __align__(16) _Float16 bst[2][2][32][SH_BCOL];
+    __shared__ double alds[SH_LCHUNK];
+    __shared__ int wg_start;
+    int mb, item;
+    xcd_map(blockIdx.x, MB * nsb, &mb, &item);
+    const int m = mb * MB + item / nsb;
+    const int sb = item % nsb;
+    if (m > mmax) return;
+    const int g = blockIdx.y;                       // group of 16 maps
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int r = lane & 31, h = lane >> 5;
+
+    // this lane's ring pair
+    const int pbase = (sb * SH_WAVES + w) * 64;
+    const int p = pbase + lane;
+    double x = 0.0, u1 = 0.0, u2 = 0.0;
+    double2 sd = make_double2(0.0, 0.0);
+    int ls = NEVER_L;
+    if (p < npairs) {
+        x = px[p];
+        ls = lseed[(int64_t)m * npairs + p];
+        if (ls <= lmax) sd = seed[(int64_t)m * npairs + p];
+    }
+    const int wstart = wmin64(ls);
+    const int winj = wmax64(ls <= lmax ? ls : -1);
+    if (tid == 0) wg_start = NEVER_L;
+    __syncthreads();
+    if (lane == 0) atomicMin(&wg_start, wstart);
+    __syncthreads();
+    const int lstart = wg_start;
+
+    const int64_t ro = rofs[m];
+    const double *al = alpha + ro - m;      // al[l] = alpha_lm (zero-padded past lmax)
+    const double *sv = sl + ro - m;
+    const double fadj = (adjoint && m > 0) ? 0.5 : 1.0;   // map2alm transpose: g/c_m
+    const int64_t i0 = alm_index(m, m, lmax) - m;   // alm[b][i0 + l]
+    // staging slots of this thread: l = c0 + (tid & 63), maps b = g*16 + tid/64 + 4 q
+    constexpr int PER = 16 / SH_WAVES;
+    const int sll = tid & 63, sb0 = tid >> 6;
+    const int spar = sll & 1, skk = sll >> 1;
+    float ssc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int b = g * 16 + sb0 + SH_WAVES * q;
+        ssc[q] = b < nmaps ? ldexpf(1.0f, kexp[(int64_t)b * (mmax + 1) + m]) : 0.0f;
+    }
+    const int bo = g * 16 + (r >> 1);
+    const float usc = bo < nmaps ? ldexpf(1.0f, -kexp[(int64_t)bo * (mmax + 1) + m]) : 0.0f;
+
+    floatx16 De[2], Do[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        De[j] = floatx16{0};
+        Do[j] = floatx16{0};
+    }
+    for (int c0 = lstart; c0 <= lmax; c0 += SH_LCHUNK) {
+        {
+            const int l = c0 + sll;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int b = g * 16 + sb0 + SH_WAVES * q;
+                float2s v = {0.0f, 0.0f};
+                if (l <= lmax && b < nmaps) {
+                    const float2 av = alm[(int64_t)b * astride + i0 + l];
+                    const double f = sv[l] * fadj;
+                    v[0] = (float)(av.x * f) * ssc[q];
+                    v[1] = (float)(av.y * f) * ssc[q];
+                }
+                const half2s hv = __builtin_convertvector(v, half2s);
+                const half2s lv = __builtin_convertvector(v - __builtin_convertvector(hv, float2s), half2s);
+                const int n = 2 * (sb0 + SH_WAVES * q);
+                bst[0][spar][n][skk] = hv[0];
+                bst[0][spar][n + 1][skk] = hv[1];
+                bst[1][spar][n][skk] = lv[0];
+                bst[1][spar][n + 1][skk] = lv[1];
+            }
+            if (tid < SH_LCHUNK) alds[tid] = al[c0 + tid];
+        }
+        __syncthreads();
+        if (c0 + SH_LCHUNK > wstart) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int l0 = c0 + 32 * s;
+                if (l0 > lmax) break;
+                // FP64 recursion of this lane's pair over l0 .. l0+31.  Past lmax
+                // the recursion runs on the next row's alpha and may leave the
+                // fp16 range: those l enter the contraction as exact zeros.
+                float v[32];
+                const int jend = lmax - l0;
+#pragma unroll
+                for (int j = 0; j < 32; ++j) {
+                    if ((j & 7) == 0 && l0 + j <= winj && ls == l0 + j) {
+                        u2 = sd.x;
+                        u1 = sd.y;
+                    }
+                    const double u = fma(alds[32 * s + j] * x, u1, -u2);
+                    u2 = u1;
+                    u1 = u;
+                    v[j] = j <= jend ? (float)u : 0.0f;
+                }
+                // split: block b, parity par holds l0 + 16 b + 2 i + par, i = 0..7
+                uint32_t ah[2][2][4], alo[2][2][4];     // [par][block][dword]
+#pragma unroll
+                for (int par = 0; par < 2; ++par)
+#pragma unroll
+                    for (int bl = 0; bl < 2; ++bl)
+#pragma unroll
+                        for (int d = 0; d < 4; ++d) {
+                            const float2s f = {v[16 * bl + 4 * d + par], v[16 * bl + 4 * d + 2 + par]};
+                            const half2s hv = __builtin_convertvector(f, half2s);
+                            const half2s lv =
+                                __builtin_convertvector(f - __builtin_convertvector(hv, float2s), half2s);
+                            ah[par][bl][d] = h2bits(hv);
+                            alo[par][bl][d] = h2bits(lv);
+                        }
+                // tile operands: swap(block 0, block 1) -> (tile 0, tile 1)
+                half8s Ah[2][2], Al[2][2];             // [par][tile]
+#pragma unroll
+                for (int par = 0; par < 2; ++par) {
+                    uint32_t t0h[4], t1h[4], t0l[4], t1l[4];
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        auto rh = __builtin_amdgcn_permlane32_swap(ah[par][0][d], ah[par][1][d], false, false);
+                        auto rl = __builtin_amdgcn_permlane32_swap(alo[par][0][d], alo[par][1][d], false, false);
+                        t0h[d] = rh[0];
+                        t1h[d] = rh[1];
+                        t0l[d] = rl[0];
+                        t1l[d] = rl[1];
+                    }
+                    Ah[par][0] = __builtin_bit_cast(half8s, t0h);
+                    Ah[par][1] = __builtin_bit_cast(half8s, t1h);
+                    Al[par][0] = __builtin_bit_cast(half8s, t0l);
+                    Al[par][1] = __builtin_bit_cast(half8s, t1l);
+                }
+                // B: column r, same-parity l 16 s + 8 h .. +7 of the chunk
+                const half8s be_h = *reinterpret_cast<const half8s *>(&bst[0][0][r][16 * s + 8 * h]);
+                const half8s be_l = *reinterpret_cast<const half8s *>(&bst[1][0][r][16 * s + 8 * h]);
+                const half8s bo_h = *reinterpret_cast<const half8s *>(&bst[0][1][r][16 * s + 8 * h]);
+                const half8s bo_l = *reinterpret_cast<const half8s *>(&bst[1][1][r][16 * s + 8 * h]);
+#pragma unroll
+                for (int T = 0; T < 2; ++T) {
+                    De[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[0][T], be_h, De[T], 0, 0, 0);
+                    Do[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[1][T], bo_h, Do[T], 0, 0, 0);
+                    De[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[0][T], be_l, De[T], 0, 0, 0);
+                    Do[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[1][T], bo_l, Do[T], 0, 0, 0);
+                    De[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al[0][T], be_h, De[T], 0, 0, 0);
+                    Do[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al[1][T], bo_h, Do[T], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // D[row = ring][col = n]: this lane holds col = r, rows (q%4) + 8 (q/4) + 4 h.
+    // Ring pair p couples ring p (north) and ring nr-1-p (south).
+    const int b = g * 16 + (r >> 1), c = r & 1;
+    if (b >= nmaps) return;
+    float *out = phase + (((int64_t)b * nmb + mb) * nr) * (2 * MB) + (m & 7) * 2 + c;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int base = pbase + 32 * j;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int row = (q & 3) + 8 * (q >> 2) + 4 * h;
+            const int pp = base + row;
+            if (pp >= npairs) continue;
+            const float ge = De[j][q] * usc, go = Do[j][q] * usc;
+            const int rn = pp, rs = nr - 1 - pp;
+            out[(int64_t)rn * (2 * MB)] = ge + go;
+            if (rs != rn) out[(int64_t)rs * (2 * MB)] = ge - go;
+        }
+    }
+}
+
 int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t astride, void *phase,
-                              int adjoint, hipStream_t s) {
+                              void *ws, int adjoint, hipStream_t s) {
     const int nsb = (p->npairs + SYN_PAIRS_WG - 1) / SYN_PAIRS_WG;
     const int nmb8 = (p->nmb + 7) / 8 * 8;
     dim3 grid(nmb8 * MB * nsb, (nmaps + 15) / 16);
     timer_mark(HBX_TIMER_LEGENDRE_SYNTHESIS, true, s);
+    if (mfma_kind_of(p) == HBX_MFMA_F16X2) {
+        int *kexp = static_cast<int *>(ws);          // [nmaps][mmax + 1]
+        hipLaunchKernelGGL(syn_exp_kernel, dim3(p->mmax + 1, nmaps), dim3(256), 0, s, (const float2 *)alm, astride,
+                           p->d_rofs, p->d_sl, p->lmax, p->mmax, adjoint, kexp);
+        hipLaunchKernelGGL(leg_synthesis_h16_kernel, grid, dim3(SH_WAVES * 64), 0, s, (const float2 *)alm, astride,
+                           (float *)phase, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs,
+                           p->nrings, p->lmax, p->mmax, p->nmb, nsb, nmaps, adjoint, kexp);
+        timer_mark(HBX_TIMER_LEGENDRE_SYNTHESIS, false, s);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_error(e, "leg_synthesis_h16_kernel launch");
+        return 0;
+    }
     hipLaunchKernelGGL(leg_synthesis_mfma_kernel, grid, dim3(SYN_WAVES * 64), 0, s, (const float2 *)alm, astride,
                        (float *)phase, p->d_px, p->d_rofs, p->d_alpha, p->d_sl,
                        p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, p->nmb, nsb, nmaps, adjoint);
