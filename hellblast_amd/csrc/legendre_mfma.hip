@@ -43,6 +43,22 @@ __device__ __forceinline__ int wmax64(int v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// fp16 pairs for the split-fp16 matrix-core kernels: v = hi + lo,
+// hi = fp16(v), lo = fp16(v - hi)
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pack_h2(half2v h) { return __builtin_bit_cast(uint32_t, h); }
+
+// lo = fp16(v - hi), the residual as one mixed-precision FMA per value
+// (v_fma_mix_f32 reads the fp16 halves directly); neg1 = -1 arrives at run
+// time so the compiler keeps the FMA form
+__device__ __forceinline__ half2v split_lo(float2v v, half2v hi, float neg1) {
+    const float r0 = __builtin_fmaf((float)hi[0], neg1, v[0]);
+    const float r1 = __builtin_fmaf((float)hi[1], neg1, v[1]);
+    return __builtin_convertvector((float2v){r0, r1}, half2v);
+}
+
 // (lo, hi) halves exchange: returns {[a.lo | b.lo], [a.hi | b.hi]}
 __device__ __forceinline__ void swap_halves(float a, float b, float *t0, float *t1) {
     auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
@@ -211,8 +227,6 @@ leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float
 // Reference counterpart: the per-(ring, m) l-sum of isht.py:15-21.
 // ---------------------------------------------------------------------------
 typedef _Float16 half8s __attribute__((ext_vector_type(8)));
-typedef _Float16 half2s __attribute__((ext_vector_type(2)));
-typedef float float2s __attribute__((ext_vector_type(2)));
 
 constexpr int SH_WAVES = 4;                     // waves per workgroup (same m), 64 ring pairs each
 constexpr int SH_LCHUNK = 64;                   // l staged per chunk (two 32-l steps)
@@ -244,14 +258,15 @@ syn_exp_kernel(const float2 *__restrict__ alm, int64_t astride, const int64_t *_
     }
 }
 
-__device__ __forceinline__ uint32_t h2bits(half2s h) { return __builtin_bit_cast(uint32_t, h); }
+__device__ __forceinline__ uint32_t h2bits(half2v h) { return __builtin_bit_cast(uint32_t, h); }
 
 __global__ void __launch_bounds__(SH_WAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 4)))
 leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float *__restrict__ phase,
                          const double *__restrict__ px, const int64_t *__restrict__ rofs,
                          const double *__restrict__ alpha, const double *__restrict__ sl,
                          const int32_t *__restrict__ lseed, const double2 *__restrict__ seed, int npairs, int nr,
-                         int lmax, int mmax, int nmb, int nsb, int nmaps, int adjoint, const int *__restrict__ kexp) {
+                         int lmax, int mmax, int nmb, int nsb, int nmaps, int adjoint, const int *__restrict__ kexp,
+                         float neg1) {
     // staged B operands: [hi/lo][parity][n][same-parity l of the chunk]
     __shared__ __align__(16) _Float16 bst[2][2][32][SH_BCOL];
     __shared__ double alds[SH_LCHUNK];
@@ -315,22 +330,22 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 const int b = g * 16 + sb0 + SH_WAVES * q;
-                float2s v = {0.0f, 0.0f};
+                float2v v = {0.0f, 0.0f};
                 if (l <= lmax && b < nmaps) {
                     const float2 av = alm[(int64_t)b * astride + i0 + l];
                     const double f = sv[l] * fadj;
                     v[0] = (float)(av.x * f) * ssc[q];
                     v[1] = (float)(av.y * f) * ssc[q];
                 }
-                const half2s hv = __builtin_convertvector(v, half2s);
-                const half2s lv = __builtin_convertvector(v - __builtin_convertvector(hv, float2s), half2s);
+                const half2v hv = __builtin_convertvector(v, half2v);
+                const half2v lv = split_lo(v, hv, neg1);
                 const int n = 2 * (sb0 + SH_WAVES * q);
                 bst[0][spar][n][skk] = hv[0];
                 bst[0][spar][n + 1][skk] = hv[1];
                 bst[1][spar][n][skk] = lv[0];
                 bst[1][spar][n + 1][skk] = lv[1];
             }
-            if (tid < SH_LCHUNK) alds[tid] = al[c0 + tid];
+            if (tid < SH_LCHUNK) alds[tid] = (c0 + tid <= lmax) ? al[c0 + tid] : 0.0;
         }
         __syncthreads();
         if (c0 + SH_LCHUNK > wstart) {
@@ -338,11 +353,9 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
             for (int s = 0; s < 2; ++s) {
                 const int l0 = c0 + 32 * s;
                 if (l0 > lmax) break;
-                // FP64 recursion of this lane's pair over l0 .. l0+31.  Past lmax
-                // the recursion runs on the next row's alpha and may leave the
-                // fp16 range: those l enter the contraction as exact zeros.
+                // FP64 recursion of this lane's pair over l0 .. l0+31 (past lmax
+                // alpha is staged as 0, so the values stay bounded: u_l = -u_{l-2})
                 float v[32];
-                const int jend = lmax - l0;
 #pragma unroll
                 for (int j = 0; j < 32; ++j) {
                     if ((j & 7) == 0 && l0 + j <= winj && ls == l0 + j) {
@@ -352,8 +365,9 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
                     const double u = fma(alds[32 * s + j] * x, u1, -u2);
                     u2 = u1;
                     u1 = u;
-                    v[j] = j <= jend ? (float)u : 0.0f;
+                    v[j] = (float)u;
                 }
+
                 // split: block b, parity par holds l0 + 16 b + 2 i + par, i = 0..7
                 uint32_t ah[2][2][4], alo[2][2][4];     // [par][block][dword]
 #pragma unroll
@@ -362,10 +376,9 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
                     for (int bl = 0; bl < 2; ++bl)
 #pragma unroll
                         for (int d = 0; d < 4; ++d) {
-                            const float2s f = {v[16 * bl + 4 * d + par], v[16 * bl + 4 * d + 2 + par]};
-                            const half2s hv = __builtin_convertvector(f, half2s);
-                            const half2s lv =
-                                __builtin_convertvector(f - __builtin_convertvector(hv, float2s), half2s);
+                            const float2v f = {v[16 * bl + 4 * d + par], v[16 * bl + 4 * d + 2 + par]};
+                            const half2v hv = __builtin_convertvector(f, half2v);
+                            const half2v lv = split_lo(f, hv, neg1);
                             ah[par][bl][d] = h2bits(hv);
                             alo[par][bl][d] = h2bits(lv);
                         }
@@ -440,7 +453,7 @@ int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t
                            p->d_rofs, p->d_sl, p->lmax, p->mmax, adjoint, kexp);
         hipLaunchKernelGGL(leg_synthesis_h16_kernel, grid, dim3(SH_WAVES * 64), 0, s, (const float2 *)alm, astride,
                            (float *)phase, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs,
-                           p->nrings, p->lmax, p->mmax, p->nmb, nsb, nmaps, adjoint, kexp);
+                           p->nrings, p->lmax, p->mmax, p->nmb, nsb, nmaps, adjoint, kexp, -1.0f);
         timer_mark(HBX_TIMER_LEGENDRE_SYNTHESIS, false, s);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_error(e, "leg_synthesis_h16_kernel launch");
@@ -674,8 +687,6 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
 // sht_fallback.py:63.
 // ---------------------------------------------------------------------------
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef _Float16 half2v __attribute__((ext_vector_type(2)));
-typedef float float2v __attribute__((ext_vector_type(2)));
 
 template <int Q> struct H16Geom {
     static constexpr int W = 4 * Q;               // waves per workgroup
@@ -688,7 +699,6 @@ template <int Q> struct H16Geom {
     static constexpr size_t LDS = 2 * (size_t)TILEB + XCH * sizeof(float) + 2 * ANA_L * sizeof(double) + 16;
 };
 
-__device__ __forceinline__ uint32_t pack_h2(half2v h) { return __builtin_bit_cast(uint32_t, h); }
 
 template <int Q>
 __global__ void __launch_bounds__(H16Geom<Q>::W * 64) __attribute__((amdgpu_waves_per_eu(4)))
@@ -697,7 +707,7 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
                         const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
                         const double *__restrict__ sl, const int32_t *__restrict__ lseed,
                         const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
-                        int64_t nlm, int nmaps, int adjoint, int mbeg) {
+                        int64_t nlm, int nmaps, int adjoint, int mbeg, float neg1) {
     using G = H16Geom<Q>;
     extern __shared__ __align__(16) unsigned char hsm[];
     unsigned char *thi = hsm, *tlo = hsm + G::TILEB;
@@ -777,7 +787,7 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
             for (int i = 0; i < 8; i += 2) {
                 const float2v v = {Sf[8 * s + i] * sc, Sf[8 * s + i + 1] * sc};
                 const half2v hv = __builtin_convertvector(v, half2v);
-                const half2v lv = __builtin_convertvector(v - __builtin_convertvector(hv, float2v), half2v);
+                const half2v lv = split_lo(v, hv, neg1);
                 Ahi[s][i] = hv[0];
                 Ahi[s][i + 1] = hv[1];
                 Alo[s][i] = lv[0];
@@ -855,7 +865,7 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
                 }
                 // packed conversions (v_cvt_pk_f16_f32, round to nearest even)
                 const half2v hh = __builtin_convertvector(f, half2v);
-                const half2v hl = __builtin_convertvector(f - __builtin_convertvector(hh, float2v), half2v);
+                const half2v hl = split_lo(f, hh, neg1);
                 const int o = (tt & 1) * G::PARB + (tt >> 1) * G::COLB + wofs;
                 *reinterpret_cast<uint32_t *>(thi + o) = pack_h2(hh);
                 *reinterpret_cast<uint32_t *>(tlo + o) = pack_h2(hl);
@@ -1333,7 +1343,7 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
     hipLaunchKernelGGL(leg_analysis_h16_kernel<QV>, grid, dim3(H16Geom<QV>::W * 64), H16Geom<QV>::LDS, s,      \
                        (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,       \
                        p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,     \
-                       p->mmax, G, p->nlm, nmaps, adjoint, mbeg)
+                       p->mmax, G, p->nlm, nmaps, adjoint, mbeg, -1.0f)
         switch (W) {
         case 4: HBX_AH(1); break;
         case 8: HBX_AH(2); break;
