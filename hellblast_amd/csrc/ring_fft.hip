@@ -379,90 +379,104 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
 // ---------------------------------------------------------------------------
 // FP64 low-m band of FP32 plans: direct DFT in FP64 for m < M_LO,
 //   F_r(m) = [m < mcut_r] scale_r e^{-i m phi0_r} sum_j f_j e^{-2 pi i m j / n}
-// (same conventions as ring_analysis_kernel), one workgroup per (ring, pair
-// of maps): the roots e^{-2 pi i j/n} and their powers are formed once per
-// pixel for both maps (a per-thread step recurrence from one sincospi pair).
+// (same conventions as ring_analysis_kernel), on the FP64 matrix cores: per
+// ring and group of 16 maps, v_mfma_f64_16x16x4f64 contracts 4 pixels per
+// instruction,
+//   D[row = 2 m + c][col = map] += A[row][k] B[k][col],
+//   A[2 m][k] = cos(2 pi m j_k / n), A[2 m + 1][k] = -sin(2 pi m j_k / n),
+//   B[k][col] = f_col(j_k).
+// Each of 4 waves takes a quarter of the ring, staging 64 pixels of the 16
+// maps at a time in LDS (coalesced loads); its lanes carry e^{-2 pi i m j/n}
+// for their (row, pixel) by an FP64 recurrence over 4-pixel steps, anchored
+// by one sincospi per 64 pixels.  Lane layout of the 16x16x4 f64
+// instruction: A lane = row + 16 k, B lane = col + 16 k, D lane = col + 16
+// (row % 4), reg = row / 4.
 // ---------------------------------------------------------------------------
-constexpr int BAND_MAPS = 2;
+typedef double dx4 __attribute__((ext_vector_type(4)));
+constexpr int RB_WAVES = 4;
+constexpr int RB_CHUNK = 64;
 
 template <typename TIn>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(RB_WAVES * 64)
 ring_band_kernel(const TIn *__restrict__ maps, int64_t map_stride, double2 *__restrict__ band, int64_t band_stride,
                  const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
                  const int32_t *__restrict__ mcut, const double *__restrict__ phi0, const double *__restrict__ scale,
                  int mtop, int nr, int nmaps) {
-    __shared__ double part[4][BAND_MAPS][2 * M_LO];
-    const int r = blockIdx.x, b0 = blockIdx.y * BAND_MAPS;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int nt = blockDim.x;
+    __shared__ TIn stg[RB_WAVES][16][RB_CHUNK + 1];
+    __shared__ double dsum[RB_WAVES][4][64];
+    const int r = blockIdx.x, g = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int n = nphi[r];
-    const TIn *f0 = maps + (int64_t)b0 * map_stride + ring_start[r];
-    const bool two = b0 + 1 < nmaps;
-    const TIn *f1 = two ? f0 + map_stride : f0;
-    double re[BAND_MAPS][M_LO], im[BAND_MAPS][M_LO];
+    const TIn *f = maps + ring_start[r];
+    const int per = ((n + RB_WAVES - 1) / RB_WAVES + 3) / 4 * 4;   // pixels per wave, a multiple of 4
+    const int jb = w * per, je = min(n, jb + per);
+    const int row = lane & 15, k = lane >> 4;
+    const int m = row >> 1, im = row & 1;
+    double sr, si;                                  // e^{-2 pi i 4 m / n}: the 4-pixel step
+    sincospi(-2.0 * (double)((4 * m) % n) / (double)n, &si, &sr);
+    dx4 D = {0.0, 0.0, 0.0, 0.0};
+    // the 16 maps' pixels of the next chunk are loaded while this one computes
+    TIn nx[16];
+    auto load = [&](int c0) {
+        const int j = c0 + lane;
 #pragma unroll
-    for (int q = 0; q < BAND_MAPS; ++q)
-#pragma unroll
-        for (int m = 0; m < M_LO; ++m) { re[q][m] = 0.0; im[q][m] = 0.0; }
-    const double inv = 2.0 / (double)n;
-    // root of pixel j = tid and the step to j + nt
-    double wr, wi, sr, si;
-    sincospi(-(double)tid * inv, &wi, &wr);
-    sincospi(-(double)nt * inv, &si, &sr);
-    for (int j = tid; j < n; j += nt) {
-        const double v0 = (double)f0[j], v1 = two ? (double)f1[j] : 0.0;
-        double pr = 1.0, pi = 0.0;
-#pragma unroll
-        for (int m = 0; m < M_LO; ++m) {
-            re[0][m] = fma(v0, pr, re[0][m]);
-            im[0][m] = fma(v0, pi, im[0][m]);
-            re[1][m] = fma(v1, pr, re[1][m]);
-            im[1][m] = fma(v1, pi, im[1][m]);
-            const double t = pr * wr - pi * wi;
-            pi = pr * wi + pi * wr;
-            pr = t;
+        for (int b = 0; b < 16; ++b) {
+            const int bb = g * 16 + b;
+            nx[b] = (j < je && bb < nmaps) ? f[(int64_t)bb * map_stride + j] : (TIn)0;
         }
-        const double t = wr * sr - wi * si;
-        wi = wr * si + wi * sr;
-        wr = t;
+    };
+    if (jb < je) load(jb);
+    for (int c0 = jb; c0 < je; c0 += RB_CHUNK) {
+#pragma unroll
+        for (int b = 0; b < 16; ++b) stg[w][b][lane] = nx[b];
+        if (c0 + RB_CHUNK < je) load(c0 + RB_CHUNK);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double cr, ci;                              // e^{-2 pi i m j / n}, j = c0 + k
+        sincospi(-2.0 * (double)((m * (c0 + k)) % n) / (double)n, &ci, &cr);
+#pragma unroll 4
+        for (int s = 0; s < RB_CHUNK / 4; ++s) {
+            D = __builtin_amdgcn_mfma_f64_16x16x4f64(im ? ci : cr, (double)stg[w][row][4 * s + k], D, 0, 0, 0);
+            const double t = cr * sr - ci * si;
+            ci = cr * si + ci * sr;
+            cr = t;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 #pragma unroll
-    for (int q = 0; q < BAND_MAPS; ++q)
-#pragma unroll
-        for (int m = 0; m < M_LO; ++m) {
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                re[q][m] += __shfl_xor(re[q][m], o, 64);
-                im[q][m] += __shfl_xor(im[q][m], o, 64);
-            }
-        }
-    if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < BAND_MAPS; ++q)
-#pragma unroll
-            for (int m = 0; m < M_LO; ++m) { part[w][q][2 * m] = re[q][m]; part[w][q][2 * m + 1] = im[q][m]; }
-    }
+    for (int q = 0; q < 4; ++q) dsum[w][q][lane] = D[q];
     __syncthreads();
-    if (tid < BAND_MAPS * M_LO) {
-        const int q = tid / M_LO, m = tid % M_LO;
-        if (m <= mtop && b0 + q < nmaps) {
+    if (tid < 16 * M_LO) {
+        const int mm = tid >> 4, b = tid & 15;       // output (m, map)
+        const int bb = g * 16 + b;
+        if (mm <= mtop && bb < nmaps) {
+            // D[row][col]: lane = col + 16 (row % 4), reg = row / 4
             double x = 0.0, y = 0.0;
-            for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { x += part[k][q][2 * m]; y += part[k][q][2 * m + 1]; }
+            const int r0 = 2 * mm, r1 = 2 * mm + 1;
+#pragma unroll
+            for (int ww = 0; ww < RB_WAVES; ++ww) {
+                x += dsum[ww][r0 >> 2][b + 16 * (r0 & 3)];
+                y += dsum[ww][r1 >> 2][b + 16 * (r1 & 3)];
+            }
             double2 F = make_double2(0.0, 0.0);
-            if (m < mcut[r]) {
+            if (mm < mcut[r]) {
                 const double sc = scale ? scale[r] : 1.0;
                 x *= sc;
                 y *= sc;
                 const double ph = phi0 ? phi0[r] : 0.0;
                 if (ph != 0.0) {
                     double sn, cs;
-                    sincos(-(double)m * ph, &sn, &cs);
+                    sincos(-(double)mm * ph, &sn, &cs);
                     F = make_double2(x * cs - y * sn, x * sn + y * cs);
                 } else {
                     F = make_double2(x, y);
                 }
             }
-            band[(int64_t)(b0 + q) * band_stride + ((int64_t)(m >> 3) * nr + r) * MB + (m & 7)] = F;
+            band[(int64_t)bb * band_stride + ((int64_t)(mm >> 3) * nr + r) * MB + (mm & 7)] = F;
         }
     }
 }
@@ -470,15 +484,15 @@ ring_band_kernel(const TIn *__restrict__ maps, int64_t map_stride, double2 *__re
 int launch_ring_band(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride, void *band,
                      const double *scale, hipStream_t s) {
     const int mtop = std::min(M_LO - 1, p->mmax);
-    dim3 grid(p->nrings, (nmaps + BAND_MAPS - 1) / BAND_MAPS);
+    dim3 grid(p->nrings, (nmaps + 15) / 16);
     if (map_dtype == HBX_DT_F64)
-        hipLaunchKernelGGL(ring_band_kernel<double>, grid, dim3(256), 0, s, (const double *)maps, map_stride,
-                           (double2 *)band, band_phase_elems(p), p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0,
-                           scale, mtop, p->nrings, nmaps);
+        hipLaunchKernelGGL(ring_band_kernel<double>, grid, dim3(RB_WAVES * 64), 0, s, (const double *)maps,
+                           map_stride, (double2 *)band, band_phase_elems(p), p->d_ring_start, p->d_nphi, p->d_mcut,
+                           p->d_phi0, scale, mtop, p->nrings, nmaps);
     else
-        hipLaunchKernelGGL(ring_band_kernel<float>, grid, dim3(256), 0, s, (const float *)maps, map_stride,
-                           (double2 *)band, band_phase_elems(p), p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0,
-                           scale, mtop, p->nrings, nmaps);
+        hipLaunchKernelGGL(ring_band_kernel<float>, grid, dim3(RB_WAVES * 64), 0, s, (const float *)maps,
+                           map_stride, (double2 *)band, band_phase_elems(p), p->d_ring_start, p->d_nphi, p->d_mcut,
+                           p->d_phi0, scale, mtop, p->nrings, nmaps);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "ring_band_kernel launch");
     return 0;
