@@ -65,9 +65,9 @@ def candidates(plan, nmaps):
     if nmaps >= 2:
         # matrix cores: split-fp16 or FP32 operands; 4 / 8 waves of 64 ring
         # pairs, or (FP32) 12 = 8 MFMA + 4 recursion waves over 512 pairs
-        waves = [w for w in (4, 8, 12) if w == 4 or min(w, 8) * 32 < npairs]
+        waves = [w for w in (4, 8, 12, 16) if w == 4 or min(w, 8) * 32 < npairs]
         out += [{"mfma_kind": _abi.HBX_MFMA_F16X2, "mfma_waves": w, "mfma_min_maps": 2} for w in waves if w != 12]
-        out += [{"mfma_kind": _abi.HBX_MFMA_F32, "mfma_waves": w, "mfma_min_maps": 2} for w in waves]
+        out += [{"mfma_kind": _abi.HBX_MFMA_F32, "mfma_waves": w, "mfma_min_maps": 2} for w in waves if w != 16]
     # the VALU kernels (also for batches: mfma_min_maps above the batch size)
     out += [{"valu_slots": s, "mfma_min_maps": nmaps + 1} for s in slots]
     return out

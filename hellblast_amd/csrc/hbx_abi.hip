@@ -362,8 +362,9 @@ int hbx_plan_set_tiles(hbx_plan *plan, const hbx_tiles *t) {
     auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
     if (t->valu_slots != 0 && (!pow2(t->valu_slots) || t->valu_slots > 16))
         return set_error(HBX_EINVAL, "valu_slots must be 0 or a power of two <= 16");
-    if (t->mfma_waves != 0 && t->mfma_waves != 4 && t->mfma_waves != 8 && t->mfma_waves != 12)
-        return set_error(HBX_EINVAL, "mfma_waves must be 0, 4, 8 or 12");
+    if (t->mfma_waves != 0 && t->mfma_waves != 4 && t->mfma_waves != 8 && t->mfma_waves != 12 &&
+        t->mfma_waves != 16)
+        return set_error(HBX_EINVAL, "mfma_waves must be 0, 4, 8, 12 or 16");
     if (t->mfma_min_maps < 0) return set_error(HBX_EINVAL, "mfma_min_maps must be >= 0");
     if (t->spin_slots != 0 && t->spin_slots != 1 && t->spin_slots != 2 && t->spin_slots != 4)
         return set_error(HBX_EINVAL, "spin_slots must be 0, 1, 2 or 4");
@@ -371,6 +372,8 @@ int hbx_plan_set_tiles(hbx_plan *plan, const hbx_tiles *t) {
         return set_error(HBX_EINVAL, "mfma_kind must be 0, HBX_MFMA_F32 or HBX_MFMA_F16X2");
     if (t->mfma_waves == 12 && t->mfma_kind != HBX_MFMA_F32)
         return set_error(HBX_EINVAL, "mfma_waves 12 needs mfma_kind HBX_MFMA_F32");
+    if (t->mfma_waves == 16 && t->mfma_kind == HBX_MFMA_F32)
+        return set_error(HBX_EINVAL, "mfma_waves 16 needs the split-fp16 matrix-core kernels");
     Plan &p = plan->impl;
     p.tile_valu_slots = t->valu_slots;
     p.tile_mfma_waves = t->mfma_waves;

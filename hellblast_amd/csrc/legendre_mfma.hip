@@ -680,9 +680,10 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
 // Lane layout of the 16x16x32 f16 instruction (checked on gfx950):
 //   A: lane = row + 16 (k/8), halves k%8   B: lane = col + 16 (k/8), halves k%8
 //   D: lane = col + 16 (row/4), reg = row%4
-// Work split: 4 Q waves = (n-tile t, parity par, K-half kh); the first 2 Q
-// waves also run the FP64 recursion, two adjacent ring pairs per lane, so the
-// hi (and lo) halves of a lane's pairs pack into one dword LDS store.
+// Work split: 4 Q waves = (n-tile t, parity par, K-part kh of 256 pairs); the
+// first 2 Q waves also run the FP64 recursion, two adjacent ring pairs per
+// lane, so the hi (and lo) halves of a lane's pairs pack into one dword LDS
+// store.  K-parts 1 .. Q-1 hand their tiles to K-part 0 through LDS.
 // Reference counterpart: the per-(m, l) ring sum of sht_raw.py:53 /
 // sht_fallback.py:63.
 // ---------------------------------------------------------------------------
@@ -695,8 +696,9 @@ template <int Q> struct H16Geom {
     static constexpr int COLB = 2 * P + 16;       // bytes per tile column: P halves + a 16-byte skew
     static constexpr int PARB = 16 * COLB;
     static constexpr int TILEB = 2 * PARB;        // bytes of the hi (or lo) tile
-    static constexpr int XCH = 4 * 256;           // floats: the K-halves' exchange (Q = 2)
-    static constexpr size_t LDS = 2 * (size_t)TILEB + XCH * sizeof(float) + 2 * ANA_L * sizeof(double) + 16;
+    static constexpr int XCH = (Q > 1 ? Q - 1 : 1) * 4 * 256;   // floats: K-parts 1.. hand over to K-part 0
+    static constexpr size_t LDS =
+        2 * (size_t)TILEB + XCH * sizeof(float) + 2 * ANA_L * sizeof(double) + (Q + 1) * sizeof(int);
 };
 
 
@@ -889,17 +891,17 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
         floatx4 D = D0 + D1;
 #pragma unroll
         for (int r = 0; r < 4; ++r) D[r] *= usc[r];
-        // 3. K-halves: the second hands its tile to the first through LDS
-        if (Q == 2 && kh == 1) {
+        // 3. K-parts 1 .. Q-1 hand their tiles to K-part 0 through LDS
+        if (Q > 1 && kh > 0) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) xch[((t * 2 + par) * 4 + r) * 64 + lane] = D[r];
+            for (int r = 0; r < 4; ++r) xch[((((kh - 1) * 2 + t) * 2 + par) * 4 + r) * 64 + lane] = D[r];
         }
         __syncthreads();
         if (kh == 0) {
-            if (Q == 2) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) D[r] += xch[((t * 2 + par) * 4 + r) * 64 + lane];
-            }
+            for (int k2 = 1; k2 < Q; ++k2)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) D[r] += xch[((((k2 - 1) * 2 + t) * 2 + par) * 4 + r) * 64 + lane];
             // D[row][col]: lane = col + 16 (row/4), reg = row%4; row = n - 16 t, col -> l
             const int l = c0 + 2 * (lane & 15) + par;
             if (l <= lmax) {
@@ -1309,9 +1311,9 @@ int launch_band_analysis_mfma(const Plan *p, int nmaps, const void *band, void *
 // operand format of the matrix-core kernels (split fp16 unless set)
 int mfma_kind_of(const Plan *p) { return p->tile_mfma_kind > 0 ? p->tile_mfma_kind : HBX_MFMA_F16X2; }
 
-// waves per matrix-core analysis workgroup: 4 or 8 (64 ring pairs per wave),
-// or 12 (FP32 only: the pipelined kernel, 512 ring pairs, 8 MFMA + 4
-// recursion waves)
+// waves per matrix-core analysis workgroup: 4, 8 or (split fp16) 16, 64
+// ring pairs per wave; or 12 (FP32 only: the pipelined kernel, 512 ring
+// pairs, 8 MFMA + 4 recursion waves)
 int mfma_analysis_waves(const Plan *p) {
     if (p->tile_mfma_waves > 0) return p->tile_mfma_waves;
     return p->npairs <= 256 ? 4 : 8;
@@ -1347,7 +1349,8 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
         switch (W) {
         case 4: HBX_AH(1); break;
         case 8: HBX_AH(2); break;
-        default: return set_error(HBX_EUNSUPPORTED, "split-fp16 analysis waves must be 4 or 8");
+        case 16: HBX_AH(4); break;
+        default: return set_error(HBX_EUNSUPPORTED, "split-fp16 analysis waves must be 4, 8 or 16");
         }
 #undef HBX_AH
     } else switch (W) {
@@ -1382,6 +1385,9 @@ int prepare_mfma_kernels() {
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_h16_kernel<2>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)H16Geom<2>::LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_h16_kernel<4>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)H16Geom<4>::LDS);
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_mfma_pipe_kernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)PipeGeom::LDS);
