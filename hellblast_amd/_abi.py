@@ -74,7 +74,7 @@ class hbx_plan_info(ctypes.Structure):
 class hbx_tiles(ctypes.Structure):
     _fields_ = [("valu_slots", ctypes.c_int32), ("mfma_waves", ctypes.c_int32),
                 ("mfma_min_maps", ctypes.c_int32), ("spin_slots", ctypes.c_int32),
-                ("mfma_kind", ctypes.c_int32)]
+                ("mfma_kind", ctypes.c_int32), ("mfma_groups", ctypes.c_int32)]
 
 
 _vp = ctypes.c_void_p

@@ -116,7 +116,8 @@ static uint64_t spin_ws_bytes(const Plan *p, int npol) {
 
 static uint64_t ws_bytes(const Plan *p, int nmaps) {
     if (p->spin != 0) return spin_ws_bytes(p, nmaps);
-    const uint64_t scaled = align256((uint64_t)nmaps * scaled_elems_per_map(p) * csize(p));
+    uint64_t scaled = align256((uint64_t)nmaps * scaled_elems_per_map(p) * csize(p));
+    if (p->precision == HBX_F32) scaled = std::max(scaled, align256(syn_prep_bytes(p, nmaps)));
     const uint64_t ana = analysis_ws(p, nmaps).total;
     return ws_tail_offset(p, nmaps) + std::max(scaled, ana);
 }
@@ -213,6 +214,7 @@ static int build_plan(Plan *p, const hbx_geom *g) {
     const double tol = p->precision == HBX_F64 ? -64.0 : -40.0;
     if ((rc = plan_build_recursion(p, px, psin, tol))) return rc;
     if (p->precision == HBX_F32 && p->spin == 0 && (rc = prepare_mfma_kernels())) return rc;
+    if (p->precision == HBX_F32 && p->spin == 0 && (rc = plan_build_syn_chunks(p))) return rc;
     return 0;
 }
 
@@ -372,6 +374,7 @@ int hbx_plan_set_tiles(hbx_plan *plan, const hbx_tiles *t) {
         return set_error(HBX_EINVAL, "mfma_kind must be 0, HBX_MFMA_F32 or HBX_MFMA_F16X2");
     if (t->mfma_waves == 12 && t->mfma_kind != HBX_MFMA_F32)
         return set_error(HBX_EINVAL, "mfma_waves 12 needs mfma_kind HBX_MFMA_F32");
+    if (t->mfma_groups < 0 || t->mfma_groups > 2) return set_error(HBX_EINVAL, "mfma_groups must be 0, 1 or 2");
     if (t->mfma_waves == 16 && t->mfma_kind == HBX_MFMA_F32)
         return set_error(HBX_EINVAL, "mfma_waves 16 needs the split-fp16 matrix-core kernels");
     Plan &p = plan->impl;
@@ -380,6 +383,7 @@ int hbx_plan_set_tiles(hbx_plan *plan, const hbx_tiles *t) {
     p.tile_mfma_min_maps = t->mfma_min_maps;
     p.tile_spin_slots = t->spin_slots;
     p.tile_mfma_kind = t->mfma_kind;
+    p.tile_syn_groups = t->mfma_groups;
     return HBX_OK;
 }
 
@@ -391,6 +395,7 @@ int hbx_plan_get_tiles(const hbx_plan *plan, hbx_tiles *t) {
     t->mfma_min_maps = mfma_min_maps(&p);
     t->spin_slots = spin_slots_of(&p);
     t->mfma_kind = mfma_kind_of(&p);
+    t->mfma_groups = p.tile_syn_groups > 0 ? p.tile_syn_groups : 1;
     return HBX_OK;
 }
 

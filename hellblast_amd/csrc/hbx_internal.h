@@ -70,6 +70,8 @@ struct Plan {
     double *d_scale_ana = nullptr;         // weight_r / nphi_r
     double *d_scale_adj = nullptr;         // weight_r / nphi_r (map2alm adjoint: omega_r)
     int64_t *d_vofs = nullptr;             // per ring offset into chirp spectra (-1: pow2)
+    int64_t *d_cofs = nullptr;             // [mmax + 2] prepared synthesis chunks of row m start here
+    int64_t syn_chunks = 0;
     double2 *d_rot = nullptr;              // [4][nrings] FP64 rotation steps of the ring kernels
     // ring pairs (device)
     int32_t *d_pair_n = nullptr, *d_pair_s = nullptr;
@@ -96,6 +98,7 @@ struct Plan {
     int tile_mfma_min_maps = 0;   // FP32 batches of at least this many maps use the matrix cores
     int tile_spin_slots = 0;      // spin analysis ring-pair slots per lane (1, 2, 4)
     int tile_mfma_kind = 0;       // HBX_MFMA_F32 or HBX_MFMA_F16X2 (default)
+    int tile_syn_groups = 0;      // 16-map groups per split-fp16 synthesis workgroup (1, 2; 0 = 1)
 };
 
 // launchers implemented in ring_fft.hip / legendre.hip
@@ -128,10 +131,12 @@ int launch_prescale(const Plan *p, int nmaps, const void *alm, int64_t alm_strid
                     void *scaled, int adjoint, hipStream_t s);
 int launch_leg_synthesis(const Plan *p, int nmaps, const void *scaled, void *phase,
                          hipStream_t s);
-// ws: nmaps * (mmax + 1) ints (the split-fp16 kernel's per-(map, m) scales)
+// ws: syn_prep_bytes (the split-fp16 kernel's prepared operands and scales)
 int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t alm_stride,
                               void *phase, void *ws, int adjoint, hipStream_t s);
 int mfma_kind_of(const Plan *p);   // HBX_MFMA_F32 or HBX_MFMA_F16X2
+int plan_build_syn_chunks(Plan *p);
+uint64_t syn_prep_bytes(const Plan *p, int nmaps);   // split-fp16 synthesis workspace
 int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *alm, int64_t alm_stride,
                              void *partial, int adjoint, hipStream_t s);
 int launch_group_reduce(const Plan *p, int nmaps, void *partial, void *alm, int64_t alm_stride,

@@ -232,43 +232,128 @@ constexpr int SH_WAVES = 4;                     // waves per workgroup (same m),
 constexpr int SH_LCHUNK = 64;                   // l staged per chunk (two 32-l steps)
 constexpr int SH_BCOL = 40;                     // halves per staged column: 32 + an 8-half skew
 
-// k_b(m) such that max_l |s_lm a_lm fadj| 2^k lies in [2^13, 2^14); one block per (m, map)
+// Synthesis operands, prepared once per call: per (m, group of 16 maps) the
+// scaled alm rows 2^k_b s_lm a_lm fadj (k_b: the row maximum lands in
+// [2^13, 2^14)) as fp16 hi/lo pairs, in 64-l chunks laid out as the kernel
+// stages them, [hi/lo][parity][n = 2 map + re/im][same-parity l] (8 KB per
+// chunk); chunk j of row m starts at l = m + 64 j, at chunk index cofs[m] + j.
+constexpr int SH_CHUNK_HALVES = 2 * 2 * 32 * 32;
+
 __global__ void __launch_bounds__(256)
-syn_exp_kernel(const float2 *__restrict__ alm, int64_t astride, const int64_t *__restrict__ rofs,
-               const double *__restrict__ sl, int lmax, int mmax, int adjoint, int *__restrict__ kexp) {
-    __shared__ float red[4];
-    const int m = blockIdx.x, b = blockIdx.y;
+syn_prep_kernel(const float2 *__restrict__ alm, int64_t astride, const int64_t *__restrict__ rofs,
+                const double *__restrict__ sl, const int64_t *__restrict__ cofs, int64_t nchunks, int lmax,
+                int mmax, int nmaps, int adjoint, float neg1, _Float16 *__restrict__ prep, int *__restrict__ kexp) {
+    __shared__ float red[4][16];
+    __shared__ float scs[16];
+    __shared__ __align__(16) _Float16 cbuf[SH_CHUNK_HALVES];
+    const int m = blockIdx.x, g = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
     const int64_t i0 = alm_index(m, m, lmax) - m;
     const double *sv = sl + rofs[m] - m;
-    const float fadj = (adjoint && m > 0) ? 0.5f : 1.0f;
-    float mx = 0.0f;
-    for (int l = m + (int)threadIdx.x; l <= lmax; l += blockDim.x) {
-        const float2 a = alm[(int64_t)b * astride + i0 + l];
-        mx = fmaxf(mx, fmaxf(fabsf(a.x), fabsf(a.y)) * (float)sv[l] * fadj);
-    }
+    const double fadj = (adjoint && m > 0) ? 0.5 : 1.0;
+    const int nb = min(16, nmaps - g * 16);          // maps of this group
+    const float2 *arow = alm + (int64_t)(g * 16) * astride + i0;
+    // 1. row maxima of the 16 maps (coalesced along l)
+    {
+        float mx[16];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+        for (int bi = 0; bi < 16; ++bi) mx[bi] = 0.0f;
+        for (int l = m + tid; l <= lmax; l += blockDim.x) {
+            const float f = (float)(sv[l] * fadj);
+#pragma unroll
+            for (int bi = 0; bi < 16; ++bi)
+                if (bi < nb) {
+                    const float2 a = arow[(int64_t)bi * astride + l];
+                    mx[bi] = fmaxf(mx[bi], fmaxf(fabsf(a.x), fabsf(a.y)) * f);
+                }
+        }
+#pragma unroll
+        for (int bi = 0; bi < 16; ++bi) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) mx[bi] = fmaxf(mx[bi], __shfl_xor(mx[bi], o, 64));
+            if (lane == 0) red[tid >> 6][bi] = mx[bi];
+        }
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) mx = fmaxf(mx, red[i]);
+    if (tid < 16) {
+        const float mx = fmaxf(fmaxf(red[0][tid], red[1][tid]), fmaxf(red[2][tid], red[3][tid]));
         int e = 0;
         (void)frexpf(mx, &e);
-        kexp[(int64_t)b * (mmax + 1) + m] = mx > 0.0f ? min(max(14 - e, -100), 100) : 0;
+        const int k = mx > 0.0f ? min(max(14 - e, -100), 100) : 0;
+        scs[tid] = tid < nb ? ldexpf(1.0f, k) : 0.0f;
+        if (tid < nb) kexp[(int64_t)(g * 16 + tid) * (mmax + 1) + m] = k;
     }
+    __syncthreads();
+    // 2. per 64-l chunk: element e = tid + 256 q -> (map e / 64, l = c0 + e % 64), split into
+    // the LDS chunk [hl][par][n][kk], then 16-byte pieces to the prepared buffer
+    const int nch = (lmax - m) / SH_LCHUNK + 1;
+    _Float16 *rowp = prep + ((int64_t)g * nchunks + cofs[m]) * SH_CHUNK_HALVES;
+    for (int j = 0; j < nch; ++j) {
+        const int c0 = m + SH_LCHUNK * j;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q, bi = e >> 6, ll = e & 63, l = c0 + ll;
+            float2v v = {0.0f, 0.0f};
+            if (l <= lmax && bi < nb) {
+                const float2 a = arow[(int64_t)bi * astride + l];
+                const double f = sv[l] * fadj;
+                v[0] = (float)(a.x * f) * scs[bi];
+                v[1] = (float)(a.y * f) * scs[bi];
+            }
+            const half2v h = __builtin_convertvector(v, half2v);
+            const half2v lo = split_lo(v, h, neg1);
+            const int par = ll & 1, kk = ll >> 1;
+            cbuf[((0 * 2 + par) * 32 + 2 * bi) * 32 + kk] = h[0];
+            cbuf[((0 * 2 + par) * 32 + 2 * bi + 1) * 32 + kk] = h[1];
+            cbuf[((1 * 2 + par) * 32 + 2 * bi) * 32 + kk] = lo[0];
+            cbuf[((1 * 2 + par) * 32 + 2 * bi + 1) * 32 + kk] = lo[1];
+        }
+        __syncthreads();
+        _Float16 *cp = rowp + (int64_t)j * SH_CHUNK_HALVES;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int pc = tid + 256 * q;
+            *reinterpret_cast<uint4 *>(cp + 8 * pc) = *reinterpret_cast<const uint4 *>(cbuf + 8 * pc);
+        }
+        __syncthreads();
+    }
+}
+
+// per-m chunk offsets of the prepared operands (plan time)
+int plan_build_syn_chunks(Plan *p) {
+    std::vector<int64_t> cofs(p->mmax + 2, 0);
+    for (int m = 0; m <= p->mmax; ++m) cofs[m + 1] = cofs[m] + ((p->lmax - m) / SH_LCHUNK + 1);
+    p->syn_chunks = cofs[p->mmax + 1];
+    hipError_t e = hipMalloc(&p->d_cofs, cofs.size() * sizeof(int64_t));
+    if (e != hipSuccess) return set_error(HBX_ENOMEM, "hipMalloc failed for synthesis chunk offsets");
+    p->allocations.push_back(p->d_cofs);
+    p->device_bytes += cofs.size() * sizeof(int64_t);
+    HBX_HIP(hipMemcpy(p->d_cofs, cofs.data(), cofs.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// workspace of the split-fp16 synthesis: prepared operands | k_b(m)
+uint64_t syn_prep_bytes(const Plan *p, int nmaps) {
+    const uint64_t ng = (uint64_t)(nmaps + 15) / 16;
+    const uint64_t a = (ng * (uint64_t)p->syn_chunks * SH_CHUNK_HALVES * 2 + 255) & ~uint64_t(255);
+    return a + (uint64_t)nmaps * (p->mmax + 1) * sizeof(int);
 }
 
 __device__ __forceinline__ uint32_t h2bits(half2v h) { return __builtin_bit_cast(uint32_t, h); }
 
+// NG groups of 16 maps per workgroup share each recursion (NG = 2: twice the
+// matrix work per lambda value; the accumulators double)
+template <int NG>
 __global__ void __launch_bounds__(SH_WAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 4)))
 leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float *__restrict__ phase,
                          const double *__restrict__ px, const int64_t *__restrict__ rofs,
                          const double *__restrict__ alpha, const double *__restrict__ sl,
                          const int32_t *__restrict__ lseed, const double2 *__restrict__ seed, int npairs, int nr,
-                         int lmax, int mmax, int nmb, int nsb, int nmaps, int adjoint, const int *__restrict__ kexp,
+                         int lmax, int mmax, int nmb, int nsb, int nmaps, const int *__restrict__ kexp,
+                         const _Float16 *__restrict__ prep, const int64_t *__restrict__ cofs, int64_t nchunks,
                          float neg1) {
+    constexpr int NC = 32 * NG;                     // n columns: NG x 16 maps x (re, im)
     // staged B operands: [hi/lo][parity][n][same-parity l of the chunk]
-    __shared__ __align__(16) _Float16 bst[2][2][32][SH_BCOL];
+    __shared__ __align__(16) _Float16 bst[2][2][NC][SH_BCOL];
     __shared__ double alds[SH_LCHUNK];
     __shared__ int wg_start;
     int mb, item;
@@ -276,7 +361,7 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
     const int m = mb * MB + item / nsb;
     const int sb = item % nsb;
     if (m > mmax) return;
-    const int g = blockIdx.y;                       // group of 16 maps
+    const int g0 = blockIdx.y * 16 * NG;            // first map of the workgroup
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int r = lane & 31, h = lane >> 5;
@@ -302,51 +387,52 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
 
     const int64_t ro = rofs[m];
     const double *al = alpha + ro - m;      // al[l] = alpha_lm (zero-padded past lmax)
-    const double *sv = sl + ro - m;
-    const double fadj = (adjoint && m > 0) ? 0.5 : 1.0;   // map2alm transpose: g/c_m
-    const int64_t i0 = alm_index(m, m, lmax) - m;   // alm[b][i0 + l]
-    // staging slots of this thread: l = c0 + (tid & 63), maps b = g*16 + tid/64 + 4 q
-    constexpr int PER = 16 / SH_WAVES;
-    const int sll = tid & 63, sb0 = tid >> 6;
-    const int spar = sll & 1, skk = sll >> 1;
-    float ssc[PER];
+    // staging: the prepared chunk of each map group, 16-byte pieces (tid + 256 i)
+    constexpr int PC = 2 * NG;                      // pieces per thread per chunk
+    const int ngr = (nmaps + 15) / 16;
+    uint4 pf[PC];
+    auto fetch = [&](int c0) {
+        const int64_t j = cofs[m] + (c0 - m) / SH_LCHUNK;
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int b = g * 16 + sb0 + SH_WAVES * q;
-        ssc[q] = b < nmaps ? ldexpf(1.0f, kexp[(int64_t)b * (mmax + 1) + m]) : 0.0f;
-    }
-    const int bo = g * 16 + (r >> 1);
-    const float usc = bo < nmaps ? ldexpf(1.0f, -kexp[(int64_t)bo * (mmax + 1) + m]) : 0.0f;
-
-    floatx16 De[2], Do[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        De[j] = floatx16{0};
-        Do[j] = floatx16{0};
-    }
-    for (int c0 = lstart; c0 <= lmax; c0 += SH_LCHUNK) {
-        {
-            const int l = c0 + sll;
-#pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                const int b = g * 16 + sb0 + SH_WAVES * q;
-                float2v v = {0.0f, 0.0f};
-                if (l <= lmax && b < nmaps) {
-                    const float2 av = alm[(int64_t)b * astride + i0 + l];
-                    const double f = sv[l] * fadj;
-                    v[0] = (float)(av.x * f) * ssc[q];
-                    v[1] = (float)(av.y * f) * ssc[q];
-                }
-                const half2v hv = __builtin_convertvector(v, half2v);
-                const half2v lv = split_lo(v, hv, neg1);
-                const int n = 2 * (sb0 + SH_WAVES * q);
-                bst[0][spar][n][skk] = hv[0];
-                bst[0][spar][n + 1][skk] = hv[1];
-                bst[1][spar][n][skk] = lv[0];
-                bst[1][spar][n + 1][skk] = lv[1];
-            }
-            if (tid < SH_LCHUNK) alds[tid] = (c0 + tid <= lmax) ? al[c0 + tid] : 0.0;
+        for (int i = 0; i < PC; ++i) {
+            const int q = tid + 256 * i, u = q >> 9, qq = q & 511;
+            const int gi = blockIdx.y * NG + u;
+            pf[i] = gi < ngr ? *reinterpret_cast<const uint4 *>(prep + ((int64_t)gi * nchunks + j) * SH_CHUNK_HALVES +
+                                                                 8 * qq)
+                             : make_uint4(0u, 0u, 0u, 0u);
         }
+    };
+    auto put = [&]() {
+#pragma unroll
+        for (int i = 0; i < PC; ++i) {
+            const int q = tid + 256 * i, u = q >> 9, qq = q & 511;
+            const int kb = qq & 3, n = (qq >> 2) & 31, par = (qq >> 7) & 1, hl = qq >> 8;
+            *reinterpret_cast<uint4 *>(&bst[hl][par][32 * u + n][8 * kb]) = pf[i];
+        }
+    };
+    float usc[NG];
+#pragma unroll
+    for (int u = 0; u < NG; ++u) {
+        const int bo = g0 + 16 * u + (r >> 1);
+        usc[u] = bo < nmaps ? ldexpf(1.0f, -kexp[(int64_t)bo * (mmax + 1) + m]) : 0.0f;
+    }
+
+    floatx16 De[2][NG], Do[2][NG];                  // [ring tile][n tile]
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int u = 0; u < NG; ++u) {
+            De[j][u] = floatx16{0};
+            Do[j][u] = floatx16{0};
+        }
+    // chunks start at l = m + 64 j (the prepared layout); the first one holds
+    // every pair's switch-on
+    const int lst = lstart <= lmax ? m + (lstart - m) / SH_LCHUNK * SH_LCHUNK : lmax + 1;
+    if (lst <= lmax) fetch(lst);
+    for (int c0 = lst; c0 <= lmax; c0 += SH_LCHUNK) {
+        put();
+        if (tid < SH_LCHUNK) alds[tid] = (c0 + tid <= lmax) ? al[c0 + tid] : 0.0;
+        if (c0 + SH_LCHUNK <= lmax) fetch(c0 + SH_LCHUNK);     // in flight during this chunk
         __syncthreads();
         if (c0 + SH_LCHUNK > wstart) {
 #pragma unroll
@@ -367,7 +453,6 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
                     u1 = u;
                     v[j] = (float)u;
                 }
-
                 // split: block b, parity par holds l0 + 16 b + 2 i + par, i = 0..7
                 uint32_t ah[2][2][4], alo[2][2][4];     // [par][block][dword]
 #pragma unroll
@@ -401,19 +486,23 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
                     Al[par][0] = __builtin_bit_cast(half8s, t0l);
                     Al[par][1] = __builtin_bit_cast(half8s, t1l);
                 }
-                // B: column r, same-parity l 16 s + 8 h .. +7 of the chunk
-                const half8s be_h = *reinterpret_cast<const half8s *>(&bst[0][0][r][16 * s + 8 * h]);
-                const half8s be_l = *reinterpret_cast<const half8s *>(&bst[1][0][r][16 * s + 8 * h]);
-                const half8s bo_h = *reinterpret_cast<const half8s *>(&bst[0][1][r][16 * s + 8 * h]);
-                const half8s bo_l = *reinterpret_cast<const half8s *>(&bst[1][1][r][16 * s + 8 * h]);
 #pragma unroll
-                for (int T = 0; T < 2; ++T) {
-                    De[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[0][T], be_h, De[T], 0, 0, 0);
-                    Do[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[1][T], bo_h, Do[T], 0, 0, 0);
-                    De[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[0][T], be_l, De[T], 0, 0, 0);
-                    Do[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[1][T], bo_l, Do[T], 0, 0, 0);
-                    De[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al[0][T], be_h, De[T], 0, 0, 0);
-                    Do[T] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al[1][T], bo_h, Do[T], 0, 0, 0);
+                for (int u = 0; u < NG; ++u) {
+                    // B: column 32 u + r, same-parity l 16 s + 8 h .. +7 of the chunk
+                    const int cn = 32 * u + r;
+                    const half8s be_h = *reinterpret_cast<const half8s *>(&bst[0][0][cn][16 * s + 8 * h]);
+                    const half8s be_l = *reinterpret_cast<const half8s *>(&bst[1][0][cn][16 * s + 8 * h]);
+                    const half8s bo_h = *reinterpret_cast<const half8s *>(&bst[0][1][cn][16 * s + 8 * h]);
+                    const half8s bo_l = *reinterpret_cast<const half8s *>(&bst[1][1][cn][16 * s + 8 * h]);
+#pragma unroll
+                    for (int T = 0; T < 2; ++T) {
+                        De[T][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[0][T], be_h, De[T][u], 0, 0, 0);
+                        Do[T][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[1][T], bo_h, Do[T][u], 0, 0, 0);
+                        De[T][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[0][T], be_l, De[T][u], 0, 0, 0);
+                        Do[T][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah[1][T], bo_l, Do[T][u], 0, 0, 0);
+                        De[T][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al[0][T], be_h, De[T][u], 0, 0, 0);
+                        Do[T][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al[1][T], bo_h, Do[T][u], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -422,21 +511,24 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
 
     // D[row = ring][col = n]: this lane holds col = r, rows (q%4) + 8 (q/4) + 4 h.
     // Ring pair p couples ring p (north) and ring nr-1-p (south).
-    const int b = g * 16 + (r >> 1), c = r & 1;
-    if (b >= nmaps) return;
-    float *out = phase + (((int64_t)b * nmb + mb) * nr) * (2 * MB) + (m & 7) * 2 + c;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int base = pbase + 32 * j;
+    for (int u = 0; u < NG; ++u) {
+        const int b = g0 + 16 * u + (r >> 1), c = r & 1;
+        if (b >= nmaps) continue;
+        float *out = phase + (((int64_t)b * nmb + mb) * nr) * (2 * MB) + (m & 7) * 2 + c;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int row = (q & 3) + 8 * (q >> 2) + 4 * h;
-            const int pp = base + row;
-            if (pp >= npairs) continue;
-            const float ge = De[j][q] * usc, go = Do[j][q] * usc;
-            const int rn = pp, rs = nr - 1 - pp;
-            out[(int64_t)rn * (2 * MB)] = ge + go;
-            if (rs != rn) out[(int64_t)rs * (2 * MB)] = ge - go;
+        for (int j = 0; j < 2; ++j) {
+            const int base = pbase + 32 * j;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int row = (q & 3) + 8 * (q >> 2) + 4 * h;
+                const int pp = base + row;
+                if (pp >= npairs) continue;
+                const float ge = De[j][u][q] * usc[u], go = Do[j][u][q] * usc[u];
+                const int rn = pp, rs = nr - 1 - pp;
+                out[(int64_t)rn * (2 * MB)] = ge + go;
+                if (rs != rn) out[(int64_t)rs * (2 * MB)] = ge - go;
+            }
         }
     }
 }
@@ -448,12 +540,26 @@ int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t
     dim3 grid(nmb8 * MB * nsb, (nmaps + 15) / 16);
     timer_mark(HBX_TIMER_LEGENDRE_SYNTHESIS, true, s);
     if (mfma_kind_of(p) == HBX_MFMA_F16X2) {
-        int *kexp = static_cast<int *>(ws);          // [nmaps][mmax + 1]
-        hipLaunchKernelGGL(syn_exp_kernel, dim3(p->mmax + 1, nmaps), dim3(256), 0, s, (const float2 *)alm, astride,
-                           p->d_rofs, p->d_sl, p->lmax, p->mmax, adjoint, kexp);
-        hipLaunchKernelGGL(leg_synthesis_h16_kernel, grid, dim3(SH_WAVES * 64), 0, s, (const float2 *)alm, astride,
-                           (float *)phase, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs,
-                           p->nrings, p->lmax, p->mmax, p->nmb, nsb, nmaps, adjoint, kexp, -1.0f);
+        // ws: prepared operands | k_b(m) [nmaps][mmax + 1]
+        _Float16 *prep = static_cast<_Float16 *>(ws);
+        const uint64_t pbytes = syn_prep_bytes(p, nmaps) - (uint64_t)nmaps * (p->mmax + 1) * sizeof(int);
+        int *kexp = reinterpret_cast<int *>(static_cast<char *>(ws) + pbytes);
+        hipLaunchKernelGGL(syn_prep_kernel, dim3(p->mmax + 1, (nmaps + 15) / 16), dim3(256), 0, s,
+                           (const float2 *)alm, astride, p->d_rofs, p->d_sl, p->d_cofs, p->syn_chunks, p->lmax,
+                           p->mmax, nmaps, adjoint, -1.0f, prep, kexp);
+        // two 16-map groups per workgroup when the batch has more than 16 maps
+        if (nmaps > 16 && p->tile_syn_groups == 2) {
+            dim3 grid2(grid.x, (nmaps + 31) / 32);
+            hipLaunchKernelGGL(leg_synthesis_h16_kernel<2>, grid2, dim3(SH_WAVES * 64), 0, s, (const float2 *)alm,
+                               astride, (float *)phase, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed,
+                               p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, p->nmb, nsb, nmaps, kexp, prep,
+                               p->d_cofs, p->syn_chunks, -1.0f);
+        } else {
+            hipLaunchKernelGGL(leg_synthesis_h16_kernel<1>, grid, dim3(SH_WAVES * 64), 0, s, (const float2 *)alm,
+                               astride, (float *)phase, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed,
+                               p->d_seed, p->npairs, p->nrings, p->lmax, p->mmax, p->nmb, nsb, nmaps, kexp, prep,
+                               p->d_cofs, p->syn_chunks, -1.0f);
+        }
         timer_mark(HBX_TIMER_LEGENDRE_SYNTHESIS, false, s);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_error(e, "leg_synthesis_h16_kernel launch");

@@ -162,7 +162,7 @@ class Plan:
             self._h = None
 
     # ------------------------------------------------------------------
-    TILE_FIELDS = ("valu_slots", "mfma_waves", "mfma_min_maps", "spin_slots", "mfma_kind")
+    TILE_FIELDS = ("valu_slots", "mfma_waves", "mfma_min_maps", "spin_slots", "mfma_kind", "mfma_groups")
 
     def set_tiles(self, **tiles):
         """Set plan tiles (0 = default); unknown names raise.  Changes the

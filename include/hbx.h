@@ -124,11 +124,14 @@ int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
  *                  HBX_MFMA_F32 (FP32 instructions) or HBX_MFMA_F16X2
  *                  (each FP32 operand as an fp16 hi/lo pair, three fp16
  *                  products per FP32 product, FP32 accumulation; the
- *                  default); with HBX_MFMA_F16X2, mfma_waves is 4 or 8 */
+ *                  default); with HBX_MFMA_F16X2, mfma_waves is 4, 8 or 16
+ *   mfma_groups    groups of 16 maps per split-fp16 synthesis workgroup:
+ *                  1 (the default), or 2 (each lambda value feeds twice the
+ *                  matrix work, at two thirds of the occupancy) */
 #define HBX_MFMA_F32 1
 #define HBX_MFMA_F16X2 2
 typedef struct hbx_tiles {
-    int32_t valu_slots, mfma_waves, mfma_min_maps, spin_slots, mfma_kind;
+    int32_t valu_slots, mfma_waves, mfma_min_maps, spin_slots, mfma_kind, mfma_groups;
 } hbx_tiles;
 int hbx_plan_set_tiles(hbx_plan *plan, const hbx_tiles *tiles);
 int hbx_plan_get_tiles(const hbx_plan *plan, hbx_tiles *tiles);

@@ -91,6 +91,24 @@ def test_autotune_caches_its_choice(gpu, tmp_path, monkeypatch):
     plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
 
 
+def test_synthesis_map_groups_agree(gpu):
+    """Two 16-map groups per split-fp16 synthesis workgroup compute the same
+    maps, bit for bit, as one group per workgroup (40 maps: a partial group)."""
+    nside, lmax, n = 64, 128, 40
+    print("[case] synthesis map groups", flush=True)
+    plan = _plan(nside, lmax, "f32")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(4)
+    alm = torch.complex(torch.randn((n, plan.nalm), generator=gen, device="cuda"),
+                        torch.randn((n, plan.nalm), generator=gen, device="cuda"))
+    plan.set_tiles(mfma_groups=1)
+    m1 = plan.alm2map(alm, dtype=torch.float32)
+    plan.set_tiles(mfma_groups=2)
+    m2 = plan.alm2map(alm, dtype=torch.float32)
+    plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
+    assert torch.equal(m1, m2)
+
+
 def _c3_specs(rng):
     """64 maps, nside allocation {2048:1, 1024:2, 512:5, 256:14, 128:42}, shuffled."""
     alloc = {2048: 1, 1024: 2, 512: 5, 256: 14, 128: 42}
