@@ -862,26 +862,35 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
         const int b = g * 16 + (n >> 1), c = n & 1;
         const float *ph = reinterpret_cast<const float *>(phase + (int64_t)min(b, nmaps - 1) * pstride +
                                                           (int64_t)mb * nr * MB + (m & 7)) + c;
-        auto sval = [&](int s, int i) -> float {
+        // the ring value of pair (s, i) this wave loads: the north ring (parity-0
+        // waves) or the south ring (parity-1 waves); the two waves of an
+        // (n-tile, K-part) swap them through the still idle lambda tiles
+        auto own = [&](int s, int i) -> float {
             const int qp = P0 + 256 * kh + 32 * s + 8 * q + i;
             const int qq = min(qp, npairs - 1);
-            const float a = ph[(int64_t)qq * (2 * MB)];
-            const float z = (nr - 1 - qq != qq) ? ph[(int64_t)(nr - 1 - qq) * (2 * MB)] : 0.0f;
-            const bool ok = qp < npairs && b < nmaps;
-            return ok ? (par ? a - z : a + z) : 0.0f;
+            const bool ok = qp < npairs && b < nmaps && (par == 0 || nr - 1 - qq != qq);
+            return ok ? ph[(int64_t)(par == 0 ? qq : nr - 1 - qq) * (2 * MB)] : 0.0f;
         };
-        // one load pass (16 loads in flight at a time), then the row maximum,
-        // then the split in place
+        float *xs = reinterpret_cast<float *>(hsm);     // [W][32][64] per round of 4 K-steps
         float Sf[64];
         float amax = 0.0f;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
+        for (int rd = 0; rd < 2; ++rd) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                Sf[8 * s + i] = sval(s, i);
-                amax = fmaxf(amax, fabsf(Sf[8 * s + i]));
+            for (int s4 = 0; s4 < 4; ++s4) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) xs[(w * 32 + 8 * s4 + i) * 64 + lane] = own(4 * rd + s4, i);
+                __builtin_amdgcn_sched_barrier(0);
             }
-            __builtin_amdgcn_sched_barrier(0);
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 32; ++j) {
+                const float mine = xs[(w * 32 + j) * 64 + lane], other = xs[((w ^ 2) * 32 + j) * 64 + lane];
+                const float a = par == 0 ? mine : other, z = par == 0 ? other : mine;
+                Sf[32 * rd + j] = par ? a - z : a + z;
+                amax = fmaxf(amax, fabsf(Sf[32 * rd + j]));
+            }
+            __syncthreads();
         }
         amax = fmaxf(amax, __shfl_xor(amax, 16, 64));
         amax = fmaxf(amax, __shfl_xor(amax, 32, 64));
