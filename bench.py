@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SHT pairs/s nside2048 lmax4095: FP32-batched vs FP64 at ≤1e-5 rel err"
 PEAK_F32 = 157.3   # TFLOP/s, MI355X FP32 (matrix / packed vector)
+PEAK_F16 = 2500.0  # TFLOP/s, MI355X FP16 matrix, dense
 PEAK_F64 = 78.6    # TFLOP/s, MI355X FP64 vector
 
 
@@ -195,6 +196,7 @@ def main():
               "(multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     import torch
+    from hellblast_amd import _abi
     from hellblast_amd.hp_compat import _plan
     from hellblast_amd.plan import nalm
 
@@ -267,19 +269,31 @@ def main():
     log("FP64 kernel means (ms):", json.dumps({k: round(v, 3) for k, v in kms64.items()}))
 
     # ---------------- roofline of the dominant kernel (FP32 batch, matrix cores)
-    # analysis covers m >= mb (rows m < mb run in the FP64 band); synthesis covers every m
+    # analysis covers m >= mb (rows m < mb run in the FP64 band); synthesis covers every m.
+    # Algorithmic flops: 2 per (ring, l, m, map, re/im) multiply-add.  The split-fp16
+    # kernels run them on the FP16 matrix path as three fp16 products each (hi*hi +
+    # hi*lo + lo*hi), so their peak is the FP16 matrix peak and the executed matrix
+    # flops are three times the algorithmic ones.
     mb = p32.fp64_band_m
     band = mb * (2 * lmax + 1 - mb) // 2 + mb
     flops = {"legendre_analysis": 4.0 * npairs * (nlm - band) * B,
              "legendre_synthesis": 4.0 * npairs * nlm * B}
-    names = {"legendre_analysis": "leg_analysis_mfma_kernel", "legendre_synthesis": "leg_synthesis_mfma_kernel"}
+    split16 = p32.tiles()["mfma_kind"] == _abi.HBX_MFMA_F16X2
+    if split16:
+        names = {"legendre_analysis": "leg_analysis_h16_kernel", "legendre_synthesis": "leg_synthesis_h16_kernel"}
+        peak, per_product = PEAK_F16, 3.0
+    else:
+        names = {"legendre_analysis": "leg_analysis_mfma_kernel", "legendre_synthesis": "leg_synthesis_mfma_kernel"}
+        peak, per_product = PEAK_F32, 1.0
     dom = max(flops, key=lambda k: kms[k])
     achieved = flops[dom] / (kms[dom] * 1e-3) / 1e12
     workload = f"nside{nside}_lmax{lmax}_b{B}"
     traffic = measured_traffic(names[dom], workload)
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_F32, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_F32, 4), "traffic": traffic, "kernel": names[dom],
-                "flops_per_launch": flops[dom], "launch_ms": round(kms[dom], 4)}
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": names[dom],
+                "flops_per_launch": flops[dom], "launch_ms": round(kms[dom], 4),
+                "executed_matrix_tflops": round(per_product * achieved, 3),
+                "executed_matrix_frac": round(per_product * achieved / peak, 4)}
     other = [k for k in flops if k != dom][0]
     other_tf = flops[other] / (kms[other] * 1e-3) / 1e12
     # ring stage: HBM bound; bytes per map per direction = npix*s + Nr*(mmax+1)*2*s (s = 4 for FP32)
@@ -313,6 +327,8 @@ def main():
                    "fp32_over_fp64": round(pairs32 / pairs64, 3),
                    "eps_max_f32_vs_f64": emax, "eps_rms_f32_vs_f64": erms, "eps_max_f32_single_map": emax_single,
                    "map_rel_err_f32": map_err, "gate_1e-5": bool(emax <= 1e-5),
+                   "legendre_matrix_operands": ("fp16 hi/lo pairs, 3 fp16 products per FP32 product, FP32 "
+                                                "accumulation" if split16 else "FP32"),
                    "kernel_ms_f32_batch": {k: round(v, 4) for k, v in kms.items()},
                    "kernel_ms_f64": {k: round(v, 4) for k, v in kms64.items()},
                    f"{other}_tflops_f32": round(other_tf, 3),
