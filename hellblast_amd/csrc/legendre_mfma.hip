@@ -358,8 +358,9 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
     __shared__ int wg_start;
     int mb, item;
     xcd_map(blockIdx.x, MB * nsb, &mb, &item);
-    const int m = mb * MB + item / nsb;
-    const int sb = item % nsb;
+    // the 8 m of one m-block (sharing 64-byte phase lines) run next to each other
+    const int m = mb * MB + item % MB;
+    const int sb = item / MB;
     if (m > mmax) return;
     const int g0 = blockIdx.y * 16 * NG;            // first map of the workgroup
     const int tid = threadIdx.x, lane = tid & 63;
@@ -825,8 +826,9 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
 
     int mb, item;
     xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
-    const int m = mb * MB + item / ngroups;
-    const int grp = item % ngroups;
+    // the 8 m of one m-block (sharing 64-byte phase lines) run next to each other
+    const int m = mb * MB + item % MB;
+    const int grp = item / MB;
     if (m > mmax || m < mbeg) return;
     const int g = blockIdx.y;                         // group of 16 maps
     const int tid = threadIdx.x, lane = tid & 63;
