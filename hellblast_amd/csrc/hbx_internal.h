@@ -49,12 +49,13 @@ __host__ __device__ inline int64_t alm_index(int l, int m, int lmax) {
 }
 inline int ilog2(int64_t v) { int r = 0; while ((int64_t(1) << r) < v) ++r; return r; }
 
-// One FFT size class: rings whose inner power-of-two FFT has size M.
+// One FFT size class: rings whose inner power-of-two FFT has size M, split
+// into rings with h = M (one FFT) and Bluestein rings (h not a power of two).
 struct FftClass {
-    int M = 0, log2M = 0, threads = 0, nb = 0;
-    int nrings = 0;
-    int32_t *d_rings = nullptr;   // ring indices in this class
-    size_t lds_bytes64 = 0, lds_bytes32 = 0;
+    int M = 0, log2M = 0, threads = 0;
+    int nrings_pow2 = 0, nrings_blue = 0;
+    int32_t *d_rings_pow2 = nullptr, *d_rings_blue = nullptr;   // ring indices
+    int64_t tw_off = 0;           // offset of this class's twiddle table
 };
 
 struct Plan {
@@ -86,8 +87,7 @@ struct Plan {
     double *d_beta = nullptr;
     double4 *d_seed4 = nullptr;
     // FFT tables (device)
-    int twn = 0, twshift = 0;
-    double2 *d_tw64 = nullptr; float2 *d_tw32 = nullptr;
+    double2 *d_ftw64 = nullptr; float2 *d_ftw32 = nullptr;   // per-class FFT pass twiddles
     double2 *d_V64 = nullptr;  float2 *d_V32 = nullptr;
     int64_t vtotal = 0;
     std::vector<FftClass> classes;

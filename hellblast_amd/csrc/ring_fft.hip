@@ -7,15 +7,22 @@
 // One workgroup transforms one ring of one map entirely in LDS:
 //   * a real ring of length n = 2h is packed as h complex values
 //     z_j = x_{2j} + i x_{2j+1} (one half-length complex DFT per ring);
-//   * h a power of two  -> radix-4/2 Stockham FFT of size M = h in LDS;
-//   * otherwise         -> Bluestein chirp-z with a power-of-two FFT of size
+//   * h a power of two  -> one FFT of size M = h;
+//   * otherwise         -> Bluestein chirp-z with FFTs of a power-of-two size
 //     M >= 2h-1 and a per-length chirp spectrum precomputed at plan time;
 //   * the half-length spectrum is untangled into X[k], k <= h, and written as
 //     F_r(m) = scale_r * e^{-i m phi0_r} X[m mod n]  for m < mcut_r, else 0,
 //     into the m-blocked ring-major phase buffer [map][m/8][ring][m%8].
-// Rings are grouped into size classes (one launch per power-of-two M) so
-// that each launch reserves exactly the LDS it needs.
+// Rings are grouped into size classes (one launch per power-of-two M and
+// kind, power of two or Bluestein) so that each launch reserves exactly the
+// LDS it needs and runs one code path.
+//
+// The FFT engine keeps 16 points per thread in registers: radix-16 passes
+// (4 radix-2 levels each, one LDS exchange per pass) and one radix-2/4/8
+// pass for the remaining levels, Stockham order, twiddles read from a
+// per-pass table of correctly rounded values.
 #include <algorithm>
+#include <cmath>
 #include <map>
 
 #include "hbx_device.h"
@@ -24,140 +31,268 @@
 namespace hbx {
 
 // ---------------------------------------------------------------------------
-// Twiddles in LDS, indexed relative to the class FFT size M:
-//   RingTw<T>::get(j) = e^{-2 pi i j / M},  0 <= j < M.
-// FP32: the first octant (j <= M/8), correctly rounded (strided from the
-// plan's FP32 table of e^{-2 pi i J / 2^twshift}, rounded once from FP64);
-// the rest of the circle follows by exact swaps and negations, so every
-// twiddle an FP32 transform uses is correctly rounded (the FP32 ring stage's
-// error budget is set by them).  M/8 + 1 entries: 8 KB for 8192-point rings.
-// FP64: a two-level table over 2^twshift, w^J = coarse[J >> f] * fine[J &
-// (2^f - 1)], f = min(7, twshift), the product formed in FP64.
+// FFT engine.  Size M = 2^L, M/16 threads (one for M < 16) hold 16 points
+// each.  Pass s has radix r_s (16 for the first floor(L/4) passes, then
+// 2^(L mod 4) if nonzero) and stride p_s = r_0 ... r_{s-1}; butterfly i
+// (0 <= i < M/r_s), k = i mod p_s, reads x_j = buf[i + j M/r_s], multiplies
+// by w_{r_s p_s}^{+-jk}, takes the r_s-point DFT y and writes y_j to
+// buf[(i - k) r_s + k + j p_s]  (Stockham autosort: natural order in and
+// out).  The first pass (p = 1) needs no twiddles and reads through a
+// caller functor; the last writes through one.
+//
+// LDS index x is padded to x + x/16 so that the stride-16 writes of the
+// first passes spread over the banks: (M + M/16) elements per workgroup.
+//
+// Twiddle table of a class: for each pass s >= 1, p_s rows of r_s entries,
+// row k = w^{0 .. r_s - 1 times k} with w = e^{-2 pi i / (r_s p_s)}, rows
+// 16-byte aligned so a thread loads its row with vector loads.  Entries are
+// computed in long double on the host and rounded once.
 // ---------------------------------------------------------------------------
-constexpr int TW_FINE_LOG = 7;
-inline int tw_fine_log(int twshift) { return twshift < TW_FINE_LOG ? twshift : TW_FINE_LOG; }
-__device__ __forceinline__ int tw_fine_log_d(int twshift) { return min(twshift, TW_FINE_LOG); }
-inline int tw_lds_entries(int twshift) {
-    const int f = tw_fine_log(twshift);
-    return (1 << (twshift - f)) + (1 << f);
+constexpr int FFT_PTS = 16;
+
+__device__ __forceinline__ int pad_idx(int x) { return x + (x >> 4); }
+inline int64_t pad_len(int64_t M) { return M + (M >> 4); }
+
+// threads of a ring workgroup of class M (also the ring kernels' stride)
+inline int class_threads(int M) { return std::max(64, M / FFT_PTS); }
+
+template <typename T> inline size_t ring_lds_bytes(int M) { return (size_t)pad_len(M) * sizeof(c2<T>); }
+
+// a * (-i) forward, a * (+i) inverse
+template <typename T, bool INV> __device__ __forceinline__ c2<T> mul_wi(c2<T> a) {
+    return INV ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
+}
+// a * (cr + i ci) forward, a * conj(cr + i ci) inverse
+template <typename T, bool INV> __device__ __forceinline__ c2<T> mul_c(c2<T> a, T cr, T ci) {
+    if (INV) ci = -ci;
+    return mk<T>(a.x * cr - a.y * ci, a.x * ci + a.y * cr);
 }
 
-template <typename T> struct RingTw;
+template <typename T, bool INV> __device__ __forceinline__ void dft4(c2<T> &x0, c2<T> &x1, c2<T> &x2, c2<T> &x3) {
+    const c2<T> s02 = cadd<T>(x0, x2), d02 = csub<T>(x0, x2);
+    const c2<T> s13 = cadd<T>(x1, x3), d13 = mul_wi<T, INV>(csub<T>(x1, x3));
+    x0 = cadd<T>(s02, s13);
+    x2 = csub<T>(s02, s13);
+    x1 = cadd<T>(d02, d13);
+    x3 = csub<T>(d02, d13);
+}
 
-template <> struct RingTw<float> {
-    const float2 *tab;
-    int qlog;   // log2(M / 4); negative for M <= 2 (only j = 0 occurs)
-    static int entries(int M) { return M >= 8 ? M / 8 + 1 : 1; }
-    static size_t lds_bytes(int M, int) { return (size_t)entries(M) * sizeof(float2); }
-    __device__ static void stage(void *lds, const void *gtab, int M, int log2M, int twshift) {
-        float2 *t = static_cast<float2 *>(lds);
-        const float2 *g = static_cast<const float2 *>(gtab);
-        const int n = M >= 8 ? M / 8 + 1 : 1, sh = twshift - log2M;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) t[i] = g[i << sh];
-    }
-    __device__ RingTw(const void *lds, int log2M, int) : tab(static_cast<const float2 *>(lds)), qlog(log2M - 2) {}
-    __device__ __forceinline__ float2 get(int j) const {
-        if (qlog < 0) return make_float2(1.f, 0.f);
-        const int q = j >> qlog, jr = j & ((1 << qlog) - 1);        // j = q M/4 + jr
-        const bool sw = qlog > 0 && jr > (1 << (qlog - 1));           // second octant
-        const float2 t = tab[sw ? (1 << qlog) - jr : jr];
-        float2 w = sw ? make_float2(-t.y, -t.x) : t;                  // e^{-i pi/2} conj(t)
-        if (q & 1) w = make_float2(w.y, -w.x);                        // times -i
-        if (q & 2) w = make_float2(-w.x, -w.y);                       // times -1
-        return w;
+// r-point DFT in place, natural order in and out
+template <typename T, int R, bool INV> struct Dft;
+
+template <typename T, bool INV> struct Dft<T, 2, INV> {
+    __device__ __forceinline__ static void run(c2<T> *x) {
+        const c2<T> a = x[0];
+        x[0] = cadd<T>(a, x[1]);
+        x[1] = csub<T>(a, x[1]);
     }
 };
 
-template <> struct RingTw<double> {
-    const double2 *tab;
-    int f, nc, sh;
-    static size_t lds_bytes(int, int twshift) { return (size_t)tw_lds_entries(twshift) * sizeof(double2); }
-    __device__ static void stage(void *lds, const void *gtab, int, int, int twshift) {
-        double2 *t = static_cast<double2 *>(lds);
-        const double2 *g = static_cast<const double2 *>(gtab);
-        const int f = tw_fine_log_d(twshift), nc = 1 << (twshift - f), nf = 1 << f;
-        for (int i = threadIdx.x; i < nc + nf; i += blockDim.x) t[i] = (i < nc) ? g[i << f] : g[i - nc];
-    }
-    __device__ RingTw(const void *lds, int log2M, int twshift)
-        : tab(static_cast<const double2 *>(lds)), f(tw_fine_log_d(twshift)), nc(1 << (twshift - f)),
-          sh(twshift - log2M) {}
-    __device__ __forceinline__ double2 get(int j) const {
-        const int J = j << sh;
-        const double2 a = tab[J >> f], b = tab[nc + (J & ((1 << f) - 1))];
-        return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <typename T, bool INV> struct Dft<T, 4, INV> {
+    __device__ __forceinline__ static void run(c2<T> *x) {
+        dft4<T, INV>(x[0], x[1], x[2], x[3]);
     }
 };
 
-// LDS bytes of a ring workgroup: M data elements + the twiddle table
-template <typename T> inline size_t ring_lds_bytes(int M, int twshift) {
-    return (size_t)M * sizeof(c2<T>) + RingTw<T>::lds_bytes(M, twshift);
+// n = 2 n1 + n2, k = k1 + 4 k2: X[k] = sum_n2 W8^{n2 k1} (-1)^{n2 k2} DFT4_n1(x[2 n1 + n2])[k1]
+template <typename T, bool INV> struct Dft<T, 8, INV> {
+    __device__ __forceinline__ static void run(c2<T> *x) {
+        const T h = (T)0.70710678118654752440;
+        dft4<T, INV>(x[0], x[2], x[4], x[6]);
+        dft4<T, INV>(x[1], x[3], x[5], x[7]);
+        const c2<T> b1 = mul_c<T, INV>(x[3], h, -h);
+        const c2<T> b2 = mul_wi<T, INV>(x[5]);
+        const c2<T> b3 = mul_c<T, INV>(x[7], -h, -h);
+        const c2<T> a0 = x[0], a1 = x[2], a2 = x[4], a3 = x[6], b0 = x[1];
+        x[0] = cadd<T>(a0, b0);
+        x[4] = csub<T>(a0, b0);
+        x[1] = cadd<T>(a1, b1);
+        x[5] = csub<T>(a1, b1);
+        x[2] = cadd<T>(a2, b2);
+        x[6] = csub<T>(a2, b2);
+        x[3] = cadd<T>(a3, b3);
+        x[7] = csub<T>(a3, b3);
+    }
+};
+
+// n = 4 n1 + n2, k = k1 + 4 k2: X[k] = DFT4_n2( W16^{n2 k1} DFT4_n1(x[4 n1 + n2])[k1] )[k2]
+template <typename T, bool INV> struct Dft<T, 16, INV> {
+    __device__ __forceinline__ static void run(c2<T> *x) {
+        const T c = (T)0.92387953251128675613, s = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) dft4<T, INV>(x[n2], x[4 + n2], x[8 + n2], x[12 + n2]);
+        // A[n2][k1] sits at x[n2 + 4 k1]; twiddle W16^{n2 k1}
+        x[5] = mul_c<T, INV>(x[5], c, -s);      // 1
+        x[9] = mul_c<T, INV>(x[9], h, -h);      // 2
+        x[13] = mul_c<T, INV>(x[13], s, -c);    // 3
+        x[6] = mul_c<T, INV>(x[6], h, -h);      // 2
+        x[10] = mul_wi<T, INV>(x[10]);          // 4
+        x[14] = mul_c<T, INV>(x[14], -h, -h);   // 6
+        x[7] = mul_c<T, INV>(x[7], s, -c);      // 3
+        x[11] = mul_c<T, INV>(x[11], -h, -h);   // 6
+        x[15] = mul_c<T, INV>(x[15], -c, s);    // 9
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) dft4<T, INV>(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]);
+        // X[k1 + 4 k2] sits at x[4 k1 + k2]: transpose the 4 x 4 index
+        c2<T> y[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) y[q] = x[4 * (q & 3) + (q >> 2)];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) x[q] = y[q];
+    }
+};
+
+// v[j] *= w^{jk} (conjugated for the inverse) from a table row of r entries
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void twiddle_row(c2<T> *v, const c2<T> *__restrict__ row) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 *w4 = reinterpret_cast<const float4 *>(row);
+#pragma unroll
+        for (int q = 0; q < R / 2; ++q) {
+            const float4 w = w4[q];
+            if (q > 0) v[2 * q] = mul_c<T, INV>(v[2 * q], w.x, w.y);
+            v[2 * q + 1] = mul_c<T, INV>(v[2 * q + 1], w.z, w.w);
+        }
+    } else {
+#pragma unroll
+        for (int j = 1; j < R; ++j) {
+            const c2<T> w = row[j];
+            v[j] = mul_c<T, INV>(v[j], w.x, w.y);
+        }
+    }
 }
 
-// ---------------------------------------------------------------------------
-// in-LDS FFT of size M (power of two), forward (e^{-i}) or inverse (e^{+i},
-// unnormalised).  Stockham radix-4 passes plus one radix-2 pass when log2 M
-// is odd; each pass stages its inputs in registers so one buffer suffices.
-// tw: the staged LDS twiddle table (visible to every thread).
-// ---------------------------------------------------------------------------
-template <typename T, int NB>
-__device__ __forceinline__ void lds_fft(c2<T> *buf, int M, int log2M, const RingTw<T> &tw, bool inv) {
-    const int nt = blockDim.x, tid = threadIdx.x;
-    const int Q = M >> 2;
-    int p = 1, lp = 0;
-    const int n4 = log2M >> 1;
-    for (int s = 0; s < n4; ++s) {
-        c2<T> v[NB][4];
+// One pass.  load(q, pq) and post(k, pk, x) get the natural index and its
+// padded LDS index.  Padded addresses advance linearly in j: the read stride
+// M/R and the write stride p of every pass after the first are multiples of
+// 16, and the first pass writes 16 consecutive elements of one 16-block.
+template <typename T, int R, bool INV, bool FIRST, bool LAST, class Load, class Post>
+__device__ __forceinline__ void fft_pass(c2<T> *buf, int M, int lp, const c2<T> *__restrict__ tw, Load &load,
+                                         Post &post) {
+    constexpr int NBF = FFT_PTS / R;                    // butterflies per thread
+    const bool full = M >= FFT_PTS;                     // every FFT thread has NBF butterflies
+    const int ntf = full ? M / FFT_PTS : 1;
+    const int nbv = full ? NBF : 1;
+    const int Mr = M / R, p = 1 << lp;
+    const int sMr = Mr + (Mr >> 4), sp = FIRST ? 1 : p + (p >> 4);
+    const int t = threadIdx.x;
+    c2<T> v[NBF][R];
+    if (t < ntf) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int i = tid + b * nt;
-            if (i < Q) {
+        for (int b = 0; b < NBF; ++b) {
+            if (b < nbv) {
+                const int i = t + b * ntf;
+                if constexpr (FIRST) {
+                    const bool lin = (Mr & 15) == 0;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) v[b][j] = buf[i + j * Q];
+                    for (int j = 0; j < R; ++j) {
+                        const int q = i + j * Mr;
+                        v[b][j] = load(q, lin ? pad_idx(i) + j * sMr : pad_idx(q));
+                    }
+                } else {
+                    const int pi = pad_idx(i);
+#pragma unroll
+                    for (int j = 0; j < R; ++j) v[b][j] = buf[pi + j * sMr];
+                    twiddle_row<T, R, INV>(v[b], tw + (size_t)(i & (p - 1)) * R);
+                }
+                Dft<T, R, INV>::run(v[b]);
             }
         }
-        __syncthreads();
-        const int sh = log2M - lp - 2;
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int i = tid + b * nt;
-            if (i < Q) {
-                const int k = i & (p - 1);
-                c2<T> w1 = tw.get(k << sh), w2 = tw.get((2 * k) << sh), w3 = tw.get((3 * k) << sh);
-                if (inv) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
-                const c2<T> x0 = v[b][0];
-                const c2<T> x1 = cmul<T>(v[b][1], w1);
-                const c2<T> x2 = cmul<T>(v[b][2], w2);
-                const c2<T> x3 = cmul<T>(v[b][3], w3);
-                const c2<T> s02 = cadd<T>(x0, x2), d02 = csub<T>(x0, x2);
-                const c2<T> s13 = cadd<T>(x1, x3), d13 = csub<T>(x1, x3);
-                const c2<T> r13 = inv ? cmul_pi<T>(d13) : cmul_mi<T>(d13);
-                const int j0 = ((i - k) << 2) + k;
-                buf[j0] = cadd<T>(s02, s13);
-                buf[j0 + p] = cadd<T>(d02, r13);
-                buf[j0 + 2 * p] = csub<T>(s02, s13);
-                buf[j0 + 3 * p] = csub<T>(d02, r13);
-            }
-        }
-        __syncthreads();
-        p <<= 2;
-        lp += 2;
     }
-    if (log2M & 1) {
-        const int H = M >> 1;  // p == H here
+    __syncthreads();
+    if (t < ntf) {
 #pragma unroll
-        for (int b = 0; b < 2 * NB; ++b) {
-            const int i = tid + b * nt;
-            if (i < H) {
-                c2<T> w = tw.get(i);
-                if (inv) w.y = -w.y;
-                const c2<T> x0 = buf[i];
-                const c2<T> x1 = cmul<T>(buf[i + H], w);
-                buf[i] = cadd<T>(x0, x1);
-                buf[i + H] = csub<T>(x0, x1);
+        for (int b = 0; b < NBF; ++b) {
+            if (b < nbv) {
+                const int i = t + b * ntf;
+                const int k = FIRST ? 0 : (i & (p - 1));
+                const int base = (i - k) * R + k;
+                const int pb = pad_idx(base);
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    if constexpr (LAST) post(base + j * p, pb + j * sp, v[b][j]);
+                    else buf[pb + j * sp] = v[b][j];
+                }
             }
         }
-        __syncthreads();
     }
+    __syncthreads();
+}
+
+// Full FFT of size M = 2^log2M (forward e^{-i}, inverse e^{+i},
+// unnormalised): input from load(q, pq), q < M; output through
+// post(k, pk, X_k); pq, pk: padded LDS indices of q, k.
+// Every thread of the workgroup must call it (it synchronises).
+template <typename T, bool INV, class Load, class Post>
+__device__ __forceinline__ void fft_run(c2<T> *buf, int M, int log2M, const c2<T> *__restrict__ tw, Load load,
+                                        Post post) {
+    const int n16 = log2M >> 2, rem = log2M & 3;
+    if (n16 == 0) {
+        switch (rem) {
+        case 0: {
+            c2<T> v = mk<T>(0, 0);
+            if (threadIdx.x == 0) v = load(0, 0);
+            __syncthreads();
+            if (threadIdx.x == 0) post(0, 0, v);
+            __syncthreads();
+            return;
+        }
+        case 1: fft_pass<T, 2, INV, true, true>(buf, M, 0, tw, load, post); return;
+        case 2: fft_pass<T, 4, INV, true, true>(buf, M, 0, tw, load, post); return;
+        default: fft_pass<T, 8, INV, true, true>(buf, M, 0, tw, load, post); return;
+        }
+    }
+    if (n16 == 1 && rem == 0) {
+        fft_pass<T, 16, INV, true, true>(buf, M, 0, tw, load, post);
+        return;
+    }
+    fft_pass<T, 16, INV, true, false>(buf, M, 0, tw, load, post);
+    int lp = 4;
+    const c2<T> *twp = tw;
+    for (int s = 1; s < n16; ++s) {
+        if (s == n16 - 1 && rem == 0) fft_pass<T, 16, INV, false, true>(buf, M, lp, twp, load, post);
+        else fft_pass<T, 16, INV, false, false>(buf, M, lp, twp, load, post);
+        twp += (size_t)16 << lp;
+        lp += 4;
+    }
+    switch (rem) {
+    case 1: fft_pass<T, 2, INV, false, true>(buf, M, lp, twp, load, post); break;
+    case 2: fft_pass<T, 4, INV, false, true>(buf, M, lp, twp, load, post); break;
+    case 3: fft_pass<T, 8, INV, false, true>(buf, M, lp, twp, load, post); break;
+    default: break;
+    }
+}
+
+// entries of the twiddle table of class M (same pass walk as fft_run)
+inline int64_t fft_table_entries(int M) {
+    const int L = ilog2(M), n16 = L >> 2, rem = L & 3;
+    int64_t e = 0;
+    int lp = 4;
+    for (int s = 1; s < n16; ++s, lp += 4) e += (int64_t)16 << lp;
+    if (n16 >= 1 && rem) e += (int64_t)(1 << rem) << lp;
+    return e;
+}
+
+// fill the table of class M: FP64 and FP32 roundings of the same long double values
+inline void fft_table_fill(int M, double2 *t64, float2 *t32) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    const int L = ilog2(M), n16 = L >> 2, rem = L & 3;
+    int64_t o = 0;
+    auto pass = [&](int r, int lp) {
+        const int64_t p = (int64_t)1 << lp, rp = r * p;
+        for (int64_t k = 0; k < p; ++k)
+            for (int j = 0; j < r; ++j) {
+                const int64_t e = (j * k) % rp;
+                const long double a = -2.0L * pi * (long double)e / (long double)rp;
+                const long double c = cosl(a), s = sinl(a);
+                t64[o] = make_double2((double)c, (double)s);
+                t32[o] = make_float2((float)c, (float)s);
+                ++o;
+            }
+    };
+    int lp = 4;
+    for (int s = 1; s < n16; ++s, lp += 4) pass(16, lp);
+    if (n16 >= 1 && rem) pass(1 << rem, lp);
 }
 
 // chirp c_j = exp(i pi j^2 / h), with j^2 reduced mod 2h exactly (plan time)
@@ -183,57 +318,60 @@ template <typename T> __device__ __forceinline__ c2<T> round_c(double2 a) { retu
 // sincos per thread by these FP64 steps (at most a few dozen per thread).
 constexpr int NROT = 4;
 
+// launch bound: FP64 classes stop at M = 8192 (512 threads) by LDS
+template <typename T> constexpr int ring_max_threads() { return sizeof(T) == 8 ? 512 : 1024; }
+
 // ---------------------------------------------------------------------------
-// analysis: maps -> phase
-// V + vofs[r]: chirp spectrum (M entries) then the chirp c_j, j < h
+// analysis: maps -> phase.  BLUE: Bluestein rings (h not a power of two),
+// V + vofs[r]: chirp spectrum (M entries) then the chirp c_j, j < h.
 // ---------------------------------------------------------------------------
-template <typename T, typename TIn, int NB>
-__global__ void __launch_bounds__(1024)
+template <typename T, typename TIn, bool BLUE>
+__global__ void __launch_bounds__(ring_max_threads<T>())
 ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__restrict__ phase,
                      int64_t phase_stride, const int32_t *__restrict__ rings,
                      const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
                      const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
-                     const double *__restrict__ scale, const double2 *__restrict__ rot, const void *tw,
-                     int twshift, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs, int M, int log2M,
-                     int mmax, int nr) {
+                     const double *__restrict__ scale, const double2 *__restrict__ rot,
+                     const c2<T> *__restrict__ tw, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
+                     int M, int log2M, int mmax, int nr) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
     const int r = rings[blockIdx.x];
     const int b = blockIdx.y;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = nphi[r], h = n >> 1;
-    const bool pow2 = (h == M);
     const TIn *f = maps + (int64_t)b * map_stride + ring_start[r];
-    RingTw<T>::stage(buf + M, tw, M, log2M, twshift);
-    const RingTw<T> twl(buf + M, log2M, twshift);
-    const c2<T> *Vr = pow2 ? nullptr : V + vofs[r];
-    const c2<T> *Cr = pow2 ? nullptr : Vr + M;
-
-    for (int j = tid; j < M; j += nt) {
-        c2<T> z = mk<T>(0, 0);
-        if (j < h) {
-            z = mk<T>(load_as<T>(f + 2 * j), load_as<T>(f + 2 * j + 1));
-            if (!pow2) z = cmul<T>(z, cconj<T>(Cr[j]));
-        }
-        buf[j] = z;
-    }
-    __syncthreads();
-    lds_fft<T, NB>(buf, M, log2M, twl, false);
-    if (!pow2) {
-        for (int k = tid; k < M; k += nt) buf[k] = cmul<T>(buf[k], Vr[k]);
-        __syncthreads();
-        lds_fft<T, NB>(buf, M, log2M, twl, true);
+    auto pixel_pair = [&](int q) { return mk<T>(load_as<T>(f + 2 * q), load_as<T>(f + 2 * q + 1)); };
+    auto lds = [&](int, int pq) { return buf[pq]; };
+    if constexpr (!BLUE) {
+        fft_run<T, false>(
+            buf, M, log2M, tw, [&](int q, int) { return pixel_pair(q); },
+            [&](int, int pk, c2<T> x) { buf[pk] = x; });
+    } else {
+        const c2<T> *Vr = V + vofs[r];
+        const c2<T> *Cr = Vr + M;
+        fft_run<T, false>(
+            buf, M, log2M, tw,
+            [&](int q, int) {
+                c2<T> z = mk<T>(0, 0);
+                if (q < h) z = cmul<T>(pixel_pair(q), cconj<T>(Cr[q]));
+                return z;
+            },
+            [&](int k, int pk, c2<T> x) { buf[pk] = cmul<T>(x, Vr[k]); });
         const T invM = (T)1 / (T)M;
-        for (int k = tid; k < h; k += nt) buf[k] = cscale<T>(cmul<T>(buf[k], cconj<T>(Cr[k])), invM);
-        __syncthreads();
+        fft_run<T, true>(buf, M, log2M, tw, lds, [&](int k, int pk, c2<T> x) {
+            if (k < h) buf[pk] = cscale<T>(cmul<T>(x, cconj<T>(Cr[k])), invM);
+        });
     }
-    // untangle: X[m] = E + W_m O with E = (Z[m] + conj Z[-m]) / 2, O = (Z[m] -
-    // conj Z[-m]) / 2i (Z of period h), W_m = e^{-2 pi i m / n}; then
-    // F(m) = scale e^{-i m phi0} X[m] for m < mcut.
+    // untangle: Z (period h) -> X[m] = E_m + W_m O_m, E_m = (Z[m] + conj Z[-m]) / 2,
+    // O_m = (Z[m] - conj Z[-m]) / 2i, W_m = e^{-2 pi i m / n}; then
+    // F(m) = scale e^{-i m phi0} X[m mod n] for m < mcut, else 0.
     const T sc = (T)(scale ? scale[r] : 1.0);
     const int cut = mcut[r];
     const double ph0 = phi0 ? phi0[r] : 0.0;
     const bool rotp = (ph0 != 0.0);
+    c2<T> *out = phase + (int64_t)b * phase_stride;
+    auto slot = [&](int m) -> c2<T> & { return out[((int64_t)(m >> 3) * nr + r) * MB + (m & 7)]; };
     double2 W, P = make_double2(1.0, 0.0);
     {
         double s, c;
@@ -245,56 +383,85 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
         }
     }
     const double2 Wst = rot[3 * nr + r], Pst = zconj(rot[r]);
-    const int hstep = nt % h;
-    int ia = tid % h;
-    c2<T> *out = phase + (int64_t)b * phase_stride;
-    for (int m = tid; m <= mmax; m += nt) {
-        c2<T> F = mk<T>(0, 0);
-        if (m < cut) {
-            const int ib = (ia == 0) ? 0 : h - ia;
-            const c2<T> za = buf[ia], zb = cconj<T>(buf[ib]);
-            const c2<T> A = cscale<T>(cadd<T>(za, zb), (T)0.5);
-            const c2<T> D = cscale<T>(csub<T>(za, zb), (T)0.5);
-            const c2<T> X = cadd<T>(A, cmul<T>(round_c<T>(W), cmul_mi<T>(D)));
-            F = cscale<T>(X, sc);
-            if (rotp) F = cmul<T>(F, round_c<T>(P));
+    if (cut <= n) {
+        // no alias copies: every m < lim is one of the n bins.  Thread pair k
+        // (0 <= k <= h/2) gives, with T = W_k O_k,
+        //   X[k] = E + T,  X[h-k] = conj(E - T),  X[h+k] = E - T,  X[n-k] = conj(E + T)
+        // (E, O of period h; W_{h+-k} = -W_{-+k}; real input).
+        const int lim = min(cut, mmax + 1);
+        const double2 Ph = zconj(rot[2 * nr + r]), Pn = zconj(rot[nr + r]);
+        auto emit = [&](int m, c2<T> X, double2 Pm) {
+            if (m < lim) {
+                c2<T> F = cscale<T>(X, sc);
+                if (rotp) F = cmul<T>(F, round_c<T>(Pm));
+                slot(m) = F;
+            }
+        };
+        for (int k = tid; 2 * k <= h; k += nt) {
+            const c2<T> za = buf[pad_idx(k)];
+            const c2<T> zb = cconj<T>(buf[pad_idx(k == 0 ? 0 : h - k)]);
+            const c2<T> E = cscale<T>(cadd<T>(za, zb), (T)0.5);
+            const c2<T> O = cmul_mi<T>(cscale<T>(csub<T>(za, zb), (T)0.5));
+            const c2<T> Tk = cmul<T>(round_c<T>(W), O);
+            const c2<T> Xp = cadd<T>(E, Tk), Xm = csub<T>(E, Tk);
+            emit(k, Xp, P);
+            if (2 * k != h) emit(h - k, cconj<T>(Xm), zmul(Ph, zconj(P)));
+            if (k != 0 && 2 * k != h) emit(h + k, Xm, zmul(Ph, P));
+            if (k != 0) emit(n - k, cconj<T>(Xp), zmul(Pn, zconj(P)));
+            W = zmul(W, Wst);
+            if (rotp) P = zmul(P, Pst);
         }
-        out[((int64_t)(m >> 3) * nr + r) * MB + (m & 7)] = F;
-        W = zmul(W, Wst);
-        if (rotp) P = zmul(P, Pst);
-        ia += hstep;
-        if (ia >= h) ia -= h;
+        for (int m = lim + tid; m <= mmax; m += nt) slot(m) = mk<T>(0, 0);
+    } else {
+        // alias copies (mcut > nphi): one m per step, bin m mod n
+        const int hstep = nt % h;
+        int ia = tid % h;
+        for (int m = tid; m <= mmax; m += nt) {
+            c2<T> F = mk<T>(0, 0);
+            if (m < cut) {
+                const int ib = (ia == 0) ? 0 : h - ia;
+                const c2<T> za = buf[pad_idx(ia)], zb = cconj<T>(buf[pad_idx(ib)]);
+                const c2<T> A = cscale<T>(cadd<T>(za, zb), (T)0.5);
+                const c2<T> D = cscale<T>(csub<T>(za, zb), (T)0.5);
+                const c2<T> X = cadd<T>(A, cmul<T>(round_c<T>(W), cmul_mi<T>(D)));
+                F = cscale<T>(X, sc);
+                if (rotp) F = cmul<T>(F, round_c<T>(P));
+            }
+            slot(m) = F;
+            W = zmul(W, Wst);
+            if (rotp) P = zmul(P, Pst);
+            ia += hstep;
+            if (ia >= h) ia -= h;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // synthesis: phase -> maps
 // ---------------------------------------------------------------------------
-template <typename T, typename TOut, int NB>
-__global__ void __launch_bounds__(1024)
+template <typename T, typename TOut, bool BLUE>
+__global__ void __launch_bounds__(ring_max_threads<T>())
 ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOut *__restrict__ maps,
                       int64_t map_stride, const int32_t *__restrict__ rings,
                       const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
                       const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
-                      const double *__restrict__ scale, const double2 *__restrict__ rot, const void *tw,
-                      int twshift, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs, int M,
-                      int log2M, int mmax, int nr) {
+                      const double *__restrict__ scale, const double2 *__restrict__ rot,
+                      const c2<T> *__restrict__ tw, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
+                      int M, int log2M, int mmax, int nr) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
     const int r = rings[blockIdx.x];
     const int b = blockIdx.y;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = nphi[r], h = n >> 1;
-    const bool pow2 = (h == M);
     const int top = min(mmax, mcut[r] - 1);
-    RingTw<T>::stage(buf + M, tw, M, log2M, twshift);
-    const RingTw<T> twl(buf + M, log2M, twshift);
-    const c2<T> *Vr = pow2 ? nullptr : V + vofs[r];
-    const c2<T> *Cr = pow2 ? nullptr : Vr + M;
+    const c2<T> *Vr = BLUE ? V + vofs[r] : nullptr;
+    const c2<T> *Cr = BLUE ? Vr + M : nullptr;
     const double ph0 = phi0 ? phi0[r] : 0.0;
     const bool rotp = (ph0 != 0.0);
     const c2<T> *G = phase + (int64_t)b * phase_stride;
     const double2 Rn = rot[nr + r];
+    auto gload = [&](int m) { return G[((int64_t)(m >> 3) * nr + r) * MB + (m & 7)]; };
 
     // X[k] = sum_{m = k mod n} G'(m) + sum_{m = -k mod n, m >= 1} conj G'(m),
     // G'(m) = G(m) e^{i m phi0} (imaginary part of G(0) dropped); Rk = e^{i k phi0}
@@ -302,7 +469,7 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
         c2<T> acc = mk<T>(0, 0);
         double2 q = Rk;
         for (int m = k; m <= top; m += n) {
-            c2<T> g = G[((int64_t)(m >> 3) * nr + r) * MB + (m & 7)];
+            c2<T> g = gload(m);
             if (m == 0) g.y = 0;
             if (rotp) {
                 g = cmul<T>(g, round_c<T>(q));
@@ -312,7 +479,7 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
         }
         q = zmul(zconj(Rk), Rn);
         for (int m = (k == 0 ? n : n - k); m <= top; m += n) {
-            c2<T> g = G[((int64_t)(m >> 3) * nr + r) * MB + (m & 7)];
+            c2<T> g = gload(m);
             if (rotp) {
                 g = cmul<T>(g, round_c<T>(q));
                 q = zmul(q, Rn);
@@ -333,46 +500,69 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
         }
     }
     const double2 S = rot[r], Rh = rot[2 * nr + r], Wst = zconj(rot[3 * nr + r]);
+    // top < n (no alias copies): bins k and h - k take m in {k, n-k} and {h-k, h+k}
+    const bool direct = top < n;
     for (int k = tid; k <= (h >> 1); k += nt) {
         const int k2 = h - k;
-        const c2<T> Xa = xsum(k, R), Xb = xsum(k2, rotp ? zmul(Rh, zconj(R)) : R);
+        c2<T> Xa, Xb;
+        if (direct) {
+            const c2<T> z0 = mk<T>(0, 0);
+            c2<T> g0 = k <= top ? gload(k) : z0;
+            c2<T> g1 = (k >= 1 && n - k <= top) ? gload(n - k) : z0;
+            c2<T> g2 = k2 <= top ? gload(k2) : z0;
+            c2<T> g3 = h + k <= top ? gload(h + k) : z0;
+            if (k == 0) g0.y = 0;
+            if (rotp) {
+                // R_{n-k} = R_n conj R_k, R_{h-k} = R_h conj R_k, R_{h+k} = R_h R_k
+                g0 = cmul<T>(g0, round_c<T>(R));
+                g1 = cmul<T>(g1, round_c<T>(zmul(Rn, zconj(R))));
+                g2 = cmul<T>(g2, round_c<T>(zmul(Rh, zconj(R))));
+                g3 = cmul<T>(g3, round_c<T>(zmul(Rh, R)));
+            }
+            Xa = cadd<T>(g0, cconj<T>(g1));
+            Xb = cadd<T>(g2, cconj<T>(g3));
+        } else {
+            Xa = xsum(k, R);
+            Xb = xsum(k2, rotp ? zmul(Rh, zconj(R)) : R);
+        }
         const c2<T> Wk = round_c<T>(W);
         const c2<T> Wk2 = mk<T>(-Wk.x, Wk.y);        // e^{2 pi i (h - k)/n} = -conj W_k
         {
             const c2<T> s = cadd<T>(Xa, cconj<T>(Xb)), d = csub<T>(Xa, cconj<T>(Xb));
             c2<T> z = cadd<T>(s, cmul_pi<T>(cmul<T>(Wk, d)));
-            if (!pow2) z = cconj<T>(cmul<T>(z, Cr[k]));
-            buf[k] = z;
+            if (BLUE) z = cconj<T>(cmul<T>(z, Cr[k]));
+            buf[pad_idx(k)] = z;
         }
         if (k != 0 && k2 != k) {
             const c2<T> s = cadd<T>(Xb, cconj<T>(Xa)), d = csub<T>(Xb, cconj<T>(Xa));
             c2<T> z = cadd<T>(s, cmul_pi<T>(cmul<T>(Wk2, d)));
-            if (!pow2) z = cconj<T>(cmul<T>(z, Cr[k2]));
-            buf[k2] = z;
+            if (BLUE) z = cconj<T>(cmul<T>(z, Cr[k2]));
+            buf[pad_idx(k2)] = z;
         }
         W = zmul(W, Wst);
         if (rotp) R = zmul(R, S);
     }
-    if (!pow2)
-        for (int j = h + tid; j < M; j += nt) buf[j] = mk<T>(0, 0);
+    if (BLUE)
+        for (int j = h + tid; j < M; j += nt) buf[pad_idx(j)] = mk<T>(0, 0);
     __syncthreads();
-    if (pow2) {
-        lds_fft<T, NB>(buf, M, log2M, twl, true);
-    } else {
-        lds_fft<T, NB>(buf, M, log2M, twl, false);
-        for (int k = tid; k < M; k += nt) buf[k] = cmul<T>(buf[k], Vr[k]);
-        __syncthreads();
-        lds_fft<T, NB>(buf, M, log2M, twl, true);
-        const T invM = (T)1 / (T)M;
-        for (int j = tid; j < h; j += nt) buf[j] = cscale<T>(cmul<T>(Cr[j], cconj<T>(buf[j])), invM);
-        __syncthreads();
-    }
     const T sc = (T)(scale ? scale[r] : 1.0);
     TOut *f = maps + (int64_t)b * map_stride + ring_start[r];
-    for (int j = tid; j < h; j += nt) {
-        const c2<T> z = buf[j];
-        f[2 * j] = (TOut)(z.x * sc);
-        f[2 * j + 1] = (TOut)(z.y * sc);
+    auto lds = [&](int, int pq) { return buf[pq]; };
+    if constexpr (!BLUE) {
+        fft_run<T, true>(buf, M, log2M, tw, lds, [&](int j, int, c2<T> z) {
+            f[2 * j] = (TOut)(z.x * sc);
+            f[2 * j + 1] = (TOut)(z.y * sc);
+        });
+    } else {
+        fft_run<T, false>(buf, M, log2M, tw, lds, [&](int k, int pk, c2<T> x) { buf[pk] = cmul<T>(x, Vr[k]); });
+        const T invM = (T)1 / (T)M;
+        fft_run<T, true>(buf, M, log2M, tw, lds, [&](int j, int, c2<T> x) {
+            if (j < h) {
+                const c2<T> z = cscale<T>(cmul<T>(Cr[j], cconj<T>(x)), invM);
+                f[2 * j] = (TOut)(z.x * sc);
+                f[2 * j + 1] = (TOut)(z.y * sc);
+            }
+        });
     }
 }
 
@@ -501,66 +691,32 @@ int launch_ring_band(const Plan *p, int nmaps, const void *maps, int map_dtype, 
 // ---------------------------------------------------------------------------
 // plan-time tables
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void twiddle_kernel(c2<T> *tw, int twn) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < twn) {
-        double s, c;
-        sincospi(-2.0 * (double)k / (double)twn, &s, &c);
-        tw[k] = mk<T>((T)c, (T)s);
-    }
-}
-
-// V_h = FFT_M(v), v_j = c_j (0 <= j < h), v_{M-j} = c_j (1 <= j < h), else 0,
-// stored at V + ofs followed by the chirp c_j itself (j < h)
-template <typename T, int NB>
-__global__ void __launch_bounds__(1024)
-chirp_spectrum_kernel(const int32_t *__restrict__ hs, const int64_t *__restrict__ ofs, c2<T> *V, const void *tw,
-                      int twshift, int M, int log2M) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
-    const int h = hs[blockIdx.x];
-    RingTw<T>::stage(buf + M, tw, M, log2M, twshift);
-    const RingTw<T> twl(buf + M, log2M, twshift);
-    c2<T> *out = V + ofs[blockIdx.x];
-    for (int j = threadIdx.x; j < M; j += blockDim.x) {
-        c2<T> v = mk<T>(0, 0);
-        if (j < h) {
-            v = chirp<T>(j, h);
-            out[M + j] = v;
-        } else if (j > M - h) {
-            v = chirp<T>(M - j, h);
-        }
-        buf[j] = v;
-    }
-    __syncthreads();
-    lds_fft<T, NB>(buf, M, log2M, twl, false);
-    for (int j = threadIdx.x; j < M; j += blockDim.x) out[j] = buf[j];
-}
-
 __global__ void round_to_f32_kernel(const double2 *__restrict__ src, float2 *__restrict__ dst, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = make_float2((float)src[i].x, (float)src[i].y);
 }
 
-// threads per ring workgroup: 2 radix-4 butterflies per thread for M >= 4096
-// (8192-point Bluestein rings need 1024 threads to stay at ~60 VGPRs, so
-// two workgroups share a CU)
-static int class_threads(int M) { return M >= 8192 ? 1024 : (M >= 4096 ? 512 : 256); }
-static int class_nb(int M) {
-    const int nt = class_threads(M);
-    return std::max(1, M / (4 * nt));
+// V_h = FFT_M(v), v_j = c_j (0 <= j < h), v_{M-j} = c_j (1 <= j < h), else 0,
+// stored at V + ofs followed by the chirp c_j itself (j < h)
+template <typename T>
+__global__ void __launch_bounds__(ring_max_threads<T>())
+chirp_spectrum_kernel(const int32_t *__restrict__ hs, const int64_t *__restrict__ ofs, c2<T> *V,
+                      const c2<T> *__restrict__ tw, int M, int log2M) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
+    const int h = hs[blockIdx.x];
+    c2<T> *out = V + ofs[blockIdx.x];
+    for (int j = threadIdx.x; j < h; j += blockDim.x) out[M + j] = chirp<T>(j, h);
+    fft_run<T, false>(
+        buf, M, log2M, tw,
+        [&](int j, int) {
+            c2<T> v = mk<T>(0, 0);
+            if (j < h) v = chirp<T>(j, h);
+            else if (j > M - h) v = chirp<T>(M - j, h);
+            return v;
+        },
+        [&](int k, int, c2<T> x) { out[k] = x; });
 }
-
-// dispatch on NB for a kernel template taking <T, ..., NB>
-#define HBX_NB_SWITCH(nb, ...)                                                \
-    switch (nb) {                                                             \
-    case 1: { constexpr int NBV = 1; __VA_ARGS__; } break;                    \
-    case 2: { constexpr int NBV = 2; __VA_ARGS__; } break;                    \
-    case 4: { constexpr int NBV = 4; __VA_ARGS__; } break;                    \
-    case 8: { constexpr int NBV = 8; __VA_ARGS__; } break;                    \
-    default: return set_error(HBX_EUNSUPPORTED, "FFT size class too large");  \
-    }
 
 template <typename T, typename K>
 static int allow_lds(K kernel, size_t bytes) {
@@ -575,8 +731,8 @@ static int allow_lds(K kernel, size_t bytes) {
 // chirp spectra and chirps of the rings of one size class M, computed at
 // precision T into V (offsets ofs_host, one entry per distinct h)
 template <typename T>
-static int build_chirp_spectra(Plan *p, const std::vector<int32_t> &hs_host, const std::vector<int64_t> &ofs_host,
-                               int M, c2<T> *V, const void *tw) {
+static int build_chirp_spectra(const std::vector<int32_t> &hs_host, const std::vector<int64_t> &ofs_host, int M,
+                               c2<T> *V, const c2<T> *tw) {
     if (hs_host.empty()) return 0;
     int32_t *d_hs = nullptr;
     int64_t *d_ofs = nullptr;
@@ -584,19 +740,14 @@ static int build_chirp_spectra(Plan *p, const std::vector<int32_t> &hs_host, con
     HBX_HIP(hipMalloc(&d_ofs, ofs_host.size() * sizeof(int64_t)));
     HBX_HIP(hipMemcpy(d_hs, hs_host.data(), hs_host.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HBX_HIP(hipMemcpy(d_ofs, ofs_host.data(), ofs_host.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-    const int nt = class_threads(M), nb = class_nb(M), lg = ilog2(M);
-    const size_t lds = ring_lds_bytes<T>(M, p->twshift);
-    int rc = 0;
-    HBX_NB_SWITCH(nb, {
-        auto k = chirp_spectrum_kernel<T, NBV>;
-        rc = allow_lds<T>(k, lds);
-        if (rc == 0) {
-            hipLaunchKernelGGL(k, dim3((unsigned)hs_host.size()), dim3(nt), lds, 0, d_hs, d_ofs, V, tw,
-                               p->twshift, M, lg);
-            hipError_t e = hipGetLastError();
-            if (e != hipSuccess) rc = hip_error(e, "chirp_spectrum_kernel");
-        }
-    });
+    const size_t lds = ring_lds_bytes<T>(M);
+    int rc = allow_lds<T>(chirp_spectrum_kernel<T>, lds);
+    if (rc == 0) {
+        hipLaunchKernelGGL(chirp_spectrum_kernel<T>, dim3((unsigned)hs_host.size()), dim3(class_threads(M)), lds, 0,
+                           d_hs, d_ofs, V, tw, M, ilog2(M));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = hip_error(e, "chirp_spectrum_kernel");
+    }
     hipError_t e = hipDeviceSynchronize();
     (void)hipFree(d_hs);
     (void)hipFree(d_ofs);
@@ -608,20 +759,20 @@ static int build_chirp_spectra(Plan *p, const std::vector<int32_t> &hs_host, con
 int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const double *phi0) {
     const int nr = p->nrings;
     // class per ring
-    std::map<int, std::vector<int32_t>> cls;            // M -> rings
+    std::map<int, std::vector<int32_t>> pow2_rings, blue_rings;   // M -> rings
     std::map<int, int64_t> vofs_of_h;                   // h -> offset (non-pow2)
     std::map<int, std::vector<int32_t>> hs_by_M;        // M -> distinct h
     std::map<int, std::vector<int64_t>> ofs_by_M;
     std::vector<int64_t> vofs(nr, -1);
     std::vector<int> ringM(nr);
-    int maxM = 4;
+    int maxM = 1;
     int64_t vtotal = 0;
     for (int r = 0; r < nr; ++r) {
         const int h = nphi[r] / 2;
         const bool pow2 = (h & (h - 1)) == 0;
         const int M = pow2 ? h : (1 << ilog2(2 * (int64_t)h - 1));
         ringM[r] = M;
-        cls[M].push_back(r);
+        (pow2 ? pow2_rings : blue_rings)[M].push_back(r);
         maxM = std::max(maxM, M);
         if (!pow2) {
             auto it = vofs_of_h.find(h);
@@ -641,15 +792,11 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
     int maxlds = 0;
     HBX_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeSharedMemPerBlockOptin, dev));
     if (maxlds <= 0) HBX_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    const int twshift = ilog2(maxM);
-    const size_t need = p->precision == HBX_F64 ? ring_lds_bytes<double>(maxM, twshift)
-                                                : ring_lds_bytes<float>(maxM, twshift);
+    const size_t need = p->precision == HBX_F64 ? ring_lds_bytes<double>(maxM) : ring_lds_bytes<float>(maxM);
     if (need > (size_t)maxlds)
         return set_error(HBX_EUNSUPPORTED, "ring FFT of size " + std::to_string(maxM) +
                                                " does not fit in LDS (" + std::to_string(maxlds) +
                                                " bytes) at this precision");
-    p->twn = maxM;
-    p->twshift = twshift;
     p->vtotal = vtotal;
     auto dalloc = [&](void **ptr, size_t bytes) -> int {
         if (bytes == 0) bytes = 16;
@@ -678,19 +825,54 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
         if ((rc = dalloc((void **)&p->d_rot, rot.size() * sizeof(double2)))) return rc;
         HBX_HIP(hipMemcpy(p->d_rot, rot.data(), rot.size() * sizeof(double2), hipMemcpyHostToDevice));
     }
-    // FP64 twiddles for every plan; FP32 plans also get them rounded once
-    if ((rc = dalloc((void **)&p->d_tw64, maxM * sizeof(double2)))) return rc;
-    hipLaunchKernelGGL(twiddle_kernel<double>, dim3((maxM + 255) / 256), dim3(256), 0, 0, p->d_tw64, maxM);
-    HBX_HIP(hipGetLastError());
-    if (p->precision == HBX_F32) {
-        if ((rc = dalloc((void **)&p->d_tw32, maxM * sizeof(float2)))) return rc;
-        hipLaunchKernelGGL(twiddle_kernel<float>, dim3((maxM + 255) / 256), dim3(256), 0, 0, p->d_tw32, maxM);
-        HBX_HIP(hipGetLastError());
+    // size classes and their twiddle tables (FP64 and FP32 roundings)
+    std::map<int, int> all_M;
+    for (auto &kv : pow2_rings) all_M[kv.first] = 1;
+    for (auto &kv : blue_rings) all_M[kv.first] = 1;
+    int64_t tw_total = 0;
+    for (auto &kv : all_M) {
+        FftClass c;
+        c.M = kv.first;
+        c.log2M = ilog2(c.M);
+        c.threads = class_threads(c.M);
+        c.tw_off = tw_total;
+        tw_total += (fft_table_entries(c.M) + 1) & ~int64_t(1);     // rows stay 16-byte aligned
+        auto up = [&](const std::vector<int32_t> &v, int32_t **d) -> int {
+            if (v.empty()) return 0;
+            int e = dalloc((void **)d, v.size() * sizeof(int32_t));
+            if (e) return e;
+            HBX_HIP(hipMemcpy(*d, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+            return 0;
+        };
+        if (pow2_rings.count(c.M)) {
+            c.nrings_pow2 = (int)pow2_rings[c.M].size();
+            if ((rc = up(pow2_rings[c.M], &c.d_rings_pow2))) return rc;
+        }
+        if (blue_rings.count(c.M)) {
+            c.nrings_blue = (int)blue_rings[c.M].size();
+            if ((rc = up(blue_rings[c.M], &c.d_rings_blue))) return rc;
+        }
+        p->classes.push_back(c);
     }
+    {
+        std::vector<double2> t64((size_t)std::max<int64_t>(tw_total, 1), make_double2(0.0, 0.0));
+        std::vector<float2> t32(t64.size(), make_float2(0.f, 0.f));
+        for (const auto &c : p->classes) fft_table_fill(c.M, t64.data() + c.tw_off, t32.data() + c.tw_off);
+        if ((rc = dalloc((void **)&p->d_ftw64, t64.size() * sizeof(double2)))) return rc;
+        if ((rc = dalloc((void **)&p->d_ftw32, t32.size() * sizeof(float2)))) return rc;
+        HBX_HIP(hipMemcpy(p->d_ftw64, t64.data(), t64.size() * sizeof(double2), hipMemcpyHostToDevice));
+        HBX_HIP(hipMemcpy(p->d_ftw32, t32.data(), t32.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    auto tw_of = [&](int M) -> int64_t {
+        for (const auto &c : p->classes)
+            if (c.M == M) return c.tw_off;
+        return 0;
+    };
     if (p->precision == HBX_F64) {
         if ((rc = dalloc((void **)&p->d_V64, vtotal * sizeof(double2)))) return rc;
         for (auto &kv : hs_by_M)
-            if ((rc = build_chirp_spectra<double>(p, kv.second, ofs_by_M[kv.first], kv.first, p->d_V64, p->d_tw64)))
+            if ((rc = build_chirp_spectra<double>(kv.second, ofs_by_M[kv.first], kv.first, p->d_V64,
+                                                  p->d_ftw64 + tw_of(kv.first))))
                 return rc;
     } else {
         if ((rc = dalloc((void **)&p->d_V32, vtotal * sizeof(float2)))) return rc;
@@ -699,14 +881,16 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
         if (vtotal > 0 && hipMalloc(&tmp, vtotal * sizeof(double2)) != hipSuccess)
             return set_error(HBX_ENOMEM, "hipMalloc failed for the FP64 chirp spectra");
         for (auto &kv : hs_by_M) {
-            if (ring_lds_bytes<double>(kv.first, twshift) <= (size_t)maxlds)
-                rc = build_chirp_spectra<double>(p, kv.second, ofs_by_M[kv.first], kv.first, tmp, p->d_tw64);
+            const bool f64 = ring_lds_bytes<double>(kv.first) <= (size_t)maxlds;
+            if (f64)
+                rc = build_chirp_spectra<double>(kv.second, ofs_by_M[kv.first], kv.first, tmp,
+                                                 p->d_ftw64 + tw_of(kv.first));
             else
-                rc = build_chirp_spectra<float>(p, kv.second, ofs_by_M[kv.first], kv.first, p->d_V32, p->d_tw32);
+                rc = build_chirp_spectra<float>(kv.second, ofs_by_M[kv.first], kv.first, p->d_V32,
+                                                p->d_ftw32 + tw_of(kv.first));
             if (rc) break;
             // round this class's entries (every distinct h of the class)
-            for (size_t i = 0; i < kv.second.size() && ring_lds_bytes<double>(kv.first, twshift) <= (size_t)maxlds;
-                 ++i) {
+            for (size_t i = 0; i < kv.second.size() && f64; ++i) {
                 const int64_t o = ofs_by_M[kv.first][i], len = kv.first + kv.second[i];
                 hipLaunchKernelGGL(round_to_f32_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, 0,
                                    tmp + o, p->d_V32 + o, len);
@@ -717,57 +901,56 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
         if (rc) return rc;
         if (e != hipSuccess) return hip_error(e, "FP32 chirp spectra");
     }
-    // size classes
-    for (auto &kv : cls) {
-        FftClass c;
-        c.M = kv.first;
-        c.log2M = ilog2(c.M);
-        c.threads = class_threads(c.M);
-        c.nb = class_nb(c.M);
-        c.nrings = (int)kv.second.size();
-        if ((rc = dalloc((void **)&c.d_rings, kv.second.size() * sizeof(int32_t)))) return rc;
-        HBX_HIP(hipMemcpy(c.d_rings, kv.second.data(), kv.second.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        c.lds_bytes64 = ring_lds_bytes<double>(c.M, p->twshift);
-        c.lds_bytes32 = ring_lds_bytes<float>(c.M, p->twshift);
-        p->classes.push_back(c);
-    }
     // raise the dynamic-LDS limit of every kernel instance we may launch
     for (const auto &c : p->classes) {
-        const size_t lds = p->precision == HBX_F64 ? c.lds_bytes64 : c.lds_bytes32;
-        if (lds <= 65536) continue;
-        HBX_NB_SWITCH(c.nb, {
-            if (p->precision == HBX_F64) {
-                if ((rc = allow_lds<double>(ring_analysis_kernel<double, double, NBV>, lds))) return rc;
-                if ((rc = allow_lds<double>(ring_analysis_kernel<double, float, NBV>, lds))) return rc;
-                if ((rc = allow_lds<double>(ring_synthesis_kernel<double, double, NBV>, lds))) return rc;
-                if ((rc = allow_lds<double>(ring_synthesis_kernel<double, float, NBV>, lds))) return rc;
-            } else {
-                if ((rc = allow_lds<float>(ring_analysis_kernel<float, double, NBV>, lds))) return rc;
-                if ((rc = allow_lds<float>(ring_analysis_kernel<float, float, NBV>, lds))) return rc;
-                if ((rc = allow_lds<float>(ring_synthesis_kernel<float, double, NBV>, lds))) return rc;
-                if ((rc = allow_lds<float>(ring_synthesis_kernel<float, float, NBV>, lds))) return rc;
-            }
-        });
+        if (p->precision == HBX_F64) {
+            const size_t lds = ring_lds_bytes<double>(c.M);
+            if ((rc = allow_lds<double>(ring_analysis_kernel<double, double, false>, lds))) return rc;
+            if ((rc = allow_lds<double>(ring_analysis_kernel<double, float, false>, lds))) return rc;
+            if ((rc = allow_lds<double>(ring_analysis_kernel<double, double, true>, lds))) return rc;
+            if ((rc = allow_lds<double>(ring_analysis_kernel<double, float, true>, lds))) return rc;
+            if ((rc = allow_lds<double>(ring_synthesis_kernel<double, double, false>, lds))) return rc;
+            if ((rc = allow_lds<double>(ring_synthesis_kernel<double, float, false>, lds))) return rc;
+            if ((rc = allow_lds<double>(ring_synthesis_kernel<double, double, true>, lds))) return rc;
+            if ((rc = allow_lds<double>(ring_synthesis_kernel<double, float, true>, lds))) return rc;
+        } else {
+            const size_t lds = ring_lds_bytes<float>(c.M);
+            if ((rc = allow_lds<float>(ring_analysis_kernel<float, double, false>, lds))) return rc;
+            if ((rc = allow_lds<float>(ring_analysis_kernel<float, float, false>, lds))) return rc;
+            if ((rc = allow_lds<float>(ring_analysis_kernel<float, double, true>, lds))) return rc;
+            if ((rc = allow_lds<float>(ring_analysis_kernel<float, float, true>, lds))) return rc;
+            if ((rc = allow_lds<float>(ring_synthesis_kernel<float, double, false>, lds))) return rc;
+            if ((rc = allow_lds<float>(ring_synthesis_kernel<float, float, false>, lds))) return rc;
+            if ((rc = allow_lds<float>(ring_synthesis_kernel<float, double, true>, lds))) return rc;
+            if ((rc = allow_lds<float>(ring_synthesis_kernel<float, float, true>, lds))) return rc;
+        }
     }
     HBX_HIP(hipDeviceSynchronize());
     return 0;
 }
 
+template <typename T> static const c2<T> *class_tw(const Plan *p, const FftClass &c);
+template <> const double2 *class_tw<double>(const Plan *p, const FftClass &c) { return p->d_ftw64 + c.tw_off; }
+template <> const float2 *class_tw<float>(const Plan *p, const FftClass &c) { return p->d_ftw32 + c.tw_off; }
+
 template <typename T, typename TIn>
 static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t map_stride,
-                           c2<T> *phase, const double *scale, const void *tw, const c2<T> *V,
-                           hipStream_t s) {
+                           c2<T> *phase, const double *scale, const c2<T> *V, hipStream_t s) {
     const int64_t pstride = phase_elems_per_map(p);
     timer_mark(HBX_TIMER_RING_ANALYSIS, true, s);
     for (const auto &c : p->classes) {
-        const size_t lds = ring_lds_bytes<T>(c.M, p->twshift);
-        dim3 grid((unsigned)c.nrings, (unsigned)nmaps);
-        HBX_NB_SWITCH(c.nb, {
-            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, NBV>), grid, dim3(c.threads), lds, s, maps,
-                               map_stride, phase, pstride, c.d_rings, p->d_ring_start, p->d_nphi, p->d_mcut,
-                               p->d_phi0, scale, p->d_rot, tw, p->twshift, V, p->d_vofs, c.M, c.log2M,
-                               p->mmax, p->nrings);
-        });
+        const size_t lds = ring_lds_bytes<T>(c.M);
+        const c2<T> *tw = class_tw<T>(p, c);
+        if (c.nrings_pow2)
+            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, false>), dim3((unsigned)c.nrings_pow2, (unsigned)nmaps),
+                               dim3(c.threads), lds, s, maps, map_stride, phase, pstride, c.d_rings_pow2,
+                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
+                               c.M, c.log2M, p->mmax, p->nrings);
+        if (c.nrings_blue)
+            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, true>), dim3((unsigned)c.nrings_blue, (unsigned)nmaps),
+                               dim3(c.threads), lds, s, maps, map_stride, phase, pstride, c.d_rings_blue,
+                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
+                               c.M, c.log2M, p->mmax, p->nrings);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_error(e, "ring_analysis_kernel launch");
     }
@@ -777,19 +960,22 @@ static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t ma
 
 template <typename T, typename TOut>
 static int ring_synthesis_t(const Plan *p, int nmaps, const c2<T> *phase, TOut *maps,
-                            int64_t map_stride, const double *scale, const void *tw,
-                            const c2<T> *V, hipStream_t s) {
+                            int64_t map_stride, const double *scale, const c2<T> *V, hipStream_t s) {
     const int64_t pstride = phase_elems_per_map(p);
     timer_mark(HBX_TIMER_RING_SYNTHESIS, true, s);
     for (const auto &c : p->classes) {
-        const size_t lds = ring_lds_bytes<T>(c.M, p->twshift);
-        dim3 grid((unsigned)c.nrings, (unsigned)nmaps);
-        HBX_NB_SWITCH(c.nb, {
-            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, NBV>), grid, dim3(c.threads), lds, s, phase,
-                               pstride, maps, map_stride, c.d_rings, p->d_ring_start, p->d_nphi, p->d_mcut,
-                               p->d_phi0, scale, p->d_rot, tw, p->twshift, V, p->d_vofs, c.M, c.log2M,
-                               p->mmax, p->nrings);
-        });
+        const size_t lds = ring_lds_bytes<T>(c.M);
+        const c2<T> *tw = class_tw<T>(p, c);
+        if (c.nrings_pow2)
+            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, false>), dim3((unsigned)c.nrings_pow2, (unsigned)nmaps),
+                               dim3(c.threads), lds, s, phase, pstride, maps, map_stride, c.d_rings_pow2,
+                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
+                               c.M, c.log2M, p->mmax, p->nrings);
+        if (c.nrings_blue)
+            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, true>), dim3((unsigned)c.nrings_blue, (unsigned)nmaps),
+                               dim3(c.threads), lds, s, phase, pstride, maps, map_stride, c.d_rings_blue,
+                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
+                               c.M, c.log2M, p->mmax, p->nrings);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_error(e, "ring_synthesis_kernel launch");
     }
@@ -802,13 +988,13 @@ int launch_ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dty
     if (p->precision == HBX_F64) {
         auto *ph = static_cast<double2 *>(phase);
         if (map_dtype == HBX_DT_F64)
-            return ring_analysis_t<double, double>(p, nmaps, (const double *)maps, map_stride, ph, scale, p->d_tw64, p->d_V64, s);
-        return ring_analysis_t<double, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_tw64, p->d_V64, s);
+            return ring_analysis_t<double, double>(p, nmaps, (const double *)maps, map_stride, ph, scale, p->d_V64, s);
+        return ring_analysis_t<double, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_V64, s);
     }
     auto *ph = static_cast<float2 *>(phase);
     if (map_dtype == HBX_DT_F64)
-        return ring_analysis_t<float, double>(p, nmaps, (const double *)maps, map_stride, ph, scale, p->d_tw32, p->d_V32, s);
-    return ring_analysis_t<float, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_tw32, p->d_V32, s);
+        return ring_analysis_t<float, double>(p, nmaps, (const double *)maps, map_stride, ph, scale, p->d_V32, s);
+    return ring_analysis_t<float, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_V32, s);
 }
 
 int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *maps, int map_dtype,
@@ -816,13 +1002,13 @@ int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *map
     if (p->precision == HBX_F64) {
         auto *ph = static_cast<const double2 *>(phase);
         if (map_dtype == HBX_DT_F64)
-            return ring_synthesis_t<double, double>(p, nmaps, ph, (double *)maps, map_stride, scale, p->d_tw64, p->d_V64, s);
-        return ring_synthesis_t<double, float>(p, nmaps, ph, (float *)maps, map_stride, scale, p->d_tw64, p->d_V64, s);
+            return ring_synthesis_t<double, double>(p, nmaps, ph, (double *)maps, map_stride, scale, p->d_V64, s);
+        return ring_synthesis_t<double, float>(p, nmaps, ph, (float *)maps, map_stride, scale, p->d_V64, s);
     }
     auto *ph = static_cast<const float2 *>(phase);
     if (map_dtype == HBX_DT_F64)
-        return ring_synthesis_t<float, double>(p, nmaps, ph, (double *)maps, map_stride, scale, p->d_tw32, p->d_V32, s);
-    return ring_synthesis_t<float, float>(p, nmaps, ph, (float *)maps, map_stride, scale, p->d_tw32, p->d_V32, s);
+        return ring_synthesis_t<float, double>(p, nmaps, ph, (double *)maps, map_stride, scale, p->d_V32, s);
+    return ring_synthesis_t<float, float>(p, nmaps, ph, (float *)maps, map_stride, scale, p->d_V32, s);
 }
 
 }  // namespace hbx
