@@ -67,6 +67,22 @@ template <typename T, bool INV> __device__ __forceinline__ c2<T> mul_c(c2<T> a, 
     if (INV) ci = -ci;
     return mk<T>(a.x * cr - a.y * ci, a.x * ci + a.y * cr);
 }
+// the same by an internal constant of the radix-8/16 DFTs (given in FP64).
+// FP32: the constant is carried as hi + lo and both parts are applied, so
+// its rounding (the same in every butterfly of every ring) adds no error
+// that is coherent across rings.  FP64: plain product.
+template <typename T, bool INV> __device__ __forceinline__ c2<T> mul_k(c2<T> a, double crd, double cid) {
+    if constexpr (sizeof(T) == 8) {
+        return mul_c<T, INV>(a, crd, cid);
+    } else {
+        const float cr = (float)crd, ci0 = (float)cid;
+        const float crl = (float)(crd - (double)cr), cil0 = (float)(cid - (double)ci0);
+        const float ci = INV ? -ci0 : ci0, cil = INV ? -cil0 : cil0;
+        const float re = fmaf(a.x, cr, fmaf(-a.y, ci, fmaf(a.x, crl, -a.y * cil)));
+        const float im = fmaf(a.x, ci, fmaf(a.y, cr, fmaf(a.x, cil, a.y * crl)));
+        return mk<T>(re, im);
+    }
+}
 
 template <typename T, bool INV> __device__ __forceinline__ void dft4(c2<T> &x0, c2<T> &x1, c2<T> &x2, c2<T> &x3) {
     const c2<T> s02 = cadd<T>(x0, x2), d02 = csub<T>(x0, x2);
@@ -97,12 +113,12 @@ template <typename T, bool INV> struct Dft<T, 4, INV> {
 // n = 2 n1 + n2, k = k1 + 4 k2: X[k] = sum_n2 W8^{n2 k1} (-1)^{n2 k2} DFT4_n1(x[2 n1 + n2])[k1]
 template <typename T, bool INV> struct Dft<T, 8, INV> {
     __device__ __forceinline__ static void run(c2<T> *x) {
-        const T h = (T)0.70710678118654752440;
+        const double h = 0.70710678118654752440;
         dft4<T, INV>(x[0], x[2], x[4], x[6]);
         dft4<T, INV>(x[1], x[3], x[5], x[7]);
-        const c2<T> b1 = mul_c<T, INV>(x[3], h, -h);
+        const c2<T> b1 = mul_k<T, INV>(x[3], h, -h);
         const c2<T> b2 = mul_wi<T, INV>(x[5]);
-        const c2<T> b3 = mul_c<T, INV>(x[7], -h, -h);
+        const c2<T> b3 = mul_k<T, INV>(x[7], -h, -h);
         const c2<T> a0 = x[0], a1 = x[2], a2 = x[4], a3 = x[6], b0 = x[1];
         x[0] = cadd<T>(a0, b0);
         x[4] = csub<T>(a0, b0);
@@ -118,19 +134,19 @@ template <typename T, bool INV> struct Dft<T, 8, INV> {
 // n = 4 n1 + n2, k = k1 + 4 k2: X[k] = DFT4_n2( W16^{n2 k1} DFT4_n1(x[4 n1 + n2])[k1] )[k2]
 template <typename T, bool INV> struct Dft<T, 16, INV> {
     __device__ __forceinline__ static void run(c2<T> *x) {
-        const T c = (T)0.92387953251128675613, s = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+        const double c = 0.92387953251128675613, s = 0.38268343236508977173, h = 0.70710678118654752440;
 #pragma unroll
         for (int n2 = 0; n2 < 4; ++n2) dft4<T, INV>(x[n2], x[4 + n2], x[8 + n2], x[12 + n2]);
         // A[n2][k1] sits at x[n2 + 4 k1]; twiddle W16^{n2 k1}
-        x[5] = mul_c<T, INV>(x[5], c, -s);      // 1
-        x[9] = mul_c<T, INV>(x[9], h, -h);      // 2
-        x[13] = mul_c<T, INV>(x[13], s, -c);    // 3
-        x[6] = mul_c<T, INV>(x[6], h, -h);      // 2
+        x[5] = mul_k<T, INV>(x[5], c, -s);      // 1
+        x[9] = mul_k<T, INV>(x[9], h, -h);      // 2
+        x[13] = mul_k<T, INV>(x[13], s, -c);    // 3
+        x[6] = mul_k<T, INV>(x[6], h, -h);      // 2
         x[10] = mul_wi<T, INV>(x[10]);          // 4
-        x[14] = mul_c<T, INV>(x[14], -h, -h);   // 6
-        x[7] = mul_c<T, INV>(x[7], s, -c);      // 3
-        x[11] = mul_c<T, INV>(x[11], -h, -h);   // 6
-        x[15] = mul_c<T, INV>(x[15], -c, s);    // 9
+        x[14] = mul_k<T, INV>(x[14], -h, -h);   // 6
+        x[7] = mul_k<T, INV>(x[7], s, -c);      // 3
+        x[11] = mul_k<T, INV>(x[11], -h, -h);   // 6
+        x[15] = mul_k<T, INV>(x[15], -c, s);    // 9
 #pragma unroll
         for (int k1 = 0; k1 < 4; ++k1) dft4<T, INV>(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]);
         // X[k1 + 4 k2] sits at x[4 k1 + k2]: transpose the 4 x 4 index
@@ -321,6 +337,17 @@ constexpr int NROT = 4;
 // launch bound: FP64 classes stop at M = 8192 (512 threads) by LDS
 template <typename T> constexpr int ring_max_threads() { return sizeof(T) == 8 ? 512 : 1024; }
 
+// Block g of a ring launch -> (ring slot, map): the maps of one ring run
+// back to back on one XCD (consecutive blocks go round-robin over the 8
+// XCDs), so its chirp spectrum, chirp and twiddle rows are read into that
+// XCD's L2 once.  Grid: ceil(ncls / 8) * 8 * nmaps blocks.
+__device__ __forceinline__ void ring_block(int g, int nmaps, int *slot, int *map) {
+    const int xcd = g & 7, k = g >> 3;
+    *map = k % nmaps;
+    *slot = (k / nmaps) * 8 + xcd;
+}
+inline unsigned ring_grid(int ncls, int nmaps) { return (unsigned)(((ncls + 7) / 8) * 8 * nmaps); }
+
 // ---------------------------------------------------------------------------
 // analysis: maps -> phase.  BLUE: Bluestein rings (h not a power of two),
 // V + vofs[r]: chirp spectrum (M entries) then the chirp c_j, j < h.
@@ -333,11 +360,13 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
                      const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
                      const double *__restrict__ scale, const double2 *__restrict__ rot,
                      const c2<T> *__restrict__ tw, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
-                     int M, int log2M, int mmax, int nr) {
+                     int M, int log2M, int mmax, int nr, int ncls, int nmaps) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
-    const int r = rings[blockIdx.x];
-    const int b = blockIdx.y;
+    int slot, b;
+    ring_block(blockIdx.x, nmaps, &slot, &b);
+    if (slot >= ncls) return;
+    const int r = rings[slot];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = nphi[r], h = n >> 1;
     const TIn *f = maps + (int64_t)b * map_stride + ring_start[r];
@@ -371,7 +400,7 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
     const double ph0 = phi0 ? phi0[r] : 0.0;
     const bool rotp = (ph0 != 0.0);
     c2<T> *out = phase + (int64_t)b * phase_stride;
-    auto slot = [&](int m) -> c2<T> & { return out[((int64_t)(m >> 3) * nr + r) * MB + (m & 7)]; };
+    auto outv = [&](int m) -> c2<T> & { return out[((int64_t)(m >> 3) * nr + r) * MB + (m & 7)]; };
     double2 W, P = make_double2(1.0, 0.0);
     {
         double s, c;
@@ -394,7 +423,7 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
             if (m < lim) {
                 c2<T> F = cscale<T>(X, sc);
                 if (rotp) F = cmul<T>(F, round_c<T>(Pm));
-                slot(m) = F;
+                outv(m) = F;
             }
         };
         for (int k = tid; 2 * k <= h; k += nt) {
@@ -411,7 +440,7 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
             W = zmul(W, Wst);
             if (rotp) P = zmul(P, Pst);
         }
-        for (int m = lim + tid; m <= mmax; m += nt) slot(m) = mk<T>(0, 0);
+        for (int m = lim + tid; m <= mmax; m += nt) outv(m) = mk<T>(0, 0);
     } else {
         // alias copies (mcut > nphi): one m per step, bin m mod n
         const int hstep = nt % h;
@@ -427,7 +456,7 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
                 F = cscale<T>(X, sc);
                 if (rotp) F = cmul<T>(F, round_c<T>(P));
             }
-            slot(m) = F;
+            outv(m) = F;
             W = zmul(W, Wst);
             if (rotp) P = zmul(P, Pst);
             ia += hstep;
@@ -447,11 +476,13 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
                       const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
                       const double *__restrict__ scale, const double2 *__restrict__ rot,
                       const c2<T> *__restrict__ tw, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
-                      int M, int log2M, int mmax, int nr) {
+                      int M, int log2M, int mmax, int nr, int ncls, int nmaps) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
-    const int r = rings[blockIdx.x];
-    const int b = blockIdx.y;
+    int slot, b;
+    ring_block(blockIdx.x, nmaps, &slot, &b);
+    if (slot >= ncls) return;
+    const int r = rings[slot];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = nphi[r], h = n >> 1;
     const int top = min(mmax, mcut[r] - 1);
@@ -942,15 +973,15 @@ static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t ma
         const size_t lds = ring_lds_bytes<T>(c.M);
         const c2<T> *tw = class_tw<T>(p, c);
         if (c.nrings_pow2)
-            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, false>), dim3((unsigned)c.nrings_pow2, (unsigned)nmaps),
+            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, false>), dim3(ring_grid(c.nrings_pow2, nmaps)),
                                dim3(c.threads), lds, s, maps, map_stride, phase, pstride, c.d_rings_pow2,
                                p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
-                               c.M, c.log2M, p->mmax, p->nrings);
+                               c.M, c.log2M, p->mmax, p->nrings, c.nrings_pow2, nmaps);
         if (c.nrings_blue)
-            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, true>), dim3((unsigned)c.nrings_blue, (unsigned)nmaps),
+            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, true>), dim3(ring_grid(c.nrings_blue, nmaps)),
                                dim3(c.threads), lds, s, maps, map_stride, phase, pstride, c.d_rings_blue,
                                p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
-                               c.M, c.log2M, p->mmax, p->nrings);
+                               c.M, c.log2M, p->mmax, p->nrings, c.nrings_blue, nmaps);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_error(e, "ring_analysis_kernel launch");
     }
@@ -967,15 +998,15 @@ static int ring_synthesis_t(const Plan *p, int nmaps, const c2<T> *phase, TOut *
         const size_t lds = ring_lds_bytes<T>(c.M);
         const c2<T> *tw = class_tw<T>(p, c);
         if (c.nrings_pow2)
-            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, false>), dim3((unsigned)c.nrings_pow2, (unsigned)nmaps),
+            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, false>), dim3(ring_grid(c.nrings_pow2, nmaps)),
                                dim3(c.threads), lds, s, phase, pstride, maps, map_stride, c.d_rings_pow2,
                                p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
-                               c.M, c.log2M, p->mmax, p->nrings);
+                               c.M, c.log2M, p->mmax, p->nrings, c.nrings_pow2, nmaps);
         if (c.nrings_blue)
-            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, true>), dim3((unsigned)c.nrings_blue, (unsigned)nmaps),
+            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, true>), dim3(ring_grid(c.nrings_blue, nmaps)),
                                dim3(c.threads), lds, s, phase, pstride, maps, map_stride, c.d_rings_blue,
                                p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
-                               c.M, c.log2M, p->mmax, p->nrings);
+                               c.M, c.log2M, p->mmax, p->nrings, c.nrings_blue, nmaps);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_error(e, "ring_synthesis_kernel launch");
     }

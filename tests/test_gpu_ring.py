@@ -20,10 +20,10 @@ NSIDE, LMAX = 2048, 4095
 TOL = {"f64": 1e-12, "f32": 2e-6}
 
 
-def _setup(precision):
+def _setup(precision, phi0="healpix"):
     from hellblast_amd.hp_compat import healpix_geom
     from hellblast_amd.plan import get_plan
-    geom = healpix_geom(NSIDE, LMAX, "dz", "healpix", "reference")
+    geom = healpix_geom(NSIDE, LMAX, "dz", phi0, "reference")
     return geom, get_plan(geom, LMAX, LMAX, precision)
 
 
@@ -44,10 +44,11 @@ def _per_ring_rel(got, ref):
     return float(np.max(d / s))
 
 
+@pytest.mark.parametrize("phi0", ["healpix", "reference"])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_ring_analysis_all_classes(gpu, precision):
-    print(f"[case] ring analysis nside={NSIDE} mmax={LMAX} {precision}", flush=True)
-    geom, plan = _setup(precision)
+def test_ring_analysis_all_classes(gpu, precision, phi0):
+    print(f"[case] ring analysis nside={NSIDE} mmax={LMAX} {precision} phi0={phi0}", flush=True)
+    geom, plan = _setup(precision, phi0)
     nmaps = 2
     rng = np.random.default_rng(7)
     rdt = torch.float64 if precision == "f64" else torch.float32
@@ -91,3 +92,39 @@ def test_ring_synthesis_all_classes(gpu, precision):
             worst = max(worst, float(np.max(np.abs(got[b, s:e] - ref[s:e])) / np.max(np.abs(ref[s:e]))))
         print(f"map {b}: per-ring rel err {worst:.2e}", flush=True)
         assert worst <= TOL[precision]
+
+
+def test_f32_gate_full_size(gpu):
+    """FP32 map2alm at nside 2048, lmax 4095 within eps_max <= 1e-5 of FP64 on
+    a red-spectrum map (C_l ~ 1/(l+1)^2): the top coefficients (l = m = 4095)
+    are the most sensitive to FP32 ring-FFT errors that are coherent across
+    the equatorial rings."""
+    print("[case] f32 gate nside=2048 lmax=4095", flush=True)
+    import math
+    from hellblast_amd.hp_compat import _plan
+    from hellblast_amd.plan import nalm
+    dev = torch.device("cuda", 0)
+    p64 = _plan(NSIDE, LMAX, LMAX, "f64", "dz", "reference", "reference")
+    p32 = _plan(NSIDE, LMAX, LMAX, "f32", "dz", "reference", "reference")
+    n = nalm(LMAX)
+    ms = torch.repeat_interleave(torch.arange(LMAX + 1, device=dev), torch.arange(LMAX + 1, 0, -1, device=dev))
+    ls = torch.arange(n, device=dev) - torch.repeat_interleave(
+        torch.cumsum(torch.arange(LMAX + 1, 0, -1, device=dev), 0) - torch.arange(LMAX + 1, 0, -1, device=dev),
+        torch.arange(LMAX + 1, 0, -1, device=dev)) + ms
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    sd = 1.0 / (ls.double() + 1.0)
+    s = torch.where(ms == 0, sd, sd / math.sqrt(2.0))
+    re = torch.randn(n, generator=g, device=dev, dtype=torch.float64)
+    im = torch.randn(n, generator=g, device=dev, dtype=torch.float64)
+    alm = torch.complex(re * s, torch.where(ms == 0, torch.zeros_like(im), im * s))
+    m_in = p64.alm2map(alm[None])[0].to(torch.float32)
+    ref = p64.map2alm(m_in.double()[None])[0]
+    p = ref.real ** 2 + ref.imag ** 2
+    cw = torch.where(ms == 0, p, 2.0 * p)
+    cl = torch.zeros(LMAX + 1, dtype=torch.float64, device=dev).index_add_(0, ls, cw)
+    cl /= 2.0 * torch.arange(LMAX + 1, device=dev, dtype=torch.float64) + 1.0
+    for name, a in (("single", p32.map2alm(m_in[None])[0]), ("batch", p32.map2alm(m_in[None].repeat(16, 1))[0])):
+        e = float(((a.to(torch.complex128) - ref).abs() / cl[ls].sqrt()).max())
+        print(f"{name}: eps_max {e:.2e}", flush=True)
+        assert e <= 1e-5
