@@ -55,7 +55,7 @@ struct FftClass {
     int M = 0, log2M = 0, threads = 0;
     int nrings_pow2 = 0, nrings_blue = 0;
     int32_t *d_rings_pow2 = nullptr, *d_rings_blue = nullptr;   // ring indices
-    int64_t tw_off = 0;           // offset of this class's twiddle table
+    int64_t tw_off64 = 0, tw_off32 = 0;   // offsets of this class's FP64 / FP32 twiddle tables
 };
 
 struct Plan {

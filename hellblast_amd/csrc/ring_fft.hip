@@ -31,32 +31,36 @@
 namespace hbx {
 
 // ---------------------------------------------------------------------------
-// FFT engine.  Size M = 2^L, M/16 threads (one for M < 16) hold 16 points
-// each.  Pass s has radix r_s (16 for the first floor(L/4) passes, then
-// 2^(L mod 4) if nonzero) and stride p_s = r_0 ... r_{s-1}; butterfly i
+// FFT engine.  Size M = 2^L, M/P threads (one for M < P) hold P points each
+// (P = 2^q: 16 in FP32, 8 in FP64).  Pass s has radix r_s (P for the first
+// floor(L/q) passes, then 2^(L mod q) if nonzero) and stride
+// p_s = r_0 ... r_{s-1}; butterfly i
 // (0 <= i < M/r_s), k = i mod p_s, reads x_j = buf[i + j M/r_s], multiplies
 // by w_{r_s p_s}^{+-jk}, takes the r_s-point DFT y and writes y_j to
 // buf[(i - k) r_s + k + j p_s]  (Stockham autosort: natural order in and
 // out).  The first pass (p = 1) needs no twiddles and reads through a
 // caller functor; the last writes through one.
 //
-// LDS index x is padded to x + x/16 so that the stride-16 writes of the
-// first passes spread over the banks: (M + M/16) elements per workgroup.
+// LDS index x is padded to x + x/P so that the stride-P writes of the
+// first passes spread over the banks: (M + M/P) elements per workgroup.
 //
 // Twiddle table of a class: for each pass s >= 1, p_s rows of r_s entries,
 // row k = w^{0 .. r_s - 1 times k} with w = e^{-2 pi i / (r_s p_s)}, rows
 // 16-byte aligned so a thread loads its row with vector loads.  Entries are
 // computed in long double on the host and rounded once.
 // ---------------------------------------------------------------------------
-constexpr int FFT_PTS = 16;
+// log2 of the points per thread: 16 in FP32, 8 in FP64 (a radix-8 pass of
+// double-complex values stays near 100 VGPRs, so 4 waves share a SIMD)
+template <typename T> constexpr int fft_lp() { return sizeof(T) == 8 ? 3 : 4; }
+template <typename T> constexpr int fft_pts() { return 1 << fft_lp<T>(); }
 
-__device__ __forceinline__ int pad_idx(int x) { return x + (x >> 4); }
-inline int64_t pad_len(int64_t M) { return M + (M >> 4); }
+template <typename T> __device__ __forceinline__ int pad_idx(int x) { return x + (x >> fft_lp<T>()); }
+template <typename T> inline int64_t pad_len(int64_t M) { return M + (M >> fft_lp<T>()); }
 
 // threads of a ring workgroup of class M (also the ring kernels' stride)
-inline int class_threads(int M) { return std::max(64, M / FFT_PTS); }
+template <typename T> inline int class_threads(int M) { return std::max(64, M / fft_pts<T>()); }
 
-template <typename T> inline size_t ring_lds_bytes(int M) { return (size_t)pad_len(M) * sizeof(c2<T>); }
+template <typename T> inline size_t ring_lds_bytes(int M) { return (size_t)pad_len<T>(M) * sizeof(c2<T>); }
 
 // a * (-i) forward, a * (+i) inverse
 template <typename T, bool INV> __device__ __forceinline__ c2<T> mul_wi(c2<T> a) {
@@ -159,21 +163,28 @@ template <typename T, bool INV> struct Dft<T, 16, INV> {
 };
 
 // v[j] *= w^{jk} (conjugated for the inverse) from a table row of r entries
-template <typename T, int R, bool INV>
-__device__ __forceinline__ void twiddle_row(c2<T> *v, const c2<T> *__restrict__ row) {
-    if constexpr (sizeof(T) == 4) {
-        const float4 *w4 = reinterpret_cast<const float4 *>(row);
+// Twiddle rows of one pass into registers: butterfly b of this thread takes
+// w[b R + j] = row (i_b mod p), entry j (P entries per thread in every pass).
+template <typename T, int R>
+__device__ __forceinline__ void load_rows(c2<T> *w, const c2<T> *__restrict__ tbl, int M, int lp) {
+    constexpr int PTS = fft_pts<T>(), NBF = PTS / R;
+    const int ntf = M / PTS, p = 1 << lp, t = threadIdx.x;
+    if (t < ntf) {
 #pragma unroll
-        for (int q = 0; q < R / 2; ++q) {
-            const float4 w = w4[q];
-            if (q > 0) v[2 * q] = mul_c<T, INV>(v[2 * q], w.x, w.y);
-            v[2 * q + 1] = mul_c<T, INV>(v[2 * q + 1], w.z, w.w);
-        }
-    } else {
+        for (int b = 0; b < NBF; ++b) {
+            const c2<T> *row = tbl + (size_t)((t + b * ntf) & (p - 1)) * R;
+            if constexpr (sizeof(T) == 4) {
+                const float4 *w4 = reinterpret_cast<const float4 *>(row);
 #pragma unroll
-        for (int j = 1; j < R; ++j) {
-            const c2<T> w = row[j];
-            v[j] = mul_c<T, INV>(v[j], w.x, w.y);
+                for (int q = 0; q < R / 2; ++q) {
+                    const float4 x = w4[q];
+                    w[b * R + 2 * q] = mk<T>(x.x, x.y);
+                    w[b * R + 2 * q + 1] = mk<T>(x.z, x.w);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < R; ++j) w[b * R + j] = row[j];
+            }
         }
     }
 }
@@ -181,16 +192,22 @@ __device__ __forceinline__ void twiddle_row(c2<T> *v, const c2<T> *__restrict__ 
 // One pass.  load(q, pq) and post(k, pk, x) get the natural index and its
 // padded LDS index.  Padded addresses advance linearly in j: the read stride
 // M/R and the write stride p of every pass after the first are multiples of
-// 16, and the first pass writes 16 consecutive elements of one 16-block.
-template <typename T, int R, bool INV, bool FIRST, bool LAST, class Load, class Post>
-__device__ __forceinline__ void fft_pass(c2<T> *buf, int M, int lp, const c2<T> *__restrict__ tw, Load &load,
-                                         Post &post) {
-    constexpr int NBF = FFT_PTS / R;                    // butterflies per thread
-    const bool full = M >= FFT_PTS;                     // every FFT thread has NBF butterflies
-    const int ntf = full ? M / FFT_PTS : 1;
+// P, and the first pass writes P consecutive elements of one P-block.
+// FP32: w holds this pass's twiddle rows (load_rows), already in registers;
+// pre() runs after the butterflies and before the barrier and fetches the
+// next pass's rows into w, so their latency overlaps the exchange.  FP64
+// (twice the registers per row): rows are read from the table tw as they
+// are applied.
+template <typename T, int R, bool INV, bool FIRST, bool LAST, class Load, class Post, class Pre>
+__device__ __forceinline__ void fft_pass(c2<T> *buf, int M, int lp, c2<T> *w, const c2<T> *__restrict__ tw,
+                                         Load &load, Post &post, Pre pre) {
+    constexpr int LP = fft_lp<T>(), PTS = 1 << LP;
+    constexpr int NBF = PTS / R;                        // butterflies per thread
+    const bool full = M >= PTS;                         // every FFT thread has NBF butterflies
+    const int ntf = full ? M / PTS : 1;
     const int nbv = full ? NBF : 1;
     const int Mr = M / R, p = 1 << lp;
-    const int sMr = Mr + (Mr >> 4), sp = FIRST ? 1 : p + (p >> 4);
+    const int sMr = Mr + (Mr >> LP), sp = FIRST ? 1 : p + (p >> LP);
     const int t = threadIdx.x;
     c2<T> v[NBF][R];
     if (t < ntf) {
@@ -199,22 +216,33 @@ __device__ __forceinline__ void fft_pass(c2<T> *buf, int M, int lp, const c2<T> 
             if (b < nbv) {
                 const int i = t + b * ntf;
                 if constexpr (FIRST) {
-                    const bool lin = (Mr & 15) == 0;
+                    const bool lin = (Mr & (PTS - 1)) == 0;
 #pragma unroll
                     for (int j = 0; j < R; ++j) {
                         const int q = i + j * Mr;
-                        v[b][j] = load(q, lin ? pad_idx(i) + j * sMr : pad_idx(q));
+                        v[b][j] = load(q, lin ? pad_idx<T>(i) + j * sMr : pad_idx<T>(q));
                     }
                 } else {
-                    const int pi = pad_idx(i);
+                    const int pi = pad_idx<T>(i);
 #pragma unroll
                     for (int j = 0; j < R; ++j) v[b][j] = buf[pi + j * sMr];
-                    twiddle_row<T, R, INV>(v[b], tw + (size_t)(i & (p - 1)) * R);
+                    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+                        for (int j = 1; j < R; ++j) v[b][j] = mul_c<T, INV>(v[b][j], w[b * R + j].x, w[b * R + j].y);
+                    } else {
+                        const c2<T> *row = tw + (size_t)(i & (p - 1)) * R;
+#pragma unroll
+                        for (int j = 1; j < R; ++j) {
+                            const c2<T> x = row[j];
+                            v[b][j] = mul_c<T, INV>(v[b][j], x.x, x.y);
+                        }
+                    }
                 }
                 Dft<T, R, INV>::run(v[b]);
             }
         }
     }
+    pre();
     __syncthreads();
     if (t < ntf) {
 #pragma unroll
@@ -223,7 +251,7 @@ __device__ __forceinline__ void fft_pass(c2<T> *buf, int M, int lp, const c2<T> 
                 const int i = t + b * ntf;
                 const int k = FIRST ? 0 : (i & (p - 1));
                 const int base = (i - k) * R + k;
-                const int pb = pad_idx(base);
+                const int pb = pad_idx<T>(base);
 #pragma unroll
                 for (int j = 0; j < R; ++j) {
                     if constexpr (LAST) post(base + j * p, pb + j * sp, v[b][j]);
@@ -242,8 +270,11 @@ __device__ __forceinline__ void fft_pass(c2<T> *buf, int M, int lp, const c2<T> 
 template <typename T, bool INV, class Load, class Post>
 __device__ __forceinline__ void fft_run(c2<T> *buf, int M, int log2M, const c2<T> *__restrict__ tw, Load load,
                                         Post post) {
-    const int n16 = log2M >> 2, rem = log2M & 3;
-    if (n16 == 0) {
+    constexpr int LP = fft_lp<T>(), PTS = 1 << LP;
+    const int nmain = log2M / LP, rem = log2M % LP;
+    c2<T> w[PTS];
+    auto none = [] {};
+    if (nmain == 0) {
         switch (rem) {
         case 0: {
             c2<T> v = mk<T>(0, 0);
@@ -253,46 +284,70 @@ __device__ __forceinline__ void fft_run(c2<T> *buf, int M, int log2M, const c2<T
             __syncthreads();
             return;
         }
-        case 1: fft_pass<T, 2, INV, true, true>(buf, M, 0, tw, load, post); return;
-        case 2: fft_pass<T, 4, INV, true, true>(buf, M, 0, tw, load, post); return;
-        default: fft_pass<T, 8, INV, true, true>(buf, M, 0, tw, load, post); return;
+        case 1: fft_pass<T, 2, INV, true, true>(buf, M, 0, w, tw, load, post, none); return;
+        case 2: fft_pass<T, 4, INV, true, true>(buf, M, 0, w, tw, load, post, none); return;
+        default:
+            if constexpr (PTS > 8) fft_pass<T, 8, INV, true, true>(buf, M, 0, w, tw, load, post, none);
+            return;
         }
     }
-    if (n16 == 1 && rem == 0) {
-        fft_pass<T, 16, INV, true, true>(buf, M, 0, tw, load, post);
+    if (nmain == 1 && rem == 0) {
+        fft_pass<T, PTS, INV, true, true>(buf, M, 0, w, tw, load, post, none);
         return;
     }
-    fft_pass<T, 16, INV, true, false>(buf, M, 0, tw, load, post);
-    int lp = 4;
+    // rows of the pass after the one with stride 2^lp (table at twp)
+    auto fetch = [&](int s_next, int lp_next, const c2<T> *twp_next) {
+        if (s_next < nmain) {
+            load_rows<T, PTS>(w, twp_next, M, lp_next);
+        } else {
+            switch (rem) {
+            case 1: load_rows<T, 2>(w, twp_next, M, lp_next); break;
+            case 2: load_rows<T, 4>(w, twp_next, M, lp_next); break;
+            case 3:
+                if constexpr (PTS > 8) load_rows<T, 8>(w, twp_next, M, lp_next);
+                break;
+            default: break;
+            }
+        }
+    };
+    // FP32: the rows of pass s + 1 are fetched before pass s's barrier
+    constexpr bool PF = sizeof(T) == 4;
     const c2<T> *twp = tw;
-    for (int s = 1; s < n16; ++s) {
-        if (s == n16 - 1 && rem == 0) fft_pass<T, 16, INV, false, true>(buf, M, lp, twp, load, post);
-        else fft_pass<T, 16, INV, false, false>(buf, M, lp, twp, load, post);
-        twp += (size_t)16 << lp;
-        lp += 4;
+    fft_pass<T, PTS, INV, true, false>(buf, M, 0, w, twp, load, post, [&] { if (PF) fetch(1, LP, twp); });
+    int lp = LP;
+    for (int s = 1; s < nmain; ++s) {
+        const c2<T> *twn = twp + ((size_t)PTS << lp);
+        auto pre = [&] { if (PF && (s + 1 < nmain || rem)) fetch(s + 1, lp + LP, twn); };
+        if (s == nmain - 1 && rem == 0) fft_pass<T, PTS, INV, false, true>(buf, M, lp, w, twp, load, post, none);
+        else fft_pass<T, PTS, INV, false, false>(buf, M, lp, w, twp, load, post, pre);
+        twp = twn;
+        lp += LP;
     }
     switch (rem) {
-    case 1: fft_pass<T, 2, INV, false, true>(buf, M, lp, twp, load, post); break;
-    case 2: fft_pass<T, 4, INV, false, true>(buf, M, lp, twp, load, post); break;
-    case 3: fft_pass<T, 8, INV, false, true>(buf, M, lp, twp, load, post); break;
+    case 1: fft_pass<T, 2, INV, false, true>(buf, M, lp, w, twp, load, post, none); break;
+    case 2: fft_pass<T, 4, INV, false, true>(buf, M, lp, w, twp, load, post, none); break;
+    case 3:
+        if constexpr (PTS > 8) fft_pass<T, 8, INV, false, true>(buf, M, lp, w, twp, load, post, none);
+        break;
     default: break;
     }
 }
 
-// entries of the twiddle table of class M (same pass walk as fft_run)
-inline int64_t fft_table_entries(int M) {
-    const int L = ilog2(M), n16 = L >> 2, rem = L & 3;
+// entries of the twiddle table of class M for 2^LP points per thread (same
+// pass walk as fft_run)
+inline int64_t fft_table_entries(int M, int LP) {
+    const int L = ilog2(M), nmain = L / LP, rem = L % LP;
     int64_t e = 0;
-    int lp = 4;
-    for (int s = 1; s < n16; ++s, lp += 4) e += (int64_t)16 << lp;
-    if (n16 >= 1 && rem) e += (int64_t)(1 << rem) << lp;
+    int lp = LP;
+    for (int s = 1; s < nmain; ++s, lp += LP) e += (int64_t)1 << (LP + lp);
+    if (nmain >= 1 && rem) e += (int64_t)(1 << rem) << lp;
     return e;
 }
 
-// fill the table of class M: FP64 and FP32 roundings of the same long double values
-inline void fft_table_fill(int M, double2 *t64, float2 *t32) {
+// fill the table of class M (2^LP points per thread), rounded from long double
+template <typename C> inline void fft_table_fill(int M, int LP, C *t) {
     const long double pi = 3.141592653589793238462643383279502884L;
-    const int L = ilog2(M), n16 = L >> 2, rem = L & 3;
+    const int L = ilog2(M), nmain = L / LP, rem = L % LP;
     int64_t o = 0;
     auto pass = [&](int r, int lp) {
         const int64_t p = (int64_t)1 << lp, rp = r * p;
@@ -300,15 +355,14 @@ inline void fft_table_fill(int M, double2 *t64, float2 *t32) {
             for (int j = 0; j < r; ++j) {
                 const int64_t e = (j * k) % rp;
                 const long double a = -2.0L * pi * (long double)e / (long double)rp;
-                const long double c = cosl(a), s = sinl(a);
-                t64[o] = make_double2((double)c, (double)s);
-                t32[o] = make_float2((float)c, (float)s);
+                t[o].x = cosl(a);
+                t[o].y = sinl(a);
                 ++o;
             }
     };
-    int lp = 4;
-    for (int s = 1; s < n16; ++s, lp += 4) pass(16, lp);
-    if (n16 >= 1 && rem) pass(1 << rem, lp);
+    int lp = LP;
+    for (int s = 1; s < nmain; ++s, lp += LP) pass(1 << LP, lp);
+    if (nmain >= 1 && rem) pass(1 << rem, lp);
 }
 
 // chirp c_j = exp(i pi j^2 / h), with j^2 reduced mod 2h exactly (plan time)
@@ -334,8 +388,8 @@ template <typename T> __device__ __forceinline__ c2<T> round_c(double2 a) { retu
 // sincos per thread by these FP64 steps (at most a few dozen per thread).
 constexpr int NROT = 4;
 
-// launch bound: FP64 classes stop at M = 8192 (512 threads) by LDS
-template <typename T> constexpr int ring_max_threads() { return sizeof(T) == 8 ? 512 : 1024; }
+// launch bound: FP32 classes stop at M = 16384, FP64 at M = 8192 (LDS)
+template <typename T> constexpr int ring_max_threads() { return 1024; }
 
 // Block g of a ring launch -> (ring slot, map): the maps of one ring run
 // back to back on one XCD (consecutive blocks go round-robin over the 8
@@ -427,8 +481,8 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
             }
         };
         for (int k = tid; 2 * k <= h; k += nt) {
-            const c2<T> za = buf[pad_idx(k)];
-            const c2<T> zb = cconj<T>(buf[pad_idx(k == 0 ? 0 : h - k)]);
+            const c2<T> za = buf[pad_idx<T>(k)];
+            const c2<T> zb = cconj<T>(buf[pad_idx<T>(k == 0 ? 0 : h - k)]);
             const c2<T> E = cscale<T>(cadd<T>(za, zb), (T)0.5);
             const c2<T> O = cmul_mi<T>(cscale<T>(csub<T>(za, zb), (T)0.5));
             const c2<T> Tk = cmul<T>(round_c<T>(W), O);
@@ -449,7 +503,7 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
             c2<T> F = mk<T>(0, 0);
             if (m < cut) {
                 const int ib = (ia == 0) ? 0 : h - ia;
-                const c2<T> za = buf[pad_idx(ia)], zb = cconj<T>(buf[pad_idx(ib)]);
+                const c2<T> za = buf[pad_idx<T>(ia)], zb = cconj<T>(buf[pad_idx<T>(ib)]);
                 const c2<T> A = cscale<T>(cadd<T>(za, zb), (T)0.5);
                 const c2<T> D = cscale<T>(csub<T>(za, zb), (T)0.5);
                 const c2<T> X = cadd<T>(A, cmul<T>(round_c<T>(W), cmul_mi<T>(D)));
@@ -562,19 +616,19 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
             const c2<T> s = cadd<T>(Xa, cconj<T>(Xb)), d = csub<T>(Xa, cconj<T>(Xb));
             c2<T> z = cadd<T>(s, cmul_pi<T>(cmul<T>(Wk, d)));
             if (BLUE) z = cconj<T>(cmul<T>(z, Cr[k]));
-            buf[pad_idx(k)] = z;
+            buf[pad_idx<T>(k)] = z;
         }
         if (k != 0 && k2 != k) {
             const c2<T> s = cadd<T>(Xb, cconj<T>(Xa)), d = csub<T>(Xb, cconj<T>(Xa));
             c2<T> z = cadd<T>(s, cmul_pi<T>(cmul<T>(Wk2, d)));
             if (BLUE) z = cconj<T>(cmul<T>(z, Cr[k2]));
-            buf[pad_idx(k2)] = z;
+            buf[pad_idx<T>(k2)] = z;
         }
         W = zmul(W, Wst);
         if (rotp) R = zmul(R, S);
     }
     if (BLUE)
-        for (int j = h + tid; j < M; j += nt) buf[pad_idx(j)] = mk<T>(0, 0);
+        for (int j = h + tid; j < M; j += nt) buf[pad_idx<T>(j)] = mk<T>(0, 0);
     __syncthreads();
     const T sc = (T)(scale ? scale[r] : 1.0);
     TOut *f = maps + (int64_t)b * map_stride + ring_start[r];
@@ -774,7 +828,7 @@ static int build_chirp_spectra(const std::vector<int32_t> &hs_host, const std::v
     const size_t lds = ring_lds_bytes<T>(M);
     int rc = allow_lds<T>(chirp_spectrum_kernel<T>, lds);
     if (rc == 0) {
-        hipLaunchKernelGGL(chirp_spectrum_kernel<T>, dim3((unsigned)hs_host.size()), dim3(class_threads(M)), lds, 0,
+        hipLaunchKernelGGL(chirp_spectrum_kernel<T>, dim3((unsigned)hs_host.size()), dim3(class_threads<T>(M)), lds, 0,
                            d_hs, d_ofs, V, tw, M, ilog2(M));
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = hip_error(e, "chirp_spectrum_kernel");
@@ -840,12 +894,14 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
     int rc;
     if ((rc = dalloc((void **)&p->d_vofs, nr * sizeof(int64_t)))) return rc;
     HBX_HIP(hipMemcpy(p->d_vofs, vofs.data(), nr * sizeof(int64_t), hipMemcpyHostToDevice));
+    // threads of a ring workgroup at this plan's precision
+    auto thr = [&](int M) { return p->precision == HBX_F64 ? class_threads<double>(M) : class_threads<float>(M); };
     // per-ring rotation steps (long double on the host, rounded to FP64)
     {
         std::vector<double2> rot((size_t)NROT * nr);
         const long double pi = 3.141592653589793238462643383279502884L;
         for (int r = 0; r < nr; ++r) {
-            const int n = nphi[r], h = n / 2, nt = class_threads(ringM[r]);
+            const int n = nphi[r], h = n / 2, nt = thr(ringM[r]);
             const long double ph = phi0 ? (long double)phi0[r] : 0.0L;
             auto e = [](long double a) { return make_double2((double)cosl(a), (double)sinl(a)); };
             rot[r] = e(nt * ph);
@@ -860,14 +916,16 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
     std::map<int, int> all_M;
     for (auto &kv : pow2_rings) all_M[kv.first] = 1;
     for (auto &kv : blue_rings) all_M[kv.first] = 1;
-    int64_t tw_total = 0;
+    int64_t tw_total64 = 0, tw_total32 = 0;
     for (auto &kv : all_M) {
         FftClass c;
         c.M = kv.first;
         c.log2M = ilog2(c.M);
-        c.threads = class_threads(c.M);
-        c.tw_off = tw_total;
-        tw_total += (fft_table_entries(c.M) + 1) & ~int64_t(1);     // rows stay 16-byte aligned
+        c.threads = thr(c.M);
+        c.tw_off64 = tw_total64;                                   // FP64 engine: 8 points per thread
+        c.tw_off32 = tw_total32;                                   // FP32 engine: 16 points per thread
+        tw_total64 += fft_table_entries(c.M, fft_lp<double>());
+        tw_total32 += (fft_table_entries(c.M, fft_lp<float>()) + 1) & ~int64_t(1);   // rows stay 16-byte aligned
         auto up = [&](const std::vector<int32_t> &v, int32_t **d) -> int {
             if (v.empty()) return 0;
             int e = dalloc((void **)d, v.size() * sizeof(int32_t));
@@ -886,24 +944,27 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
         p->classes.push_back(c);
     }
     {
-        std::vector<double2> t64((size_t)std::max<int64_t>(tw_total, 1), make_double2(0.0, 0.0));
-        std::vector<float2> t32(t64.size(), make_float2(0.f, 0.f));
-        for (const auto &c : p->classes) fft_table_fill(c.M, t64.data() + c.tw_off, t32.data() + c.tw_off);
+        std::vector<double2> t64((size_t)std::max<int64_t>(tw_total64, 1), make_double2(0.0, 0.0));
+        std::vector<float2> t32((size_t)std::max<int64_t>(tw_total32, 2), make_float2(0.f, 0.f));
+        for (const auto &c : p->classes) {
+            fft_table_fill(c.M, fft_lp<double>(), t64.data() + c.tw_off64);
+            fft_table_fill(c.M, fft_lp<float>(), t32.data() + c.tw_off32);
+        }
         if ((rc = dalloc((void **)&p->d_ftw64, t64.size() * sizeof(double2)))) return rc;
         if ((rc = dalloc((void **)&p->d_ftw32, t32.size() * sizeof(float2)))) return rc;
         HBX_HIP(hipMemcpy(p->d_ftw64, t64.data(), t64.size() * sizeof(double2), hipMemcpyHostToDevice));
         HBX_HIP(hipMemcpy(p->d_ftw32, t32.data(), t32.size() * sizeof(float2), hipMemcpyHostToDevice));
     }
-    auto tw_of = [&](int M) -> int64_t {
+    auto tw_of = [&](int M, bool f64) -> int64_t {
         for (const auto &c : p->classes)
-            if (c.M == M) return c.tw_off;
+            if (c.M == M) return f64 ? c.tw_off64 : c.tw_off32;
         return 0;
     };
     if (p->precision == HBX_F64) {
         if ((rc = dalloc((void **)&p->d_V64, vtotal * sizeof(double2)))) return rc;
         for (auto &kv : hs_by_M)
             if ((rc = build_chirp_spectra<double>(kv.second, ofs_by_M[kv.first], kv.first, p->d_V64,
-                                                  p->d_ftw64 + tw_of(kv.first))))
+                                                  p->d_ftw64 + tw_of(kv.first, true))))
                 return rc;
     } else {
         if ((rc = dalloc((void **)&p->d_V32, vtotal * sizeof(float2)))) return rc;
@@ -915,10 +976,10 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
             const bool f64 = ring_lds_bytes<double>(kv.first) <= (size_t)maxlds;
             if (f64)
                 rc = build_chirp_spectra<double>(kv.second, ofs_by_M[kv.first], kv.first, tmp,
-                                                 p->d_ftw64 + tw_of(kv.first));
+                                                 p->d_ftw64 + tw_of(kv.first, true));
             else
                 rc = build_chirp_spectra<float>(kv.second, ofs_by_M[kv.first], kv.first, p->d_V32,
-                                                p->d_ftw32 + tw_of(kv.first));
+                                                p->d_ftw32 + tw_of(kv.first, false));
             if (rc) break;
             // round this class's entries (every distinct h of the class)
             for (size_t i = 0; i < kv.second.size() && f64; ++i) {
@@ -961,8 +1022,8 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
 }
 
 template <typename T> static const c2<T> *class_tw(const Plan *p, const FftClass &c);
-template <> const double2 *class_tw<double>(const Plan *p, const FftClass &c) { return p->d_ftw64 + c.tw_off; }
-template <> const float2 *class_tw<float>(const Plan *p, const FftClass &c) { return p->d_ftw32 + c.tw_off; }
+template <> const double2 *class_tw<double>(const Plan *p, const FftClass &c) { return p->d_ftw64 + c.tw_off64; }
+template <> const float2 *class_tw<float>(const Plan *p, const FftClass &c) { return p->d_ftw32 + c.tw_off32; }
 
 template <typename T, typename TIn>
 static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t map_stride,
