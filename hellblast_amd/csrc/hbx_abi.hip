@@ -80,7 +80,7 @@ static void *band_phase_ptr(const Plan *p, int nmaps, const void *phase) {
 }
 
 struct AnalysisWs {
-    uint64_t partial = 0, band_alm = 0, band_partial = 0, total = 0;
+    uint64_t partial = 0, band_alm = 0, band_partial = 0, lfirst = 0, total = 0;
 };
 
 static AnalysisWs analysis_ws(const Plan *p, int nmaps) {
@@ -97,6 +97,9 @@ static AnalysisWs analysis_ws(const Plan *p, int nmaps) {
         w.band_partial = off;
         off += align256(g64 > 1 ? (uint64_t)nmaps * g64 * band_nlm(p) * 16 : 0);
     }
+    // first l with a nonzero partial sum, per (m, ring-pair group)
+    w.lfirst = off;
+    off += align256(g > 1 ? (uint64_t)(p->mmax + 1) * g * sizeof(int32_t) : 0);
     w.total = off;
     return w;
 }
@@ -259,7 +262,8 @@ static int legendre_analysis(const Plan *p, int nmaps, const void *phase, void *
     char *t = static_cast<char *>(tail);
     int rc;
     if (p->precision == HBX_F32 && nmaps >= mfma_min_maps(p))
-        rc = launch_leg_analysis_mfma(p, nmaps, phase, alm, alm_stride, t + w.partial, adjoint, s);
+        rc = launch_leg_analysis_mfma(p, nmaps, phase, alm, alm_stride, t + w.partial,
+                                      reinterpret_cast<int32_t *>(t + w.lfirst), adjoint, s);
     else
         rc = launch_leg_analysis(p, nmaps, phase, alm, alm_stride, t + w.partial, adjoint, s);
     if (rc || p->precision != HBX_F32) return rc;

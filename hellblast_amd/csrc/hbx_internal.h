@@ -138,10 +138,12 @@ int mfma_kind_of(const Plan *p);   // HBX_MFMA_F32 or HBX_MFMA_F16X2
 int plan_build_syn_chunks(Plan *p);
 uint64_t syn_prep_bytes(const Plan *p, int nmaps);   // split-fp16 synthesis workspace
 int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *alm, int64_t alm_stride,
-                             void *partial, int adjoint, hipStream_t s);
+                             void *partial, int32_t *lfirst, int adjoint, hipStream_t s);
+// lfirst (may be null): [m][group] first l whose partial sum was written;
+// below it the group's partial is zero and is not read
 int launch_group_reduce(const Plan *p, int nmaps, void *partial, void *alm, int64_t alm_stride,
                         int ngroups, int64_t nlm_stride, int mbeg, int mend, bool f64, int adjoint,
-                        hipStream_t s);
+                        hipStream_t s, const int32_t *lfirst = nullptr);
 int launch_leg_analysis_band(const Plan *p, int nmaps, const void *band_phase, void *alm,
                              int64_t alm_stride, void *band_alm, void *band_partial, int adjoint,
                              hipStream_t s);

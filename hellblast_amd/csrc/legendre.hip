@@ -362,11 +362,13 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
     }
 }
 
-// alm[idx(l,m)] = s_lm * sum_g partial[g][idx(l,m)] (fixed summation order)
+// alm[idx(l,m)] = s_lm * sum_g partial[g][idx(l,m)] (fixed summation order);
+// with lfirst, group g's partial is read only for l >= lfirst[m][g] (zero below)
 template <typename T>
 __global__ void group_reduce_kernel(const c2<T> *__restrict__ partial, c2<T> *__restrict__ alm, int64_t astride,
                                     const int64_t *__restrict__ rofs, const double *__restrict__ sl, int lmax,
-                                    int ngroups, int64_t nlm, int adjoint, int mbeg) {
+                                    int ngroups, int64_t nlm, int adjoint, int mbeg,
+                                    const int32_t *__restrict__ lfirst) {
     const int m = mbeg + blockIdx.x, b = blockIdx.y;
     const int64_t i0 = alm_index(m, m, lmax);
     const int64_t ro = rofs[m];
@@ -375,6 +377,7 @@ __global__ void group_reduce_kernel(const c2<T> *__restrict__ partial, c2<T> *__
     for (int j = threadIdx.x; j <= lmax - m; j += blockDim.x) {
         double re = 0.0, im = 0.0;
         for (int g = 0; g < ngroups; ++g) {
+            if (lfirst && m + j < lfirst[(int64_t)m * ngroups + g]) continue;
             const c2<T> v = src[(int64_t)g * nlm + j];
             re += (double)v.x;
             im += (double)v.y;
@@ -553,14 +556,17 @@ static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, int64_t 
 }
 
 int launch_group_reduce(const Plan *p, int nmaps, void *partial, void *alm, int64_t astride, int ngroups,
-                        int64_t nlm_stride, int mbeg, int mend, bool f64, int adjoint, hipStream_t s) {
+                        int64_t nlm_stride, int mbeg, int mend, bool f64, int adjoint, hipStream_t s,
+                        const int32_t *lfirst) {
     dim3 grid(mend - mbeg + 1, nmaps);
     if (f64)
         hipLaunchKernelGGL(group_reduce_kernel<double>, grid, dim3(256), 0, s, (const double2 *)partial,
-                           (double2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups, nlm_stride, adjoint, mbeg);
+                           (double2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups, nlm_stride, adjoint, mbeg,
+                           lfirst);
     else
         hipLaunchKernelGGL(group_reduce_kernel<float>, grid, dim3(256), 0, s, (const float2 *)partial,
-                           (float2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups, nlm_stride, adjoint, mbeg);
+                           (float2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups, nlm_stride, adjoint, mbeg,
+                           lfirst);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "group_reduce_kernel launch");
     return 0;

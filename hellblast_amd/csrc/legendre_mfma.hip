@@ -816,7 +816,7 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
                         const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
                         const double *__restrict__ sl, const int32_t *__restrict__ lseed,
                         const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
-                        int64_t nlm, int nmaps, int adjoint, int mbeg, float neg1) {
+                        int64_t nlm, int nmaps, int adjoint, int mbeg, float neg1, int32_t *__restrict__ lfirst) {
     using G = H16Geom<Q>;
     extern __shared__ __align__(16) unsigned char hsm[];
     unsigned char *thi = hsm, *tlo = hsm + G::TILEB;
@@ -945,9 +945,13 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
             partial[((int64_t)b * ngroups + grp) * nlm + i0 + l] = v;
         }
     };
-    // coefficients below every ring pair's switch-on are zero
+    // coefficients below every ring pair's switch-on are zero: written to alm
+    // by a single group; partial sums record where theirs start instead
     const int lz = lstart > lmax ? lmax + 1 : lstart;
-    for (int e = tid; e < (lz - m) * 16; e += G::W * 64) store(m + (e >> 4), g * 16 + (e & 15), make_float2(0.f, 0.f));
+    if (direct)
+        for (int e = tid; e < (lz - m) * 16; e += G::W * 64) store(m + (e >> 4), g * 16 + (e & 15), make_float2(0.f, 0.f));
+    else if (g == 0 && tid == 0)
+        lfirst[(int64_t)m * ngroups + grp] = lz;
     if (lstart > lmax) return;
 
     if (tid < ANA_L) alds[tid] = al[lstart + tid];
@@ -1443,7 +1447,7 @@ int mfma_analysis_groups(const Plan *p) {
 }
 
 int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *alm, int64_t astride, void *partial,
-                             int adjoint, hipStream_t s) {
+                             int32_t *lfirst, int adjoint, hipStream_t s) {
     const int mbeg = std::min(M_LO, p->mmax + 1);    // m < M_LO run in the FP64 band
     if (mbeg > p->mmax) return 0;
     const int W = mfma_analysis_waves(p);
@@ -1462,7 +1466,7 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
     hipLaunchKernelGGL(leg_analysis_h16_kernel<QV>, grid, dim3(H16Geom<QV>::W * 64), H16Geom<QV>::LDS, s,      \
                        (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,       \
                        p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,     \
-                       p->mmax, G, p->nlm, nmaps, adjoint, mbeg, -1.0f)
+                       p->mmax, G, p->nlm, nmaps, adjoint, mbeg, -1.0f, lfirst)
         switch (W) {
         case 4: HBX_AH(1); break;
         case 8: HBX_AH(2); break;
@@ -1485,7 +1489,9 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
     timer_mark(HBX_TIMER_LEGENDRE_ANALYSIS, false, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "leg_analysis_mfma_kernel launch");
-    if (G > 1) return launch_group_reduce(p, nmaps, partial, alm, astride, G, p->nlm, mbeg, p->mmax, false, adjoint, s);
+    if (G > 1)
+        return launch_group_reduce(p, nmaps, partial, alm, astride, G, p->nlm, mbeg, p->mmax, false, adjoint, s,
+                                   mfma_kind_of(p) == HBX_MFMA_F16X2 ? lfirst : nullptr);
     return 0;
 }
 
