@@ -106,16 +106,35 @@ static AnalysisWs analysis_ws(const Plan *p, int nmaps) {
 
 static uint64_t ws_tail_offset(const Plan *p, int nmaps) { return phase_total_bytes(p, nmaps); }
 
-// Spin plans, per polarisation pair: phase of Q and U | tail = scaled (E, B)
-// or the partial sums of both.
-static uint64_t spin_ws_bytes(const Plan *p, int npol) {
+// Spin plans, per polarisation pair: phase of Q and U | (FP32 plans) the FP64
+// band: phase, alm and partial sums of m < M_LO | tail = scaled (E, B) or the
+// partial sums of both.  Synthesis puts scaled right after the phase.
+struct SpinWs {
+    uint64_t phase = 0, band_phase = 0, band_alm = 0, band_partial = 0, tail = 0, total = 0;
+};
+
+static SpinWs spin_ws(const Plan *p, int npol) {
+    SpinWs w;
     const uint64_t cs = csize(p), n2 = 2 * (uint64_t)npol;
-    const uint64_t phase = align256(n2 * phase_elems_per_map(p) * cs);
-    const uint64_t scaled = align256(n2 * scaled_elems_per_map(p) * cs);
     const int g = spin_groups(p);
+    w.phase = align256(n2 * phase_elems_per_map(p) * cs);
+    uint64_t off = w.phase;
+    if (p->precision == HBX_F32) {
+        w.band_phase = off;
+        off += align256(n2 * band_phase_elems(p) * 16);
+        w.band_alm = off;
+        off += align256(n2 * band_nlm(p) * 16);
+        w.band_partial = off;
+        off += align256(g > 1 ? n2 * g * band_nlm(p) * 16 : 0);
+    }
+    w.tail = off;
+    const uint64_t scaled = align256(n2 * scaled_elems_per_map(p) * cs);
     const uint64_t part = align256(g > 1 ? n2 * g * p->nlm * cs : 0);
-    return phase + std::max(scaled, part);
+    w.total = std::max(w.phase + scaled, w.tail + part);
+    return w;
 }
+
+static uint64_t spin_ws_bytes(const Plan *p, int npol) { return spin_ws(p, npol).total; }
 
 static uint64_t ws_bytes(const Plan *p, int nmaps) {
     if (p->spin != 0) return spin_ws_bytes(p, nmaps);
@@ -211,6 +230,8 @@ static int build_plan(Plan *p, const hbx_geom *g) {
     if ((rc = upload(p, &p->d_pair_n, pn.data(), np))) return rc;
     if ((rc = upload(p, &p->d_pair_s, ps.data(), np))) return rc;
     if ((rc = upload(p, &p->d_px, px.data(), np))) return rc;
+    // FP32 spin plans analyse Q/U rings in FP64 (output rounded once to FP32)
+    p->ring_f64 = p->precision == HBX_F32 && p->spin != 0;
     if ((rc = plan_build_fft_tables(p, nphi, any_phi0 ? g->phi0 : nullptr))) return rc;
     // switch-on tolerance: FP64 plans 2^-64, FP32 plans 2^-40 (relative
     // contributions far below each format's rounding error)
@@ -470,12 +491,18 @@ static int spin_analysis(const hbx_plan *plan, int32_t npol, const void *maps, i
     const Plan *p = &plan->impl;
     if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    const uint64_t phase = align256(2 * (uint64_t)npol * phase_elems_per_map(p) * csize(p));
-    void *tail = static_cast<char *>(ws) + phase;
-    if ((rc = launch_ring_analysis(p, 2 * npol, maps, map_dtype, map_stride, ws, adjoint ? nullptr : p->d_scale_ana,
-                                   s)))
+    const SpinWs w = spin_ws(p, npol);
+    char *base = static_cast<char *>(ws);
+    const double *scale = adjoint ? nullptr : p->d_scale_ana;
+    if ((rc = launch_ring_analysis(p, 2 * npol, maps, map_dtype, map_stride, ws, scale, s))) return rc;
+    if (p->precision == HBX_F32 &&
+        (rc = launch_ring_band(p, 2 * npol, maps, map_dtype, map_stride, base + w.band_phase, scale, s)))
         return rc;
-    return launch_spin_analysis(p, npol, ws, alm, alm_stride, tail, adjoint, s);
+    if ((rc = launch_spin_analysis(p, npol, ws, alm, alm_stride, base + w.tail, adjoint, s))) return rc;
+    if (p->precision != HBX_F32) return 0;
+    // m < M_LO in FP64 (the low-m rows carry most of the power of red spectra)
+    return launch_spin_analysis_band(p, npol, base + w.band_phase, alm, alm_stride, base + w.band_alm,
+                                     base + w.band_partial, adjoint, s);
 }
 
 static int spin_synthesis(const hbx_plan *plan, int32_t npol, const void *alm, int64_t alm_stride, void *maps,

@@ -74,6 +74,11 @@ struct Plan {
     int64_t *d_cofs = nullptr;             // [mmax + 2] prepared synthesis chunks of row m start here
     int64_t syn_chunks = 0;
     double2 *d_rot = nullptr;              // [4][nrings] FP64 rotation steps of the ring kernels
+    double2 *d_rot64 = nullptr;            // the same for FP64 analysis transforms of an FP32 plan
+    // FP32 plan whose ring analysis runs in FP64 (output rounded once to FP32):
+    // spin plans, where B's weak high-l signal sits below the FP32 FFT's
+    // absolute error
+    bool ring_f64 = false;
     // ring pairs (device)
     int32_t *d_pair_n = nullptr, *d_pair_s = nullptr;
     double *d_px = nullptr;                // cos theta of the north ring of each pair
@@ -118,6 +123,9 @@ int plan_build_spin(Plan *p, const std::vector<double> &pair_cos, const std::vec
                     double tol_log2);
 int spin_groups(const Plan *p);
 int spin_slots_of(const Plan *p);
+int launch_spin_analysis_band(const Plan *p, int npol, const void *band_phase, void *alm, int64_t alm_stride,
+                              void *band_alm, void *band_partial, int adjoint, hipStream_t s);
+int launch_band_to_f32(const Plan *p, int nrows, const void *band_alm, void *alm, int64_t alm_stride, hipStream_t s);
 int launch_spin_analysis(const Plan *p, int npol, const void *phase, void *alm, int64_t alm_stride, void *partial,
                          int adjoint, hipStream_t s);
 int launch_spin_prescale(const Plan *p, int npol, const void *alm, int64_t alm_stride, void *scaled, int adjoint,

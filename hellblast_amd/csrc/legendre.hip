@@ -606,7 +606,12 @@ int launch_leg_analysis_band(const Plan *p, int nmaps, const void *band_phase, v
         rc = leg_analysis_t<double>(p, nmaps, (const double2 *)band_phase, band_phase_elems(p), (double2 *)band_alm,
                                     nb, (double2 *)band_partial, nb, 0, mend, adjoint, s, 0, band_slots(p));
     if (rc) return rc;
-    hipLaunchKernelGGL(band_to_f32_kernel, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024), nmaps),
+    return launch_band_to_f32(p, nmaps, band_alm, alm, astride, s);
+}
+
+int launch_band_to_f32(const Plan *p, int nrows, const void *band_alm, void *alm, int64_t astride, hipStream_t s) {
+    const int64_t nb = band_nlm(p);
+    hipLaunchKernelGGL(band_to_f32_kernel, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024), nrows),
                        dim3(256), 0, s, (const double2 *)band_alm, nb, (float2 *)alm, astride);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "band_to_f32_kernel launch");

@@ -405,10 +405,12 @@ inline unsigned ring_grid(int ncls, int nmaps) { return (unsigned)(((ncls + 7) /
 // ---------------------------------------------------------------------------
 // analysis: maps -> phase.  BLUE: Bluestein rings (h not a power of two),
 // V + vofs[r]: chirp spectrum (M entries) then the chirp c_j, j < h.
+// TP: precision of the phase written (FP64 transforms of FP32 spin plans
+// round their output once to FP32).
 // ---------------------------------------------------------------------------
-template <typename T, typename TIn, bool BLUE>
+template <typename T, typename TIn, bool BLUE, typename TP = T>
 __global__ void __launch_bounds__(ring_max_threads<T>())
-ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__restrict__ phase,
+ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<TP> *__restrict__ phase,
                      int64_t phase_stride, const int32_t *__restrict__ rings,
                      const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
                      const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
@@ -453,8 +455,8 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
     const int cut = mcut[r];
     const double ph0 = phi0 ? phi0[r] : 0.0;
     const bool rotp = (ph0 != 0.0);
-    c2<T> *out = phase + (int64_t)b * phase_stride;
-    auto outv = [&](int m) -> c2<T> & { return out[((int64_t)(m >> 3) * nr + r) * MB + (m & 7)]; };
+    c2<TP> *out = phase + (int64_t)b * phase_stride;
+    auto put_out = [&](int m, c2<T> F) { out[((int64_t)(m >> 3) * nr + r) * MB + (m & 7)] = mk<TP>((TP)F.x, (TP)F.y); };
     double2 W, P = make_double2(1.0, 0.0);
     {
         double s, c;
@@ -477,7 +479,7 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
             if (m < lim) {
                 c2<T> F = cscale<T>(X, sc);
                 if (rotp) F = cmul<T>(F, round_c<T>(Pm));
-                outv(m) = F;
+                put_out(m, F);
             }
         };
         for (int k = tid; 2 * k <= h; k += nt) {
@@ -494,7 +496,7 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
             W = zmul(W, Wst);
             if (rotp) P = zmul(P, Pst);
         }
-        for (int m = lim + tid; m <= mmax; m += nt) outv(m) = mk<T>(0, 0);
+        for (int m = lim + tid; m <= mmax; m += nt) put_out(m, mk<T>(0, 0));
     } else {
         // alias copies (mcut > nphi): one m per step, bin m mod n
         const int hstep = nt % h;
@@ -510,7 +512,7 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<T> *__
                 F = cscale<T>(X, sc);
                 if (rotp) F = cmul<T>(F, round_c<T>(P));
             }
-            outv(m) = F;
+            put_out(m, F);
             W = zmul(W, Wst);
             if (rotp) P = zmul(P, Pst);
             ia += hstep;
@@ -878,6 +880,7 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
     HBX_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeSharedMemPerBlockOptin, dev));
     if (maxlds <= 0) HBX_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     const size_t need = p->precision == HBX_F64 ? ring_lds_bytes<double>(maxM) : ring_lds_bytes<float>(maxM);
+    if (p->ring_f64 && ring_lds_bytes<double>(maxM) > (size_t)maxlds) p->ring_f64 = false;
     if (need > (size_t)maxlds)
         return set_error(HBX_EUNSUPPORTED, "ring FFT of size " + std::to_string(maxM) +
                                                " does not fit in LDS (" + std::to_string(maxlds) +
@@ -911,6 +914,17 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
         }
         if ((rc = dalloc((void **)&p->d_rot, rot.size() * sizeof(double2)))) return rc;
         HBX_HIP(hipMemcpy(p->d_rot, rot.data(), rot.size() * sizeof(double2), hipMemcpyHostToDevice));
+        if (p->ring_f64) {          // FP64 analysis transforms of an FP32 plan: FP64 thread counts
+            for (int r = 0; r < nr; ++r) {
+                const int n = nphi[r], nt = class_threads<double>(ringM[r]);
+                const long double ph = phi0 ? (long double)phi0[r] : 0.0L;
+                auto e = [](long double a) { return make_double2((double)cosl(a), (double)sinl(a)); };
+                rot[r] = e(nt * ph);
+                rot[3 * (size_t)nr + r] = e(-2.0L * pi * (long double)(nt % n) / (long double)n);
+            }
+            if ((rc = dalloc((void **)&p->d_rot64, rot.size() * sizeof(double2)))) return rc;
+            HBX_HIP(hipMemcpy(p->d_rot64, rot.data(), rot.size() * sizeof(double2), hipMemcpyHostToDevice));
+        }
     }
     // size classes and their twiddle tables (FP64 and FP32 roundings)
     std::map<int, int> all_M;
@@ -989,7 +1003,13 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
             }
         }
         hipError_t e = hipDeviceSynchronize();
-        if (tmp) (void)hipFree(tmp);
+        if (tmp && p->ring_f64 && rc == 0) {        // kept: the FP64 analysis transforms use them
+            p->d_V64 = tmp;
+            p->allocations.push_back(tmp);
+            p->device_bytes += vtotal * sizeof(double2);
+        } else if (tmp) {
+            (void)hipFree(tmp);
+        }
         if (rc) return rc;
         if (e != hipSuccess) return hip_error(e, "FP32 chirp spectra");
     }
@@ -1015,6 +1035,13 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
             if ((rc = allow_lds<float>(ring_synthesis_kernel<float, float, false>, lds))) return rc;
             if ((rc = allow_lds<float>(ring_synthesis_kernel<float, double, true>, lds))) return rc;
             if ((rc = allow_lds<float>(ring_synthesis_kernel<float, float, true>, lds))) return rc;
+            if (p->ring_f64) {
+                const size_t l64 = ring_lds_bytes<double>(c.M);
+                if ((rc = allow_lds<double>(ring_analysis_kernel<double, double, false, float>, l64))) return rc;
+                if ((rc = allow_lds<double>(ring_analysis_kernel<double, float, false, float>, l64))) return rc;
+                if ((rc = allow_lds<double>(ring_analysis_kernel<double, double, true, float>, l64))) return rc;
+                if ((rc = allow_lds<double>(ring_analysis_kernel<double, float, true, float>, l64))) return rc;
+            }
         }
     }
     HBX_HIP(hipDeviceSynchronize());
@@ -1025,23 +1052,26 @@ template <typename T> static const c2<T> *class_tw(const Plan *p, const FftClass
 template <> const double2 *class_tw<double>(const Plan *p, const FftClass &c) { return p->d_ftw64 + c.tw_off64; }
 template <> const float2 *class_tw<float>(const Plan *p, const FftClass &c) { return p->d_ftw32 + c.tw_off32; }
 
-template <typename T, typename TIn>
+template <typename T, typename TIn, typename TP = T>
 static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t map_stride,
-                           c2<T> *phase, const double *scale, const c2<T> *V, hipStream_t s) {
+                           c2<TP> *phase, const double *scale, const c2<T> *V, hipStream_t s) {
     const int64_t pstride = phase_elems_per_map(p);
+    // FP64 transforms of an FP32 plan use their own rotation steps (thread count)
+    const double2 *rot = (sizeof(T) != sizeof(TP)) ? p->d_rot64 : p->d_rot;
     timer_mark(HBX_TIMER_RING_ANALYSIS, true, s);
     for (const auto &c : p->classes) {
         const size_t lds = ring_lds_bytes<T>(c.M);
         const c2<T> *tw = class_tw<T>(p, c);
+        const int nt = class_threads<T>(c.M);
         if (c.nrings_pow2)
-            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, false>), dim3(ring_grid(c.nrings_pow2, nmaps)),
-                               dim3(c.threads), lds, s, maps, map_stride, phase, pstride, c.d_rings_pow2,
-                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
+            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, false, TP>), dim3(ring_grid(c.nrings_pow2, nmaps)),
+                               dim3(nt), lds, s, maps, map_stride, phase, pstride, c.d_rings_pow2,
+                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, rot, tw, V, p->d_vofs,
                                c.M, c.log2M, p->mmax, p->nrings, c.nrings_pow2, nmaps);
         if (c.nrings_blue)
-            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, true>), dim3(ring_grid(c.nrings_blue, nmaps)),
-                               dim3(c.threads), lds, s, maps, map_stride, phase, pstride, c.d_rings_blue,
-                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
+            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, true, TP>), dim3(ring_grid(c.nrings_blue, nmaps)),
+                               dim3(nt), lds, s, maps, map_stride, phase, pstride, c.d_rings_blue,
+                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, rot, tw, V, p->d_vofs,
                                c.M, c.log2M, p->mmax, p->nrings, c.nrings_blue, nmaps);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_error(e, "ring_analysis_kernel launch");
@@ -1084,6 +1114,13 @@ int launch_ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dty
         return ring_analysis_t<double, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_V64, s);
     }
     auto *ph = static_cast<float2 *>(phase);
+    if (p->ring_f64) {
+        if (map_dtype == HBX_DT_F64)
+            return ring_analysis_t<double, double, float>(p, nmaps, (const double *)maps, map_stride, ph, scale,
+                                                          p->d_V64, s);
+        return ring_analysis_t<double, float, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_V64,
+                                                     s);
+    }
     if (map_dtype == HBX_DT_F64)
         return ring_analysis_t<float, double>(p, nmaps, (const double *)maps, map_stride, ph, scale, p->d_V32, s);
     return ring_analysis_t<float, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_V32, s);

@@ -543,6 +543,35 @@ static int spin_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, c2<T> *
     return 0;
 }
 
+// FP64 band of FP32 spin plans: E/B rows m < M_LO from the FP64 band phase
+// of Q and U (ring_band_kernel), with the FP64 kernel, then rounded into alm
+int launch_spin_analysis_band(const Plan *p, int npol, const void *band_phase, void *alm, int64_t astride,
+                              void *band_alm, void *band_partial, int adjoint, hipStream_t s) {
+    const int R = spin_slots_of(p);
+    const int G = spin_groups(p);
+    const int mtop = std::min(M_LO - 1, p->mmax);
+    const int64_t nb = band_nlm(p);
+    dim3 grid(8 * MB * G, npol);                   // xcd_map over 8 m-blocks; only m-block 0 is work
+#define HBX_SB(RV)                                                                                            \
+    hipLaunchKernelGGL((spin_analysis_kernel<double, RV>), grid, dim3(64), 0, s, (const double2 *)band_phase,   \
+                       band_phase_elems(p), (double2 *)band_alm, nb, (double2 *)band_partial, p->d_px,         \
+                       p->d_rofs, p->d_alpha, p->d_beta, p->d_sl, p->d_lseed, p->d_seed4, p->npairs, p->nrings, \
+                       p->lmax, mtop, G, nb, p->spin, adjoint)
+    switch (R) {
+    case 1: HBX_SB(1); break;
+    case 2: HBX_SB(2); break;
+    case 4: HBX_SB(4); break;
+    default: return set_error(HBX_EUNSUPPORTED, "spin slot count");
+    }
+#undef HBX_SB
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "spin_analysis_kernel (band) launch");
+    int rc;
+    if (G > 1 && (rc = launch_group_reduce(p, 2 * npol, band_partial, band_alm, nb, G, nb, 0, mtop, true, adjoint, s)))
+        return rc;
+    return launch_band_to_f32(p, 2 * npol, band_alm, alm, astride, s);
+}
+
 int launch_spin_analysis(const Plan *p, int nmaps, const void *phase, void *alm, int64_t astride, void *partial,
                          int adjoint, hipStream_t s) {
     if (p->precision == HBX_F64)

@@ -2,8 +2,10 @@
 the FP64 oracle (oracle/spin.py, pinned in test_oracle_spin.py).
 
 FP64 bar: relative max error <= 1e-11.  FP32 plans (FP32 data and
-contraction, FP64 recursion): relative max error <= 2e-5 of the largest
-coefficient (spin transforms have no FP64 low-m band).
+contraction, FP64 recursion, FP64 band m < 8 in the analysis): relative max
+error <= 2e-5 of the largest coefficient at nside 32, and eps_max =
+max |da_lm| / sqrt(C_l) <= 1e-5 for E and for B at nside 2048, lmax 4095
+(4 Q/U pairs, C_l^BB = 0.1 C_l^EE).
 """
 import math
 
@@ -203,3 +205,43 @@ def test_spin_plan_kind_errors(gpu):
         scalar.map2alm_spin(torch.zeros(2, 768, device="cuda", dtype=torch.float64))
     with pytest.raises(HbxValueError):
         spin.map2alm(torch.zeros(768, device="cuda", dtype=torch.float64))
+
+
+def test_spin_f32_gate_full_size(gpu):
+    """Spin-2 FP32 analysis at nside 2048, lmax 4095 on 4 Q/U pairs:
+    eps_max <= 1e-5 against the FP64 plan for E and for B, with
+    C_l^EE = 1/(l+1)^2 (l >= 2) and C_l^BB = 0.1 C_l^EE (B, ten times weaker,
+    is the sensitive one)."""
+    print("[case] spin f32 gate nside=2048 lmax=4095", flush=True)
+    nside, lmax = 2048, 4095
+    dev = torch.device("cuda", 0)
+    p64 = _plan(nside, lmax, spin=2, precision="f64")
+    p32 = _plan(nside, lmax, spin=2, precision="f32")
+    n = (lmax + 1) * (lmax + 2) // 2
+    rows = torch.arange(lmax + 1, 0, -1, device=dev)
+    ms = torch.repeat_interleave(torch.arange(lmax + 1, device=dev), rows)
+    ls = torch.arange(n, device=dev) - torch.repeat_interleave(torch.cumsum(rows, 0) - rows, rows) + ms
+    g = torch.Generator(device=dev)
+    g.manual_seed(21)
+
+    def draw(amp):
+        sd = torch.where(ls < 2, torch.zeros(n, device=dev, dtype=torch.float64), amp / (ls.double() + 1.0))
+        sc = torch.where(ms == 0, sd, sd / math.sqrt(2.0))
+        re = torch.randn(n, generator=g, device=dev, dtype=torch.float64)
+        im = torch.randn(n, generator=g, device=dev, dtype=torch.float64)
+        return torch.complex(re * sc, torch.where(ms == 0, torch.zeros_like(im), im * sc))
+
+    eb = torch.stack([draw(1.0), draw(math.sqrt(0.1))])[None]
+    qu32 = p64.alm2map_spin(eb).to(torch.float32)
+    ref = p64.map2alm_spin(qu32.double())[0]
+    got = p32.map2alm_spin(qu32.repeat(4, 1, 1))
+    for k, name in enumerate("EB"):
+        r = ref[k]
+        pw = r.real ** 2 + r.imag ** 2
+        cl = torch.zeros(lmax + 1, dtype=torch.float64, device=dev).index_add_(0, ls, torch.where(ms == 0, pw, 2 * pw))
+        cl /= 2.0 * torch.arange(lmax + 1, device=dev, dtype=torch.float64) + 1.0
+        ok = cl[ls] > 0
+        for b in (0, 3):
+            e = float(((got[b, k].to(torch.complex128) - r).abs()[ok] / cl[ls][ok].sqrt()).max())
+            print(f"{name} pair {b}: eps_max {e:.2e}", flush=True)
+            assert e <= 1e-5
