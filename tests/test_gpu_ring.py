@@ -128,3 +128,39 @@ def test_f32_gate_full_size(gpu):
         e = float(((a.to(torch.complex128) - ref).abs() / cl[ls].sqrt()).max())
         print(f"{name}: eps_max {e:.2e}", flush=True)
         assert e <= 1e-5
+
+
+def test_ring_stage_nside4096_f32(gpu):
+    """FP32 ring analysis and synthesis at nside 4096 (mmax 2047): the
+    16384-point class (equatorial rings of 16384 pixels, 1024 threads) and
+    Bluestein rings up to a 16384-point inner FFT, against NumPy."""
+    print("[case] ring stage nside=4096 f32", flush=True)
+    from hellblast_amd.hp_compat import healpix_geom
+    from hellblast_amd.plan import get_plan
+    nside, mmax = 4096, 2047
+    geom = healpix_geom(nside, mmax, "dz", "healpix", "reference")
+    plan = get_plan(geom, mmax, mmax, "f32")
+    rng = np.random.default_rng(9)
+    f = rng.standard_normal(plan.npix)
+    phase = plan.phase_buffer(1)
+    plan.ring_analysis(torch.as_tensor(f, dtype=torch.float32, device="cuda")[None], phase)
+    nr, nmb = plan.geom.nrings, (mmax + 8) // 8
+    v = phase[:nmb * nr * 8 * 8].view(torch.complex64).reshape(nmb, nr, 8)
+    got = v.permute(1, 0, 2).reshape(nr, nmb * 8)[:, :mmax + 1].cpu().numpy()
+    f32 = f.astype(np.float32).astype(np.float64)
+    ref = OS.ring_phase(f32, geom.ring_start, geom.nphi, mmax, geom.phi0, geom.mcut) * geom.weight[:, None]
+    err = _per_ring_rel(got, ref)
+    print(f"analysis per-ring rel err {err:.2e}", flush=True)
+    assert err <= TOL["f32"]
+    G = (rng.standard_normal((nr, mmax + 1)) + 1j * rng.standard_normal((nr, mmax + 1))).astype(np.complex64)
+    Gp = np.zeros((nr, nmb * 8), dtype=np.complex64)
+    Gp[:, :mmax + 1] = G
+    v.copy_(torch.as_tensor(Gp.reshape(nr, nmb, 8).transpose(1, 0, 2).copy()))
+    maps = plan.ring_synthesis(phase, 1, dtype=torch.float32)[0].double().cpu().numpy()
+    refm = OS.ring_synthesis(G.astype(np.complex128), geom.ring_start, geom.nphi, mmax, geom.phi0, geom.mcut)
+    worst = 0.0
+    for r in range(0, nr, 7):
+        s, e = geom.ring_start[r], geom.ring_start[r + 1]
+        worst = max(worst, float(np.max(np.abs(maps[s:e] - refm[s:e])) / np.max(np.abs(refm[s:e]))))
+    print(f"synthesis per-ring rel err (every 7th ring) {worst:.2e}", flush=True)
+    assert worst <= TOL["f32"]
