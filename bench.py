@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed", action="store_true",
+                    help="instead of the headline: 64 FP32 maps of mixed nside (128..2048, lmax 2 nside) "
+                         "in one HIP graph with autotuned tiles; prints its own JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB -> 1 GiB elementwise copy kernel (HBM counter calibration)")
     return ap.parse_args()
@@ -189,12 +192,85 @@ def cpu_baseline(nside, lmax, seconds):
     }
 
 
+def mixed_main(args):
+    """64 FP32 maps, nside {2048:1, 1024:2, 512:5, 256:14, 128:42} (seeded
+    shuffle), lmax = 2 nside, alm ~ C_l = (l+10)^-2: map2alm + alm2map of the
+    whole batch per step, replayed from one HIP graph (MixedBatch)."""
+    import numpy as np
+    import torch
+    from hellblast_amd.batch import MixedBatch
+    from hellblast_amd.hp_compat import _plan
+    from hellblast_amd.plan import nalm
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(3)
+    alloc = {2048: 1, 1024: 2, 512: 5, 256: 14, 128: 42}
+    sides = [n for n, k in alloc.items() for _ in range(k)]
+    rng.shuffle(sides)
+    specs = [(n, 2 * n) for n in sides]
+    t0 = time.time()
+    batch = MixedBatch(specs, precision="f32", tune=True)
+    log(f"mixed batch of {len(specs)} maps planned and tuned in {time.time() - t0:.1f}s: {batch.tiles}")
+    maps = []
+    for i, (nside, lmax) in enumerate(specs):
+        n = nalm(lmax)
+        ls = torch.cat([torch.arange(m, lmax + 1, device=dev) for m in range(lmax + 1)])
+        m0 = torch.zeros(n, dtype=torch.bool, device=dev)
+        m0[:lmax + 1] = True
+        g = torch.Generator(device=dev)
+        g.manual_seed(1000 + i)
+        sd = 1.0 / (ls.double() + 10.0)
+        sc = torch.where(m0, sd, sd / math.sqrt(2.0))
+        re = torch.randn(n, generator=g, device=dev, dtype=torch.float64)
+        im = torch.randn(n, generator=g, device=dev, dtype=torch.float64)
+        alm = torch.complex(re * sc, torch.where(m0, torch.zeros_like(im), im * sc))
+        maps.append(_plan(nside, lmax, lmax, "f64", "dz", "reference", "reference").alm2map(alm[None])[0].float())
+    batch.set_maps(maps)
+    batch.capture()
+    torch.cuda.synchronize()
+
+    def timed(fn, k):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(k):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    ms_graph = timed(batch.replay, args.steps)
+    ms_eager = timed(batch.run_eager, args.steps)
+    worst = 0.0
+    for i, (nside, lmax) in enumerate(specs):
+        p64 = _plan(nside, lmax, lmax, "f64", "dz", "reference", "reference")
+        ref = p64.map2alm(batch.map(i).double()[None])[0]
+        ls = torch.cat([torch.arange(m, lmax + 1, device=dev) for m in range(lmax + 1)])
+        emax, _ = eps_metrics(torch, batch.alm_of(i), ref, ls, lmax)
+        worst = max(worst, emax)
+    npix = sum(12 * n * n for n in sides)
+    res = {"metric": "mixed-batch SHT pairs/s: 64 FP32 maps, nside 128..2048, lmax 2 nside, one HIP graph",
+           "value": round(len(specs) / (ms_graph / 1e3), 3), "unit": "pairs/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_graph, 3),
+           "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "64 maps nside {2048:1,1024:2,512:5,256:14,128:42}, lmax 2 nside, "
+                                  "C_l=(l+10)^-2, map2alm+alm2map per map",
+                      "pixels_per_step": npix, "eager_ms_per_step": round(ms_eager, 3),
+                      "graph_over_eager": round(ms_eager / ms_graph, 3), "eps_max_worst_map": worst,
+                      "tiles": {f"nside{k[0]}_lmax{k[1]}": v for k, v in batch.tiles.items()}}}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
               "(multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.mixed:
+        return mixed_main(args)
     import torch
     from hellblast_amd import _abi
     from hellblast_amd.hp_compat import _plan
