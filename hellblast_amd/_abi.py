@@ -105,6 +105,8 @@ SIGNATURES = {
     "hbx_timer_arm": (ctypes.c_int, [ctypes.c_uint32]),
     "hbx_timer_read": (ctypes.c_int, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(ctypes.c_double)]),
     "hbx_alm2cl": (ctypes.c_int, [_i32, _i32, _i32, _vp, _i32, _i64, _vp, _i64, _vp]),
+    "hbx_ang2pix_ring": (ctypes.c_int, [_i64, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "hbx_pix2ang_ring": (ctypes.c_int, [_i64, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "hbx_phase_bytes": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_u64)]),
     "hbx_ring_analysis": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _u64, _vp]),
     "hbx_legendre_analysis": (ctypes.c_int, [_vp, _i32, _vp, _u64, _vp, _i64, _vp, _u64, _vp]),

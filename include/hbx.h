@@ -20,6 +20,9 @@
  *                          sht_fallback.map2alm_gpu   sht_fallback.py:36-65
  *   hbx_alm2map          - isht.alm2map_gpu           isht.py:5-26
  *   hbx_alm2cl           - alm_ops.alm2cl             alm_ops.py:19-32
+ *   hbx_ang2pix_ring     - coords_ring ang2pix_ring_with_rings
+ *                          coords_ring.cu:9-32 (+ ring search, coords_ring.py:18-24)
+ *   hbx_pix2ang_ring     - coords_ring pix2ang_ring_batch coords_ring.cu:33-54
  *   hbx_map2alm_spin     - spin.map2alm_spin_gpu      spin.py:32-71
  *                          spin1.map2alm_spin1_gpu    spin1.py:5-38
  *   hbx_alm2map_spin     - spin.qu_from_eb_gpu        spin.py:80-115
@@ -222,6 +225,21 @@ int hbx_ring_synthesis(const hbx_plan *plan, int32_t nmaps, const void *phase,
 int hbx_alm2cl(int32_t lmax, int32_t mmax, int32_t nmaps, const void *alm,
                int32_t alm_dtype, int64_t alm_stride, double *cl,
                int64_t cl_stride, void *stream);
+
+/* RING pixel <-> angle (device arrays; tables as in HealpixRingTables:
+ * z_edges[nrings+1] decreasing from 1 to -1, ring_start[nrings+1], nphi,
+ * phi0 = phi of pixel 0 of each ring).  ang2pix: ring = the z-edge cell of
+ * cos(theta), pixel = nearest centre phi0_r + j 2 pi / nphi_r.  pix2ang:
+ * (theta_centers[r], phi0_r + j 2 pi / nphi_r wrapped to [0, 2 pi)); an
+ * index outside [0, ring_start[nrings]) gives NaN angles.                   */
+int hbx_ang2pix_ring(int64_t n, const double *theta, const double *phi,
+                     int32_t nrings, const double *z_edges,
+                     const int64_t *ring_start, const int32_t *nphi,
+                     const double *phi0, int64_t *ipix, void *stream);
+int hbx_pix2ang_ring(int64_t n, const int64_t *ipix, int32_t nrings,
+                     const double *theta_centers, const int64_t *ring_start,
+                     const int32_t *nphi, const double *phi0, double *theta,
+                     double *phi, void *stream);
 
 /* Kernel timers (diagnostics; no reference counterpart).  hbx_timer_arm(mask)
  * arms the kernels whose bits (1 << HBX_TIMER_*) are set and clears their
