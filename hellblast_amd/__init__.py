@@ -3,7 +3,8 @@
 Drop-in for the SHT hot path of hellblast: ``map2alm_gpu`` (sht.py),
 ``alm2map_gpu`` (isht.py), ``anafast_ring`` (anafast.py), the ``hp_compat``
 SHT and spin wrappers, ``spin`` / ``spin1`` (Q/U <-> E/B) and
-``HealpixRingTables``, running hand-written gfx950 HIP
+``HealpixRingTables``; next to the path, beam / pixel-window smoothing,
+``synfast_tqu`` and RING ``ang2pix`` / ``pix2ang``; running hand-written gfx950 HIP
 kernels (libhbx.so, C ABI in include/hbx.h) on PyTorch-ROCm tensors.
 
 Package aliases as in the reference (``__init__.py:125-127``):
@@ -16,6 +17,10 @@ from .anafast import anafast_ring
 from .alm_ops import alm2cl, almxfl, synalm, alm_index, getidx, getsize, to_reference_layout, from_reference_layout
 from .spin import map2alm_spin_gpu, eb_from_qu_gpu, qu_from_eb_gpu
 from .spin1 import map2alm_spin1_gpu, alm2map_spin1_gpu
+from .smoothing import gaussian_beam_fl, smooth_alm_gaussian
+from .beams import pixel_window_fl, smooth_alm_with_beam, smooth_map
+from .synfast_tqu import synfast_tqu
+from .coords_ring import RingMapperGPU, vec2pix_ring_gpu
 from . import hp_compat
 
 map2alm = map2alm_gpu
@@ -26,5 +31,6 @@ __all__ = [
     "HealpixRingTables", "map2alm_gpu", "alm2map_gpu", "anafast_ring", "alm2cl", "almxfl", "synalm",
     "alm_index", "getidx", "getsize", "to_reference_layout", "from_reference_layout", "hp_compat",
     "map2alm", "alm2map", "anafast", "map2alm_spin_gpu", "eb_from_qu_gpu", "qu_from_eb_gpu",
-    "map2alm_spin1_gpu", "alm2map_spin1_gpu",
+    "map2alm_spin1_gpu", "alm2map_spin1_gpu", "gaussian_beam_fl", "smooth_alm_gaussian", "pixel_window_fl",
+    "smooth_alm_with_beam", "smooth_map", "synfast_tqu", "RingMapperGPU", "vec2pix_ring_gpu",
 ]
