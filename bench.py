@@ -267,7 +267,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+              "(this build runs one GPU per process; batches shard by map across processes)", file=sys.stderr)
         sys.exit(2)
     if args.mixed:
         return mixed_main(args)

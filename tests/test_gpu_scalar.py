@@ -2,7 +2,7 @@
 
 FP64 bar: relative max error <= 1e-11 against the FP64 CPU oracle (which is
 itself pinned to the reference formulas in test_oracle.py).  FP32 bar
-(the BASELINE gate): eps_max <= 1e-5 against the FP64 oracle.
+(the FP32 gate): eps_max <= 1e-5 against the FP64 oracle.
 """
 import math
 
