@@ -164,3 +164,28 @@ def test_ring_stage_nside4096_f32(gpu):
         worst = max(worst, float(np.max(np.abs(maps[s:e] - refm[s:e])) / np.max(np.abs(refm[s:e]))))
     print(f"synthesis per-ring rel err (every 7th ring) {worst:.2e}", flush=True)
     assert worst <= TOL["f32"]
+
+
+@pytest.mark.parametrize("lmax", [40, 63, 100])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_gauss_legendre_round_trip_any_ring_length(gpu, lmax, precision):
+    """Synthesis then analysis on a Gauss-Legendre grid (lmax + 1 rings of
+    nphi = 2 lmax + 2 pixels: half-lengths 41 (odd, Bluestein), 64 (power of
+    two) and 101) returns the band-limited input: quadrature exact, so the
+    round trip is limited by rounding only."""
+    import math
+    from hellblast_amd.plan import Geometry, get_plan
+    print(f"[case] GL round trip lmax={lmax} {precision}", flush=True)
+    x, w = np.polynomial.legendre.leggauss(lmax + 1)
+    x, w = x[::-1].copy(), w[::-1].copy()
+    nphi = np.full(lmax + 1, 2 * lmax + 2, dtype=np.int32)
+    geom = Geometry(nphi, cos_theta=x, sin_theta=np.sqrt((1 - x) * (1 + x)), weight=2 * math.pi * w)
+    plan = get_plan(geom, lmax, lmax, precision)
+    alm = OS.synalm(1.0 / (np.arange(lmax + 1) + 1.0) ** 2, lmax, seed=lmax)
+    a = torch.as_tensor(alm, device="cuda")
+    if precision == "f32":
+        a = a.to(torch.complex64)
+    back = plan.map2alm(plan.alm2map(a[None].repeat(3, 1), dtype=torch.float64 if precision == "f64" else torch.float32))
+    err = float((back - a[None]).abs().max() / a.abs().max())
+    print(f"round trip rel err {err:.2e}", flush=True)
+    assert err <= (1e-12 if precision == "f64" else 2e-6)
