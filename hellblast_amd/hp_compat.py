@@ -162,3 +162,36 @@ def iqu2eb(Q, U, lmax=None, nest=False, **kw):
 def eb2iqu(E, B, nside=None, lmax=None, nest=False, **kw):
     """(Q, U) of spin-2 (E, B) (``hp_compat.py:96-102``)."""
     return alm2map_spin(E, B, 2, nside=nside, lmax=lmax, nest=nest, **kw)
+
+
+# ---------- FITS helpers (``hp_compat.py:204-225``) ----------
+def read_map(path, hdu=1, to_gpu=True, dtype=np.float64):
+    from .io_fits import read_map_fits
+    return read_map_fits(path, hdu=hdu, to_gpu=to_gpu, dtype=dtype)
+
+
+def write_map(path, m, header=None, overwrite=True):
+    from .io_fits import write_map_fits
+    return write_map_fits(path, m, header=header, overwrite=overwrite)
+
+
+def read_alm(path, hdu=1, to_gpu=True, dtype=np.complex128):
+    from .io_fits import read_alm_fits
+    return read_alm_fits(path, hdu=hdu, to_gpu=to_gpu, dtype=dtype)
+
+
+def write_alm(path, alm, header=None, overwrite=True):
+    from .io_fits import write_alm_fits
+    return write_alm_fits(path, alm, header=header, overwrite=overwrite)
+
+
+def read_tqu(path, to_gpu=True):
+    from .io_fits_multi import read_tqu_fits
+    return read_tqu_fits(path, to_gpu=to_gpu)
+
+
+def write_tqu(path, T=None, Q=None, U=None, nside=None, ordering="RING", coordsys="G", overwrite=True,
+              extra_header=None):
+    from .io_fits_multi import write_tqu_fits
+    return write_tqu_fits(path, T=T, Q=Q, U=U, nside=nside, ordering=ordering, coordsys=coordsys,
+                          overwrite=overwrite, extra_header=extra_header)
