@@ -58,7 +58,12 @@ def candidates(plan, nmaps):
         cover *= 2
     slots = [s for s in (1, 2, 4, 8, 16) if s <= cover] or [1]
     if plan.spin:
-        return [{"spin_slots": s} for s in (1, 2, 4) if s <= max(cover, 1)]
+        valu = [{"spin_slots": s} for s in (1, 2, 4) if s <= max(cover, 1)]
+        if plan.precision == _abi.HBX_F64:
+            return valu
+        # FP32: the matrix-core analysis, or the VALU kernel (batches below
+        # mfma_min_maps maps take it)
+        return [{"mfma_min_maps": 2}] + [dict(t, mfma_min_maps=2 * nmaps + 1) for t in valu]
     if plan.precision == _abi.HBX_F64:
         return [{"valu_slots": s} for s in slots if s <= 8]
     out = []

@@ -110,7 +110,7 @@ static uint64_t ws_tail_offset(const Plan *p, int nmaps) { return phase_total_by
 // band: phase, alm and partial sums of m < M_LO | tail = scaled (E, B) or the
 // partial sums of both.  Synthesis puts scaled right after the phase.
 struct SpinWs {
-    uint64_t phase = 0, band_phase = 0, band_alm = 0, band_partial = 0, tail = 0, total = 0;
+    uint64_t phase = 0, band_phase = 0, band_alm = 0, band_partial = 0, tail = 0, lfirst = 0, total = 0;
 };
 
 static SpinWs spin_ws(const Plan *p, int npol) {
@@ -129,8 +129,13 @@ static SpinWs spin_ws(const Plan *p, int npol) {
     }
     w.tail = off;
     const uint64_t scaled = align256(n2 * scaled_elems_per_map(p) * cs);
-    const uint64_t part = align256(g > 1 ? n2 * g * p->nlm * cs : 0);
-    w.total = std::max(w.phase + scaled, w.tail + part);
+    // matrix-core analysis (FP32): its own ring-pair groups and the first
+    // l of each group's partial sums per m
+    const int gm = spin_uses_mfma(p, npol) ? spin_mfma_groups(p) : 1;
+    const uint64_t part = align256(std::max(g, gm) > 1 ? n2 * std::max(g, gm) * p->nlm * cs : 0);
+    w.lfirst = w.tail + part;
+    const uint64_t lf = align256(gm > 1 ? (uint64_t)(p->mmax + 1) * gm * sizeof(int32_t) : 0);
+    w.total = std::max(w.phase + scaled, w.lfirst + lf);
     return w;
 }
 
@@ -498,7 +503,12 @@ static int spin_analysis(const hbx_plan *plan, int32_t npol, const void *maps, i
     if (p->precision == HBX_F32 &&
         (rc = launch_ring_band(p, 2 * npol, maps, map_dtype, map_stride, base + w.band_phase, scale, s)))
         return rc;
-    if ((rc = launch_spin_analysis(p, npol, ws, alm, alm_stride, base + w.tail, adjoint, s))) return rc;
+    if (spin_uses_mfma(p, npol))
+        rc = launch_spin_analysis_mfma(p, npol, ws, alm, alm_stride, base + w.tail,
+                                       reinterpret_cast<int32_t *>(base + w.lfirst), adjoint, s);
+    else
+        rc = launch_spin_analysis(p, npol, ws, alm, alm_stride, base + w.tail, adjoint, s);
+    if (rc) return rc;
     if (p->precision != HBX_F32) return 0;
     // m < M_LO in FP64 (the low-m rows carry most of the power of red spectra)
     return launch_spin_analysis_band(p, npol, base + w.band_phase, alm, alm_stride, base + w.band_alm,

@@ -126,6 +126,10 @@ int spin_slots_of(const Plan *p);
 int launch_spin_analysis_band(const Plan *p, int npol, const void *band_phase, void *alm, int64_t alm_stride,
                               void *band_alm, void *band_partial, int adjoint, hipStream_t s);
 int launch_band_to_f32(const Plan *p, int nrows, const void *band_alm, void *alm, int64_t alm_stride, hipStream_t s);
+bool spin_uses_mfma(const Plan *p, int npol);
+int spin_mfma_groups(const Plan *p);
+int launch_spin_analysis_mfma(const Plan *p, int npol, const void *phase, void *alm, int64_t astride, void *partial,
+                              int32_t *lfirst, int adjoint, hipStream_t s);
 int launch_spin_analysis(const Plan *p, int npol, const void *phase, void *alm, int64_t alm_stride, void *partial,
                          int adjoint, hipStream_t s);
 int launch_spin_prescale(const Plan *p, int npol, const void *alm, int64_t alm_stride, void *scaled, int adjoint,

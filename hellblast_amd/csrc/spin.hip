@@ -383,6 +383,207 @@ spin_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__
     }
 }
 
+// ---------------------------------------------------------------------------
+// analysis on the matrix cores (FP32 plans, m >= M_LO).  Per m the sums of
+// spin_analysis_kernel,
+//   S_a(l) = sum_p a_l(p) P_n(p) + eps b_l(p) P_s(p),
+//   S_b(l) = sum_p b_l(p) M_n(p) + eps a_l(p) M_s(p),     eps = (-1)^{l+m},
+// are two products over the ring pairs p.  With 16 output rows n = 4 j + c
+// (polarisation pair j of a group of 4, c = S_a.re, S_a.im, S_b.re, S_b.im)
+//   D[n][l] = sum_p A_a[n][p] a_l(p) + A_b[n][p] b_l(p),
+//   A_a[n][p] = (P_n.re, P_n.im, eps M_s.re, eps M_s.im)[c],
+//   A_b[n][p] = (eps P_s.re, eps P_s.im, M_n.re, M_n.im)[c],
+// on v_mfma_f32_16x16x4f32: FP32 operands (the FP32 values the VALU kernel
+// multiplies) and FP32 sums.  A workgroup is 4 waves over P = 128 ring pairs
+// of one m.  Per 32-l chunk waves 0-1 run the FP64 a-recursion and waves 2-3
+// the b-recursion of the 128 pairs (one pair per lane) into two lambda tiles;
+// then wave (par, fn) contracts A_fn with tile fn over the l of parity par
+// (chunks start at l = m mod 2, so eps = (-1)^par), and the b-waves hand
+// their D tiles to the a-waves through LDS.  Tiles as in the scalar matrix
+// kernel: [par][col][k][s], pair 4 s + k.
+// Lane layout of the 16x16x4 f32 instruction (checked on gfx950):
+//   A: lane = row + 16 k   B: lane = col + 16 k   D: lane = col + 16 (row/4), reg = row%4
+// ---------------------------------------------------------------------------
+typedef float sfloatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int SPL = 32;                   // l per chunk: 16 even + 16 odd columns
+
+struct SpinMfGeom {
+    static constexpr int W = 4;                        // waves per workgroup
+    static constexpr int P = 128;                      // ring pairs per workgroup
+    static constexpr int RUN = P / 4 + 16;             // stride between the k runs of a column
+    static constexpr int COL = P + 68;                 // column stride (= 4 mod 64 banks)
+    static constexpr int PAR = 16 * COL;
+    static constexpr int TILE = 2 * PAR;               // floats per lambda function
+    static constexpr int XCH = 2 * 4 * 64;             // b-waves -> a-waves: parity x reg x lane
+    static constexpr size_t LDS = (size_t)(2 * TILE + XCH) * sizeof(float) + 2 * 2 * SPL * sizeof(double) + 16;
+};
+
+__global__ void __launch_bounds__(SpinMfGeom::W * 64)
+spin_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, float2 *__restrict__ alm,
+                          int64_t astride, float2 *__restrict__ partial, const double *__restrict__ px,
+                          const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
+                          const double *__restrict__ beta, const double *__restrict__ sl,
+                          const int32_t *__restrict__ lseed, const double4 *__restrict__ seed, int npairs, int nr,
+                          int lmax, int mmax, int ngroups, int64_t nlm, int npol, int s, int adjoint, int mbeg,
+                          int32_t *__restrict__ lfirst) {
+    using G = SpinMfGeom;
+    extern __shared__ __align__(16) float sms[];
+    float *lam = sms;                                                  // [2][TILE]: a, b
+    float *xch = sms + 2 * G::TILE;                                    // [XCH]
+    double *cf = reinterpret_cast<double *>(xch + G::XCH);             // [2][alpha SPL | beta SPL]
+    int *start = reinterpret_cast<int *>(cf + 4 * SPL);
+
+    int mb, item;
+    xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
+    // the 8 m of one m-block (sharing 64-byte phase lines) run next to each other
+    const int m = mb * MB + item % MB;
+    const int grp = item / MB;
+    if (m > mmax || m < mbeg) return;
+    const int g = blockIdx.y;                          // group of 4 polarisation pairs
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int par = w & 1, fn = w >> 1;                // contraction: parity; lambda function (0: a, 1: b)
+    const int P0 = grp * G::P;
+
+    // recursion: function fn of pair P0 + 64 (w & 1) + lane
+    const int qr = 64 * (w & 1) + lane;
+    const int pr = P0 + qr;
+    double x = 0.0, u1 = 0.0, u2 = 0.0;
+    double2 sd = make_double2(0.0, 0.0);
+    int ls = SNEVER;
+    if (pr < npairs) {
+        x = px[pr];
+        ls = lseed[(int64_t)m * npairs + pr];
+        if (ls <= lmax) {
+            const double4 q = seed[(int64_t)m * npairs + pr];
+            sd = fn ? make_double2(q.z, q.w) : make_double2(q.x, q.y);
+        }
+    }
+    const int wstart = swmin(ls);
+    const int winj = swmax(ls <= lmax ? ls : -1);
+
+    // A operand of this wave: A_fn[n = lane%16][pair P0 + 4 s + lane/16], s = 0..31
+    float Sv[32];
+    {
+        const int row = lane & 15, k = lane >> 4;
+        const int j = row >> 2, c = row & 3;
+        const int gb = g * 4 + j;
+        const bool south = (fn == 0) == (c >= 2);      // a: P_n, M_s; b: P_s, M_n
+        const int cq = c & 1, cu = cq ^ 1;             // Q part re/im, U part im/re
+        // P = Q + iU = (Q.re - U.im, Q.im + U.re), M = Q - iU = (Q.re + U.im, Q.im - U.re)
+        const float su = ((c >= 2) != (cq == 1)) ? 1.0f : -1.0f;
+        const float sg = (south && par) ? -1.0f : 1.0f;
+        const float *pq = reinterpret_cast<const float *>(phase + (int64_t)(2 * min(gb, npol - 1)) * pstride +
+                                                          (int64_t)mb * nr * MB + (m & 7));
+        const float *pu = pq + 2 * pstride;
+#pragma unroll
+        for (int s4 = 0; s4 < 32; ++s4) {
+            const int q = P0 + 4 * s4 + k;
+            const int qq = min(q, npairs - 1);
+            const int r = south ? nr - 1 - qq : qq;
+            const bool ok = q < npairs && gb < npol && !(south && nr - 1 - qq == qq);
+            const float vq = pq[(int64_t)r * (2 * MB) + cq], vu = pu[(int64_t)r * (2 * MB) + cu];
+            Sv[s4] = ok ? sg * (vq + su * vu) : 0.0f;
+        }
+    }
+    if (tid == 0) start[0] = SNEVER;
+    __syncthreads();
+    if (lane == 0) atomicMin(&start[0], wstart);
+    __syncthreads();
+    const int lstart = start[0];
+
+    const bool direct = (ngroups == 1);
+    const int64_t i0 = alm_index(m, m, lmax) - m;       // + l
+    const int64_t ro = rofs[m] - m;                      // table index of l
+    const double *sv = sl + ro;
+    // out(l, b): b = alm row (E of pair j: 8 g + 2 j, B: 8 g + 2 j + 1)
+    auto store = [&](int l, int b, double re, double im) {
+        if (b >= 2 * npol) return;
+        if (direct) {
+            const double f = sv[l] * ((adjoint && m > 0) ? 2.0 : 1.0);
+            alm[(int64_t)b * astride + i0 + l] = make_float2((float)(re * f), (float)(im * f));
+        } else {
+            partial[((int64_t)b * ngroups + grp) * nlm + i0 + l] = make_float2((float)re, (float)im);
+        }
+    };
+    // coefficients below every ring pair's switch-on are zero: written to alm
+    // by a single group; partial sums record where theirs start instead
+    const int lz = lstart > lmax ? lmax + 1 : lstart;
+    if (direct)
+        for (int e = tid; e < (lz - m) * 8; e += G::W * 64) store(m + (e >> 3), g * 8 + (e & 7), 0.0, 0.0);
+    else if (g == 0 && tid == 0)
+        lfirst[(int64_t)m * ngroups + grp] = lz;
+    if (lstart > lmax) return;
+
+    // alpha and beta of a chunk staged in LDS (double-buffered), loaded a
+    // whole chunk ahead; rows are zero past lmax within the row, and the
+    // tables carry TAILPAD zeros after the last row
+    auto coef = [&](int l) -> double { return tid < SPL ? alpha[ro + l + tid] : beta[ro + l + tid - SPL]; };
+    if (tid < 2 * SPL) cf[tid] = coef(lstart);
+    double cnext = (tid < 2 * SPL) ? coef(lstart + SPL) : 0.0;
+    __syncthreads();
+    // this lane's writes: pair qr -> [.][.][k = qr%4][s = qr/4] of tile fn
+    float *lw = lam + fn * G::TILE + (qr & 3) * G::RUN + (qr >> 2);
+    // this lane's B reads: col = lane%16, k = lane/16, s = 0..31 of tile fn
+    const float *lr = lam + fn * G::TILE + par * G::PAR + (lane & 15) * G::COL + (lane >> 4) * G::RUN;
+    const double sigma = (s & 1) ? -1.0 : 1.0;
+    int buf = 0;
+    for (int c0 = lstart; c0 <= lmax; c0 += SPL, buf ^= 1) {
+        const double *ca = cf + buf * 2 * SPL;
+        const double cnext2 = (tid < 2 * SPL) ? coef(c0 + 2 * SPL) : 0.0;   // in flight a chunk
+        // 1. recursion of this lane's (pair, function) over l = c0 .. c0+31
+#pragma unroll
+        for (int tt = 0; tt < SPL; ++tt) {
+            if ((tt & 7) == 0 && c0 <= winj && ls == c0 + tt) {
+                u2 = sd.x;
+                u1 = sd.y;
+            }
+            const double t = ca[tt] * x;
+            const double bl = fn ? -ca[SPL + tt] : ca[SPL + tt];
+            const double u = fma(t + bl, u1, -u2);
+            u2 = u1;
+            u1 = u;
+            lw[(tt & 1) * G::PAR + (tt >> 1) * G::COL] = (float)u;
+        }
+        if (tid < 2 * SPL) cf[(buf ^ 1) * 2 * SPL + tid] = cnext;
+        cnext = cnext2;
+        __syncthreads();
+        // 2. this wave's product over the 128 pairs; four chains of 32 pairs
+        sfloatx4 D0 = sfloatx4{0.f, 0.f, 0.f, 0.f}, D1 = D0, D2 = D0, D3 = D0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float4 bq = *reinterpret_cast<const float4 *>(lr + 4 * j);
+            sfloatx4 &Dj = (j < 2) ? D0 : (j < 4) ? D1 : (j < 6) ? D2 : D3;
+            Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j], bq.x, Dj, 0, 0, 0);
+            Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j + 1], bq.y, Dj, 0, 0, 0);
+            Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j + 2], bq.z, Dj, 0, 0, 0);
+            Dj = __builtin_amdgcn_mfma_f32_16x16x4f32(Sv[4 * j + 3], bq.w, Dj, 0, 0, 0);
+        }
+        sfloatx4 D = (D0 + D1) + (D2 + D3);
+        // 3. the b-waves hand their tiles to the a-waves
+        if (fn == 1) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) xch[(par * 4 + r) * 64 + lane] = D[r];
+        }
+        __syncthreads();
+        if (fn == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) D[r] += xch[(par * 4 + r) * 64 + lane];
+            // D[row][col]: lane = col + 16 (row/4), reg = row%4 -> pair j = lane/16,
+            // (S_a.re, S_a.im, S_b.re, S_b.im) of l = c0 + 2 col + par
+            const int l = c0 + 2 * (lane & 15) + par;
+            if (l <= lmax) {
+                const int b0 = g * 8 + 2 * (lane >> 4);
+                const double sar = D[0], sai = D[1], sbr = D[2], sbi = D[3];
+                // E = -(S_a + sigma S_b)/2, B = i (S_a - sigma S_b)/2
+                store(l, b0, -0.5 * (sar + sigma * sbr), -0.5 * (sai + sigma * sbi));
+                store(l, b0 + 1, -0.5 * (sai - sigma * sbi), 0.5 * (sar - sigma * sbr));
+            }
+        }
+    }
+}
+
 // synthesis pre-pass: row 2b = s_l (E + iB), row 2b+1 = s_l (E - iB) (halved
 // for m > 0 in the map2alm adjoint), zero-padded to the row length
 template <typename T>
@@ -579,6 +780,34 @@ int launch_spin_analysis(const Plan *p, int nmaps, const void *phase, void *alm,
                                        adjoint, s);
     return spin_analysis_t<float>(p, nmaps, (const float2 *)phase, (float2 *)alm, astride, (float2 *)partial,
                                   adjoint, s);
+}
+
+// FP32 batches of at least mfma_min_maps maps (Q and U count) contract on the
+// matrix cores; FP64 plans and smaller batches run spin_analysis_kernel
+bool spin_uses_mfma(const Plan *p, int npol) { return p->precision == HBX_F32 && 2 * npol >= mfma_min_maps(p); }
+
+int spin_mfma_groups(const Plan *p) { return (p->npairs + SpinMfGeom::P - 1) / SpinMfGeom::P; }
+
+// rows m >= M_LO of FP32 plans (the FP64 band covers m < M_LO); lfirst holds
+// (mmax + 1) x spin_mfma_groups entries
+int launch_spin_analysis_mfma(const Plan *p, int npol, const void *phase, void *alm, int64_t astride, void *partial,
+                              int32_t *lfirst, int adjoint, hipStream_t s) {
+    if (p->mmax < M_LO) return 0;
+    const int G = spin_mfma_groups(p);
+    const int nmb8 = (p->nmb + 7) / 8 * 8;
+    dim3 grid(nmb8 * MB * G, (npol + 3) / 4);
+    timer_mark(HBX_TIMER_SPIN_ANALYSIS, true, s);
+    hipLaunchKernelGGL(spin_analysis_mfma_kernel, grid, dim3(SpinMfGeom::W * 64), SpinMfGeom::LDS, s,
+                       (const float2 *)phase, phase_elems_per_map(p), (float2 *)alm, astride, (float2 *)partial,
+                       p->d_px, p->d_rofs, p->d_alpha, p->d_beta, p->d_sl, p->d_lseed, p->d_seed4, p->npairs,
+                       p->nrings, p->lmax, p->mmax, G, p->nlm, npol, p->spin, adjoint, M_LO, lfirst);
+    timer_mark(HBX_TIMER_SPIN_ANALYSIS, false, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "spin_analysis_mfma_kernel launch");
+    if (G > 1)
+        return launch_group_reduce(p, 2 * npol, partial, alm, astride, G, p->nlm, M_LO, p->mmax, false, adjoint, s,
+                                   lfirst);
+    return 0;
 }
 
 int launch_spin_prescale(const Plan *p, int npol, const void *alm, int64_t astride, void *scaled, int adjoint,

@@ -121,8 +121,10 @@ int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
  *                  8 MFMA waves fed by 4 lambda-recursion waves one chunk
  *                  ahead (double-buffered LDS tile)
  *   mfma_min_maps  FP32 batches of at least this many maps use the
- *                  matrix-core Legendre kernels (default 2)
- *   spin_slots     spin Legendre analysis: ring-pair slots per lane, 1, 2, 4
+ *                  matrix-core Legendre kernels (default 2; spin analysis
+ *                  counts the Q and U maps)
+ *   spin_slots     VALU spin Legendre analysis (FP64 plans, and FP32 batches
+ *                  below mfma_min_maps): ring-pair slots per lane, 1, 2, 4
  *   mfma_kind      matrix-core operand format of FP32 batches:
  *                  HBX_MFMA_F32 (FP32 instructions) or HBX_MFMA_F16X2
  *                  (each FP32 operand as an fp16 hi/lo pair, three fp16

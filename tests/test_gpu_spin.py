@@ -2,7 +2,8 @@
 the FP64 oracle (oracle/spin.py, pinned in test_oracle_spin.py).
 
 FP64 bar: relative max error <= 1e-11.  FP32 plans (FP32 data and
-contraction, FP64 recursion, FP64 band m < 8 in the analysis): relative max
+contraction -- on the matrix cores for the analysis of batches --, FP64
+recursion, FP64 band m < 8 in the analysis): relative max
 error <= 2e-5 of the largest coefficient at nside 32, and eps_max =
 max |da_lm| / sqrt(C_l) <= 1e-5 for E and for B at nside 2048, lmax 4095
 (4 Q/U pairs, C_l^BB = 0.1 C_l^EE).
@@ -145,6 +146,39 @@ def test_spin_f32_against_f64_oracle(gpu, s):
     q, u = OSp.alm2map_spin(Ea, Ba, tabs, lmax, spin=s)
     gm = plan.alm2map_spin(torch.as_tensor(np.stack([Ea, Ba]), device="cuda"), dtype=torch.float32).cpu().numpy()
     assert _rel(gm[0], q) <= F32_TOL and _rel(gm[1], u) <= F32_TOL
+
+
+def test_spin_f32_matrix_core_analysis(gpu):
+    """FP32 spin analysis on the matrix cores (128 ring pairs per workgroup,
+    4 Q/U pairs per product) at nside 128 (256 ring pairs: two groups whose
+    partial sums are reduced) on 5 Q/U pairs (a second, partly filled pair
+    group): against the FP64 oracle; equal to the VALU kernel (selected by
+    mfma_min_maps above the batch) to FP32 rounding, for the weighted and
+    the unweighted (adjoint) analysis; one pair alone gives the bits it gets
+    inside the batch."""
+    nside, lmax, s = 128, 200, 2
+    print("[case] spin f32 matrix-core analysis nside=128 lmax=200 5 pairs", flush=True)
+    tabs = OG.ring_tables(nside)
+    qu = _qu(tabs, 31, n=5)
+    plan = _plan(nside, lmax, spin=s, precision="f32")
+    x = torch.as_tensor(qu, device="cuda", dtype=torch.float32)
+    got = plan.map2alm_spin(x).cpu().numpy()
+    q32 = qu.astype(np.float32).astype(np.float64)
+    for b in (0, 4):
+        E, B = OSp.map2alm_spin(q32[b, 0], q32[b, 1], tabs, lmax, spin=s)
+        print(f"pair {b}: E {_rel(got[b, 0], E):.2e} B {_rel(got[b, 1], B):.2e}", flush=True)
+        assert _rel(got[b, 0], E) <= F32_TOL and _rel(got[b, 1], B) <= F32_TOL
+    one = plan.map2alm_spin(x[4:5]).cpu().numpy()
+    assert np.array_equal(one[0], got[4])
+    adj = plan.alm2map_spin_adjoint(x).cpu().numpy()
+    plan.set_tiles(mfma_min_maps=1000)
+    try:
+        valu = plan.map2alm_spin(x).cpu().numpy()
+        vadj = plan.alm2map_spin_adjoint(x).cpu().numpy()
+    finally:
+        plan.set_tiles(mfma_min_maps=0)
+    print(f"matrix vs VALU: {_rel(got, valu):.2e}, adjoint {_rel(adj, vadj):.2e}", flush=True)
+    assert _rel(got, valu) <= 4e-6 and _rel(adj, vadj) <= 4e-6
 
 
 def _gl_geometry(lmax):
