@@ -700,6 +700,172 @@ spin_synthesis_kernel(const c2<T> *__restrict__ scaled, int64_t nrow, c2<T> *__r
 }
 
 // ---------------------------------------------------------------------------
+// synthesis on the matrix cores (FP32 plans).  Per m the l-sums of
+// spin_synthesis_kernel are four products over l,
+//   X_{f,c}[p][n] = sum_{l in class c} f_l(p) Y[l][n],   f = a, b;
+//   c = e (l - l0 = 0 mod 2), o (odd);  Y[l][n] = (u.re, u.im, v.re, v.im)[n%4]
+// of polarisation pair n/4 (u, v: the scaled rows of spin_prescale_kernel),
+// and with sg = (-1)^{l0+m}, sigma = (-1)^s
+//   P_n = -(X_ae + X_ao)[u],        M_n = -sigma (X_be + X_bo)[v],
+//   P_s = -sg (X_be - X_bo)[u],     M_s = -sigma sg (X_ae - X_ao)[v],
+//   G_Q = (P + M)/2,  G_U = -i (P - M)/2.
+// On v_mfma_f32_32x32x2f32 (rows: 32 ring pairs of a wave, columns: 8
+// polarisation pairs x 4): lanes 0-31 run the FP64 a-recursion and lanes
+// 32-63 the b-recursion of the wave's 32 ring pairs over a 4-l step, and
+// v_permlane32_swap of (l0, l0+2) and (l0+1, l0+3) gives the A operands of
+// the four products; Y is staged in LDS per 64-l chunk.  The four lanes of
+// one polarisation pair (columns 4 j .. 4 j + 3) combine their sums with an
+// exchange across lane^2.
+// Lane layout of the 32x32x2 f32 instruction (checked on gfx950):
+//   A: lane = row + 32 k     B: lane = col + 32 k
+//   D[row][col]: lane = col + 32 ((row/4)%2), reg = (row%4) + 4 (row/8)
+// and v_permlane32_swap(a, b) returns ([a lanes 0-31 | b lanes 0-31],
+// [a lanes 32-63 | b lanes 32-63]).
+// ---------------------------------------------------------------------------
+typedef float sfloatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int SS_WAVES = 4;               // waves per workgroup (same m), 32 ring pairs each
+constexpr int SS_LCHUNK = 64;             // l staged per chunk
+
+__device__ __forceinline__ void sswap(float a, float b, float *t0, float *t1) {
+    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    *t0 = __uint_as_float(r[0]);
+    *t1 = __uint_as_float(r[1]);
+}
+
+__global__ void __launch_bounds__(SS_WAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 4)))
+spin_synthesis_mfma_kernel(const float2 *__restrict__ scaled, int64_t nrow, float2 *__restrict__ phase,
+                           int64_t pstride, const double *__restrict__ px, const int64_t *__restrict__ rofs,
+                           const double *__restrict__ alpha, const double *__restrict__ beta,
+                           const int32_t *__restrict__ lseed, const double4 *__restrict__ seed, int npairs, int nr,
+                           int lmax, int mmax, int nsb, int npol, int s) {
+    __shared__ float stage[SS_LCHUNK][33];
+    __shared__ double cf[2][SS_LCHUNK];     // alpha, beta of the chunk
+    __shared__ int wg_start;
+    int mb, item;
+    xcd_map(blockIdx.x, MB * nsb, &mb, &item);
+    const int m = mb * MB + item / nsb;
+    const int sb = item % nsb;
+    if (m > mmax) return;
+    const int g = blockIdx.y;                       // group of 8 polarisation pairs
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int r = lane & 31, h = lane >> 5;         // ring row; function (0: a, 1: b)
+
+    const int pbase = (sb * SS_WAVES + w) * 32;
+    const int p = pbase + r;
+    double x = 0.0, u1 = 0.0, u2 = 0.0;
+    double2 sd = make_double2(0.0, 0.0);
+    int ls = SNEVER;
+    if (p < npairs) {
+        x = px[p];
+        ls = lseed[(int64_t)m * npairs + p];
+        if (ls <= lmax) {
+            const double4 q = seed[(int64_t)m * npairs + p];
+            sd = h ? make_double2(q.z, q.w) : make_double2(q.x, q.y);
+        }
+    }
+    const int wstart = swmin(ls);
+    const int winj = swmax(ls <= lmax ? ls : -1);
+    if (tid == 0) wg_start = SNEVER;
+    __syncthreads();
+    if (lane == 0) atomicMin(&wg_start, wstart);
+    __syncthreads();
+    const int lstart = wg_start;
+
+    const int l0 = max(m, abs(s));
+    const int64_t ro = rofs[m] - m;                 // table / scaled-row index of l
+    // stage slots of this thread: l = c0 + (tid & 63), pairs j = tid/64 + 4 q
+    const int sll = tid & 63, sj0 = tid >> 6;
+
+    sfloatx16 Dae = sfloatx16{0}, Dao = Dae, Dbe = Dae, Dbo = Dae;
+    for (int c0 = lstart; c0 <= lmax; c0 += SS_LCHUNK) {
+        {
+            const int l = c0 + sll;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int j = sj0 + 4 * q, gb = g * 8 + j;
+                float2 u = make_float2(0.f, 0.f), v = u;
+                if (l <= lmax && gb < npol) {
+                    u = scaled[(int64_t)(2 * gb) * nrow + ro + l];
+                    v = scaled[(int64_t)(2 * gb + 1) * nrow + ro + l];
+                }
+                stage[sll][4 * j] = u.x;
+                stage[sll][4 * j + 1] = u.y;
+                stage[sll][4 * j + 2] = v.x;
+                stage[sll][4 * j + 3] = v.y;
+            }
+            if (tid < 2 * SS_LCHUNK) {
+                const int i = tid & (SS_LCHUNK - 1);
+                cf[tid >> 6][i] = (tid < SS_LCHUNK) ? alpha[ro + c0 + i] : beta[ro + c0 + i];
+            }
+        }
+        __syncthreads();
+        if (c0 + SS_LCHUNK > wstart) {
+            for (int kk = 0; kk < SS_LCHUNK / 4; ++kk) {
+                const int lk = c0 + 4 * kk;
+                if (lk > lmax) break;
+                if (lk <= winj && ls == lk) {
+                    u2 = sd.x;
+                    u1 = sd.y;
+                }
+                double v[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const double t = cf[0][4 * kk + i] * x;
+                    const double bl = h ? -cf[1][4 * kk + i] : cf[1][4 * kk + i];
+                    v[i] = fma(t + bl, u1, -u2);
+                    u2 = u1;
+                    u1 = v[i];
+                }
+                float ae, be_, ao, bo_;
+                sswap((float)v[0], (float)v[2], &ae, &be_);
+                sswap((float)v[1], (float)v[3], &ao, &bo_);
+                const float ye = stage[4 * kk + 2 * h][r];
+                const float yo = stage[4 * kk + 2 * h + 1][r];
+                Dae = __builtin_amdgcn_mfma_f32_32x32x2f32(ae, ye, Dae, 0, 0, 0);
+                Dao = __builtin_amdgcn_mfma_f32_32x32x2f32(ao, yo, Dao, 0, 0, 0);
+                Dbe = __builtin_amdgcn_mfma_f32_32x32x2f32(be_, ye, Dbe, 0, 0, 0);
+                Dbo = __builtin_amdgcn_mfma_f32_32x32x2f32(bo_, yo, Dbo, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    // this lane: column r = 4 j + c of pair j, rows (q%4) + 8 (q/4) + 4 h
+    const int j = r >> 2, c = r & 3, gb = g * 8 + j;
+    const float sigma = (s & 1) ? -1.0f : 1.0f;
+    const float sg = ((l0 + m) & 1) ? -1.0f : 1.0f;
+    // c < 2: north P, south P (u columns); c >= 2: north M, south M (v columns)
+    const float fn_ = c < 2 ? -1.0f : -sigma, fs_ = fn_ * sg;
+    // lanes c = 0, 1 write the north ring, c = 2, 3 the south ring
+    const bool north = c < 2;
+    float *oq = reinterpret_cast<float *>(phase + (int64_t)(2 * min(gb, npol - 1)) * pstride + (int64_t)mb * nr * MB +
+                                          (m & 7));
+    float *ou = oq + 2 * pstride;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int row = (q & 3) + 8 * (q >> 2) + 4 * h;
+        const int pp = pbase + row;
+        // P columns take the a-sums north and the b-sums south; M columns the reverse
+        const float nv = fn_ * (c < 2 ? Dae[q] + Dao[q] : Dbe[q] + Dbo[q]);
+        const float sv_ = fs_ * (c < 2 ? Dbe[q] - Dbo[q] : Dae[q] - Dao[q]);
+        // partner column c ^ 2: P <-> M of the same part (re / im)
+        const float pn = __shfl_xor(nv, 2, 64), ps = __shfl_xor(sv_, 2, 64);
+        if (pp >= npairs || gb >= npol) continue;
+        const int rn = pp, rs = nr - 1 - pp;
+        if (!north && rs == rn) continue;
+        // own = P (c < 2) or M (c >= 2) of part c & 1 on this lane's ring
+        const float P = north ? nv : ps, M = north ? pn : sv_;
+        const int ring = north ? rn : rs;
+        const int part = c & 1;
+        // G_Q = (P + M)/2; G_U = -i (P - M)/2: re(G_U) = (P.im - M.im)/2, im(G_U) = -(P.re - M.re)/2
+        oq[(int64_t)ring * (2 * MB) + part] = 0.5f * (P + M);
+        ou[(int64_t)ring * (2 * MB) + (part ^ 1)] = part ? 0.5f * (P - M) : -0.5f * (P - M);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 // ring-pair slots per lane: the plan's tile, else up to 4 (more costs
@@ -783,7 +949,7 @@ int launch_spin_analysis(const Plan *p, int nmaps, const void *phase, void *alm,
 }
 
 // FP32 batches of at least mfma_min_maps maps (Q and U count) contract on the
-// matrix cores; FP64 plans and smaller batches run spin_analysis_kernel
+// matrix cores; FP64 plans and smaller batches run the VALU kernels
 bool spin_uses_mfma(const Plan *p, int npol) { return p->precision == HBX_F32 && 2 * npol >= mfma_min_maps(p); }
 
 int spin_mfma_groups(const Plan *p) { return (p->npairs + SpinMfGeom::P - 1) / SpinMfGeom::P; }
@@ -825,8 +991,21 @@ int launch_spin_prescale(const Plan *p, int npol, const void *alm, int64_t astri
 }
 
 int launch_spin_synthesis(const Plan *p, int nmaps, const void *scaled, void *phase, hipStream_t s) {
-    const int npb = (p->npairs + 63) / 64;
     const int nmb8 = (p->nmb + 7) / 8 * 8;
+    if (spin_uses_mfma(p, nmaps)) {
+        const int nsb = (p->npairs + SS_WAVES * 32 - 1) / (SS_WAVES * 32);
+        dim3 grid(nmb8 * MB * nsb, (nmaps + 7) / 8);
+        timer_mark(HBX_TIMER_SPIN_SYNTHESIS, true, s);
+        hipLaunchKernelGGL(spin_synthesis_mfma_kernel, grid, dim3(SS_WAVES * 64), 0, s, (const float2 *)scaled,
+                           p->nrow, (float2 *)phase, phase_elems_per_map(p), p->d_px, p->d_rofs, p->d_alpha,
+                           p->d_beta, p->d_lseed, p->d_seed4, p->npairs, p->nrings, p->lmax, p->mmax, nsb, nmaps,
+                           p->spin);
+        timer_mark(HBX_TIMER_SPIN_SYNTHESIS, false, s);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_error(e, "spin_synthesis_mfma_kernel launch");
+        return 0;
+    }
+    const int npb = (p->npairs + 63) / 64;
     dim3 grid(nmb8 * npb, nmaps);
     const int64_t pstride = phase_elems_per_map(p);
     timer_mark(HBX_TIMER_SPIN_SYNTHESIS, true, s);

@@ -2,8 +2,8 @@
 the FP64 oracle (oracle/spin.py, pinned in test_oracle_spin.py).
 
 FP64 bar: relative max error <= 1e-11.  FP32 plans (FP32 data and
-contraction -- on the matrix cores for the analysis of batches --, FP64
-recursion, FP64 band m < 8 in the analysis): relative max
+contraction -- on the matrix cores for batches --, FP64 recursion, FP64
+band m < 8 in the analysis): relative max
 error <= 2e-5 of the largest coefficient at nside 32, and eps_max =
 max |da_lm| / sqrt(C_l) <= 1e-5 for E and for B at nside 2048, lmax 4095
 (4 Q/U pairs, C_l^BB = 0.1 C_l^EE).
@@ -179,6 +179,40 @@ def test_spin_f32_matrix_core_analysis(gpu):
         plan.set_tiles(mfma_min_maps=0)
     print(f"matrix vs VALU: {_rel(got, valu):.2e}, adjoint {_rel(adj, vadj):.2e}", flush=True)
     assert _rel(got, valu) <= 4e-6 and _rel(adj, vadj) <= 4e-6
+
+
+def test_spin_f32_matrix_core_synthesis(gpu):
+    """FP32 spin synthesis on the matrix cores (32 ring pairs per wave, 8
+    polarisation pairs per product) at nside 128, lmax 200 for s = 1, 2, 3
+    (s = 1, 3: the rows m < s start at an l of the other parity) on 5 Q/U
+    pairs (a partly filled pair group): against the FP64 oracle; equal to
+    the VALU kernel to FP32 rounding, for the synthesis and its weighted
+    transpose (map2alm_spin_adjoint); one pair alone gives the bits it gets
+    inside the batch."""
+    nside, lmax = 128, 200
+    tabs = OG.ring_tables(nside)
+    for s in (1, 2, 3):
+        print(f"[case] spin f32 matrix-core synthesis nside=128 lmax=200 s={s} 5 pairs", flush=True)
+        plan = _plan(nside, lmax, spin=s, precision="f32")
+        eb = np.stack([np.stack(OSp.synalm_eb(lmax, spin=s, seed=40 + b)) for b in range(5)])
+        a = torch.as_tensor(eb, device="cuda").to(torch.complex64)
+        got = plan.alm2map_spin(a, dtype=torch.float32).double().cpu().numpy()
+        e32 = eb.astype(np.complex64).astype(np.complex128)
+        for b in (0, 4):
+            q, u = OSp.alm2map_spin(e32[b, 0], e32[b, 1], tabs, lmax, spin=s)
+            print(f"pair {b}: Q {_rel(got[b, 0], q):.2e} U {_rel(got[b, 1], u):.2e}", flush=True)
+            assert _rel(got[b, 0], q) <= F32_TOL and _rel(got[b, 1], u) <= F32_TOL
+        one = plan.alm2map_spin(a[4:5], dtype=torch.float32).double().cpu().numpy()
+        assert np.array_equal(one[0], got[4])
+        adj = plan.map2alm_spin_adjoint(a, dtype=torch.float32).double().cpu().numpy()
+        plan.set_tiles(mfma_min_maps=1000)
+        try:
+            valu = plan.alm2map_spin(a, dtype=torch.float32).double().cpu().numpy()
+            vadj = plan.map2alm_spin_adjoint(a, dtype=torch.float32).double().cpu().numpy()
+        finally:
+            plan.set_tiles(mfma_min_maps=0)
+        print(f"matrix vs VALU: {_rel(got, valu):.2e}, adjoint {_rel(adj, vadj):.2e}", flush=True)
+        assert _rel(got, valu) <= 4e-6 and _rel(adj, vadj) <= 4e-6
 
 
 def _gl_geometry(lmax):
