@@ -702,55 +702,50 @@ spin_synthesis_kernel(const c2<T> *__restrict__ scaled, int64_t nrow, c2<T> *__r
 // ---------------------------------------------------------------------------
 // synthesis on the matrix cores (FP32 plans).  Per m the l-sums of
 // spin_synthesis_kernel are four products over l,
-//   X_{f,c}[p][n] = sum_{l in class c} f_l(p) Y[l][n],   f = a, b;
-//   c = e (l - l0 = 0 mod 2), o (odd);  Y[l][n] = (u.re, u.im, v.re, v.im)[n%4]
-// of polarisation pair n/4 (u, v: the scaled rows of spin_prescale_kernel),
-// and with sg = (-1)^{l0+m}, sigma = (-1)^s
+//   X_{f,c}[n][p] = sum_{l in class c} Y[n][l] f_l(p),   f = a, b;
+//   c = e (l - l0 even), o (odd);  Y[n][l] = (u.re, u.im, v.re, v.im)[n%4]
+// of polarisation pair n/4 of a group of 4 (u, v: the scaled rows of
+// spin_prescale_kernel), and with sg = (-1)^{l0+m}, sigma = (-1)^s
 //   P_n = -(X_ae + X_ao)[u],        M_n = -sigma (X_be + X_bo)[v],
 //   P_s = -sg (X_be - X_bo)[u],     M_s = -sigma sg (X_ae - X_ao)[v],
 //   G_Q = (P + M)/2,  G_U = -i (P - M)/2.
-// On v_mfma_f32_32x32x2f32 (rows: 32 ring pairs of a wave, columns: 8
-// polarisation pairs x 4): lanes 0-31 run the FP64 a-recursion and lanes
-// 32-63 the b-recursion of the wave's 32 ring pairs over a 4-l step, and
-// v_permlane32_swap of (l0, l0+2) and (l0+1, l0+3) gives the A operands of
-// the four products; Y is staged in LDS per 64-l chunk.  The four lanes of
-// one polarisation pair (columns 4 j .. 4 j + 3) combine their sums with an
-// exchange across lane^2.
-// Lane layout of the 32x32x2 f32 instruction (checked on gfx950):
-//   A: lane = row + 32 k     B: lane = col + 32 k
-//   D[row][col]: lane = col + 32 ((row/4)%2), reg = (row%4) + 4 (row/8)
-// and v_permlane32_swap(a, b) returns ([a lanes 0-31 | b lanes 0-31],
-// [a lanes 32-63 | b lanes 32-63]).
+// On v_mfma_f32_16x16x4f32 with A = Y (16 rows n, 4 l of one class), B =
+// lambda (4 l, 16 ring pairs): every wave works alone on 32 ring pairs of
+// one m.  Per 16-l chunk lanes 0-31 run the FP64 a-recursion and lanes 32-63
+// the b-recursion of the wave's pairs into the wave's own LDS tiles
+// [f][l][pair]; after a wave barrier the wave contracts them (2 column tiles
+// x 2 functions x 2 classes, 2 l-steps of 8).  Small per-wave tiles keep 5
+// waves per SIMD resident, so one wave's recursion overlaps another's
+// products.  The output lane holds the
+// four columns of one polarisation pair for one ring pair: D lane =
+// col + 16 (row/4), reg = row%4, so it combines them without exchanges.
+// Lane layout of the 16x16x4 f32 instruction (checked on gfx950):
+//   A: lane = row + 16 k   B: lane = col + 16 k   D: lane = col + 16 (row/4), reg = row%4
 // ---------------------------------------------------------------------------
-typedef float sfloatx16 __attribute__((ext_vector_type(16)));
-
 constexpr int SS_WAVES = 4;               // waves per workgroup (same m), 32 ring pairs each
-constexpr int SS_LCHUNK = 64;             // l staged per chunk
+constexpr int SS_L = 16;                  // l per chunk
+constexpr int SS_ROW = 40;                // floats per tile row (32 pairs + 8): 2 rows = 16 mod 64 banks
+constexpr int SS_TILE = SS_L * SS_ROW + 32;   // floats per function tile (b tile 32 banks from a's)
+constexpr int SS_WAVE_LDS = 2 * SS_TILE;      // floats per wave
 
-__device__ __forceinline__ void sswap(float a, float b, float *t0, float *t1) {
-    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
-    *t0 = __uint_as_float(r[0]);
-    *t1 = __uint_as_float(r[1]);
-}
-
-__global__ void __launch_bounds__(SS_WAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 4)))
+__global__ void __launch_bounds__(SS_WAVES * 64)
 spin_synthesis_mfma_kernel(const float2 *__restrict__ scaled, int64_t nrow, float2 *__restrict__ phase,
                            int64_t pstride, const double *__restrict__ px, const int64_t *__restrict__ rofs,
                            const double *__restrict__ alpha, const double *__restrict__ beta,
                            const int32_t *__restrict__ lseed, const double4 *__restrict__ seed, int npairs, int nr,
                            int lmax, int mmax, int nsb, int npol, int s) {
-    __shared__ float stage[SS_LCHUNK][33];
-    __shared__ double cf[2][SS_LCHUNK];     // alpha, beta of the chunk
-    __shared__ int wg_start;
+    __shared__ float tiles[SS_WAVES * SS_WAVE_LDS];
+    __shared__ double coefs[SS_WAVES][2][SS_L];     // per wave: alpha, beta of the chunk
     int mb, item;
     xcd_map(blockIdx.x, MB * nsb, &mb, &item);
     const int m = mb * MB + item / nsb;
     const int sb = item % nsb;
     if (m > mmax) return;
-    const int g = blockIdx.y;                       // group of 8 polarisation pairs
+    const int g = blockIdx.y;                       // group of 4 polarisation pairs
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int r = lane & 31, h = lane >> 5;         // ring row; function (0: a, 1: b)
+    const int r = lane & 31, h = lane >> 5;         // recursion: pair r of the wave, function h (0: a, 1: b)
+    float *tw = tiles + w * SS_WAVE_LDS;
 
     const int pbase = (sb * SS_WAVES + w) * 32;
     const int p = pbase + r;
@@ -765,103 +760,118 @@ spin_synthesis_mfma_kernel(const float2 *__restrict__ scaled, int64_t nrow, floa
             sd = h ? make_double2(q.z, q.w) : make_double2(q.x, q.y);
         }
     }
-    const int wstart = swmin(ls);
+    const int lstart = swmin(ls);
     const int winj = swmax(ls <= lmax ? ls : -1);
-    if (tid == 0) wg_start = SNEVER;
-    __syncthreads();
-    if (lane == 0) atomicMin(&wg_start, wstart);
-    __syncthreads();
-    const int lstart = wg_start;
 
     const int l0 = max(m, abs(s));
     const int64_t ro = rofs[m] - m;                 // table / scaled-row index of l
-    // stage slots of this thread: l = c0 + (tid & 63), pairs j = tid/64 + 4 q
-    const int sll = tid & 63, sj0 = tid >> 6;
+    // A operand of this lane: Y[n = lane%16][l], k = lane/16
+    const int n = lane & 15, k = lane >> 4;
+    const int gb = g * 4 + (n >> 2), comp = n & 3;
+    const float *yrow = reinterpret_cast<const float *>(scaled + (int64_t)(2 * min(gb, npol - 1) + (comp >> 1)) * nrow + ro) +
+                        (comp & 1);
+    const bool yok = gb < npol;
+    float *lw = tw + h * SS_TILE + r;               // recursion writes: [h][l][r]
+    const float *lr = tw + k * 2 * SS_ROW + n;      // B reads: [f][l = 8 st + 2 k + c][16 t + col]
 
-    sfloatx16 Dae = sfloatx16{0}, Dao = Dae, Dbe = Dae, Dbo = Dae;
-    for (int c0 = lstart; c0 <= lmax; c0 += SS_LCHUNK) {
-        {
-            const int l = c0 + sll;
+    sfloatx4 D[2][2][2];                            // [column tile][function][class]
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int j = sj0 + 4 * q, gb = g * 8 + j;
-                float2 u = make_float2(0.f, 0.f), v = u;
-                if (l <= lmax && gb < npol) {
-                    u = scaled[(int64_t)(2 * gb) * nrow + ro + l];
-                    v = scaled[(int64_t)(2 * gb + 1) * nrow + ro + l];
-                }
-                stage[sll][4 * j] = u.x;
-                stage[sll][4 * j + 1] = u.y;
-                stage[sll][4 * j + 2] = v.x;
-                stage[sll][4 * j + 3] = v.y;
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) D[t][f][c] = sfloatx4{0.f, 0.f, 0.f, 0.f};
+    // chunk operands, loaded a chunk ahead: Y of this lane (SS_L/4 l) and the
+    // chunk's alpha, beta (lanes i and i + 32 hold l = c0 + i), which the
+    // wave stages in LDS for broadcast reads
+    float yv[SS_L / 4];
+    double av = 0.0, bv = 0.0;
+    auto load = [&](int c0) {
+#pragma unroll
+        for (int st = 0; st < SS_L / 8; ++st)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int l = c0 + 8 * st + 2 * k + c;
+                yv[2 * st + c] = (yok && l <= lmax) ? yrow[2 * (int64_t)l] : 0.0f;
             }
-            if (tid < 2 * SS_LCHUNK) {
-                const int i = tid & (SS_LCHUNK - 1);
-                cf[tid >> 6][i] = (tid < SS_LCHUNK) ? alpha[ro + c0 + i] : beta[ro + c0 + i];
+        av = alpha[ro + c0 + (r & (SS_L - 1))];
+        bv = beta[ro + c0 + (r & (SS_L - 1))];
+    };
+    const double sgn = h ? -1.0 : 1.0;              // b: beta -> -beta
+    const double *ca = coefs[w][0], *cb = coefs[w][1];
+    if (lstart <= lmax) load(lstart);
+    for (int c0 = lstart; c0 <= lmax; c0 += SS_L) {
+        float yc[SS_L / 4];
+#pragma unroll
+        for (int i = 0; i < SS_L / 4; ++i) yc[i] = yv[i];
+        if (r < SS_L) coefs[w][h][r] = h ? bv : av;
+        if (c0 + SS_L <= lmax) load(c0 + SS_L);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // 1. recursion of this lane's (pair, function) over l = c0 .. c0+SS_L-1;
+        // past lmax the tables run into the next row: those values stay out
+        // of the tile (Y is zero there, but 0 * inf would not be)
+        const bool tail = c0 + SS_L > lmax + 1;
+#pragma unroll
+        for (int tt = 0; tt < SS_L; ++tt) {
+            if ((tt & 7) == 0 && c0 <= winj && ls == c0 + tt) {
+                u2 = sd.x;
+                u1 = sd.y;
+            }
+            const double t = ca[tt] * x;
+            const double u = fma(fma(sgn, cb[tt], t), u1, -u2);
+            u2 = u1;
+            u1 = u;
+            lw[tt * SS_ROW] = (!tail || c0 + tt <= lmax) ? (float)u : 0.0f;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // 2. the wave's products over the chunk: 4 steps of 8 l
+#pragma unroll
+        for (int st = 0; st < SS_L / 8; ++st) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                for (int f = 0; f < 2; ++f)
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        const float lam = lr[f * SS_TILE + (8 * st + c) * SS_ROW + 16 * t];
+                        D[t][f][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(yc[2 * st + c], lam, D[t][f][c], 0, 0, 0);
+                    }
             }
         }
-        __syncthreads();
-        if (c0 + SS_LCHUNK > wstart) {
-            for (int kk = 0; kk < SS_LCHUNK / 4; ++kk) {
-                const int lk = c0 + 4 * kk;
-                if (lk > lmax) break;
-                if (lk <= winj && ls == lk) {
-                    u2 = sd.x;
-                    u1 = sd.y;
-                }
-                double v[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const double t = cf[0][4 * kk + i] * x;
-                    const double bl = h ? -cf[1][4 * kk + i] : cf[1][4 * kk + i];
-                    v[i] = fma(t + bl, u1, -u2);
-                    u2 = u1;
-                    u1 = v[i];
-                }
-                float ae, be_, ao, bo_;
-                sswap((float)v[0], (float)v[2], &ae, &be_);
-                sswap((float)v[1], (float)v[3], &ao, &bo_);
-                const float ye = stage[4 * kk + 2 * h][r];
-                const float yo = stage[4 * kk + 2 * h + 1][r];
-                Dae = __builtin_amdgcn_mfma_f32_32x32x2f32(ae, ye, Dae, 0, 0, 0);
-                Dao = __builtin_amdgcn_mfma_f32_32x32x2f32(ao, yo, Dao, 0, 0, 0);
-                Dbe = __builtin_amdgcn_mfma_f32_32x32x2f32(be_, ye, Dbe, 0, 0, 0);
-                Dbo = __builtin_amdgcn_mfma_f32_32x32x2f32(bo_, yo, Dbo, 0, 0, 0);
-            }
-        }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 
-    // this lane: column r = 4 j + c of pair j, rows (q%4) + 8 (q/4) + 4 h
-    const int j = r >> 2, c = r & 3, gb = g * 8 + j;
+    // this lane: ring pair pbase + 16 t + (lane & 15), polarisation pair
+    // g*4 + lane/16, regs = (u.re, u.im, v.re, v.im) columns
+    const int jo = g * 4 + (lane >> 4);
+    if (jo >= npol) return;
     const float sigma = (s & 1) ? -1.0f : 1.0f;
     const float sg = ((l0 + m) & 1) ? -1.0f : 1.0f;
-    // c < 2: north P, south P (u columns); c >= 2: north M, south M (v columns)
-    const float fn_ = c < 2 ? -1.0f : -sigma, fs_ = fn_ * sg;
-    // lanes c = 0, 1 write the north ring, c = 2, 3 the south ring
-    const bool north = c < 2;
-    float *oq = reinterpret_cast<float *>(phase + (int64_t)(2 * min(gb, npol - 1)) * pstride + (int64_t)mb * nr * MB +
-                                          (m & 7));
-    float *ou = oq + 2 * pstride;
+    float2 *oq = phase + (int64_t)(2 * jo) * pstride + (int64_t)mb * nr * MB + (m & 7);
+    float2 *ou = oq + pstride;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int row = (q & 3) + 8 * (q >> 2) + 4 * h;
-        const int pp = pbase + row;
-        // P columns take the a-sums north and the b-sums south; M columns the reverse
-        const float nv = fn_ * (c < 2 ? Dae[q] + Dao[q] : Dbe[q] + Dbo[q]);
-        const float sv_ = fs_ * (c < 2 ? Dbe[q] - Dbo[q] : Dae[q] - Dao[q]);
-        // partner column c ^ 2: P <-> M of the same part (re / im)
-        const float pn = __shfl_xor(nv, 2, 64), ps = __shfl_xor(sv_, 2, 64);
-        if (pp >= npairs || gb >= npol) continue;
+    for (int t = 0; t < 2; ++t) {
+        const int pp = pbase + 16 * t + (lane & 15);
+        if (pp >= npairs) continue;
+        const sfloatx4 ae = D[t][0][0], ao = D[t][0][1], be = D[t][1][0], bo = D[t][1][1];
+        const float pnr = -(ae[0] + ao[0]), pni = -(ae[1] + ao[1]);
+        const float mnr = -sigma * (be[2] + bo[2]), mni = -sigma * (be[3] + bo[3]);
+        const float psr = -sg * (be[0] - bo[0]), psi = -sg * (be[1] - bo[1]);
+        const float msr = -sigma * sg * (ae[2] - ao[2]), msi = -sigma * sg * (ae[3] - ao[3]);
         const int rn = pp, rs = nr - 1 - pp;
-        if (!north && rs == rn) continue;
-        // own = P (c < 2) or M (c >= 2) of part c & 1 on this lane's ring
-        const float P = north ? nv : ps, M = north ? pn : sv_;
-        const int ring = north ? rn : rs;
-        const int part = c & 1;
-        // G_Q = (P + M)/2; G_U = -i (P - M)/2: re(G_U) = (P.im - M.im)/2, im(G_U) = -(P.re - M.re)/2
-        oq[(int64_t)ring * (2 * MB) + part] = 0.5f * (P + M);
-        ou[(int64_t)ring * (2 * MB) + (part ^ 1)] = part ? 0.5f * (P - M) : -0.5f * (P - M);
+        // G_Q = (P + M)/2, G_U = -i (P - M)/2
+        oq[(int64_t)rn * MB] = make_float2(0.5f * (pnr + mnr), 0.5f * (pni + mni));
+        ou[(int64_t)rn * MB] = make_float2(0.5f * (pni - mni), -0.5f * (pnr - mnr));
+        if (rs != rn) {
+            oq[(int64_t)rs * MB] = make_float2(0.5f * (psr + msr), 0.5f * (psi + msi));
+            ou[(int64_t)rs * MB] = make_float2(0.5f * (psi - msi), -0.5f * (psr - msr));
+        }
     }
 }
 
@@ -994,7 +1004,7 @@ int launch_spin_synthesis(const Plan *p, int nmaps, const void *scaled, void *ph
     const int nmb8 = (p->nmb + 7) / 8 * 8;
     if (spin_uses_mfma(p, nmaps)) {
         const int nsb = (p->npairs + SS_WAVES * 32 - 1) / (SS_WAVES * 32);
-        dim3 grid(nmb8 * MB * nsb, (nmaps + 7) / 8);
+        dim3 grid(nmb8 * MB * nsb, (nmaps + 3) / 4);
         timer_mark(HBX_TIMER_SPIN_SYNTHESIS, true, s);
         hipLaunchKernelGGL(spin_synthesis_mfma_kernel, grid, dim3(SS_WAVES * 64), 0, s, (const float2 *)scaled,
                            p->nrow, (float2 *)phase, phase_elems_per_map(p), p->d_px, p->d_rofs, p->d_alpha,
