@@ -11,6 +11,8 @@ ring stage's HBM rate and the CPU oracle timed on a bounded sample.  Prints
 ONE JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+    python bench.py --spin      (spin-2: 4 Q/U pairs FP32 vs 1 pair FP64)
+    python bench.py --mixed     (64 maps of mixed nside in one HIP graph)
 """
 import argparse
 import json
@@ -41,6 +43,8 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spin", action="store_true",
+                    help="spin-2 workload: 4 Q/U pairs FP32 vs 1 pair FP64 at nside 2048, lmax 4095")
     ap.add_argument("--mixed", action="store_true",
                     help="instead of the headline: 64 FP32 maps of mixed nside (128..2048, lmax 2 nside) "
                          "in one HIP graph with autotuned tiles; prints its own JSON line")
@@ -263,6 +267,103 @@ def mixed_main(args):
     print(json.dumps(res), flush=True)
 
 
+def spin_main(args):
+    """Spin-2 Q/U <-> E/B at nside 2048, lmax 4095: one step = map2alm_spin +
+    alm2map_spin of 4 Q/U pairs in FP32 (FP32 maps, alm and contraction on
+    the matrix cores; FP64 recursion), the FP64 drop-in (1 pair) timed
+    beside it.  E/B drawn with C_l^EE = 1/(l+1)^2 (l >= 2), C_l^BB = 0.1
+    C_l^EE; Q/U = FP64 synthesis rounded to FP32; eps_max of E and B against
+    the FP64 analysis of the same FP32 maps."""
+    import torch
+    from hellblast_amd.hp_compat import _plan
+    from hellblast_amd.plan import KernelTimer, nalm
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    nside = args.nside
+    lmax = 2 * nside - 1 if args.lmax is None else args.lmax
+    npol = 4
+    t0 = time.time()
+    p64 = _plan(nside, lmax, lmax, "f64", "dz", "reference", "reference", 2)
+    p32 = _plan(nside, lmax, lmax, "f32", "dz", "reference", "reference", 2)
+    torch.cuda.synchronize()
+    log(f"spin plans built in {time.time() - t0:.1f}s")
+    n = nalm(lmax)
+    rows = torch.arange(lmax + 1, 0, -1, device=dev)
+    ms_ = torch.repeat_interleave(torch.arange(lmax + 1, device=dev), rows)
+    ls = torch.arange(n, device=dev) - torch.repeat_interleave(torch.cumsum(rows, 0) - rows, rows) + ms_
+    g = torch.Generator(device=dev)
+    g.manual_seed(21)
+
+    def draw(amp):
+        sd = torch.where(ls < 2, torch.zeros(n, device=dev, dtype=torch.float64), amp / (ls.double() + 1.0))
+        sc = torch.where(ms_ == 0, sd, sd / math.sqrt(2.0))
+        re = torch.randn(n, generator=g, device=dev, dtype=torch.float64)
+        im = torch.randn(n, generator=g, device=dev, dtype=torch.float64)
+        return torch.complex(re * sc, torch.where(ms_ == 0, torch.zeros_like(im), im * sc))
+
+    eb = torch.stack([torch.stack([draw(1.0), draw(math.sqrt(0.1))]) for _ in range(npol)])
+    qu = p64.alm2map_spin(eb).to(torch.float32)                  # [4, 2, npix]
+    eb32 = eb.to(torch.complex64)
+    got = p32.map2alm_spin(qu)
+    eps = {"E": 0.0, "B": 0.0}
+    for b in range(npol):
+        ref = p64.map2alm_spin(qu[b].double())
+        for k, name in enumerate("EB"):
+            eps[name] = max(eps[name], eps_metrics(torch, got[b, k], ref[k], ls, lmax)[0])
+    del got, ref
+    log(f"FP32 vs FP64: eps_max E {eps['E']:.2e} B {eps['B']:.2e}")
+
+    out_eb = torch.empty((npol, 2, n), dtype=torch.complex64, device=dev)
+    out_qu = torch.empty_like(qu)
+
+    def step32():
+        p32.map2alm_spin(qu, out=out_eb)
+        p32.alm2map_spin(eb32, dtype=torch.float32, out=out_qu)
+
+    kt = KernelTimer("spin_analysis", "spin_synthesis")
+    ms32 = time_steps(torch, step32, args.steps, args.warmup, kt)
+    kms = {k: kt.mean_ms(k) for k in kt.kernels}
+    q64 = qu[:1].double()
+    out64a = torch.empty((1, 2, n), dtype=torch.complex128, device=dev)
+    out64m = torch.empty((1, 2, p64.npix), dtype=torch.float64, device=dev)
+
+    def step64():
+        p64.map2alm_spin(q64, out=out64a)
+        p64.alm2map_spin(eb[:1], out=out64m)
+
+    kt64 = KernelTimer("spin_analysis", "spin_synthesis")
+    ms64 = time_steps(torch, step64, args.fp64_steps, 1, kt64)
+    kms64 = {k: kt64.mean_ms(k) for k in kt64.kernels}
+    log("kernel means (ms): FP32", json.dumps({k: round(v, 3) for k, v in kms.items()}),
+        "FP64", json.dumps({k: round(v, 3) for k, v in kms64.items()}))
+    # algorithmic flops: 16 per (ring pair, l, m) per Q/U pair and direction;
+    # the FP32 analysis kernel covers m >= 8 (the band m < 8 runs in FP64)
+    npairs = 2 * nside
+    mb = p32.fp64_band_m
+    band = mb * (2 * lmax + 1 - mb) // 2 + mb
+    flops = {"spin_analysis": 16.0 * npairs * (n - band) * npol, "spin_synthesis": 16.0 * npairs * n * npol}
+    tf = {k: flops[k] / (kms[k] * 1e-3) / 1e12 for k in flops}
+    dom = max(flops, key=lambda k: kms[k])
+    names = {"spin_analysis": "spin_analysis_mfma_kernel", "spin_synthesis": "spin_synthesis_mfma_kernel"}
+    pairs32, pairs64 = npol * 1000.0 / ms32, 1000.0 / ms64
+    res = {"metric": "spin-2 SHT pairs/s nside2048 lmax4095 (Q/U pairs): FP32-batched vs FP64",
+           "value": round(pairs32, 3), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms32, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"nside{nside} lmax{lmax} spin-2: 4 Q/U pairs map2alm_spin+alm2map_spin, "
+                                  "C_l^EE=1/(l+1)^2, C_l^BB=0.1 C_l^EE",
+                      "fp64_pairs_per_s": round(pairs64, 3), "fp32_over_fp64": round(pairs32 / pairs64, 3),
+                      "eps_max_E": eps["E"], "eps_max_B": eps["B"], "gate_1e-5": bool(max(eps.values()) <= 1e-5),
+                      "kernel_ms_f32": {k: round(v, 4) for k, v in kms.items()},
+                      "kernel_ms_f64": {k: round(v, 4) for k, v in kms64.items()},
+                      "tflops_f32": {k: round(v, 3) for k, v in tf.items()}},
+           "roofline": {"bound": "mfma", "achieved": round(tf[dom], 3), "peak": PEAK_F32, "unit": "TFLOP/s",
+                        "frac": round(tf[dom] / PEAK_F32, 4), "traffic": None, "kernel": names[dom],
+                        "flops_per_launch": flops[dom], "launch_ms": round(kms[dom], 4)},
+           "cpu_baseline": None}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -271,6 +372,8 @@ def main():
         sys.exit(2)
     if args.mixed:
         return mixed_main(args)
+    if args.spin:
+        return spin_main(args)
     import torch
     from hellblast_amd import _abi
     from hellblast_amd.hp_compat import _plan
