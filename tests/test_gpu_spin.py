@@ -215,6 +215,35 @@ def test_spin_f32_matrix_core_synthesis(gpu):
         assert _rel(got, valu) <= 4e-6 and _rel(adj, vadj) <= 4e-6
 
 
+@pytest.mark.parametrize("s", [1, 3])
+def test_spin_f32_matrix_cores_options(gpu, s):
+    """The FP32 matrix-core spin kernels on the other spins (l0 = max(m, s)),
+    mmax < lmax with mmax + 1 not a multiple of 8 (a partial last m-block),
+    HEALPix phi0 and the periodic alias fold, 3 Q/U pairs (one partly filled
+    pair group): analysis and synthesis against the FP64 oracle."""
+    nside, lmax, mmax = 64, 150, 100
+    print(f"[case] spin f32 matrix cores options s={s} nside=64 lmax=150 mmax=100", flush=True)
+    tabs = OG.ring_tables(nside)
+    ph = tabs["phi0_healpix"]
+    mc = np.full(tabs["Nr"], mmax + 1)
+    plan = _plan(nside, lmax, mmax, spin=s, precision="f32", phi0="healpix", alias="fold")
+    qu = _qu(tabs, 50 + s, n=3)
+    got = plan.map2alm_spin(torch.as_tensor(qu, device="cuda", dtype=torch.float32)).cpu().numpy()
+    q32 = qu.astype(np.float32).astype(np.float64)
+    for b in (0, 2):
+        E, B = OSp.map2alm_spin(q32[b, 0], q32[b, 1], tabs, lmax, mmax, spin=s, phi0=ph, mcut=mc)
+        print(f"analysis pair {b}: E {_rel(got[b, 0], E):.2e} B {_rel(got[b, 1], B):.2e}", flush=True)
+        assert _rel(got[b, 0], E) <= F32_TOL and _rel(got[b, 1], B) <= F32_TOL
+    eb = np.stack([np.stack(OSp.synalm_eb(lmax, mmax, spin=s, seed=60 + b)) for b in range(3)])
+    a = torch.as_tensor(eb, device="cuda").to(torch.complex64)
+    gm = plan.alm2map_spin(a, dtype=torch.float32).double().cpu().numpy()
+    e32 = eb.astype(np.complex64).astype(np.complex128)
+    for b in (0, 2):
+        q, u = OSp.alm2map_spin(e32[b, 0], e32[b, 1], tabs, lmax, mmax, spin=s, phi0=ph, mcut=mc)
+        print(f"synthesis pair {b}: Q {_rel(gm[b, 0], q):.2e} U {_rel(gm[b, 1], u):.2e}", flush=True)
+        assert _rel(gm[b, 0], q) <= F32_TOL and _rel(gm[b, 1], u) <= F32_TOL
+
+
 def _gl_geometry(lmax):
     from hellblast_amd.plan import Geometry
     x, w = np.polynomial.legendre.leggauss(lmax + 1)
