@@ -215,6 +215,28 @@ def test_spin_f32_matrix_core_synthesis(gpu):
         assert _rel(got, valu) <= 4e-6 and _rel(adj, vadj) <= 4e-6
 
 
+@pytest.mark.parametrize("nside,lmax,s", [(1, 2, 2), (2, 5, 2), (4, 8, 1), (8, 24, 3), (16, 40, 2)])
+def test_spin_f32_small_sizes(gpu, nside, lmax, s):
+    """FP32 spin batches at small sizes: mmax < 8 (the analysis is all FP64
+    band, the synthesis runs the matrix kernel on every m), one partly used
+    ring-pair tile, the self-paired equator ring; 2 Q/U pairs."""
+    print(f"[case] spin f32 small nside={nside} lmax={lmax} s={s}", flush=True)
+    tabs = OG.ring_tables(nside)
+    plan = _plan(nside, lmax, spin=s, precision="f32")
+    qu = _qu(tabs, 70 + nside, n=2)
+    got = plan.map2alm_spin(torch.as_tensor(qu, device="cuda", dtype=torch.float32)).cpu().numpy()
+    q32 = qu.astype(np.float32).astype(np.float64)
+    eb = np.stack([np.stack(OSp.synalm_eb(lmax, spin=s, seed=80 + b)) for b in range(2)])
+    gm = plan.alm2map_spin(torch.as_tensor(eb, device="cuda").to(torch.complex64),
+                           dtype=torch.float32).double().cpu().numpy()
+    e32 = eb.astype(np.complex64).astype(np.complex128)
+    for b in range(2):
+        E, B = OSp.map2alm_spin(q32[b, 0], q32[b, 1], tabs, lmax, spin=s)
+        assert _rel(got[b, 0], E) <= F32_TOL and _rel(got[b, 1], B) <= F32_TOL
+        q, u = OSp.alm2map_spin(e32[b, 0], e32[b, 1], tabs, lmax, spin=s)
+        assert _rel(gm[b, 0], q) <= F32_TOL and _rel(gm[b, 1], u) <= F32_TOL
+
+
 @pytest.mark.parametrize("s", [1, 3])
 def test_spin_f32_matrix_cores_options(gpu, s):
     """The FP32 matrix-core spin kernels on the other spins (l0 = max(m, s)),
