@@ -164,6 +164,59 @@ def eb2iqu(E, B, nside=None, lmax=None, nest=False, **kw):
     return alm2map_spin(E, B, 2, nside=nside, lmax=lmax, nest=nest, **kw)
 
 
+# ---------- alm utilities (``hp_compat.py:113-120``) ----------
+def almxfl(alm, fl, lmax=None, mmax=None):
+    """a_lm f_l on m-major alm (``alm_ops.py:7-17``)."""
+    from .alm_ops import almxfl as _almxfl
+    return _almxfl(alm, fl, lmax=lmax, mmax=mmax)
+
+
+def synalm(cl, seed=None, lmax=None, mmax=None):
+    """Gaussian m-major alm with spectrum cl (``alm_ops.py:34-54``; drawn with
+    NumPy ``default_rng(seed)``)."""
+    from .alm_ops import synalm as _synalm
+    return _synalm(cl, lmax=lmax, mmax=mmax, seed=seed)
+
+
+# ---------- RING geometry (``hp_compat.py:131-149``) ----------
+def _check_nest_pix(nest):
+    if nest:
+        raise NotImplementedError("NESTED pixel indexing is not part of this package; use nest=False (RING)")
+
+
+def ang2pix(nside, theta, phi, nest=False):
+    """RING pixel of (theta, phi) (``coords_ring.cu:9-28``); NESTED is not built."""
+    _check_nest_pix(nest)
+    from .coords_ring import RingMapperGPU
+    return RingMapperGPU(int(nside)).ang2pix(theta, phi)
+
+
+def pix2ang(nside, ipix, nest=False):
+    """(theta, phi) of RING pixels (``coords_ring.cu:30-54``); NESTED is not built."""
+    _check_nest_pix(nest)
+    from .coords_ring import RingMapperGPU
+    return RingMapperGPU(int(nside)).pix2ang(ipix)
+
+
+def pix2vec(nside, ipix, nest=False):
+    """Unit vectors of RING pixel centres (``hp_compat.py:143-149``)."""
+    th, ph = pix2ang(nside, ipix, nest=nest)
+    return torch.sin(th) * torch.cos(ph), torch.sin(th) * torch.sin(ph), torch.cos(th)
+
+
+# ---------- smoothing and synfast (``hp_compat.py:228-236``) ----------
+def smooth_map(m, nside, lmax=None, fwhm=None, bl=None, use_pixel_window=True, nest=False, **kw):
+    from .beams import smooth_map as _smooth_map
+    return _smooth_map(m, nside, lmax=lmax, fwhm=fwhm, bl=bl, use_pixel_window=use_pixel_window, nest=nest, **kw)
+
+
+def synfast(cls, nside, lmax=None, fwhm=None, use_pixel_window=True, seed=None, sigmaT=None, sigmaQ=None,
+            sigmaU=None, return_alm=False, **kw):
+    from .synfast_tqu import synfast_tqu
+    return synfast_tqu(cls, nside, lmax=lmax, fwhm=fwhm, use_pixel_window=use_pixel_window, seed=seed,
+                       sigmaT=sigmaT, sigmaQ=sigmaQ, sigmaU=sigmaU, return_alm=return_alm, **kw)
+
+
 # ---------- FITS helpers (``hp_compat.py:204-225``) ----------
 def read_map(path, hdu=1, to_gpu=True, dtype=np.float64):
     from .io_fits import read_map_fits

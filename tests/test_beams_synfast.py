@@ -96,3 +96,27 @@ def test_synfast_matches_oracle_composition(gpu):
     assert torch.equal(T, T2)
     Tn, Qn, _ = synfast_tqu({"tt": ctt}, nside, lmax=lmax, seed=3, sigmaT=0.1, sigmaQ=0.2)
     assert float(Qn.std()) > 0.15 and float(Tn.std()) > 0.05
+
+
+@pytest.mark.gpu
+def test_hp_compat_consumer_wrappers(gpu):
+    """hp_compat.almxfl / synalm / smooth_map / synfast (hp_compat.py:113-120,
+    228-236) are the package functions."""
+    print("[case] hp_compat consumer wrappers nside=16", flush=True)
+    from hellblast_amd import hp_compat
+    from hellblast_amd.alm_ops import almxfl, synalm
+    from hellblast_amd.beams import smooth_map
+    from hellblast_amd.synfast_tqu import synfast_tqu
+    nside, lmax = 16, 40
+    cl = 1.0 / (np.arange(lmax + 1) + 1.0) ** 2
+    a = hp_compat.synalm(cl, seed=4)
+    assert torch.equal(torch.as_tensor(a), torch.as_tensor(synalm(cl, seed=4)))
+    fl = np.linspace(1.0, 0.5, lmax + 1)
+    assert torch.equal(torch.as_tensor(hp_compat.almxfl(a, fl)), torch.as_tensor(almxfl(a, fl)))
+    f = torch.as_tensor(np.random.default_rng(5).standard_normal(12 * nside * nside), device="cuda")
+    fw = math.radians(6.0)
+    assert torch.equal(hp_compat.smooth_map(f, nside, lmax=lmax, fwhm=fw), smooth_map(f, nside, lmax=lmax, fwhm=fw))
+    cls = {"tt": cl, "ee": 0.3 * cl, "bb": 0.05 * cl, "te": 0.4 * cl}
+    got = hp_compat.synfast(cls, nside, lmax=lmax, seed=9)
+    want = synfast_tqu(cls, nside, lmax=lmax, seed=9)
+    assert all(torch.equal(g, w) for g, w in zip(got, want))

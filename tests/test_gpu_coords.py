@@ -42,3 +42,23 @@ def test_ring_coords_match_oracle(gpu, nside, phi0):
     ang = np.arctan2(y, x)
     ang = np.where(ang < 0, ang + 2 * math.pi, ang)
     assert np.array_equal(v, OC.ang2pix(np.arccos(np.clip(z, -1, 1)), ang, t, p0))
+
+
+def test_hp_compat_geometry_wrappers(gpu):
+    """hp_compat.ang2pix / pix2ang / pix2vec (hp_compat.py:131-149) on the
+    RING kernels: the reference's phi centres, unit vectors, and NESTED
+    refused."""
+    print("[case] hp_compat ang2pix/pix2ang/pix2vec nside=32", flush=True)
+    from hellblast_amd import hp_compat
+    nside = 32
+    t = OG.ring_tables(nside)
+    pix = np.arange(t["npix"])
+    th, ph = hp_compat.pix2ang(nside, torch.as_tensor(pix))
+    et, ep = OC.pix2ang(pix, t, t["phi_center0"])
+    assert np.array_equal(th.cpu().numpy(), et) and np.array_equal(ph.cpu().numpy(), ep)
+    assert torch.equal(hp_compat.ang2pix(nside, th, ph).cpu(), torch.as_tensor(pix))
+    x, y, z = hp_compat.pix2vec(nside, torch.as_tensor(pix))
+    assert float((x * x + y * y + z * z - 1).abs().max()) <= 1e-15
+    assert np.allclose(z.cpu().numpy(), np.cos(et), rtol=0, atol=1e-15)
+    with pytest.raises(NotImplementedError):
+        hp_compat.ang2pix(nside, th, ph, nest=True)
