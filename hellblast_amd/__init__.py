@@ -11,7 +11,7 @@ on PyTorch-ROCm tensors.
 Package aliases as in the reference (``__init__.py:125-127``):
 ``map2alm``, ``alm2map``, ``anafast``.
 """
-from .ring_tables import HealpixRingTables
+from .ring_tables import HealpixRingTables, pack_rings_to_flat, unpack_flat_to_rings
 from .sht import map2alm_gpu
 from .isht import alm2map_gpu
 from .anafast import anafast_ring
@@ -37,4 +37,5 @@ __all__ = [
     "map2alm_spin1_gpu", "alm2map_spin1_gpu", "gaussian_beam_fl", "smooth_alm_gaussian", "pixel_window_fl",
     "smooth_alm_with_beam", "smooth_map", "synfast_tqu", "RingMapperGPU", "vec2pix_ring_gpu",
     "read_map_fits", "write_map_fits", "read_alm_fits", "write_alm_fits", "read_tqu_fits", "write_tqu_fits",
+    "pack_rings_to_flat", "unpack_flat_to_rings",
 ]

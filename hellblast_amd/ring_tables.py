@@ -93,3 +93,27 @@ class HealpixRingTables:
             "nphi": h["nphi"].copy(), "phi_center0": h["phi_center0"].copy(),
             "ring_start": h["ring_start"].copy(), "z_edges": h["z_edges"].copy(),
         }
+
+
+def unpack_flat_to_rings(flat, tables):
+    """Per-ring views of a flat RING map (``udgrade.py:96-102``): ring r is
+    ``flat[..., ring_start[r] : ring_start[r] + nphi[r]]``.  One split on the
+    host-side ring lengths; no copies, no per-ring device reads."""
+    t = torch.as_tensor(flat)
+    if t.shape[-1] != tables.npix:
+        raise ValueError(f"map has {t.shape[-1]} pixels but the ring tables describe {tables.npix}")
+    return list(torch.split(t, [int(n) for n in tables._host["nphi"]], dim=-1))
+
+
+def pack_rings_to_flat(rings, tables):
+    """Flat RING map from per-ring arrays (``udgrade.py:86-94``), one
+    concatenation; ring r must hold nphi[r] pixels."""
+    nphi = tables._host["nphi"]
+    if len(rings) != tables.Nr:
+        raise ValueError(f"{len(rings)} rings given for {tables.Nr} ring tables")
+    parts = [torch.as_tensor(r) for r in rings]
+    for r, (p, n) in enumerate(zip(parts, nphi)):
+        if p.shape[-1] != int(n):
+            raise ValueError(f"ring {r} has {p.shape[-1]} pixels, expected {int(n)}")
+    dev = parts[0].device
+    return torch.cat([p.to(dev) for p in parts], dim=-1)
