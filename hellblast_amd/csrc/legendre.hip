@@ -500,13 +500,16 @@ __global__ void alm2cl_kernel(const c2<T> *__restrict__ alm, int64_t astride, do
 // Ring-pair slots per lane: FP64 holds 8 (E/O in registers cost 8 VGPRs
 // each), FP32 holds 16; fewer when the geometry has fewer pairs.
 // ring-pair slots (64 pairs each) per lane of the VALU analysis: the plan's
-// tile if set (clamped to what the type allows), else enough to cover every
-// pair with one wave up to 8 (FP64) / 16 (FP32)
+// tile if set (clamped to 8 for FP64, 16 for FP32), else enough to cover
+// every pair with one wave up to 8 (FP64) / 4 (FP32).  Single-map map2alm,
+// nside 2048 / 1024: FP64 R = 8 7.5 / 1.28 ms (R = 4: 9.5 / 1.59); FP32
+// R = 4 6.6 / 1.39 ms (R = 8: 6.8 / 1.49, R = 16: 10.5 / 1.98)
 template <typename T> static int analysis_slots(const Plan *p) {
     const int rmax = sizeof(T) == 8 ? 8 : 16;
+    const int rdef = sizeof(T) == 8 ? 8 : 4;
     if (p->tile_valu_slots > 0) return std::min(p->tile_valu_slots, rmax);
     int r = 1;
-    while (r < rmax && (int64_t)r * 64 < p->npairs) r *= 2;
+    while (r < rdef && (int64_t)r * 64 < p->npairs) r *= 2;
     return r;
 }
 
