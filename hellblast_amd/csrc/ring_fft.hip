@@ -758,9 +758,91 @@ ring_band_kernel(const TIn *__restrict__ maps, int64_t map_stride, double2 *__re
     }
 }
 
+// The band for batches below the matrix-core threshold (single maps): one
+// workgroup per (ring, map) on the VALU, each thread summing its pixels
+// j = tid + 256 t against e^{-2 pi i m j / n}, m < M_LO (one sincospi per
+// pixel, the higher m by complex multiplication), then a fixed-order
+// reduction over the wave (xor shuffles) and the 4 waves (LDS).  Same
+// conventions and output as ring_band_kernel.
+template <typename TIn>
+__global__ void __launch_bounds__(256)
+ring_band_valu_kernel(const TIn *__restrict__ maps, int64_t map_stride, double2 *__restrict__ band,
+                      int64_t band_stride, const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
+                      const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
+                      const double *__restrict__ scale, int mtop, int nr) {
+    __shared__ double part[4][2 * M_LO];
+    const int r = blockIdx.x, b = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int n = nphi[r];
+    const TIn *f = maps + (int64_t)b * map_stride + ring_start[r];
+    double acc[2 * M_LO];
+#pragma unroll
+    for (int i = 0; i < 2 * M_LO; ++i) acc[i] = 0.0;
+    for (int j = tid; j < n; j += 256) {
+        const double v = (double)f[j];
+        double s1, c1;                              // e^{-2 pi i j / n}
+        sincospi(-2.0 * (double)j / (double)n, &s1, &c1);
+        double cr = 1.0, ci = 0.0;
+#pragma unroll
+        for (int m = 0; m < M_LO; ++m) {
+            acc[2 * m] = fma(v, cr, acc[2 * m]);
+            acc[2 * m + 1] = fma(v, ci, acc[2 * m + 1]);
+            const double t = cr * c1 - ci * s1;
+            ci = cr * s1 + ci * c1;
+            cr = t;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2 * M_LO; ++i) {
+        double v = acc[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        acc[i] = v;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 2 * M_LO; ++i) part[w][i] = acc[i];
+    }
+    __syncthreads();
+    if (tid <= mtop) {
+        const int mm = tid;
+        double x = ((part[0][2 * mm] + part[1][2 * mm]) + part[2][2 * mm]) + part[3][2 * mm];
+        double y = ((part[0][2 * mm + 1] + part[1][2 * mm + 1]) + part[2][2 * mm + 1]) + part[3][2 * mm + 1];
+        double2 F = make_double2(0.0, 0.0);
+        if (mm < mcut[r]) {
+            const double sc = scale ? scale[r] : 1.0;
+            x *= sc;
+            y *= sc;
+            const double ph = phi0 ? phi0[r] : 0.0;
+            if (ph != 0.0) {
+                double sn, cs;
+                sincos(-(double)mm * ph, &sn, &cs);
+                F = make_double2(x * cs - y * sn, x * sn + y * cs);
+            } else {
+                F = make_double2(x, y);
+            }
+        }
+        band[(int64_t)b * band_stride + ((int64_t)(mm >> 3) * nr + r) * MB + (mm & 7)] = F;
+    }
+}
+
 int launch_ring_band(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride, void *band,
                      const double *scale, hipStream_t s) {
     const int mtop = std::min(M_LO - 1, p->mmax);
+    if (nmaps < mfma_min_maps(p)) {
+        dim3 grid(p->nrings, nmaps);
+        if (map_dtype == HBX_DT_F64)
+            hipLaunchKernelGGL(ring_band_valu_kernel<double>, grid, dim3(256), 0, s, (const double *)maps,
+                               map_stride, (double2 *)band, band_phase_elems(p), p->d_ring_start, p->d_nphi,
+                               p->d_mcut, p->d_phi0, scale, mtop, p->nrings);
+        else
+            hipLaunchKernelGGL(ring_band_valu_kernel<float>, grid, dim3(256), 0, s, (const float *)maps,
+                               map_stride, (double2 *)band, band_phase_elems(p), p->d_ring_start, p->d_nphi,
+                               p->d_mcut, p->d_phi0, scale, mtop, p->nrings);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_error(e, "ring_band_valu_kernel launch");
+        return 0;
+    }
     dim3 grid(p->nrings, (nmaps + 15) / 16);
     if (map_dtype == HBX_DT_F64)
         hipLaunchKernelGGL(ring_band_kernel<double>, grid, dim3(RB_WAVES * 64), 0, s, (const double *)maps,
