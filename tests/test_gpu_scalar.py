@@ -310,3 +310,29 @@ def test_map2alm_iter_matches_oracle_jacobi(gpu):
         errs.append(_rel(got, alm))
     print("round trip error by iter:", errs, flush=True)
     assert errs[1] < 0.2 * errs[0] and errs[2] < 1e-3 * errs[0]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_empty_batches(gpu, precision):
+    """Batches of zero maps / pairs / pixels run nothing and return empty
+    outputs of the right shape on every entry point."""
+    print(f"[case] empty batches {precision}", flush=True)
+    from hellblast_amd.coords_ring import RingMapperGPU
+    from hellblast_amd.hp_compat import _plan
+    nside, lmax = 8, 16
+    rdt = torch.float64 if precision == "f64" else torch.float32
+    p = _plan(nside, lmax, lmax, precision, "dz", "reference", "reference")
+    ps = _plan(nside, lmax, lmax, precision, "dz", "reference", "reference", 2)
+    npix = 12 * nside * nside
+    a = p.map2alm(torch.zeros(0, npix, device="cuda", dtype=rdt))
+    assert a.shape == (0, p.nalm) and a.dtype == p.alm_dtype
+    m = p.alm2map(torch.zeros(0, p.nalm, device="cuda", dtype=p.alm_dtype), dtype=rdt)
+    assert m.shape == (0, npix)
+    assert p.alm2map_adjoint(torch.zeros(0, npix, device="cuda", dtype=rdt)).shape == (0, p.nalm)
+    assert ps.map2alm_spin(torch.zeros(0, 2, npix, device="cuda", dtype=rdt)).shape == (0, 2, ps.nalm)
+    assert ps.alm2map_spin(torch.zeros(0, 2, ps.nalm, device="cuda", dtype=ps.alm_dtype),
+                           dtype=rdt).shape == (0, 2, npix)
+    mp = RingMapperGPU(nside)
+    assert mp.ang2pix(torch.zeros(0, dtype=torch.float64), torch.zeros(0, dtype=torch.float64)).numel() == 0
+    th, ph = mp.pix2ang(torch.zeros(0, dtype=torch.int64))
+    assert th.numel() == 0 and ph.numel() == 0
