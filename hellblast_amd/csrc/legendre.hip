@@ -520,9 +520,12 @@ int analysis_groups(const Plan *p, bool f64) {
     return (int)((p->npairs + 64 * r - 1) / (64 * r));
 }
 
-// FP64 band of FP32 plans: the FP64 slot count of the plan (the per-chunk
-// wave reduction is amortised over the slots, so fewer slots cost more)
-static int band_slots(const Plan *p) { return analysis_slots<double>(p); }
+// FP64 band of FP32 plans on the VALU (single maps): up to 4 slots of 64
+// ring pairs per lane.  The band has only 8 rows of m, so fewer slots than
+// the full FP64 analysis give it more waves; single-map FP32 map2alm with
+// 1 / 2 / 4 / 8 slots: 7.27 / 6.29 / 5.86 / 6.02 ms at nside 2048, 1.41 /
+// 1.25 / 1.24 / 1.33 ms at nside 1024, 0.47 / 0.45 / 0.46 / 0.50 ms at 512
+static int band_slots(const Plan *p) { return std::min(4, analysis_slots<double>(p)); }
 int band_groups(const Plan *p) { return (p->npairs + 64 * band_slots(p) - 1) / (64 * band_slots(p)); }
 
 template <typename T>
