@@ -168,9 +168,17 @@ int launch_band_analysis_mfma(const Plan *p, int nmaps, const void *band, void *
                               int adjoint, hipStream_t s);
 int mfma_analysis_waves(const Plan *p);
 int prepare_mfma_kernels();
-// by default, batches at least this large use the matrix-core Legendre kernels (FP32 plans)
-constexpr int MFMA_MIN_MAPS = 2;
-inline int mfma_min_maps(const Plan *p) { return p->tile_mfma_min_maps > 0 ? p->tile_mfma_min_maps : MFMA_MIN_MAPS; }
+// by default, batches at least this large use the matrix-core Legendre
+// kernels (FP32 plans): scalar 3 maps, spin 6 maps (3 Q/U pairs).  Map2alm +
+// alm2map at nside 2048, matrix / VALU: 2 maps 19.4 / 19.1 ms, 3 maps 20.2 /
+// 28.1 ms; spin 1 pair 53.0 / 27.5 ms, 2 pairs 56.1 / 53.3 ms, 4 pairs 65 /
+// 106 ms (a matrix tile serves 16 maps or 4 pairs at nearly the same cost)
+constexpr int MFMA_MIN_MAPS = 3;
+constexpr int SPIN_MFMA_MIN_MAPS = 6;
+inline int mfma_min_maps(const Plan *p) {
+    if (p->tile_mfma_min_maps > 0) return p->tile_mfma_min_maps;
+    return p->spin != 0 ? SPIN_MFMA_MIN_MAPS : MFMA_MIN_MAPS;
+}
 int analysis_slots_of(const Plan *p, bool f64);   // effective VALU analysis slots
 int launch_alm2cl(int lmax, int mmax, int nmaps, const void *alm, int dt,
                   int64_t alm_stride, double *cl, int64_t cl_stride, hipStream_t s);

@@ -121,8 +121,8 @@ int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
  *                  8 MFMA waves fed by 4 lambda-recursion waves one chunk
  *                  ahead (double-buffered LDS tile)
  *   mfma_min_maps  FP32 batches of at least this many maps use the
- *                  matrix-core Legendre kernels (default 2; spin analysis
- *                  counts the Q and U maps)
+ *                  matrix-core Legendre kernels (default 3; spin plans count
+ *                  the Q and U maps, default 6 = 3 Q/U pairs)
  *   spin_slots     VALU spin Legendre analysis (FP64 plans, and FP32 batches
  *                  below mfma_min_maps): ring-pair slots per lane, 1, 2, 4
  *   mfma_kind      matrix-core operand format of FP32 batches:

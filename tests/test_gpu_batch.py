@@ -35,7 +35,7 @@ def test_tiles_set_get_and_validation(gpu):
     from hellblast_amd._abi import HbxValueError
     plan = _plan(32, 64, "f32")
     base = plan.tiles()
-    assert base["mfma_waves"] == 4 and base["mfma_min_maps"] == 2
+    assert base["mfma_waves"] == 4 and base["mfma_min_maps"] == 3
     assert base["mfma_kind"] == _abi.HBX_MFMA_F16X2
     plan.set_tiles(mfma_waves=4, valu_slots=2)
     t = plan.tiles()

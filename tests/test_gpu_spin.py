@@ -161,6 +161,14 @@ def test_spin_f32_matrix_core_analysis(gpu):
     tabs = OG.ring_tables(nside)
     qu = _qu(tabs, 31, n=5)
     plan = _plan(nside, lmax, spin=s, precision="f32")
+    plan.set_tiles(mfma_min_maps=2)          # every batch, one pair included, on the matrix cores
+    try:
+        _matrix_core_analysis_checks(plan, qu, tabs, lmax, s)
+    finally:
+        plan.set_tiles(mfma_min_maps=0)
+
+
+def _matrix_core_analysis_checks(plan, qu, tabs, lmax, s):
     x = torch.as_tensor(qu, device="cuda", dtype=torch.float32)
     got = plan.map2alm_spin(x).cpu().numpy()
     q32 = qu.astype(np.float32).astype(np.float64)
@@ -176,7 +184,7 @@ def test_spin_f32_matrix_core_analysis(gpu):
         valu = plan.map2alm_spin(x).cpu().numpy()
         vadj = plan.alm2map_spin_adjoint(x).cpu().numpy()
     finally:
-        plan.set_tiles(mfma_min_maps=0)
+        plan.set_tiles(mfma_min_maps=2)
     print(f"matrix vs VALU: {_rel(got, valu):.2e}, adjoint {_rel(adj, vadj):.2e}", flush=True)
     assert _rel(got, valu) <= 4e-6 and _rel(adj, vadj) <= 4e-6
 
@@ -194,6 +202,7 @@ def test_spin_f32_matrix_core_synthesis(gpu):
     for s in (1, 2, 3):
         print(f"[case] spin f32 matrix-core synthesis nside=128 lmax=200 s={s} 5 pairs", flush=True)
         plan = _plan(nside, lmax, spin=s, precision="f32")
+        plan.set_tiles(mfma_min_maps=2)      # every batch, one pair included, on the matrix cores
         eb = np.stack([np.stack(OSp.synalm_eb(lmax, spin=s, seed=40 + b)) for b in range(5)])
         a = torch.as_tensor(eb, device="cuda").to(torch.complex64)
         got = plan.alm2map_spin(a, dtype=torch.float32).double().cpu().numpy()
@@ -223,6 +232,14 @@ def test_spin_f32_small_sizes(gpu, nside, lmax, s):
     print(f"[case] spin f32 small nside={nside} lmax={lmax} s={s}", flush=True)
     tabs = OG.ring_tables(nside)
     plan = _plan(nside, lmax, spin=s, precision="f32")
+    plan.set_tiles(mfma_min_maps=2)          # the matrix kernels, which 2 pairs would not get by default
+    try:
+        _small_size_checks(plan, tabs, nside, lmax, s)
+    finally:
+        plan.set_tiles(mfma_min_maps=0)
+
+
+def _small_size_checks(plan, tabs, nside, lmax, s):
     qu = _qu(tabs, 70 + nside, n=2)
     got = plan.map2alm_spin(torch.as_tensor(qu, device="cuda", dtype=torch.float32)).cpu().numpy()
     q32 = qu.astype(np.float32).astype(np.float64)
@@ -364,3 +381,11 @@ def test_spin_f32_gate_full_size(gpu):
             e = float(((got[b, k].to(torch.complex128) - r).abs()[ok] / cl[ls][ok].sqrt()).max())
             print(f"{name} pair {b}: eps_max {e:.2e}", flush=True)
             assert e <= 1e-5
+
+
+def test_default_matrix_thresholds(gpu):
+    """Effective defaults: scalar FP32 batches of 3 or more maps and spin FP32
+    batches of 3 or more Q/U pairs (6 maps) use the matrix-core kernels."""
+    print("[case] default matrix thresholds", flush=True)
+    assert _plan(8, 16, precision="f32", spin=0).tiles()["mfma_min_maps"] == 3
+    assert _plan(8, 16, precision="f32", spin=2).tiles()["mfma_min_maps"] == 6
