@@ -7,8 +7,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# HBX_LIBRARY selects another build of the library (e.g. a debug build)
-LIB_PATH = os.environ.get("HBX_LIBRARY") or os.path.join(_HERE, "libhbx.so")
+LIB_PATH = os.path.join(_HERE, "libhbx.so")
 
 HBX_OK = 0
 HBX_EINVAL = -1
@@ -92,7 +91,7 @@ SIGNATURES = {
     "hbx_plan_destroy": (ctypes.c_int, [_vp]),
     "hbx_plan_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(hbx_plan_info)]),
     "hbx_workspace_bytes": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_u64)]),
-    "hbx_plan_set_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(hbx_tiles)]),
+    "hbx_plan_derive": (ctypes.c_int, [_vp, ctypes.POINTER(hbx_tiles), ctypes.POINTER(_vp)]),
     "hbx_plan_get_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(hbx_tiles)]),
     "hbx_map2alm": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _i64, _vp, _u64, _vp]),
     "hbx_alm2map": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i32, _i64, _vp, _u64, _vp]),

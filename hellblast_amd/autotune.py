@@ -1,8 +1,9 @@
 """Tile autotuner for libhbx plans (no reference counterpart).
 
 For one plan and batch size it times map2alm + alm2map (spin plans: the
-spin pair) under each candidate tile set (``hbx_plan_set_tiles``) and keeps
-the fastest.  Winners are cached in a JSON file keyed by device name,
+spin pair) under each candidate tile set (a plan derived with
+``hbx_plan_derive``, sharing the plan's device tables) and keeps the
+fastest.  Winners are cached in a JSON file keyed by device name,
 geometry, lmax, mmax, precision, spin and batch size, so a tuned
 configuration is reused across processes:
 ``$HBX_TUNE_CACHE`` or ``~/.cache/hellblast_amd/tiles.json``.
@@ -102,27 +103,24 @@ def _time(plan, nmaps, reps):
 
 
 def tune(plan, nmaps, reps=3, force=False, verbose=False):
-    """Apply the best tiles for (plan, nmaps), timing candidates unless cached."""
+    """The best tiles for (plan, nmaps), timing candidates unless cached.
+
+    Returns the cache entry {"tiles", "ms", "timed"}.  ``plan`` itself is not
+    changed (plans are immutable); run with ``plan.with_tiles(**entry["tiles"])``.
+    """
     db = _load()
     k = key(plan, nmaps)
     if not force and k in db:
-        plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
-        plan.set_tiles(**db[k]["tiles"])
         return db[k]
     results = []
     for cand in candidates(plan, nmaps):
-        plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
-        plan.set_tiles(**cand)
-        ms = _time(plan, nmaps, reps)
+        ms = _time(plan.with_tiles(**{f: 0 for f in plan.TILE_FIELDS}).with_tiles(**cand), nmaps, reps)
         results.append((ms, cand))
         if verbose:
             print(f"[autotune] {k} {cand} {ms:.3f} ms", flush=True)
     ms, best = min(results, key=lambda r: r[0])
-    plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
-    plan.set_tiles(**best)
     entry = {"tiles": best, "ms": ms, "timed": [[m, c] for m, c in results]}
     db = _load()
     db[k] = entry
     _save(db)
     return entry
-

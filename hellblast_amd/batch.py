@@ -38,7 +38,6 @@ class MixedBatch:
             for key, idx in self.groups.items():
                 nside, lmax = key
                 plan = _plan(nside, lmax, lmax, precision, weights, phi0, alias)
-                self.plans[key] = plan
                 rdt = torch.float64 if plan.alm_dtype == torch.complex128 else torch.float32
                 n = len(idx)
                 self.maps[key] = torch.zeros((n, plan.npix), dtype=rdt, device=self.device)
@@ -49,6 +48,8 @@ class MixedBatch:
                 if tune:
                     from .autotune import tune as _tune
                     self.tiles[key] = _tune(plan, n)["tiles"]
+                    plan = plan.with_tiles(**self.tiles[key])
+                self.plans[key] = plan
         self.graph = None
         self.streams = [torch.cuda.Stream(device=self.device) for _ in self.groups]
 

@@ -196,14 +196,17 @@ static int upload(Plan *p, X **dst, const X *src, size_t n) {
     return 0;
 }
 
-static void destroy_plan(Plan *p) {
+DeviceAllocs::~DeviceAllocs() {
+    if (ptrs.empty()) return;
     int cur = 0;
     const bool have = hipGetDevice(&cur) == hipSuccess;
-    if (have && cur != p->device) (void)hipSetDevice(p->device);
-    for (void *ptr : p->allocations) (void)hipFree(ptr);
-    p->allocations.clear();
-    if (have && cur != p->device) (void)hipSetDevice(cur);
+    if (have && cur != device) (void)hipSetDevice(device);
+    for (void *ptr : ptrs) (void)hipFree(ptr);
+    if (have && cur != device) (void)hipSetDevice(cur);
 }
+
+// drops this plan's reference to the shared device tables
+static void destroy_plan(Plan *p) { p->allocations.s.reset(); }
 
 static int build_plan(Plan *p, const hbx_geom *g) {
     const int nr = p->nrings;
@@ -345,6 +348,7 @@ int hbx_plan_create(const hbx_geom *g, int32_t lmax, int32_t mmax, int32_t spin,
         delete hp;
         return set_error(HBX_EDEVICE, "hipGetDevice failed");
     }
+    p->allocations.s->device = p->device;
     p->nrings = g->nrings;
     p->npairs = (g->nrings + 1) / 2;
     p->lmax = lmax;
@@ -389,8 +393,10 @@ int hbx_plan_get_info(const hbx_plan *plan, hbx_plan_info *info) {
     return HBX_OK;
 }
 
-int hbx_plan_set_tiles(hbx_plan *plan, const hbx_tiles *t) {
-    if (!plan || !t) return set_error(HBX_EINVAL, "NULL argument");
+int hbx_plan_derive(const hbx_plan *base, const hbx_tiles *t, hbx_plan **out) {
+    if (!out) return set_error(HBX_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!base || !t) return set_error(HBX_EINVAL, "NULL argument");
     auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
     if (t->valu_slots != 0 && (!pow2(t->valu_slots) || t->valu_slots > 16))
         return set_error(HBX_EINVAL, "valu_slots must be 0 or a power of two <= 16");
@@ -407,13 +413,16 @@ int hbx_plan_set_tiles(hbx_plan *plan, const hbx_tiles *t) {
     if (t->mfma_groups < 0 || t->mfma_groups > 2) return set_error(HBX_EINVAL, "mfma_groups must be 0, 1 or 2");
     if (t->mfma_waves == 16 && t->mfma_kind == HBX_MFMA_F32)
         return set_error(HBX_EINVAL, "mfma_waves 16 needs the split-fp16 matrix-core kernels");
-    Plan &p = plan->impl;
+    auto *hp = new hbx_plan();
+    hp->impl = base->impl;            // shares the device tables (reference counted)
+    Plan &p = hp->impl;
     p.tile_valu_slots = t->valu_slots;
     p.tile_mfma_waves = t->mfma_waves;
     p.tile_mfma_min_maps = t->mfma_min_maps;
     p.tile_spin_slots = t->spin_slots;
     p.tile_mfma_kind = t->mfma_kind;
     p.tile_syn_groups = t->mfma_groups;
+    *out = hp;
     return HBX_OK;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -58,6 +59,22 @@ struct FftClass {
     int64_t tw_off64 = 0, tw_off32 = 0;   // offsets of this class's FP64 / FP32 twiddle tables
 };
 
+// Device allocations of a plan.  Plans derived from it with hbx_plan_derive
+// share them (read-only tables); the last plan destroyed frees them on their
+// device.
+struct DeviceAllocs {
+    int device = 0;
+    std::vector<void *> ptrs;
+    DeviceAllocs() = default;
+    DeviceAllocs(const DeviceAllocs &) = delete;
+    DeviceAllocs &operator=(const DeviceAllocs &) = delete;
+    ~DeviceAllocs();
+};
+struct AllocRef {
+    std::shared_ptr<DeviceAllocs> s = std::make_shared<DeviceAllocs>();
+    void push_back(void *ptr) { s->ptrs.push_back(ptr); }
+};
+
 struct Plan {
     int device = 0;
     int nrings = 0, npairs = 0, lmax = 0, mmax = 0, nmb = 0, spin = 0, precision = 0;
@@ -96,8 +113,8 @@ struct Plan {
     double2 *d_V64 = nullptr;  float2 *d_V32 = nullptr;
     int64_t vtotal = 0;
     std::vector<FftClass> classes;
-    std::vector<void *> allocations;
-    // tiles (hbx_plan_set_tiles); 0 = size-based default
+    AllocRef allocations;
+    // tiles (fixed when the plan is created or derived); 0 = size-based default
     int tile_valu_slots = 0;      // VALU analysis ring-pair slots per lane
     int tile_mfma_waves = 0;      // waves per matrix-core analysis workgroup (4, 8, 12)
     int tile_mfma_min_maps = 0;   // FP32 batches of at least this many maps use the matrix cores

@@ -243,8 +243,8 @@ __global__ void __launch_bounds__(256)
 syn_prep_kernel(const float2 *__restrict__ alm, int64_t astride, const int64_t *__restrict__ rofs,
                 const double *__restrict__ sl, const int64_t *__restrict__ cofs, int64_t nchunks, int lmax,
                 int mmax, int nmaps, int adjoint, float neg1, _Float16 *__restrict__ prep, int *__restrict__ kexp) {
-    __shared__ float red[4][16];
-    __shared__ float scs[16];
+    __shared__ double red[4][16];
+    __shared__ double scs[16];
     __shared__ __align__(16) _Float16 cbuf[SH_CHUNK_HALVES];
     const int m = blockIdx.x, g = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
     const int64_t i0 = alm_index(m, m, lmax) - m;
@@ -252,34 +252,37 @@ syn_prep_kernel(const float2 *__restrict__ alm, int64_t astride, const int64_t *
     const double fadj = (adjoint && m > 0) ? 0.5 : 1.0;
     const int nb = min(16, nmaps - g * 16);          // maps of this group
     const float2 *arow = alm + (int64_t)(g * 16) * astride + i0;
-    // 1. row maxima of the 16 maps (coalesced along l)
+    // 1. row maxima of the 16 maps (coalesced along l), in FP64 so that every
+    // finite FP32 row keeps its full precision
     {
-        float mx[16];
+        double mx[16];
 #pragma unroll
-        for (int bi = 0; bi < 16; ++bi) mx[bi] = 0.0f;
+        for (int bi = 0; bi < 16; ++bi) mx[bi] = 0.0;
         for (int l = m + tid; l <= lmax; l += blockDim.x) {
-            const float f = (float)(sv[l] * fadj);
+            const double f = fabs(sv[l] * fadj);
 #pragma unroll
             for (int bi = 0; bi < 16; ++bi)
                 if (bi < nb) {
                     const float2 a = arow[(int64_t)bi * astride + l];
-                    mx[bi] = fmaxf(mx[bi], fmaxf(fabsf(a.x), fabsf(a.y)) * f);
+                    mx[bi] = fmax(mx[bi], (double)fmaxf(fabsf(a.x), fabsf(a.y)) * f);
                 }
         }
 #pragma unroll
         for (int bi = 0; bi < 16; ++bi) {
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) mx[bi] = fmaxf(mx[bi], __shfl_xor(mx[bi], o, 64));
+            for (int o = 32; o > 0; o >>= 1) mx[bi] = fmax(mx[bi], __shfl_xor(mx[bi], o, 64));
             if (lane == 0) red[tid >> 6][bi] = mx[bi];
         }
     }
     __syncthreads();
     if (tid < 16) {
-        const float mx = fmaxf(fmaxf(red[0][tid], red[1][tid]), fmaxf(red[2][tid], red[3][tid]));
+        // row maximum scaled to [2^13, 2^14); the scale rides on the FP64
+        // factor s_lm, so no FP32 intermediate can leave the FP32 range
+        const double mx = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
         int e = 0;
-        (void)frexpf(mx, &e);
-        const int k = mx > 0.0f ? min(max(14 - e, -100), 100) : 0;
-        scs[tid] = tid < nb ? ldexpf(1.0f, k) : 0.0f;
+        (void)frexp(mx, &e);
+        const int k = mx > 0.0 ? 14 - e : 0;
+        scs[tid] = tid < nb ? ldexp(fadj, k) : 0.0;
         if (tid < nb) kexp[(int64_t)(g * 16 + tid) * (mmax + 1) + m] = k;
     }
     __syncthreads();
@@ -295,9 +298,9 @@ syn_prep_kernel(const float2 *__restrict__ alm, int64_t astride, const int64_t *
             float2v v = {0.0f, 0.0f};
             if (l <= lmax && bi < nb) {
                 const float2 a = arow[(int64_t)bi * astride + l];
-                const double f = sv[l] * fadj;
-                v[0] = (float)(a.x * f) * scs[bi];
-                v[1] = (float)(a.y * f) * scs[bi];
+                const double f = sv[l] * scs[bi];
+                v[0] = (float)(a.x * f);
+                v[1] = (float)(a.y * f);
             }
             const half2v h = __builtin_convertvector(v, half2v);
             const half2v lo = split_lo(v, h, neg1);
@@ -411,11 +414,16 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
             *reinterpret_cast<uint4 *>(&bst[hl][par][32 * u + n][8 * kb]) = pf[i];
         }
     };
-    float usc[NG];
+    // 2^-k of this lane's output column as two FP32 factors, exact while
+    // |k| <= 250 (k >= -114 for finite rows; outputs of rows with k > 250
+    // lie below the FP32 range whatever the scaling)
+    float usc1[NG], usc2[NG];
 #pragma unroll
     for (int u = 0; u < NG; ++u) {
         const int bo = g0 + 16 * u + (r >> 1);
-        usc[u] = bo < nmaps ? ldexpf(1.0f, -kexp[(int64_t)bo * (mmax + 1) + m]) : 0.0f;
+        const int k = bo < nmaps ? kexp[(int64_t)bo * (mmax + 1) + m] : 0;
+        usc1[u] = ldexpf(1.0f, -(k >> 1));
+        usc2[u] = ldexpf(1.0f, (k >> 1) - k);
     }
 
     floatx16 De[2][NG], Do[2][NG];                  // [ring tile][n tile]
@@ -525,10 +533,10 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
                 const int row = (q & 3) + 8 * (q >> 2) + 4 * h;
                 const int pp = base + row;
                 if (pp >= npairs) continue;
-                const float ge = De[j][u][q] * usc[u], go = Do[j][u][q] * usc[u];
+                const float ge = De[j][u][q], go = Do[j][u][q];
                 const int rn = pp, rs = nr - 1 - pp;
-                out[(int64_t)rn * (2 * MB)] = ge + go;
-                if (rs != rn) out[(int64_t)rs * (2 * MB)] = ge - go;
+                out[(int64_t)rn * (2 * MB)] = ((ge + go) * usc1[u]) * usc2[u];
+                if (rs != rn) out[(int64_t)rs * (2 * MB)] = ((ge - go) * usc1[u]) * usc2[u];
             }
         }
     }
@@ -777,11 +785,12 @@ leg_analysis_mfma_kernel(const float2 *__restrict__ phase, int64_t pstride, floa
 // v = hi + lo (hi = fp16(v), lo = fp16(v - hi): 22 significant bits), and
 //   S lambda = S_hi lambda_hi + S_hi lambda_lo + S_lo lambda_hi
 // (the dropped S_lo lambda_lo is ~2^-22 relative); products of fp16 values
-// are exact in the FP32 accumulators.  Range: each wave scales its A rows
-// (one n = map re/im over its 256 ring pairs) by 2^k so the row maximum sits
-// in [2^13, 2^14), and unscales its output rows by 2^-k (exact); the
-// rescaled recursion values stay far below the fp16 maximum (|mu| < 100 up
-// to lmax 8191).
+// are exact in the FP32 accumulators.  Range: the A rows of one map (re and
+// im, over the workgroup's ring pairs of one parity) are scaled by 2^k so
+// their maximum sits in [2^13, 2^14); k is taken without a clamp, so every
+// finite FP32 input keeps its 22-bit operands, and the outputs are unscaled
+// by 2^-k in FP64 at the store (exact); the rescaled recursion values stay
+// far below the fp16 maximum (|mu| < 100 up to lmax 8191).
 //   A[row = n][k = pair]   = S_par(pair)[n]   (registers, hi/lo, whole kernel)
 //   B[k = pair][col]       = mu_{c0+2col+par}(pair)   (LDS tile, hi/lo)
 // Lane layout of the 16x16x32 f16 instruction (checked on gfx950):
@@ -857,7 +866,7 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
 
     // A operands: S_par[pair P0 + 256 kh + 32 s + 8 q + i][n = 16 t + row], q = lane/16, row = lane%16
     half8 Ahi[8], Alo[8];
-    float usc[4];                                      // 2^-k of the output rows 4 q + j
+    int kr[4];                                         // exponent k of the output rows 4 q + j
     {
         const int row = lane & 15, q = lane >> 4;
         const int n = 16 * t + row;
@@ -894,17 +903,30 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
             }
             __syncthreads();
         }
+        // one exponent per map (rows 2i, 2i+1 = re, im) and (n-tile, parity):
+        // the K-parts (waves w & 3 + 4 j) add their products before unscaling
+        amax = fmaxf(amax, __shfl_xor(amax, 1, 64));
         amax = fmaxf(amax, __shfl_xor(amax, 16, 64));
         amax = fmaxf(amax, __shfl_xor(amax, 32, 64));
+        if (Q > 1) {
+            float *am = xch;                            // [W][16], idle until the chunk loop
+            if (lane < 16) am[w * 16 + lane] = amax;
+            __syncthreads();
+#pragma unroll
+            for (int k2 = 0; k2 < Q; ++k2) amax = fmaxf(amax, am[((w & 3) + 4 * k2) * 16 + row]);
+        }
+        // map maximum scaled to [2^13, 2^14): k covers every finite FP32 row
+        // (k in [-114, 162]), applied in two exact power-of-two steps
         int e = 0;
         (void)frexpf(amax, &e);
-        const int k = amax > 0.0f ? min(max(14 - e, -100), 100) : 0;
-        const float sc = ldexpf(1.0f, k);
+        const int k = amax > 0.0f ? 14 - e : 0;
+        const int k1 = k >> 1;
+        const float sc1 = ldexpf(1.0f, k1), sc2 = ldexpf(1.0f, k - k1);
 #pragma unroll
         for (int s = 0; s < 8; ++s)
 #pragma unroll
             for (int i = 0; i < 8; i += 2) {
-                const float2v v = {Sf[8 * s + i] * sc, Sf[8 * s + i + 1] * sc};
+                const float2v v = {(Sf[8 * s + i] * sc1) * sc2, (Sf[8 * s + i + 1] * sc1) * sc2};
                 const half2v hv = __builtin_convertvector(v, half2v);
                 const half2v lv = split_lo(v, hv, neg1);
                 Ahi[s][i] = hv[0];
@@ -912,9 +934,8 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
                 Alo[s][i] = lv[0];
                 Alo[s][i + 1] = lv[1];
             }
-        const float inv = ldexpf(1.0f, -k);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) usc[j] = __shfl(inv, 4 * q + j, 64);
+        for (int j = 0; j < 4; ++j) kr[j] = __shfl(k, 4 * q + j, 64);
     }
     if (tid < 1 + Q) starts[tid] = NEVER_L;
     __syncthreads();
@@ -931,10 +952,11 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
     const int64_t ro = rofs[m];
     const double *al = alpha + ro - m;                  // zero-padded past lmax
     const double *sv = sl + ro - m;
-    auto store = [&](int l, int b, float2 v) {
+    // v: (re, im) of one map scaled by 2^k (the map's k); unscaled in FP64
+    auto store = [&](int l, int b, float2 v, int k) {
         if (b >= nmaps) return;
         if (direct) {
-            double f = sv[l];
+            double f = ldexp(sv[l], -k);
             double re = (double)v.x * f, im = (double)v.y * f;
             if (adjoint) {
                 if (m > 0) { re *= 2.0; im *= 2.0; }
@@ -942,14 +964,16 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
             }
             alm[(int64_t)b * astride + i0 + l] = make_float2((float)re, (float)im);
         } else {
-            partial[((int64_t)b * ngroups + grp) * nlm + i0 + l] = v;
+            partial[((int64_t)b * ngroups + grp) * nlm + i0 + l] =
+                make_float2((float)ldexp((double)v.x, -k), (float)ldexp((double)v.y, -k));
         }
     };
     // coefficients below every ring pair's switch-on are zero: written to alm
     // by a single group; partial sums record where theirs start instead
     const int lz = lstart > lmax ? lmax + 1 : lstart;
     if (direct)
-        for (int e = tid; e < (lz - m) * 16; e += G::W * 64) store(m + (e >> 4), g * 16 + (e & 15), make_float2(0.f, 0.f));
+        for (int e = tid; e < (lz - m) * 16; e += G::W * 64)
+            store(m + (e >> 4), g * 16 + (e & 15), make_float2(0.f, 0.f), 0);
     else if (g == 0 && tid == 0)
         lfirst[(int64_t)m * ngroups + grp] = lz;
     if (lstart > lmax) return;
@@ -1010,9 +1034,7 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
             }
         }
         floatx4 D = D0 + D1;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) D[r] *= usc[r];
-        // 3. K-parts 1 .. Q-1 hand their tiles to K-part 0 through LDS
+        // 3. K-parts 1 .. Q-1 hand their tiles to K-part 0 through LDS (same row scales)
         if (Q > 1 && kh > 0) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) xch[((((kh - 1) * 2 + t) * 2 + par) * 4 + r) * 64 + lane] = D[r];
@@ -1024,11 +1046,12 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
 #pragma unroll
                 for (int r = 0; r < 4; ++r) D[r] += xch[((((k2 - 1) * 2 + t) * 2 + par) * 4 + r) * 64 + lane];
             // D[row][col]: lane = col + 16 (row/4), reg = row%4; row = n - 16 t, col -> l
+            // (rows 4q, 4q+1 are re, im of one map and share its scale)
             const int l = c0 + 2 * (lane & 15) + par;
             if (l <= lmax) {
                 const int b0 = g * 16 + 8 * t + 2 * (lane >> 4);
-                store(l, b0, make_float2(D[0], D[1]));
-                store(l, b0 + 1, make_float2(D[2], D[3]));
+                store(l, b0, make_float2(D[0], D[1]), kr[0]);
+                store(l, b0 + 1, make_float2(D[2], D[3]), kr[2]);
             }
         }
     }

@@ -111,15 +111,25 @@ def _check_spin(spin):
     return spin
 
 
-def map2alm_spin(Q, U, spin=2, lmax=None, mmax=None, nest=False, *, precision="f64", weights="dz",
-                 phi0="reference", alias="reference"):
-    """(E, B) m-major from flat RING Q, U maps (``hp_compat.py:57-69``).
+def _wants_eb(spin, eb):
+    """The reference's pair convention: spin 1 -> (1_a, -1_a) (hp_compat.py:66-67,
+    81-82), spin 2 -> (E, B) (hp_compat.py:64-65, 79-80); ``eb`` overrides it
+    (True: always (E, B); False: always the spin pair (s_a, -s_a))."""
+    return spin != 1 if eb is None else bool(eb)
 
-    Unlike the reference, whose spin=1 branch returns (1_a, -1_a)
-    (``hp_compat.py:66-67``), every spin returns (E, B); the pair is
-    ``spin.eb_to_spin_pair(E, B, spin)``.  Batched [nmaps, npix] inputs give
-    [nmaps, nalm] outputs.
+
+def map2alm_spin(Q, U, spin=2, lmax=None, mmax=None, nest=False, *, eb=None, precision="f64", weights="dz",
+                 phi0="reference", alias="reference"):
+    """Spin-weighted analysis of flat RING Q, U maps (``hp_compat.py:57-69``), m-major.
+
+    Returns (E, B) for spin 2 and, as the reference's spin=1 branch
+    (``map2alm_spin1_gpu``, ``hp_compat.py:66-67``), the pair (1_a, -1_a)
+    for spin 1, with s_a = -(E + iB) and -s_a = -(-1)^s (E - iB).  Spins
+    3..8 (no reference counterpart) return (E, B).  ``eb=True`` / ``False``
+    selects (E, B) / the spin pair for any spin.  Batched [nmaps, npix]
+    inputs give [nmaps, nalm] outputs.
     """
+    from .spin import eb_to_spin_pair
     _check_nest(nest)
     spin = _check_spin(spin)
     q = _util.flat_maps(Q, Q.shape[-1] if hasattr(Q, "shape") else len(Q))
@@ -128,40 +138,48 @@ def map2alm_spin(Q, U, spin=2, lmax=None, mmax=None, nest=False, *, precision="f
     lmax = 3 * nside if lmax is None else int(lmax)
     mmax = lmax if mmax is None else int(mmax)
     plan = _plan(nside, lmax, mmax, precision, weights, phi0, alias, spin)
-    eb = plan.map2alm_spin(torch.stack([q, u], dim=-2))
-    return eb[..., 0, :], eb[..., 1, :]
+    out = plan.map2alm_spin(torch.stack([q, u], dim=-2))
+    E, B = out[..., 0, :], out[..., 1, :]
+    return (E, B) if _wants_eb(spin, eb) else eb_to_spin_pair(E, B, spin)
 
 
-def alm2map_spin(E, B, spin=2, nside=None, lmax=None, nest=False, mmax=None, *, precision="f64",
-                 dtype=torch.float64, phi0="reference", alias="reference"):
-    """Flat RING (Q, U) from m-major (E, B) (``hp_compat.py:71-84``).
+def alm2map_spin(Aplus, Aminus, spin=2, nside=None, lmax=None, nest=False, mmax=None, *, eb=None,
+                 precision="f64", dtype=torch.float64, phi0="reference", alias="reference"):
+    """Flat RING (Q, U) from m-major spin alm (``hp_compat.py:71-84``).
 
-    nside defaults to ceil(lmax/3) as in the reference (``hp_compat.py:72-75``).
+    (Aplus, Aminus) are (E, B) for spin 2 (``qu_from_eb_gpu``) and the pair
+    (1_a, -1_a) for spin 1 (``alm2map_spin1_gpu``), as in the reference;
+    ``eb`` overrides this as in ``map2alm_spin``.  nside defaults to
+    ceil(lmax/3) as in the reference (``hp_compat.py:72-75``).
     """
+    from .spin import spin_pair_to_eb
     _check_nest(nest)
     spin = _check_spin(spin)
-    e = torch.as_tensor(E)
-    b = torch.as_tensor(B)
+    a = torch.as_tensor(Aplus)
+    b = torch.as_tensor(Aminus)
     if lmax is None:
-        lmax = lmax_from_size(e.shape[-1])
+        lmax = lmax_from_size(a.shape[-1])
     mmax = lmax if mmax is None else int(mmax)
-    if e.shape != b.shape or e.shape[-1] != nalm(int(lmax), mmax):
-        raise ValueError("E and B must both be m-major alm of lmax/mmax")
+    if a.shape != b.shape or a.shape[-1] != nalm(int(lmax), mmax):
+        raise ValueError("Aplus and Aminus must both be m-major alm of lmax/mmax")
     nside = max(1, -(-int(lmax) // 3)) if nside is None else int(nside)
     plan = _plan(nside, lmax, mmax, precision, "dz", phi0, alias, spin)
     dev = plan.device
-    qu = plan.alm2map_spin(torch.stack([e.to(dev), b.to(dev)], dim=-2), dtype=dtype)
+    a, b = a.to(dev), b.to(dev)
+    if not _wants_eb(spin, eb):
+        a, b = spin_pair_to_eb(a, b, spin)
+    qu = plan.alm2map_spin(torch.stack([a, b], dim=-2), dtype=dtype)
     return qu[..., 0, :], qu[..., 1, :]
 
 
 def iqu2eb(Q, U, lmax=None, nest=False, **kw):
     """(E, B) of a spin-2 field (``hp_compat.py:87-94``)."""
-    return map2alm_spin(Q, U, 2, lmax=lmax, nest=nest, **kw)
+    return map2alm_spin(Q, U, 2, lmax=lmax, nest=nest, eb=True, **kw)
 
 
 def eb2iqu(E, B, nside=None, lmax=None, nest=False, **kw):
     """(Q, U) of spin-2 (E, B) (``hp_compat.py:96-102``)."""
-    return alm2map_spin(E, B, 2, nside=nside, lmax=lmax, nest=nest, **kw)
+    return alm2map_spin(E, B, 2, nside=nside, lmax=lmax, nest=nest, eb=True, **kw)
 
 
 # ---------- alm utilities (``hp_compat.py:113-120``) ----------

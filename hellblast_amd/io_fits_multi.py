@@ -10,11 +10,6 @@ import numpy as np
 
 from .io_fits import _to_device, _to_host, image_hdu_bytes, list_hdus, read_image, write_file
 
-# header keywords kept when a header is carried over (``io_fits_multi.py:4-7``)
-KEY_KEEP = ["NSIDE", "ORDERING", "COORDSYS", "EXTNAME", "OBJECT", "DATE", "CREATOR", "HISTORY", "COMMENT",
-            "TTYPE1", "TTYPE2", "TTYPE3", "BUNIT", "TELESCOP", "INSTRUME", "OBS_MODE"]
-
-
 def write_tqu_fits(path, T=None, Q=None, U=None, nside=None, ordering="RING", coordsys="G", overwrite=True,
                    extra_header=None):
     """(``io_fits_multi.py:24-41``)"""

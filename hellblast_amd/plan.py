@@ -150,6 +150,9 @@ class Plan:
         self.device_bytes = int(info.device_bytes)
         self.fp64_band_m = int(info.fp64_band_m)
         self.alm_dtype = torch.complex128 if self.precision == _abi.HBX_F64 else torch.complex64
+        self._req_tiles = {k: 0 for k in self.TILE_FIELDS}
+        self._derived = {}
+        self._root = None
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -164,22 +167,42 @@ class Plan:
     # ------------------------------------------------------------------
     TILE_FIELDS = ("valu_slots", "mfma_waves", "mfma_min_maps", "spin_slots", "mfma_kind", "mfma_groups")
 
-    def set_tiles(self, **tiles):
-        """Set plan tiles (0 = default); unknown names raise.  Changes the
-        workspace size: do not call while work on this plan is in flight."""
-        cur = self.tiles(effective=False)
+    def with_tiles(self, **tiles):
+        """This plan with other tiles (0 = default; unknown names raise).
+
+        Plans are immutable: the result is another plan (``hbx_plan_derive``)
+        that shares this plan's device tables, cached per tile set on this
+        plan, so callers that tune tiles never change the kernels or the
+        summation order of anyone else's plan.
+        """
+        req = dict(self._req_tiles)
         for k, v in tiles.items():
             if k not in self.TILE_FIELDS:
                 raise ValueError(f"unknown tile {k!r}")
-            cur[k] = int(v)
-        t = _abi.hbx_tiles(**cur)
-        _abi.check(_abi.lib().hbx_plan_set_tiles(self._h, ctypes.byref(t)))
-        self._tiles = dict(cur)
+            req[k] = int(v)
+        key = tuple(req[k] for k in self.TILE_FIELDS)
+        if key == tuple(self._req_tiles[k] for k in self.TILE_FIELDS):
+            return self
+        root = self._root if self._root is not None else self
+        hit = root._derived.get(key)
+        if hit is not None:
+            return hit
+        t = _abi.hbx_tiles(**req)
+        handle = ctypes.c_void_p()
+        _abi.check(_abi.lib().hbx_plan_derive(self._h, ctypes.byref(t), ctypes.byref(handle)))
+        q = object.__new__(Plan)
+        q.__dict__.update({k: v for k, v in self.__dict__.items() if k not in ("_h", "_derived", "_root")})
+        q._h = handle
+        q._req_tiles = req
+        q._derived = {}
+        q._root = root
+        root._derived[key] = q
+        return q
 
     def tiles(self, effective=True):
         """Effective tiles (library defaults resolved), or the requested ones."""
         if not effective:
-            return dict(getattr(self, "_tiles", {k: 0 for k in self.TILE_FIELDS}))
+            return dict(self._req_tiles)
         t = _abi.hbx_tiles()
         _abi.check(_abi.lib().hbx_plan_get_tiles(self._h, ctypes.byref(t)))
         return {k: int(getattr(t, k)) for k in self.TILE_FIELDS}
@@ -216,26 +239,54 @@ class Plan:
             raise ValueError(f"alm has {t.shape[-1]} coefficients, plan expects {self.nalm}")
         return t.contiguous(), squeeze
 
+    def _out(self, out, shape, dtype):
+        """A new output tensor, or the caller's ``out`` after checking that the
+        kernels can write ``shape`` of ``dtype`` into it."""
+        if out is None:
+            return torch.empty(shape, dtype=dtype, device=self.device)
+        if not isinstance(out, torch.Tensor):
+            raise TypeError("out must be a torch tensor")
+        if out.device != self.device:
+            raise ValueError(f"out is on {out.device}, the plan on {self.device}")
+        if out.dtype != dtype:
+            raise ValueError(f"out has dtype {out.dtype}, expected {dtype}")
+        if not out.is_contiguous():
+            raise ValueError("out must be contiguous")
+        if out.numel() != math.prod(shape) or out.dim() < 1 or out.shape[-1] != shape[-1]:
+            raise ValueError(f"out has shape {tuple(out.shape)}; {tuple(shape)} (any leading shape) expected")
+        return out
+
+    def _buffer(self, buf, what):
+        """Byte size of a caller's phase buffer / workspace (device, contiguous)."""
+        if not isinstance(buf, torch.Tensor) or buf.device != self.device or not buf.is_contiguous():
+            raise ValueError(f"{what} must be a contiguous tensor on {self.device}")
+        return buf.numel() * buf.element_size()
+
     def _analysis(self, fn, maps, out=None):
         m, squeeze = self._as_maps(maps)
         nb = m.shape[0]
-        alm = torch.empty((nb, self.nalm), dtype=self.alm_dtype, device=self.device) if out is None else out
+        alm = self._out(out, (nb, self.nalm), self.alm_dtype)
         ws = self.workspace(nb)
         with torch.cuda.device(self.device):
             _abi.check(fn(self._h, nb, ctypes.c_void_p(m.data_ptr()), _map_dtype_code(m.dtype), m.shape[1],
                           ctypes.c_void_p(alm.data_ptr()), alm.shape[-1], ctypes.c_void_p(ws.data_ptr()),
                           ws.numel(), _stream_ptr(self.device)))
+        if out is not None:
+            return out
         return alm[0] if squeeze else alm
 
     def _synthesis(self, fn, alm, dtype=torch.float64, out=None):
         a, squeeze = self._as_alm(alm)
         nb = a.shape[0]
-        maps = torch.empty((nb, self.npix), dtype=dtype, device=self.device) if out is None else out
+        _map_dtype_code(dtype)
+        maps = self._out(out, (nb, self.npix), dtype)
         ws = self.workspace(nb)
         with torch.cuda.device(self.device):
             _abi.check(fn(self._h, nb, ctypes.c_void_p(a.data_ptr()), a.shape[1], ctypes.c_void_p(maps.data_ptr()),
                           _map_dtype_code(maps.dtype), maps.shape[-1], ctypes.c_void_p(ws.data_ptr()),
                           ws.numel(), _stream_ptr(self.device)))
+        if out is not None:
+            return out
         return maps[0] if squeeze else maps
 
     # Public transforms are differentiable: an input that requires grad (with
@@ -297,12 +348,14 @@ class Plan:
             t = t.to(torch.float64)
         m, squeeze = self._pairs(t, self.npix)
         npol = m.shape[0]
-        alm = torch.empty((npol, 2, self.nalm), dtype=self.alm_dtype, device=self.device) if out is None else out
+        alm = self._out(out, (npol, 2, self.nalm), self.alm_dtype)
         ws = self.workspace(npol)
         with torch.cuda.device(self.device):
             _abi.check(fn(self._h, npol, ctypes.c_void_p(m.data_ptr()), _map_dtype_code(m.dtype), self.npix,
                           ctypes.c_void_p(alm.data_ptr()), self.nalm, ctypes.c_void_p(ws.data_ptr()), ws.numel(),
                           _stream_ptr(self.device)))
+        if out is not None:
+            return out
         return alm[0] if squeeze else alm
 
     def _spin_synthesis(self, fn, alm, dtype=torch.float64, out=None):
@@ -311,12 +364,15 @@ class Plan:
         t = t.to(self.alm_dtype) if t.dtype != self.alm_dtype else t
         a, squeeze = self._pairs(t, self.nalm)
         npol = a.shape[0]
-        maps = torch.empty((npol, 2, self.npix), dtype=dtype, device=self.device) if out is None else out
+        _map_dtype_code(dtype)
+        maps = self._out(out, (npol, 2, self.npix), dtype)
         ws = self.workspace(npol)
         with torch.cuda.device(self.device):
             _abi.check(fn(self._h, npol, ctypes.c_void_p(a.data_ptr()), self.nalm, ctypes.c_void_p(maps.data_ptr()),
                           _map_dtype_code(maps.dtype), self.npix, ctypes.c_void_p(ws.data_ptr()), ws.numel(),
                           _stream_ptr(self.device)))
+        if out is not None:
+            return out
         return maps[0] if squeeze else maps
 
     def map2alm_spin(self, maps, out=None):
@@ -369,37 +425,42 @@ class Plan:
 
     def ring_analysis(self, maps, phase):
         m, _ = self._as_maps(maps)
+        pb = self._buffer(phase, "phase")
         with torch.cuda.device(self.device):
             _abi.check(_abi.lib().hbx_ring_analysis(
                 self._h, m.shape[0], ctypes.c_void_p(m.data_ptr()), _map_dtype_code(m.dtype), m.shape[1],
-                ctypes.c_void_p(phase.data_ptr()), phase.numel(), _stream_ptr(self.device)))
+                ctypes.c_void_p(phase.data_ptr()), pb, _stream_ptr(self.device)))
         return phase
 
     def legendre_analysis(self, phase, nmaps, out=None, ws=None):
-        alm = torch.empty((nmaps, self.nalm), dtype=self.alm_dtype, device=self.device) if out is None else out
+        alm = self._out(out, (int(nmaps), self.nalm), self.alm_dtype)
         ws = self.workspace(nmaps) if ws is None else ws
+        pb, wb = self._buffer(phase, "phase"), self._buffer(ws, "ws")
         with torch.cuda.device(self.device):
             _abi.check(_abi.lib().hbx_legendre_analysis(
-                self._h, int(nmaps), ctypes.c_void_p(phase.data_ptr()), phase.numel(),
-                ctypes.c_void_p(alm.data_ptr()), alm.shape[-1], ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                self._h, int(nmaps), ctypes.c_void_p(phase.data_ptr()), pb,
+                ctypes.c_void_p(alm.data_ptr()), self.nalm, ctypes.c_void_p(ws.data_ptr()), wb,
                 _stream_ptr(self.device)))
         return alm
 
     def legendre_synthesis(self, alm, phase, ws=None):
         a, _ = self._as_alm(alm)
         ws = self.workspace(a.shape[0]) if ws is None else ws
+        pb, wb = self._buffer(phase, "phase"), self._buffer(ws, "ws")
         with torch.cuda.device(self.device):
             _abi.check(_abi.lib().hbx_legendre_synthesis(
                 self._h, a.shape[0], ctypes.c_void_p(a.data_ptr()), a.shape[1], ctypes.c_void_p(phase.data_ptr()),
-                phase.numel(), ctypes.c_void_p(ws.data_ptr()), ws.numel(), _stream_ptr(self.device)))
+                pb, ctypes.c_void_p(ws.data_ptr()), wb, _stream_ptr(self.device)))
         return phase
 
     def ring_synthesis(self, phase, nmaps, dtype=torch.float64, out=None):
-        maps = torch.empty((nmaps, self.npix), dtype=dtype, device=self.device) if out is None else out
+        _map_dtype_code(dtype)
+        maps = self._out(out, (int(nmaps), self.npix), dtype)
+        pb = self._buffer(phase, "phase")
         with torch.cuda.device(self.device):
             _abi.check(_abi.lib().hbx_ring_synthesis(
-                self._h, int(nmaps), ctypes.c_void_p(phase.data_ptr()), phase.numel(),
-                ctypes.c_void_p(maps.data_ptr()), _map_dtype_code(maps.dtype), maps.shape[-1],
+                self._h, int(nmaps), ctypes.c_void_p(phase.data_ptr()), pb,
+                ctypes.c_void_p(maps.data_ptr()), _map_dtype_code(maps.dtype), self.npix,
                 _stream_ptr(self.device)))
         return maps
 

@@ -8,8 +8,10 @@
  * (contiguous, complex arrays interleaved re/im like torch complex64/128).
  * Execute functions never allocate and never synchronise: they enqueue on the
  * given hipStream_t (NULL = default stream) and are safe to capture in a HIP
- * graph.  Plans are immutable after creation and bound to the device that was
- * current when they were created.
+ * graph.  Plans are immutable after creation (tiles included: another tile
+ * choice is another plan, made by hbx_plan_derive) and bound to the device
+ * that was current when they were created; one plan may be used from several
+ * threads and streams at once.
  *
  * Reference interfaces replaced (file:line in Luce-Morningstar/hellblast):
  *   hbx_plan_create      - per-call setup in sht_raw.py:30-45 (ring FFT prep,
@@ -42,7 +44,7 @@
 extern "C" {
 #endif
 
-#define HBX_ABI_VERSION 1
+#define HBX_ABI_VERSION 2
 
 typedef enum hbx_status {
     HBX_OK = 0,
@@ -110,10 +112,11 @@ int hbx_plan_get_info(const hbx_plan *plan, hbx_plan_info *info);
 int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
 
 /* Tiles (no reference counterpart; the autotuner's hook).  0 = the
- * size-based default.  Tiles change the workspace size and the summation
- * order (results agree to rounding), so set them before the plan is used and
- * never while work using the plan is in flight.  get_tiles returns the
- * effective values.
+ * size-based default.  hbx_plan_derive makes a new plan with the given tiles
+ * that shares the device tables of base (no device allocation; base and the
+ * derived plan may be destroyed in any order, the tables are freed with the
+ * last of them).  Tiles change the workspace size and the summation order
+ * (results agree to rounding).  get_tiles returns the effective values.
  *   valu_slots     VALU Legendre analysis: ring-pair slots of 64 pairs per lane
  *                  (power of two <= 16; FP64 kernels clamp to 8)
  *   mfma_waves     matrix-core analysis: waves (64 ring pairs each) per
@@ -138,7 +141,7 @@ int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
 typedef struct hbx_tiles {
     int32_t valu_slots, mfma_waves, mfma_min_maps, spin_slots, mfma_kind, mfma_groups;
 } hbx_tiles;
-int hbx_plan_set_tiles(hbx_plan *plan, const hbx_tiles *tiles);
+int hbx_plan_derive(const hbx_plan *base, const hbx_tiles *tiles, hbx_plan **out);
 int hbx_plan_get_tiles(const hbx_plan *plan, hbx_tiles *tiles);
 
 /* Scalar transforms.  maps: nmaps maps of plan npix values (dtype

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 import torch
 
-from hellblast_amd import hp_compat, io_fits, io_fits_multi
+from hellblast_amd import hp_compat, io_fits
 from hellblast_amd.alm_ops import from_reference_layout, getidx
 
 
@@ -176,7 +176,6 @@ def test_tqu_round_trip(tmp_path):
     (t, q, u), meta = hp_compat.read_tqu(p, to_gpu=False)
     assert q is None and np.array_equal(t, T) and u.dtype == np.float32 and np.array_equal(u, U)
     assert meta == {"NSIDE": 4, "ORDERING": "RING", "COORDSYS": "G"}
-    assert "NSIDE" in io_fits_multi.KEY_KEEP
 
 
 def test_header_values_and_errors(tmp_path):

@@ -30,26 +30,46 @@ def _eps_max(a, ref, lmax):
     return float(np.max(np.abs(a - ref)[ok] / np.sqrt(c[ok])))
 
 
-def test_tiles_set_get_and_validation(gpu):
+def test_tiles_derive_get_and_validation(gpu):
+    """Tiles make a derived plan (hbx_plan_derive); the base plan keeps its
+    own tiles, so tuning never changes anyone else's kernels."""
     from hellblast_amd import _abi
     from hellblast_amd._abi import HbxValueError
     plan = _plan(32, 64, "f32")
     base = plan.tiles()
     assert base["mfma_waves"] == 4 and base["mfma_min_maps"] == 3
     assert base["mfma_kind"] == _abi.HBX_MFMA_F16X2
-    plan.set_tiles(mfma_waves=4, valu_slots=2)
-    t = plan.tiles()
+    d = plan.with_tiles(mfma_waves=4, valu_slots=2)
+    t = d.tiles()
     assert t["mfma_waves"] == 4 and t["valu_slots"] == 2
-    with pytest.raises(HbxValueError):
-        plan.set_tiles(mfma_waves=5)
-    with pytest.raises(HbxValueError):
-        plan.set_tiles(valu_slots=3)
-    with pytest.raises(HbxValueError):
-        plan.set_tiles(mfma_kind=3)
-    with pytest.raises(HbxValueError):
-        plan.set_tiles(mfma_waves=12, mfma_kind=_abi.HBX_MFMA_F16X2)
-    plan.set_tiles(mfma_waves=0, valu_slots=0)
     assert plan.tiles() == base
+    assert plan.with_tiles(mfma_waves=4, valu_slots=2) is d          # cached per tile set
+    assert d.with_tiles(mfma_waves=0, valu_slots=0).tiles() == base
+    for bad in ({"mfma_waves": 5}, {"valu_slots": 3}, {"mfma_kind": 3},
+                {"mfma_waves": 12, "mfma_kind": _abi.HBX_MFMA_F16X2}):
+        with pytest.raises(HbxValueError):
+            plan.with_tiles(**bad)
+    with pytest.raises(ValueError):
+        plan.with_tiles(no_such_tile=1)
+
+
+def test_derived_plan_outlives_its_base(gpu):
+    """A derived plan shares the base's device tables and keeps them alive."""
+    import gc
+    from hellblast_amd.hp_compat import healpix_geom
+    from hellblast_amd.plan import Plan
+    nside, lmax = 16, 32
+    geom = healpix_geom(nside, lmax)
+    base = Plan(geom, lmax, precision="f32")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(8)
+    maps = torch.randn((4, base.npix), generator=gen, device="cuda", dtype=torch.float32)
+    ref = base.map2alm(maps)
+    d = base.with_tiles(mfma_min_maps=100)            # the VALU kernels
+    del base
+    gc.collect()
+    got = d.map2alm(maps)
+    assert float((got - ref).abs().max()) <= 2e-6 * float(ref.abs().max())
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -62,33 +82,32 @@ def test_every_candidate_tile_gives_the_same_transform(gpu, precision):
     gen = torch.Generator(device="cuda")
     gen.manual_seed(3)
     maps = torch.randn((n, plan.npix), generator=gen, device="cuda", dtype=rdt)
-    plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
     ref_a = plan.map2alm(maps)
     ref_m = plan.alm2map(ref_a, dtype=rdt)
     tol = 1e-13 if precision == "f64" else 2e-6
     for cand in candidates(plan, n):
-        plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
-        plan.set_tiles(**cand)
-        a = plan.map2alm(maps)
-        m = plan.alm2map(ref_a, dtype=rdt)
+        p = plan.with_tiles(**cand)
+        a = p.map2alm(maps)
+        m = p.alm2map(ref_a, dtype=rdt)
         assert float((a - ref_a).abs().max()) <= tol * float(ref_a.abs().max()), cand
         assert float((m - ref_m).abs().max()) <= tol * float(ref_m.abs().max()), cand
-    plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
 
 
 def test_autotune_caches_its_choice(gpu, tmp_path, monkeypatch):
     from hellblast_amd import autotune
     monkeypatch.setenv(autotune.CACHE_ENV, str(tmp_path / "tiles.json"))
     plan = _plan(64, 128, "f32")
+    before = plan.tiles()
     first = autotune.tune(plan, 4, reps=1)
     assert (tmp_path / "tiles.json").exists()
     assert len(first["timed"]) == len(autotune.candidates(plan, 4))
+    assert plan.tiles() == before                      # tuning leaves the plan alone
     second = autotune.tune(plan, 4, reps=1)
     assert second["tiles"] == first["tiles"]
-    eff = plan.tiles()
+    eff = plan.with_tiles(**first["tiles"]).tiles()
     for k, v in first["tiles"].items():
-        assert eff[k] == v
-    plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
+        if v:
+            assert eff[k] == v
 
 
 def test_synthesis_map_groups_agree(gpu):
@@ -101,11 +120,8 @@ def test_synthesis_map_groups_agree(gpu):
     gen.manual_seed(4)
     alm = torch.complex(torch.randn((n, plan.nalm), generator=gen, device="cuda"),
                         torch.randn((n, plan.nalm), generator=gen, device="cuda"))
-    plan.set_tiles(mfma_groups=1)
-    m1 = plan.alm2map(alm, dtype=torch.float32)
-    plan.set_tiles(mfma_groups=2)
-    m2 = plan.alm2map(alm, dtype=torch.float32)
-    plan.set_tiles(**{f: 0 for f in plan.TILE_FIELDS})
+    m1 = plan.with_tiles(mfma_groups=1).alm2map(alm, dtype=torch.float32)
+    m2 = plan.with_tiles(mfma_groups=2).alm2map(alm, dtype=torch.float32)
     assert torch.equal(m1, m2)
 
 

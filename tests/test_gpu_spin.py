@@ -160,12 +160,9 @@ def test_spin_f32_matrix_core_analysis(gpu):
     print("[case] spin f32 matrix-core analysis nside=128 lmax=200 5 pairs", flush=True)
     tabs = OG.ring_tables(nside)
     qu = _qu(tabs, 31, n=5)
-    plan = _plan(nside, lmax, spin=s, precision="f32")
-    plan.set_tiles(mfma_min_maps=2)          # every batch, one pair included, on the matrix cores
-    try:
-        _matrix_core_analysis_checks(plan, qu, tabs, lmax, s)
-    finally:
-        plan.set_tiles(mfma_min_maps=0)
+    # every batch, one pair included, on the matrix cores
+    plan = _plan(nside, lmax, spin=s, precision="f32").with_tiles(mfma_min_maps=2)
+    _matrix_core_analysis_checks(plan, qu, tabs, lmax, s)
 
 
 def _matrix_core_analysis_checks(plan, qu, tabs, lmax, s):
@@ -179,12 +176,9 @@ def _matrix_core_analysis_checks(plan, qu, tabs, lmax, s):
     one = plan.map2alm_spin(x[4:5]).cpu().numpy()
     assert np.array_equal(one[0], got[4])
     adj = plan.alm2map_spin_adjoint(x).cpu().numpy()
-    plan.set_tiles(mfma_min_maps=1000)
-    try:
-        valu = plan.map2alm_spin(x).cpu().numpy()
-        vadj = plan.alm2map_spin_adjoint(x).cpu().numpy()
-    finally:
-        plan.set_tiles(mfma_min_maps=2)
+    vplan = plan.with_tiles(mfma_min_maps=1000)
+    valu = vplan.map2alm_spin(x).cpu().numpy()
+    vadj = vplan.alm2map_spin_adjoint(x).cpu().numpy()
     print(f"matrix vs VALU: {_rel(got, valu):.2e}, adjoint {_rel(adj, vadj):.2e}", flush=True)
     assert _rel(got, valu) <= 4e-6 and _rel(adj, vadj) <= 4e-6
 
@@ -201,8 +195,8 @@ def test_spin_f32_matrix_core_synthesis(gpu):
     tabs = OG.ring_tables(nside)
     for s in (1, 2, 3):
         print(f"[case] spin f32 matrix-core synthesis nside=128 lmax=200 s={s} 5 pairs", flush=True)
-        plan = _plan(nside, lmax, spin=s, precision="f32")
-        plan.set_tiles(mfma_min_maps=2)      # every batch, one pair included, on the matrix cores
+        # every batch, one pair included, on the matrix cores
+        plan = _plan(nside, lmax, spin=s, precision="f32").with_tiles(mfma_min_maps=2)
         eb = np.stack([np.stack(OSp.synalm_eb(lmax, spin=s, seed=40 + b)) for b in range(5)])
         a = torch.as_tensor(eb, device="cuda").to(torch.complex64)
         got = plan.alm2map_spin(a, dtype=torch.float32).double().cpu().numpy()
@@ -214,12 +208,9 @@ def test_spin_f32_matrix_core_synthesis(gpu):
         one = plan.alm2map_spin(a[4:5], dtype=torch.float32).double().cpu().numpy()
         assert np.array_equal(one[0], got[4])
         adj = plan.map2alm_spin_adjoint(a, dtype=torch.float32).double().cpu().numpy()
-        plan.set_tiles(mfma_min_maps=1000)
-        try:
-            valu = plan.alm2map_spin(a, dtype=torch.float32).double().cpu().numpy()
-            vadj = plan.map2alm_spin_adjoint(a, dtype=torch.float32).double().cpu().numpy()
-        finally:
-            plan.set_tiles(mfma_min_maps=0)
+        vplan = plan.with_tiles(mfma_min_maps=1000)
+        valu = vplan.alm2map_spin(a, dtype=torch.float32).double().cpu().numpy()
+        vadj = vplan.map2alm_spin_adjoint(a, dtype=torch.float32).double().cpu().numpy()
         print(f"matrix vs VALU: {_rel(got, valu):.2e}, adjoint {_rel(adj, vadj):.2e}", flush=True)
         assert _rel(got, valu) <= 4e-6 and _rel(adj, vadj) <= 4e-6
 
@@ -231,12 +222,9 @@ def test_spin_f32_small_sizes(gpu, nside, lmax, s):
     ring-pair tile, the self-paired equator ring; 2 Q/U pairs."""
     print(f"[case] spin f32 small nside={nside} lmax={lmax} s={s}", flush=True)
     tabs = OG.ring_tables(nside)
-    plan = _plan(nside, lmax, spin=s, precision="f32")
-    plan.set_tiles(mfma_min_maps=2)          # the matrix kernels, which 2 pairs would not get by default
-    try:
-        _small_size_checks(plan, tabs, nside, lmax, s)
-    finally:
-        plan.set_tiles(mfma_min_maps=0)
+    # the matrix kernels, which 2 pairs would not get by default
+    plan = _plan(nside, lmax, spin=s, precision="f32").with_tiles(mfma_min_maps=2)
+    _small_size_checks(plan, tabs, nside, lmax, s)
 
 
 def _small_size_checks(plan, tabs, nside, lmax, s):
