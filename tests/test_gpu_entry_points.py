@@ -164,14 +164,16 @@ def test_spin1_module_reference_signatures(gpu):
 @pytest.mark.parametrize("nside,lmax", [(16, 40), (32, 95)])
 def test_anafast_ring_matches_oracle(gpu, nside, lmax):
     """anafast.anafast_ring (anafast.py:5-7): C_l of map2alm_gpu on ring lists."""
-    from hellblast_amd import anafast
+    # (the package attribute ``anafast`` is the function alias, as in the
+    # reference's __init__.py:125-127; the module is hellblast_amd.anafast)
+    from hellblast_amd.anafast import anafast_ring
     print(f"[case] anafast_ring nside={nside} lmax={lmax}", flush=True)
     tabs = OG.ring_tables(nside)
     rng = np.random.default_rng(600 + nside)
     f = rng.standard_normal(tabs["npix"])
     rs = tabs["ring_start"]
     rings = [torch.as_tensor(f[rs[r]:rs[r + 1]], device="cuda") for r in range(tabs["Nr"])]
-    cl = anafast.anafast_ring(rings, tabs["theta_centers"], tabs["nphi"], tabs["weights_dz"], lmax)
+    cl = anafast_ring(rings, tabs["theta_centers"], tabs["nphi"], tabs["weights_dz"], lmax)
     ref = OS.alm2cl(OS.map2alm(f, tabs, lmax), lmax)
     assert cl.shape == (lmax + 1,)
     assert _rel(cl.cpu().numpy(), ref) <= TOL
