@@ -17,6 +17,7 @@ HBX_EWORKSPACE = -4
 HBX_EUNSUPPORTED = -5
 HBX_EPLAN = -6
 HBX_MAX_SPIN = 8
+HBX_STATIC_SCRATCH_BYTES = 64 << 20
 HBX_MFMA_F32 = 1
 HBX_MFMA_F16X2 = 2
 HBX_TIMER_LEGENDRE_ANALYSIS = 1
@@ -95,6 +96,7 @@ SIGNATURES = {
     "hbx_plan_get_tiles": (ctypes.c_int, [_vp, ctypes.POINTER(hbx_tiles)]),
     "hbx_map2alm": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _i64, _vp, _u64, _vp]),
     "hbx_alm2map": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i32, _i64, _vp, _u64, _vp]),
+    "hbx_map2alm_static": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _i64, _vp]),
     "hbx_alm2map_adjoint": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _i64, _vp, _u64, _vp]),
     "hbx_map2alm_adjoint": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i32, _i64, _vp, _u64, _vp]),
     "hbx_map2alm_spin": (ctypes.c_int, [_vp, _i32, _vp, _i32, _i64, _vp, _i64, _vp, _u64, _vp]),

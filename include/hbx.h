@@ -154,6 +154,20 @@ int hbx_map2alm(const hbx_plan *plan, int32_t nmaps, const void *maps,
 int hbx_alm2map(const hbx_plan *plan, int32_t nmaps, const void *alm,
                 int64_t alm_stride, void *maps, int32_t map_dtype,
                 int64_t map_stride, void *ws, uint64_t ws_bytes, void *stream);
+/* hbx_map2alm with no workspace argument that allocates nothing: its
+ * workspace is a static device array of HBX_STATIC_SCRATCH_BYTES, which each
+ * GPU holds from the first call on it until the process ends.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  A plan and batch whose
+ * hbx_workspace_bytes() exceeds the scratch get HBX_EUNSUPPORTED (64 MiB
+ * holds e.g. one FP64 map at nside 512, lmax 1023).  All calls on one GPU
+ * share the scratch: calls that overlap on one GPU (other streams or
+ * threads) race.                                                           */
+#define HBX_STATIC_SCRATCH_BYTES (64u << 20)
+int hbx_map2alm_static(const hbx_plan *plan, int32_t nmaps, const void *maps,
+                       int32_t map_dtype, int64_t map_stride, void *alm,
+                       int64_t alm_stride, void *stream);
 /* Transposes: adjoint of alm2map (c_m-weighted unweighted analysis) and of
  * map2alm (weighted synthesis of g/c_m), PyTorch complex-gradient
  * convention grad = dL/dRe + i dL/dIm.                                      */
