@@ -883,20 +883,21 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
             return ok ? ph[(int64_t)(par == 0 ? qq : nr - 1 - qq) * (2 * MB)] : 0.0f;
         };
         float *xs = reinterpret_cast<float *>(hsm);     // [W][32][64] per round of 4 K-steps
+        // every load is issued before the first exchange
         float Sf[64];
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) Sf[8 * s + i] = own(s, i);
         float amax = 0.0f;
 #pragma unroll
         for (int rd = 0; rd < 2; ++rd) {
 #pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) xs[(w * 32 + 8 * s4 + i) * 64 + lane] = own(4 * rd + s4, i);
-                __builtin_amdgcn_sched_barrier(0);
-            }
+            for (int j = 0; j < 32; ++j) xs[(w * 32 + j) * 64 + lane] = Sf[32 * rd + j];
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < 32; ++j) {
-                const float mine = xs[(w * 32 + j) * 64 + lane], other = xs[((w ^ 2) * 32 + j) * 64 + lane];
+                const float mine = Sf[32 * rd + j], other = xs[((w ^ 2) * 32 + j) * 64 + lane];
                 const float a = par == 0 ? mine : other, z = par == 0 ? other : mine;
                 Sf[32 * rd + j] = par ? a - z : a + z;
                 amax = fmaxf(amax, fabsf(Sf[32 * rd + j]));
@@ -978,48 +979,57 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
         lfirst[(int64_t)m * ngroups + grp] = lz;
     if (lstart > lmax) return;
 
-    if (tid < ANA_L) alds[tid] = al[lstart + tid];
-    double anext = (tid < ANA_L) ? al[lstart + ANA_L + tid] : 0.0;
-    __syncthreads();
     // this lane's recursion stores: byte 2 (128 w + 2 lane) of a column
     const int wofs = 256 * w + 4 * lane;
     // this lane's B reads: column lane%16, halves 256 kh + 32 s + 8 q
     const int rofs16 = par * G::PARB + (lane & 15) * G::COLB + 2 * (256 * kh + 8 * (lane >> 4));
-    int buf = 0;
-    for (int c0 = lstart; c0 <= lmax; c0 += ANA_L, buf ^= 1) {
-        const double *ab = alds + buf * ANA_L;
-        const double anext2 = (tid < ANA_L) ? al[c0 + 2 * ANA_L + tid] : 0.0;   // in flight a chunk
-        // 1. recursion of this lane's two pairs over l = c0 .. c0+31 into the hi/lo tiles
-        if (rec) {
+    // alpha_l is wave-uniform: two banks of 16 scalar registers, each loaded
+    // 16 steps before its first use (a load used at once would stall the
+    // wave; broadcast LDS reads would take LDS bandwidth from the operands)
+    const int lst = __builtin_amdgcn_readfirstlane(lstart);
+    double aA[16], aB[16];
+    auto load16 = [&](double *dst, int l0) {
 #pragma unroll
-            for (int tt = 0; tt < ANA_L; ++tt) {
-                if ((tt & 7) == 0 && c0 <= winj) {
+        for (int j = 0; j < 16; ++j) dst[j] = al[l0 + j];
+    };
+    // one recursion step of this lane's two pairs at l = c0 + tt into the
+    // hi/lo tiles; switch-on seeds are injected at l = m + 8 j
+    auto step = [&](int c0, int tt, double a) {
+        if ((tt & 7) == 0 && c0 <= winj) {
 #pragma unroll
-                    for (int h = 0; h < 2; ++h)
-                        if (ls[h] == c0 + tt) {
-                            u2[h] = sd[h].x;
-                            u1[h] = sd[h].y;
-                        }
+            for (int h = 0; h < 2; ++h)
+                if (ls[h] == c0 + tt) {
+                    u2[h] = sd[h].x;
+                    u1[h] = sd[h].y;
                 }
-                const double a = ab[tt];
-                float2v f;
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const double u = fma(a * x[h], u1[h], -u2[h]);
-                    u2[h] = u1[h];
-                    u1[h] = u;
-                    f[h] = (float)u;
-                }
-                // packed conversions (v_cvt_pk_f16_f32, round to nearest even)
-                const half2v hh = __builtin_convertvector(f, half2v);
-                const half2v hl = split_lo(f, hh, neg1);
-                const int o = (tt & 1) * G::PARB + (tt >> 1) * G::COLB + wofs;
-                *reinterpret_cast<uint32_t *>(thi + o) = pack_h2(hh);
-                *reinterpret_cast<uint32_t *>(tlo + o) = pack_h2(hl);
-            }
         }
-        if (tid < ANA_L) alds[(buf ^ 1) * ANA_L + tid] = anext;
-        anext = anext2;
+        float2v f;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const double u = fma(a * x[h], u1[h], -u2[h]);
+            u2[h] = u1[h];
+            u1[h] = u;
+            f[h] = (float)u;
+        }
+        // packed conversions (v_cvt_pk_f16_f32, round to nearest even)
+        const half2v hh = __builtin_convertvector(f, half2v);
+        const half2v hl = split_lo(f, hh, neg1);
+        const int o = (tt & 1) * G::PARB + (tt >> 1) * G::COLB + wofs;
+        *reinterpret_cast<uint32_t *>(thi + o) = pack_h2(hh);
+        *reinterpret_cast<uint32_t *>(tlo + o) = pack_h2(hl);
+    };
+    // 1. recursion over l = c0 .. c0+31 (aA: steps 0..15 of this chunk on
+    // entry, of the next chunk on exit)
+    if (rec) load16(aA, lst);
+    for (int c0 = lst; c0 <= lmax; c0 += ANA_L) {
+        if (rec) {
+            load16(aB, c0 + 16);
+#pragma unroll
+            for (int tt = 0; tt < 16; ++tt) step(c0, tt, aA[tt]);
+            load16(aA, c0 + ANA_L);
+#pragma unroll
+            for (int tt = 16; tt < ANA_L; ++tt) step(c0, tt, aB[tt - 16]);
+        }
         __syncthreads();
         // 2. this wave's output tile over its 256 pairs (skipped before any switches on)
         floatx4 D0 = floatx4{0.f, 0.f, 0.f, 0.f}, D1 = D0;
