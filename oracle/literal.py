@@ -8,6 +8,12 @@ only where the reference is finite: l + m <= 170 for the host norm
 i.e. lmax <= 85.  Output is the reference layout l(l+1)+m
 (``alm_ops.py:4-5``).  The only deviation: ``np.math.factorial`` (removed in
 NumPy 2, ``sht_fallback.py:8``) is ``math.factorial``.
+
+Timing (the CPU baseline): ``norm=alm_norm_safe`` replaces the reference's
+``float(factorial(l+m))``, which raises for l+m >= 171, by an exact integer
+division (same statements and cost, no exception); ``m_stride`` visits every
+k-th m only.  Above lmax 85 the results are not meaningful (P_mm overflows at
+m >= 151, the norm underflows), and only the elapsed time is used.
 """
 import math
 
@@ -21,7 +27,13 @@ def _alm_norm(l, m):
     return np.sqrt((2 * l + 1) / (4 * np.pi) * float(math.factorial(l - m)) / float(math.factorial(l + m)))
 
 
-def sht_fallback_map2alm(rings, thetas, nphi_per_ring, weights_dz, lmax, mmax=None):
+def alm_norm_safe(l, m):
+    """``_alm_norm`` with the factorial ratio as one exact integer division
+    (finite for every l, m; underflows to 0 where the ratio does)."""
+    return np.sqrt((2 * l + 1) / (4 * np.pi) * (math.factorial(l - m) / math.factorial(l + m)))
+
+
+def sht_fallback_map2alm(rings, thetas, nphi_per_ring, weights_dz, lmax, mmax=None, norm=_alm_norm, m_stride=1):
     """``sht_fallback.map2alm_gpu`` (sht_fallback.py:36-65)."""
     thetas = np.asarray(thetas, dtype=np.float64)
     nphi_per_ring = np.asarray(nphi_per_ring, dtype=np.int32)
@@ -37,12 +49,12 @@ def sht_fallback_map2alm(rings, thetas, nphi_per_ring, weights_dz, lmax, mmax=No
         Fm.append([F[m] for m in range(min(mmax, nphi - 1) + 1)])
     alm = np.zeros(((lmax + 1) * (lmax + 1),), dtype=np.complex128)
     x = np.cos(thetas)
-    for m in range(0, mmax + 1):                              # :59-64
+    for m in range(0, mmax + 1, m_stride):                    # :59-64
         Pl = plm_literal(x, lmax, m)
         Fcol = np.asarray([Fm[r][m] if m < len(Fm[r]) else 0.0 + 0.0j for r in range(Nr)])
         for l in range(m, lmax + 1):
             val = np.sum((2 * np.pi) * Fcol * Pl[:, l] * w)
-            alm[l * (l + 1) + m] = _alm_norm(l, m) * val
+            alm[l * (l + 1) + m] = norm(l, m) * val
     return alm
 
 
@@ -75,7 +87,7 @@ def sht_raw_map2alm(rings, thetas, nphi_per_ring, weights_dz, lmax, mmax=None):
     return alm
 
 
-def isht_alm2map(alm, thetas, nphi_per_ring, lmax, mmax=None, dtype=np.float64):
+def isht_alm2map(alm, thetas, nphi_per_ring, lmax, mmax=None, dtype=np.float64, norm=_alm_norm):
     """``isht.alm2map_gpu`` (isht.py:5-26), overwrite semantics and real part.
 
     ``cp.asrealarray`` (isht.py:25) does not exist; the real part is assumed.
@@ -92,7 +104,7 @@ def isht_alm2map(alm, thetas, nphi_per_ring, lmax, mmax=None, dtype=np.float64):
         nphi = int(nphi_per_ring[r])
         F = np.zeros((nphi,), dtype=np.complex128)
         for m in range(0, min(mmax, nphi - 1) + 1):
-            norms = np.asarray([_alm_norm(l, m) for l in range(m, lmax + 1)])
+            norms = np.asarray([norm(l, m) for l in range(m, lmax + 1)])
             coeff = np.sum(norms * blocks[m][r] * np.asarray([alm[l * (l + 1) + m] for l in range(m, lmax + 1)]))
             F[m] = coeff
             if m > 0:
