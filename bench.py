@@ -256,7 +256,9 @@ def cpu_baseline(nside, lmax, seconds):
     """cpu_baseline for the JSON line: the vectorised FP64 oracle on a bounded
     sample at the target (value), with the literal reference fallback's fitted
     extrapolation beside it."""
-    v = cpu_vectorised(nside, lmax, seconds)
+    # 32 m spread over 0..lmax: short rows (large m) cost more per (l, ring)
+    # unit than long ones, so a sample of small m only would flatter the CPU
+    v = cpu_vectorised(nside, lmax, seconds, m_stride=max(1, (lmax + 1) // 32))
     lit = cpu_literal(nside, lmax)
     cores = _cores()
     return {

@@ -174,6 +174,59 @@ def test_eb_separation_gradient(gpu):
     assert worst <= 1e-5
 
 
+def test_eb_separation_gradient_c5_size(gpu):
+    """E/B separation at C5's size (nside 2048, lmax 4095, FP64 forward):
+    L = sum_p w_p |alm2map_spin(E, B)_p - d_p|^2 with the |b| < 20 deg mask,
+    C^EE = 1/(l+1)^2, C^BB = 0.1 C^EE; the autograd gradient (through the
+    spin adjoint kernels) against central differences on 40 coefficients of
+    E and B.  The difference L(a + e) - L(a - e) is summed as
+    sum_p w_p (r_+ - r_-)(r_+ + r_-), the same number as the difference of
+    the two sums in exact arithmetic, so that the 5e7-pixel sums do not
+    cancel at h = 1e-6."""
+    nside, lmax = 2048, 4095
+    print(f"[case] E/B separation FD nside={nside} lmax={lmax}", flush=True)
+    plan = _plan(nside, lmax, spin=2)
+    dev = torch.device("cuda")
+    n = plan.nalm
+    rows = torch.arange(lmax + 1, 0, -1, device=dev)
+    ms = torch.repeat_interleave(torch.arange(lmax + 1, device=dev), rows)
+    ls = torch.arange(n, device=dev) - torch.repeat_interleave(torch.cumsum(rows, 0) - rows, rows) + ms
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(55)
+
+    def draw(amp):
+        sd = amp / (ls.double() + 1.0)
+        sc = torch.where(ms == 0, sd, sd / math.sqrt(2.0))
+        re = torch.randn(n, generator=gen, device=dev, dtype=torch.float64)
+        im = torch.randn(n, generator=gen, device=dev, dtype=torch.float64)
+        return torch.complex(re * sc, torch.where(ms == 0, torch.zeros_like(im), im * sc))
+
+    eb = torch.stack([draw(1.0), draw(math.sqrt(0.1))]).requires_grad_(True)
+    with torch.no_grad():
+        d = plan.alm2map_spin(torch.stack([draw(1.0), draw(math.sqrt(0.1))]))
+    w = torch.as_tensor(_mask(OG.ring_tables(nside)), device=dev)
+    assert w.numel() == plan.npix
+    loss = ((plan.alm2map_spin(eb) - d) ** 2 * w).sum()
+    loss.backward()
+    g = eb.grad.detach()
+    rng = np.random.default_rng(12)
+    h = 1e-6
+    worst = 0.0
+    gscale = float(g.abs().max())
+    with torch.no_grad():
+        for _ in range(40):
+            c, i, part = int(rng.integers(2)), int(rng.integers(n)), int(rng.integers(2))
+            e = torch.zeros_like(eb)
+            e[c, i] = h if part == 0 else 1j * h
+            rp = plan.alm2map_spin(eb + e) - d
+            rm = plan.alm2map_spin(eb - e) - d
+            fd = float((w * (rp - rm) * (rp + rm)).sum()) / (2 * h)
+            an = float(g[c, i].real if part == 0 else g[c, i].imag)
+            worst = max(worst, abs(fd - an) / max(abs(an), 1e-3 * gscale))
+    print(f"worst FD relative error {worst:.2e}", flush=True)
+    assert worst <= 1e-5
+
+
 def test_spin2_transpose_c5_size_f32(gpu):
     """4 Q/U pairs at nside 2048, lmax 4095 (FP32 plan): the spin adjoint is
     the transpose of the spin synthesis to FP32 accuracy."""
