@@ -434,7 +434,7 @@ int hbx_plan_get_tiles(const hbx_plan *plan, hbx_tiles *t) {
     t->mfma_min_maps = mfma_min_maps(&p);
     t->spin_slots = spin_slots_of(&p);
     t->mfma_kind = mfma_kind_of(&p);
-    t->mfma_groups = p.tile_syn_groups > 0 ? p.tile_syn_groups : 1;
+    t->mfma_groups = p.tile_syn_groups;       // 0: chosen per batch
     return HBX_OK;
 }
 

@@ -120,7 +120,7 @@ struct Plan {
     int tile_mfma_min_maps = 0;   // FP32 batches of at least this many maps use the matrix cores
     int tile_spin_slots = 0;      // spin analysis ring-pair slots per lane (1, 2, 4)
     int tile_mfma_kind = 0;       // HBX_MFMA_F32 or HBX_MFMA_F16X2 (default)
-    int tile_syn_groups = 0;      // 16-map groups per split-fp16 synthesis workgroup (1, 2; 0 = 1)
+    int tile_syn_groups = 0;      // 16-map groups per split-fp16 synthesis workgroup (1, 2; 0 = per batch)
 };
 
 // launchers implemented in ring_fft.hip / legendre.hip

@@ -556,8 +556,11 @@ int launch_leg_synthesis_mfma(const Plan *p, int nmaps, const void *alm, int64_t
         hipLaunchKernelGGL(syn_prep_kernel, dim3(p->mmax + 1, (nmaps + 15) / 16), dim3(256), 0, s,
                            (const float2 *)alm, astride, p->d_rofs, p->d_sl, p->d_cofs, p->syn_chunks, p->lmax,
                            p->mmax, nmaps, adjoint, -1.0f, prep, kexp);
-        // two 16-map groups per workgroup when the batch has more than 16 maps
-        if (nmaps > 16 && p->tile_syn_groups == 2) {
+        // two 16-map groups per workgroup share each recursion; automatic
+        // (tile 0): when the batch fills whole pairs of 16-map groups, where
+        // the pair's cost (measured ~1.74 x one group's) is below two groups'
+        const int ng = p->tile_syn_groups > 0 ? p->tile_syn_groups : (((nmaps + 15) / 16) % 2 == 0 ? 2 : 1);
+        if (nmaps > 16 && ng == 2) {
             dim3 grid2(grid.x, (nmaps + 31) / 32);
             hipLaunchKernelGGL(leg_synthesis_h16_kernel<2>, grid2, dim3(SH_WAVES * 64), 0, s, (const float2 *)alm,
                                astride, (float *)phase, p->d_px, p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed,

@@ -60,7 +60,8 @@ template <typename T> inline int64_t pad_len(int64_t M) { return M + (M >> fft_l
 // threads of a ring workgroup of class M (also the ring kernels' stride)
 template <typename T> inline int class_threads(int M) { return std::max(64, M / fft_pts<T>()); }
 
-template <typename T> inline size_t ring_lds_bytes(int M) { return (size_t)pad_len<T>(M) * sizeof(c2<T>); }
+// (one element more than the FFT: the analysis untangle writes X[h] there)
+template <typename T> inline size_t ring_lds_bytes(int M) { return (size_t)(pad_len<T>(M) + 1) * sizeof(c2<T>); }
 
 // a * (-i) forward, a * (+i) inverse
 template <typename T, bool INV> __device__ __forceinline__ c2<T> mul_wi(c2<T> a) {
@@ -448,76 +449,63 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<TP> *_
             if (k < h) buf[pk] = cscale<T>(cmul<T>(x, cconj<T>(Cr[k])), invM);
         });
     }
-    // untangle: Z (period h) -> X[m] = E_m + W_m O_m, E_m = (Z[m] + conj Z[-m]) / 2,
-    // O_m = (Z[m] - conj Z[-m]) / 2i, W_m = e^{-2 pi i m / n}; then
-    // F(m) = scale e^{-i m phi0} X[m mod n] for m < mcut, else 0.
-    const T sc = (T)(scale ? scale[r] : 1.0);
-    const int cut = mcut[r];
-    const double ph0 = phi0 ? phi0[r] : 0.0;
-    const bool rotp = (ph0 != 0.0);
-    c2<TP> *out = phase + (int64_t)b * phase_stride;
-    auto put_out = [&](int m, c2<T> F) { out[((int64_t)(m >> 3) * nr + r) * MB + (m & 7)] = mk<TP>((TP)F.x, (TP)F.y); };
-    double2 W, P = make_double2(1.0, 0.0);
+    // untangle in place: Z (period h) -> X[k] = E_k + W_k O_k for k = 0..h,
+    //   E_k = (Z[k] + conj Z[-k]) / 2,  O_k = (Z[k] - conj Z[-k]) / 2i,
+    //   W_k = e^{-2 pi i k / n};
+    // thread k <= h/2 overwrites the two slots it reads, k and h-k, with X[k]
+    // and X[h-k] = conj(E_k - W_k O_k) (k = 0: X[h] into slot h, one past the
+    // FFT).  The rest of the spectrum is X[n-k] = conj X[k] (real input).
     {
+        double2 W;
         double s, c;
         sincospi(-2.0 * (double)(tid % n) / (double)n, &s, &c);
         W = make_double2(c, s);
-        if (rotp) {
-            sincos(-(double)tid * ph0, &s, &c);
-            P = make_double2(c, s);
-        }
-    }
-    const double2 Wst = rot[3 * nr + r], Pst = zconj(rot[r]);
-    if (cut <= n) {
-        // no alias copies: every m < lim is one of the n bins.  Thread pair k
-        // (0 <= k <= h/2) gives, with T = W_k O_k,
-        //   X[k] = E + T,  X[h-k] = conj(E - T),  X[h+k] = E - T,  X[n-k] = conj(E + T)
-        // (E, O of period h; W_{h+-k} = -W_{-+k}; real input).
-        const int lim = min(cut, mmax + 1);
-        const double2 Ph = zconj(rot[2 * nr + r]), Pn = zconj(rot[nr + r]);
-        auto emit = [&](int m, c2<T> X, double2 Pm) {
-            if (m < lim) {
-                c2<T> F = cscale<T>(X, sc);
-                if (rotp) F = cmul<T>(F, round_c<T>(Pm));
-                put_out(m, F);
-            }
-        };
+        const double2 Wst = rot[3 * nr + r];
         for (int k = tid; 2 * k <= h; k += nt) {
             const c2<T> za = buf[pad_idx<T>(k)];
             const c2<T> zb = cconj<T>(buf[pad_idx<T>(k == 0 ? 0 : h - k)]);
             const c2<T> E = cscale<T>(cadd<T>(za, zb), (T)0.5);
             const c2<T> O = cmul_mi<T>(cscale<T>(csub<T>(za, zb), (T)0.5));
             const c2<T> Tk = cmul<T>(round_c<T>(W), O);
-            const c2<T> Xp = cadd<T>(E, Tk), Xm = csub<T>(E, Tk);
-            emit(k, Xp, P);
-            if (2 * k != h) emit(h - k, cconj<T>(Xm), zmul(Ph, zconj(P)));
-            if (k != 0 && 2 * k != h) emit(h + k, Xm, zmul(Ph, P));
-            if (k != 0) emit(n - k, cconj<T>(Xp), zmul(Pn, zconj(P)));
+            buf[pad_idx<T>(k)] = cadd<T>(E, Tk);
+            if (2 * k != h) buf[pad_idx<T>(h - k)] = cconj<T>(csub<T>(E, Tk));
             W = zmul(W, Wst);
-            if (rotp) P = zmul(P, Pst);
         }
-        for (int m = lim + tid; m <= mmax; m += nt) put_out(m, mk<T>(0, 0));
-    } else {
-        // alias copies (mcut > nphi): one m per step, bin m mod n
-        const int hstep = nt % h;
-        int ia = tid % h;
-        for (int m = tid; m <= mmax; m += nt) {
-            c2<T> F = mk<T>(0, 0);
-            if (m < cut) {
-                const int ib = (ia == 0) ? 0 : h - ia;
-                const c2<T> za = buf[pad_idx<T>(ia)], zb = cconj<T>(buf[pad_idx<T>(ib)]);
-                const c2<T> A = cscale<T>(cadd<T>(za, zb), (T)0.5);
-                const c2<T> D = cscale<T>(csub<T>(za, zb), (T)0.5);
-                const c2<T> X = cadd<T>(A, cmul<T>(round_c<T>(W), cmul_mi<T>(D)));
-                F = cscale<T>(X, sc);
-                if (rotp) F = cmul<T>(F, round_c<T>(P));
-            }
-            put_out(m, F);
-            W = zmul(W, Wst);
-            if (rotp) P = zmul(P, Pst);
-            ia += hstep;
-            if (ia >= h) ia -= h;
+    }
+    __syncthreads();
+    // F(m) = [m < mcut] scale e^{-i m phi0} X[m mod n], m = 0..mmax: thread
+    // tid takes m = tid + j nt (nt a multiple of 8, so its stores step by a
+    // fixed number of m-blocks; lanes fill 64-byte m-blocks), the rotation
+    // times the scale by FP64 steps e^{-i nt phi0} from one sincos, rounded
+    // once per output.
+    const double scd = scale ? scale[r] : 1.0;
+    const int cut = mcut[r];
+    const double ph0 = phi0 ? phi0[r] : 0.0;
+    const bool rotp = (ph0 != 0.0);
+    const T sc = (T)scd;
+    c2<TP> *o = phase + (int64_t)b * phase_stride + ((int64_t)(tid >> 3) * nr + r) * MB + (tid & 7);
+    const int64_t ostep = (int64_t)(nt >> 3) * nr * MB;
+    double2 P = make_double2(scd, 0.0);
+    if (rotp) {
+        double s, c;
+        sincos(-(double)tid * ph0, &s, &c);
+        P = make_double2(scd * c, scd * s);
+    }
+    const double2 Pst = zconj(rot[r]);                  // e^{-i nt phi0}
+    const int mstep = nt % n;
+    int mm = tid % n;                                   // m mod n
+    for (int m = tid; m <= mmax; m += nt, o += ostep) {
+        c2<T> F = mk<T>(0, 0);
+        if (m < cut) {
+            const bool low = mm <= h;
+            c2<T> X = buf[pad_idx<T>(low ? mm : n - mm)];
+            if (!low) X.y = -X.y;
+            F = rotp ? cmul<T>(X, round_c<T>(P)) : cscale<T>(X, sc);
         }
+        *o = mk<TP>((TP)F.x, (TP)F.y);
+        if (rotp) P = zmul(P, Pst);
+        mm += mstep;
+        if (mm >= n) mm -= n;
     }
 }
 
