@@ -10,7 +10,7 @@ HIP events recorded around it on its own stream over the timed steps), the
 ring stage's HBM rate and the CPU oracle timed on a bounded sample.  Prints
 ONE JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B (32)]
     python bench.py --spin      (spin-2: 4 Q/U pairs FP32 vs 1 pair FP64)
     python bench.py --mixed     (64 maps of mixed nside in one HIP graph)
 """
@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--nside", type=int, default=2048)
     ap.add_argument("--lmax", type=int, default=None)
     ap.add_argument("--fp64-steps", type=int, default=3)

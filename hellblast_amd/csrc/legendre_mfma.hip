@@ -1071,6 +1071,274 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
 }
 
 // ---------------------------------------------------------------------------
+// Split-fp16 analysis with two 16-map groups per workgroup (32 maps; tile
+// mfma_groups = 2, not the automatic choice): the
+// contraction of leg_analysis_h16_kernel<2> (512 ring pairs of one m, two
+// K-halves of 256) for both groups, 16 waves = (map group u, n-tile t,
+// parity par, K-half kh), sharing one lambda tile per 32-l chunk, so every
+// FP64 recursion step feeds twice the matrix work.  The recursion runs on
+// the 8 waves of group 0 (two per SIMD), one ring pair per lane; adjacent
+// lanes trade their fp16 (hi, lo) halves with one DPP move, and the even lane
+// stores the two pairs' hi halves, the odd lane their lo halves, as one
+// dword each (the tile layout of leg_analysis_h16_kernel<2>).  The A
+// operands of each wave are loaded and scaled as there (the north/south
+// exchange in four rounds of 16 K-steps, so that it fits in the idle tiles).
+// ---------------------------------------------------------------------------
+struct H16x2Geom {
+    static constexpr int W = 16;                  // waves per workgroup
+    static constexpr int P = 512;                 // ring pairs per workgroup
+    static constexpr int RW = 8;                  // recursion waves (64 ring pairs each)
+    static constexpr int COLB = H16Geom<2>::COLB, PARB = H16Geom<2>::PARB, TILEB = H16Geom<2>::TILEB;
+    static constexpr int XCH = 8 * 256;           // floats: the K-half 1 tiles of the 8 (u, t, par)
+    static constexpr size_t LDS = 2 * (size_t)TILEB + XCH * sizeof(float) + 4 * sizeof(int);
+};
+static_assert((size_t)H16x2Geom::W * 16 * 64 * sizeof(float) <= 2 * (size_t)H16x2Geom::TILEB,
+              "the A-operand exchange fits in the lambda tiles");
+
+__global__ void __launch_bounds__(H16x2Geom::W * 64) __attribute__((amdgpu_waves_per_eu(4)))
+leg_analysis_h16x2_kernel(const float2 *__restrict__ phase, int64_t pstride, float2 *__restrict__ alm,
+                          int64_t astride, float2 *__restrict__ partial, const double *__restrict__ px,
+                          const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
+                          const double *__restrict__ sl, const int32_t *__restrict__ lseed,
+                          const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
+                          int64_t nlm, int nmaps, int adjoint, int mbeg, float neg1, int32_t *__restrict__ lfirst) {
+    using G = H16x2Geom;
+    extern __shared__ __align__(16) unsigned char hsm[];
+    unsigned char *thi = hsm, *tlo = hsm + G::TILEB;
+    float *xch = reinterpret_cast<float *>(hsm + 2 * G::TILEB);
+    int *starts = reinterpret_cast<int *>(xch + G::XCH);          // [0]: workgroup, [1 + kh]: K-half
+
+    int mb, item;
+    xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
+    // the 8 m of one m-block (sharing 64-byte phase lines) run next to each other
+    const int m = mb * MB + item % MB;
+    const int grp = item / MB;
+    if (m > mmax || m < mbeg) return;
+    const int g = blockIdx.y;                         // group of 32 maps
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int t = w & 1, par = (w >> 1) & 1, kh = (w >> 2) & 1, u = w >> 3;
+    const int P0 = grp * G::P;
+    const bool rec = (w < G::RW);
+
+    // recursion: pair P0 + 64 w + lane (recursion waves only)
+    double x = 0.0, u1 = 0.0, u2 = 0.0;
+    double2 sd = make_double2(0.0, 0.0);
+    int ls = NEVER_L;
+    if (rec) {
+        const int pr = P0 + 64 * w + lane;
+        if (pr < npairs) {
+            x = px[pr];
+            ls = lseed[(int64_t)m * npairs + pr];
+            if (ls <= lmax) sd = seed[(int64_t)m * npairs + pr];
+        }
+    }
+    const int wstart = wmin64(ls);
+    const int winj = wmax64(ls <= lmax ? ls : -1);
+
+    // A operands: S_par[pair P0 + 256 kh + 32 s + 8 q + i][n = 16 t + row] of map group u
+    half8 Ahi[8], Alo[8];
+    int kr[4];                                         // exponent k of the output rows 4 q + j
+    {
+        const int row = lane & 15, q = lane >> 4;
+        const int n = 16 * t + row;
+        const int b = g * 32 + 16 * u + (n >> 1), c = n & 1;
+        const float *ph = reinterpret_cast<const float *>(phase + (int64_t)min(b, nmaps - 1) * pstride +
+                                                          (int64_t)mb * nr * MB + (m & 7)) + c;
+        auto own = [&](int s, int i) -> float {
+            const int qp = P0 + 256 * kh + 32 * s + 8 * q + i;
+            const int qq = min(qp, npairs - 1);
+            const bool ok = qp < npairs && b < nmaps && (par == 0 || nr - 1 - qq != qq);
+            return ok ? ph[(int64_t)(par == 0 ? qq : nr - 1 - qq) * (2 * MB)] : 0.0f;
+        };
+        float *xs = reinterpret_cast<float *>(hsm);     // [W][16][64] per round of 2 K-steps
+        float Sf[64];
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) Sf[8 * s + i] = own(s, i);
+        float amax = 0.0f;
+#pragma unroll
+        for (int rd = 0; rd < 4; ++rd) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) xs[(w * 16 + j) * 64 + lane] = Sf[16 * rd + j];
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const float mine = Sf[16 * rd + j], other = xs[((w ^ 2) * 16 + j) * 64 + lane];
+                const float a = par == 0 ? mine : other, z = par == 0 ? other : mine;
+                Sf[16 * rd + j] = par ? a - z : a + z;
+                amax = fmaxf(amax, fabsf(Sf[16 * rd + j]));
+            }
+            __syncthreads();
+        }
+        // one exponent per map and (map group, n-tile, parity): the two K-halves
+        // add their products before unscaling
+        amax = fmaxf(amax, __shfl_xor(amax, 1, 64));
+        amax = fmaxf(amax, __shfl_xor(amax, 16, 64));
+        amax = fmaxf(amax, __shfl_xor(amax, 32, 64));
+        float *am = xch;                                // [W][16], idle until the chunk loop
+        if (lane < 16) am[w * 16 + lane] = amax;
+        __syncthreads();
+        amax = fmaxf(am[(w & ~4) * 16 + row], am[(w | 4) * 16 + row]);
+        // map maximum scaled to [2^13, 2^14), applied in two exact power-of-two steps
+        int e = 0;
+        (void)frexpf(amax, &e);
+        const int k = amax > 0.0f ? 14 - e : 0;
+        const int k1 = k >> 1;
+        const float sc1 = ldexpf(1.0f, k1), sc2 = ldexpf(1.0f, k - k1);
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+#pragma unroll
+            for (int i = 0; i < 8; i += 2) {
+                const float2v v = {(Sf[8 * s + i] * sc1) * sc2, (Sf[8 * s + i + 1] * sc1) * sc2};
+                const half2v hv = __builtin_convertvector(v, half2v);
+                const half2v lv = split_lo(v, hv, neg1);
+                Ahi[s][i] = hv[0];
+                Ahi[s][i + 1] = hv[1];
+                Alo[s][i] = lv[0];
+                Alo[s][i + 1] = lv[1];
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) kr[j] = __shfl(k, 4 * q + j, 64);
+    }
+    if (tid < 3) starts[tid] = NEVER_L;
+    __syncthreads();
+    if (rec && lane == 0) {
+        atomicMin(&starts[0], wstart);
+        atomicMin(&starts[1 + (w >> 2)], wstart);     // recursion waves 4j .. 4j+3 cover K-half j
+    }
+    __syncthreads();
+    const int lstart = starts[0];
+    const int kstart = starts[1 + kh];
+
+    const bool direct = (ngroups == 1);
+    const int64_t i0 = alm_index(m, m, lmax) - m;       // + l
+    const int64_t ro = rofs[m];
+    const double *al = alpha + ro - m;                  // zero-padded past lmax
+    const double *sv = sl + ro - m;
+    // v: (re, im) of one map scaled by 2^k (the map's k); unscaled in FP64
+    auto store = [&](int l, int b, float2 v, int k) {
+        if (b >= nmaps) return;
+        if (direct) {
+            double f = ldexp(sv[l], -k);
+            double re = (double)v.x * f, im = (double)v.y * f;
+            if (adjoint) {
+                if (m > 0) { re *= 2.0; im *= 2.0; }
+                else im = 0.0;
+            }
+            alm[(int64_t)b * astride + i0 + l] = make_float2((float)re, (float)im);
+        } else {
+            partial[((int64_t)b * ngroups + grp) * nlm + i0 + l] =
+                make_float2((float)ldexp((double)v.x, -k), (float)ldexp((double)v.y, -k));
+        }
+    };
+    // coefficients below every ring pair's switch-on are zero: written to alm
+    // by a single group; partial sums record where theirs start instead
+    const int lz = lstart > lmax ? lmax + 1 : lstart;
+    if (direct)
+        for (int e = tid; e < (lz - m) * 32; e += G::W * 64)
+            store(m + (e >> 5), g * 32 + (e & 31), make_float2(0.f, 0.f), 0);
+    else if (g == 0 && tid == 0)
+        lfirst[(int64_t)m * ngroups + grp] = lz;
+    if (lstart > lmax) return;
+
+    // this lane's recursion store: the dword of its lane pair in a column, in
+    // the hi tile (even lanes) or the lo tile (odd lanes)
+    unsigned char *wbase = ((lane & 1) ? tlo : thi) + 128 * w + 4 * (lane >> 1);
+    // v_perm_b32 selectors, sources (other, mine): even lanes take the two hi
+    // halves (mine, other), odd lanes the two lo halves (other, mine)
+    const uint32_t psel = (lane & 1) ? 0x03020706u : 0x05040100u;
+    // this lane's B reads: column lane%16, halves 256 kh + 32 s + 8 q
+    const int rofs16 = par * G::PARB + (lane & 15) * G::COLB + 2 * (256 * kh + 8 * (lane >> 4));
+    // alpha_l is wave-uniform: two banks of 16 scalar registers, each loaded
+    // 16 steps before its first use
+    const int lst = __builtin_amdgcn_readfirstlane(lstart);
+    double aA[16], aB[16];
+    auto load16 = [&](double *dst, int l0) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) dst[j] = al[l0 + j];
+    };
+    // two recursion steps of this lane's pair at l = c0 + tt, c0 + tt + 1 (tt
+    // even) into the hi/lo tiles; switch-on seeds are injected at l = m + 8 j
+    auto step2 = [&](int c0, int tt, double a0, double a1) {
+        if ((tt & 7) == 0 && c0 <= winj && ls == c0 + tt) {
+            u2 = sd.x;
+            u1 = sd.y;
+        }
+        float2v f;
+        {
+            const double v = fma(a0 * x, u1, -u2);
+            u2 = u1;
+            u1 = v;
+            f[0] = (float)v;
+        }
+        {
+            const double v = fma(a1 * x, u1, -u2);
+            u2 = u1;
+            u1 = v;
+            f[1] = (float)v;
+        }
+        const half2v hh = __builtin_convertvector(f, half2v);
+        const half2v hl = split_lo(f, hh, neg1);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const uint32_t mine = pack_h2((half2v){hh[j], hl[j]});
+            const uint32_t other = (uint32_t)__builtin_amdgcn_mov_dpp((int)mine, 0xB1, 0xF, 0xF, false);
+            const uint32_t v = __builtin_amdgcn_perm(other, mine, psel);
+            const int o = j * G::PARB + (tt >> 1) * G::COLB;      // l = c0 + tt + j: parity j
+            *reinterpret_cast<uint32_t *>(wbase + o) = v;
+        }
+    };
+    // 1. recursion over l = c0 .. c0+31 (aA: steps 0..15 of this chunk on
+    // entry, of the next chunk on exit)
+    if (rec) load16(aA, lst);
+    for (int c0 = lst; c0 <= lmax; c0 += ANA_L) {
+        if (rec) {
+            load16(aB, c0 + 16);
+#pragma unroll
+            for (int tt = 0; tt < 16; tt += 2) step2(c0, tt, aA[tt], aA[tt + 1]);
+            load16(aA, c0 + ANA_L);
+#pragma unroll
+            for (int tt = 16; tt < ANA_L; tt += 2) step2(c0, tt, aB[tt - 16], aB[tt - 15]);
+        }
+        __syncthreads();
+        // 2. this wave's output tile over its 256 pairs (skipped before any switches on)
+        floatx4 D0 = floatx4{0.f, 0.f, 0.f, 0.f}, D1 = D0;
+        if (c0 + ANA_L > kstart) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const half8 bh = *reinterpret_cast<const half8 *>(thi + rofs16 + 64 * s);
+                const half8 bl = *reinterpret_cast<const half8 *>(tlo + rofs16 + 64 * s);
+                D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[s], bh, D0, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[s], bl, D1, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[s], bh, D1, 0, 0, 0);
+            }
+        }
+        floatx4 D = D0 + D1;
+        // 3. K-half 1 hands its tile to K-half 0 through LDS (same row scales)
+        const int xo = ((u * 2 + t) * 2 + par) * 256;
+        if (kh == 1) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) xch[xo + r * 64 + lane] = D[r];
+        }
+        __syncthreads();
+        if (kh == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) D[r] += xch[xo + r * 64 + lane];
+            // D[row][col]: lane = col + 16 (row/4), reg = row%4; row = n - 16 t, col -> l
+            // (rows 4q, 4q+1 are re, im of one map and share its scale)
+            const int l = c0 + 2 * (lane & 15) + par;
+            if (l <= lmax) {
+                const int b0 = g * 32 + 16 * u + 8 * t + 2 * (lane >> 4);
+                store(l, b0, make_float2(D[0], D[1]), kr[0]);
+                store(l, b0 + 1, make_float2(D[2], D[3]), kr[2]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Pipelined matrix-core analysis (12-wave workgroups): the contraction of
 // leg_analysis_mfma_kernel<2> -- 512 ring pairs of one m per workgroup, eight
 // MFMA waves = 2 n-tiles x 2 parities x 2 K-halves of 256 pairs -- plus four
@@ -1503,7 +1771,18 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
                        (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,       \
                        p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,     \
                        p->mmax, G, p->nlm, nmaps, adjoint, mbeg, -1.0f, lfirst)
-        switch (W) {
+        // two 16-map groups per workgroup (8-wave geometry) only on request
+        // (tile mfma_groups = 2): at nside 2048 its single 16-wave workgroup
+        // per CU alternates recursion and matrix phases with nothing to
+        // overlap them (22.9 ms for 32 maps against 20.6 ms as two 16-map
+        // groups), so the automatic choice keeps one group
+        if (W == 8 && nmaps > 16 && p->tile_syn_groups == 2) {
+            dim3 grid2(grid.x, (nmaps + 31) / 32);
+            hipLaunchKernelGGL(leg_analysis_h16x2_kernel, grid2, dim3(H16x2Geom::W * 64), H16x2Geom::LDS, s,
+                               (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,
+                               p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings,
+                               p->lmax, p->mmax, G, p->nlm, nmaps, adjoint, mbeg, -1.0f, lfirst);
+        } else switch (W) {
         case 4: HBX_AH(1); break;
         case 8: HBX_AH(2); break;
         case 16: HBX_AH(4); break;
@@ -1547,6 +1826,9 @@ int prepare_mfma_kernels() {
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_h16_kernel<4>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)H16Geom<4>::LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_h16x2_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)H16x2Geom::LDS);
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_mfma_pipe_kernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)PipeGeom::LDS);

@@ -125,6 +125,24 @@ def test_synthesis_map_groups_agree(gpu):
     assert torch.equal(m1, m2)
 
 
+@pytest.mark.parametrize("nside,lmax,n", [(64, 128, 40), (512, 1023, 32), (1024, 2047, 48)])
+def test_analysis_map_groups_agree(gpu, nside, lmax, n):
+    """Two 16-map groups per split-fp16 analysis workgroup (16 waves, one
+    lambda tile shared by 32 maps) compute the same alm, bit for bit, as one
+    group per workgroup: nside 64 with a partial map group (40 maps, one
+    ring-pair group), nside 512 (two ring-pair groups of 512 pairs: partial
+    sums), nside 1024 (four ring-pair groups, 48 maps: a partial map group)."""
+    print(f"[case] analysis map groups nside={nside} n={n}", flush=True)
+    plan = _plan(nside, lmax, "f32")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(14 + nside)
+    maps = torch.randn((n, plan.npix), generator=gen, device="cuda")
+    a1 = plan.with_tiles(mfma_groups=1).map2alm(maps)
+    a2 = plan.with_tiles(mfma_groups=2).map2alm(maps)
+    assert torch.equal(a1, a2)
+    assert torch.equal(plan.map2alm(maps), a1)           # the automatic choice
+
+
 def _c3_specs(rng):
     """64 maps, nside allocation {2048:1, 1024:2, 512:5, 256:14, 128:42}, shuffled."""
     alloc = {2048: 1, 1024: 2, 512: 5, 256: 14, 128: 42}
