@@ -518,25 +518,42 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
         __syncthreads();
     }
 
-    // D[row = ring][col = n]: this lane holds col = r, rows (q%4) + 8 (q/4) + 4 h.
-    // Ring pair p couples ring p (north) and ring nr-1-p (south).
+    // D[row = ring][col = n]: this lane holds col = r (map r/2, re or im as
+    // r is even or odd), rows (q%4) + 8 (q/4) + 4 h.  Ring pair p couples
+    // ring p (north) and ring nr-1-p (south).  Adjacent lanes (re, im of one
+    // map) trade one value per two rows, so that the even lane stores the
+    // complex value of row q and the odd lane that of row q+1, 8 bytes each.
+    const bool odd = (r & 1) != 0;
 #pragma unroll
     for (int u = 0; u < NG; ++u) {
-        const int b = g0 + 16 * u + (r >> 1), c = r & 1;
-        if (b >= nmaps) continue;
-        float *out = phase + (((int64_t)b * nmb + mb) * nr) * (2 * MB) + (m & 7) * 2 + c;
+        const int b = g0 + 16 * u + (r >> 1);
+        const bool live = b < nmaps;
+        float2 *out = reinterpret_cast<float2 *>(phase) + (((int64_t)min(b, nmaps - 1) * nmb + mb) * nr) * MB + (m & 7);
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const int base = pbase + 32 * j;
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int row = (q & 3) + 8 * (q >> 2) + 4 * h;
+            for (int q = 0; q < 16; q += 2) {
+                // north (ge + go) and south (ge - go) values of rows q, q+1
+                float vn[2], vs[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const float ge = De[j][u][q + i], go = Do[j][u][q + i];
+                    vn[i] = ((ge + go) * usc1[u]) * usc2[u];
+                    vs[i] = ((ge - go) * usc1[u]) * usc2[u];
+                }
+                // even lane keeps row q and sends row q+1; odd keeps q+1, sends q
+                const float sn = odd ? vn[0] : vn[1], ss = odd ? vs[0] : vs[1];
+                const float rn_ = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(sn), 0xB1, 0xF, 0xF, false));
+                const float rs_ = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(ss), 0xB1, 0xF, 0xF, false));
+                const float2 cn = odd ? make_float2(rn_, vn[1]) : make_float2(vn[0], rn_);
+                const float2 cs = odd ? make_float2(rs_, vs[1]) : make_float2(vs[0], rs_);
+                const int row = ((q + (odd ? 1 : 0)) & 3) + 8 * (q >> 2) + 4 * h;
                 const int pp = base + row;
-                if (pp >= npairs) continue;
-                const float ge = De[j][u][q], go = Do[j][u][q];
+                if (!live || pp >= npairs) continue;
                 const int rn = pp, rs = nr - 1 - pp;
-                out[(int64_t)rn * (2 * MB)] = ((ge + go) * usc1[u]) * usc2[u];
-                if (rs != rn) out[(int64_t)rs * (2 * MB)] = ((ge - go) * usc1[u]) * usc2[u];
+                out[(int64_t)rn * MB] = cn;
+                if (rs != rn) out[(int64_t)rs * MB] = cs;
             }
         }
     }
