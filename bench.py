@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-full", action="store_true",
                     help="time the CPU oracle at C2 (whole pair) and the target (every 16th m); no GPU")
+    ap.add_argument("--configs", action="store_true",
+                    help="C1 (nside 64, lmax 128) and C2 (nside 1024, lmax 2047) single-map figures")
     ap.add_argument("--spin", action="store_true",
                     help="spin-2 workload: 4 Q/U pairs FP32 vs 1 pair FP64 at nside 2048, lmax 4095")
     ap.add_argument("--mixed", action="store_true",
@@ -349,6 +351,98 @@ def cpu_full_main(args):
                       "literal_reference_fallback": lit}), flush=True)
 
 
+def configs_main(args):
+    """--configs: throughput at C1 (nside 64, lmax 128, one FP64 map: the
+    reference's CPU-runnable case) and C2 (nside 1024, lmax 2047, one map,
+    FP64 and FP32), with C2's FP32-vs-FP64 errors and the CPU baselines at C1
+    (the literal reference fallback and the vectorised oracle, whole pairs).
+    The accuracy against the oracle is checked by tests/ (C1:
+    test_config1_roundtrip, C2: test_config2_single_map_f32_vs_f64)."""
+    import torch
+    from hellblast_amd.hp_compat import _plan
+    from hellblast_amd.plan import alm2cl, nalm
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    out = {"configs": True}
+
+    def pair_ms(plan, maps, alm, dtype, reps):
+        om = torch.empty_like(maps)
+        oa = torch.empty_like(alm)
+        def step():
+            plan.map2alm(maps, out=oa)
+            plan.alm2map(alm, dtype=dtype, out=om)
+        return time_steps(torch, step, reps, 2)
+
+    # C1
+    nside, lmax = 64, 128
+    p64 = _plan(nside, lmax, lmax, "f64", "dz", "reference", "reference")
+    alm, l_t = synth_alm(torch, lmax, 1, 1, torch.complex128, dev)
+    alm[:, 0] = 0
+    maps = p64.alm2map(alm)
+    ms = pair_ms(p64, maps, alm, torch.float64, 50)
+    rt = p64.map2alm(maps)
+    emax, erms = eps_metrics(torch, rt[0], alm[0], l_t, lmax)
+    out["C1"] = {"nside": nside, "lmax": lmax, "fp64_pairs_per_s": round(1000.0 / ms, 1), "ms_per_pair": round(ms, 4),
+                 "round_trip_eps_max": emax, "round_trip_eps_rms": erms}
+    log(f"C1: {1000.0 / ms:.1f} FP64 pairs/s, round trip eps_max {emax:.2e}")
+    if not args.no_cpu:
+        from oracle import geometry as OG
+        from oracle import literal as OL
+        from oracle import sht as OS
+        t = OG.ring_tables(nside)
+        f = maps[0].cpu().numpy()
+        rings = [f[t["ring_start"][r]:t["ring_start"][r + 1]] for r in range(t["Nr"])]
+        a_np = alm[0].cpu().numpy()
+        old = np.seterr(all="ignore")
+        t0 = time.perf_counter()
+        OL.sht_fallback_map2alm(rings, t["theta_centers"], t["nphi"], t["weights_dz"], lmax, norm=OL.alm_norm_safe)
+        OL.isht_alm2map(OS.mmajor_to_ref(a_np, lmax), t["theta_centers"], t["nphi"], lmax, norm=OL.alm_norm_safe)
+        t_lit = time.perf_counter() - t0
+        np.seterr(**old)
+        t0 = time.perf_counter()
+        OS.alm2map(OS.map2alm(f, t, lmax), t, lmax)
+        t_vec = time.perf_counter() - t0
+        out["C1"]["cpu"] = {"literal_fallback_s_per_pair": round(t_lit, 3), "vectorised_oracle_s_per_pair": round(t_vec, 3),
+                            **_cores(), "note": "whole pairs; literal: sht_fallback + isht restated, one thread; "
+                                                "vectorised: BLAS on all cores"}
+        log(f"C1 CPU: literal {t_lit:.2f} s, vectorised {t_vec:.2f} s per pair")
+    # C2
+    nside, lmax = 1024, 2047
+    p64 = _plan(nside, lmax, lmax, "f64", "dz", "reference", "reference")
+    p32 = _plan(nside, lmax, lmax, "f32", "dz", "reference", "reference")
+    n = nalm(lmax)
+    ls = np.concatenate([np.arange(m, lmax + 1) for m in range(lmax + 1)])
+    l_t = torch.as_tensor(ls, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(2)
+    lf = l_t.double()
+    sd = torch.where(l_t >= 2, 1.0 / torch.sqrt((lf * (lf + 1.0)).clamp(min=1.0)), torch.zeros_like(lf))
+    m0 = torch.as_tensor(np.concatenate([np.full(lmax + 1 - m, m) for m in range(lmax + 1)]) == 0, device=dev)
+    s_ = torch.where(m0, sd, sd / math.sqrt(2.0))
+    re = torch.randn((1, n), generator=g, device=dev, dtype=torch.float64)
+    im = torch.randn((1, n), generator=g, device=dev, dtype=torch.float64)
+    alm = torch.complex(re * s_, torch.where(m0, torch.zeros_like(im), im * s_))
+    maps = p64.alm2map(alm) + 0.01 * torch.randn((1, p64.npix), generator=g, device=dev, dtype=torch.float64)
+    ms64 = pair_ms(p64, maps, alm, torch.float64, 10)
+    ms32 = pair_ms(p32, maps.float(), alm.to(torch.complex64), torch.float32, 10)
+    a64 = p64.map2alm(maps)[0]
+    a32 = p32.map2alm(maps)[0]
+    emax, erms = eps_metrics(torch, a32, a64, l_t, lmax)
+    c64, c32 = alm2cl(a64, lmax), alm2cl(a32.to(torch.complex128), lmax)
+    ecl = float(((c32[2:] - c64[2:]).abs() / c64[2:]).max())
+    m64 = p64.alm2map(a64)
+    m32 = p32.alm2map(a32, dtype=torch.float64)
+    emap = float((m32 - m64).abs().max() / m64.abs().max())
+    rt = p64.map2alm(p64.alm2map(alm))
+    rt_max, rt_rms = eps_metrics(torch, rt[0], alm[0], l_t, lmax)
+    out["C2"] = {"nside": nside, "lmax": lmax, "fp64_pairs_per_s": round(1000.0 / ms64, 2),
+                 "fp32_single_map_pairs_per_s": round(1000.0 / ms32, 2), "fp32_over_fp64": round(ms64 / ms32, 3),
+                 "eps_max_f32_vs_f64": emax, "eps_rms_f32_vs_f64": erms, "cl_max_rel_f32_vs_f64": ecl,
+                 "map_rel_f32_vs_f64": emap, "fp64_round_trip_eps_max": rt_max, "fp64_round_trip_eps_rms": rt_rms}
+    log(f"C2: FP64 {1000.0 / ms64:.2f} pairs/s, FP32 {1000.0 / ms32:.2f} pairs/s, eps_max {emax:.2e}")
+    print(json.dumps(out), flush=True)
+
+
 def mixed_main(args):
     """64 FP32 maps, nside {2048:1, 1024:2, 512:5, 256:14, 128:42} (seeded
     shuffle), lmax = 2 nside, alm ~ C_l = (l+10)^-2: map2alm + alm2map of the
@@ -530,6 +624,8 @@ def main():
         sys.exit(2)
     if args.cpu_full:
         return cpu_full_main(args)
+    if args.configs:
+        return configs_main(args)
     if args.mixed:
         return mixed_main(args)
     if args.spin:

@@ -173,10 +173,14 @@ def test_config1_roundtrip(gpu):
     alm[0] = 0.0
     tabs = OG.ring_tables(nside)
     m = hp_compat.alm2map(torch.as_tensor(alm, device="cuda"), nside, lmax=lmax)
-    assert _rel(m.cpu().numpy(), OS.alm2map(alm, tabs, lmax)) <= F64_TOL
+    emap = _rel(m.cpu().numpy(), OS.alm2map(alm, tabs, lmax))
     a2 = hp_compat.map2alm(m, lmax=lmax)
     ref2 = OS.map2alm(m.cpu().numpy(), tabs, lmax)
-    assert _rel(a2.cpu().numpy(), ref2) <= F64_TOL
+    ealm = _rel(a2.cpu().numpy(), ref2)
+    rt_max, rt_rms = OS.alm_errors(a2.cpu().numpy(), alm, lmax)
+    print(f"C1: GPU vs oracle: alm2map rel {emap:.2e}, map2alm rel {ealm:.2e}; "
+          f"round trip alm -> map -> alm (iter 0): eps_max {rt_max:.3e} eps_rms {rt_rms:.3e}", flush=True)
+    assert emap <= F64_TOL and ealm <= F64_TOL
 
 
 @pytest.mark.parametrize("nside,lmax", [(32, 64), (64, 128), (128, 256)])
