@@ -21,9 +21,13 @@ def test_flags_parse():
     old = sys.argv
     try:
         for argv in (["bench.py"], ["bench.py", "--spin", "--steps", "3"], ["bench.py", "--mixed", "--warmup", "0"],
-                     ["bench.py", "--batch", "8", "--nside", "512", "--no-cpu", "--copy-calibration"]):
+                     ["bench.py", "--batch", "8", "--nside", "512", "--no-cpu", "--copy-calibration"],
+                     ["bench.py", "--configs", "--no-cpu"], ["bench.py", "--cpu-full"]):
             sys.argv = argv
             a = bench.parse()
             assert a.gpus == 1 and a.steps >= 1
+        sys.argv = ["bench.py"]
+        assert bench.parse().batch == 32
     finally:
         sys.argv = old
+
