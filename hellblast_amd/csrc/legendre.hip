@@ -239,7 +239,11 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
                     const double *__restrict__ sl, const int32_t *__restrict__ lseed,
                     const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
                     int64_t nlm, int adjoint, int mbeg) {
-    __shared__ T red[2 * CHUNK * 64];
+    // transpose buffer: row v (one of the 2 CHUNK sums) of the 64 lanes, rows
+    // 65 elements apart so that the reads below (4 lanes per row, 16
+    // consecutive elements each) fall on distinct banks
+    constexpr int RED_ROW = 65;
+    __shared__ T red[2 * CHUNK * RED_ROW];
     int mb, item;
     xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
     const int m = mb * MB + item / ngroups;
@@ -331,13 +335,13 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
         }
         // wave-level transpose-reduce of the 16 values
 #pragma unroll
-        for (int v = 0; v < 2 * CHUNK; ++v) red[v * 64 + lane] = acc[v];
+        for (int v = 0; v < 2 * CHUNK; ++v) red[v * RED_ROW + lane] = acc[v];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         T s = (T)0;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) s += red[v_id * 64 + q0 + q];
+        for (int q = 0; q < 16; ++q) s += red[v_id * RED_ROW + q0 + q];
         s += __shfl_xor(s, 1, 64);
         s += __shfl_xor(s, 2, 64);
         if ((lane & 3) == 0) {
