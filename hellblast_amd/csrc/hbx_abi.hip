@@ -54,7 +54,8 @@ void timer_mark(int kernel, bool begin, hipStream_t s) {
     }
 }
 
-int64_t phase_elems_per_map(const Plan *p) { return (int64_t)p->nmb * p->nrings * MB; }
+// blocks of the larger of the two layouts (2 pairs8(npairs) >= nrings)
+int64_t phase_elems_per_map(const Plan *p) { return (int64_t)p->nmb * syn_block_elems(p->npairs); }
 int64_t scaled_elems_per_map(const Plan *p) { return p->nrow; }
 
 static uint64_t align256(uint64_t v) { return (v + 255) & ~uint64_t(255); }

@@ -41,6 +41,23 @@ constexpr int TAILPAD = 128;   // zero coefficients after the last recursion row
 constexpr int M_LO = 8;
 constexpr int NMB_LO = M_LO / MB;
 
+// Phase buffer layouts.  Analysis direction (ring analysis writes, Legendre
+// analysis reads): [map][m/8][ring][m%8], m-blocks nr * MB elements apart;
+// one ring workgroup writes 8 m of its ring per 64-byte line.  Synthesis
+// direction (Legendre synthesis writes, ring synthesis reads):
+// [map][m/8][half][pair/8][m%8][pair%8], half 0 holding ring p and half 1
+// ring nr-1-p of ring pair p (half 1 has no entry for the ring pair of one
+// ring); one Legendre workgroup (one m, 64 or more ring pairs) writes 8 ring
+// pairs of one m per 64-byte line.  Maps lie phase_elems_per_map() apart in
+// both directions.
+__host__ __device__ inline int pairs8(int npairs) { return (npairs + 7) & ~7; }
+__host__ __device__ inline int64_t syn_block_elems(int npairs) { return (int64_t)2 * pairs8(npairs) * MB; }
+// offset of ring pair p, half (0: ring p, 1: ring nr-1-p), m in one map
+__host__ __device__ inline int64_t syn_ofs(int m, int half, int p, int npairs) {
+    return (int64_t)(m >> 3) * syn_block_elems(npairs) + (int64_t)half * pairs8(npairs) * MB +
+           (int64_t)(p >> 3) * (8 * MB) + (m & 7) * 8 + (p & 7);
+}
+
 inline int64_t nalm(int lmax, int mmax) {
     return (int64_t)(mmax + 1) * (lmax + 1) - (int64_t)mmax * (mmax + 1) / 2;
 }

@@ -472,10 +472,10 @@ leg_synthesis_kernel(const c2<T> *__restrict__ scaled, int64_t nrow, c2<T> *__re
         }
     }
     if (!valid) return;
-    c2<T> *out = phase + (int64_t)b * pstride + (int64_t)mb * nr * MB + w;
-    const int rn = pair_n[p], rs = pair_s[p];
-    out[(int64_t)rn * MB] = cadd<T>(ge, go);
-    if (rs != rn) out[(int64_t)rs * MB] = csub<T>(ge, go);
+    // synthesis layout: ring pair p, north (ring pair_n[p]) and south (pair_s[p]) halves
+    c2<T> *out = phase + (int64_t)b * pstride + syn_ofs(m, 0, p, npairs);
+    out[0] = cadd<T>(ge, go);
+    if (pair_s[p] != pair_n[p]) out[(int64_t)pairs8(npairs) * MB] = csub<T>(ge, go);
 }
 
 // ---------------------------------------------------------------------------

@@ -191,7 +191,9 @@ leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float
     // Ring pair p couples ring p (north) and ring nr-1-p (south).
     const int b = g * 16 + (r >> 1), c = r & 1;
     if (b >= nmaps) return;
-    float *out = phase + (((int64_t)b * nmb + mb) * nr) * (2 * MB) + (m & 7) * 2 + c;
+    // synthesis layout (floats): [map][m/8][half][pair/8][m%8][pair%8][re, im]
+    const int64_t sbe = syn_block_elems(npairs), south = 2 * (int64_t)pairs8(npairs) * MB;
+    float *out = phase + ((int64_t)b * nmb + mb) * sbe * 2 + (m & 7) * 16 + c;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const int base = pbase + 32 * j;
@@ -201,9 +203,9 @@ leg_synthesis_mfma_kernel(const float2 *__restrict__ alm, int64_t astride, float
             const int pp = base + row;
             if (pp >= npairs) continue;
             const float ge = De[j][q], go = Do[j][q];
-            const int rn = pp, rs = nr - 1 - pp;
-            out[(int64_t)rn * (2 * MB)] = ge + go;
-            if (rs != rn) out[(int64_t)rs * (2 * MB)] = ge - go;
+            const int64_t o = 2 * ((pp >> 3) * (8 * MB) + (pp & 7));
+            out[o] = ge + go;
+            if (nr - 1 - pp != pp) out[south + o] = ge - go;
         }
     }
 }
@@ -523,12 +525,16 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
     // ring p (north) and ring nr-1-p (south).  Adjacent lanes (re, im of one
     // map) trade one value per two rows, so that the even lane stores the
     // complex value of row q and the odd lane that of row q+1, 8 bytes each.
+    // Rows 8 k .. 8 k + 7 (q < 4 over both h) are 8 ring pairs of this m:
+    // one 64-byte line of the synthesis layout per map and hemisphere,
+    // written whole by this wave.
     const bool odd = (r & 1) != 0;
 #pragma unroll
     for (int u = 0; u < NG; ++u) {
         const int b = g0 + 16 * u + (r >> 1);
         const bool live = b < nmaps;
-        float2 *out = reinterpret_cast<float2 *>(phase) + (((int64_t)min(b, nmaps - 1) * nmb + mb) * nr) * MB + (m & 7);
+        const int64_t sbe = syn_block_elems(npairs), south = (int64_t)pairs8(npairs) * MB;
+        float2 *out = reinterpret_cast<float2 *>(phase) + ((int64_t)min(b, nmaps - 1) * nmb + mb) * sbe + (m & 7) * 8;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const int base = pbase + 32 * j;
@@ -551,9 +557,9 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
                 const int row = ((q + (odd ? 1 : 0)) & 3) + 8 * (q >> 2) + 4 * h;
                 const int pp = base + row;
                 if (!live || pp >= npairs) continue;
-                const int rn = pp, rs = nr - 1 - pp;
-                out[(int64_t)rn * MB] = cn;
-                if (rs != rn) out[(int64_t)rs * MB] = cs;
+                const int64_t o = (pp >> 3) * (8 * MB) + (pp & 7);
+                out[o] = cn;
+                if (nr - 1 - pp != pp) out[south + o] = cs;
             }
         }
     }

@@ -402,6 +402,16 @@ __device__ __forceinline__ void ring_block(int g, int nmaps, int *slot, int *map
     *slot = (k / nmaps) * 8 + xcd;
 }
 inline unsigned ring_grid(int ncls, int nmaps) { return (unsigned)(((ncls + 7) / 8) * 8 * nmaps); }
+// synthesis: each XCD takes a contiguous run of slots (octets q of C per
+// XCD), map-major, the 8 slots of an octet next to each other in time, so
+// that the 64-byte lines of the synthesis layout (8 ring pairs of one m,
+// shared by 8 consecutive rings of a class) meet in that XCD's L2
+__device__ __forceinline__ void ring_block_syn(int g, int nmaps, int C, int *slot, int *map) {
+    const int xcd = g & 7, k = g >> 3;
+    *map = (k >> 3) / C;
+    *slot = ((xcd * C + (k >> 3) % C) << 3) + (k & 7);
+}
+inline unsigned ring_grid_syn(int ncls, int nmaps) { return (unsigned)(((ncls + 63) / 64) * 64 * nmaps); }
 
 // ---------------------------------------------------------------------------
 // analysis: maps -> phase.  BLUE: Bluestein rings (h not a power of two),
@@ -524,7 +534,7 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
     extern __shared__ __align__(16) unsigned char smem_raw[];
     c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
     int slot, b;
-    ring_block(blockIdx.x, nmaps, &slot, &b);
+    ring_block_syn(blockIdx.x, nmaps, (ncls + 63) >> 6, &slot, &b);
     if (slot >= ncls) return;
     const int r = rings[slot];
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -534,9 +544,13 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
     const c2<T> *Cr = BLUE ? Vr + M : nullptr;
     const double ph0 = phi0 ? phi0[r] : 0.0;
     const bool rotp = (ph0 != 0.0);
-    const c2<T> *G = phase + (int64_t)b * phase_stride;
+    // synthesis layout: ring r is half (r >= npairs) of ring pair p
+    const int npairs = (nr + 1) >> 1;
+    const int half = r >= npairs ? 1 : 0, pr = half ? nr - 1 - r : r;
+    const int64_t sbe = syn_block_elems(npairs);
+    const c2<T> *G = phase + (int64_t)b * phase_stride + syn_ofs(0, half, pr, npairs);
     const double2 Rn = rot[nr + r];
-    auto gload = [&](int m) { return G[((int64_t)(m >> 3) * nr + r) * MB + (m & 7)]; };
+    auto gload = [&](int m) { return G[(int64_t)(m >> 3) * sbe + (m & 7) * 8]; };
 
     // X[k] = sum_{m = k mod n} G'(m) + sum_{m = -k mod n, m >= 1} conj G'(m),
     // G'(m) = G(m) e^{i m phi0} (imaginary part of G(0) dropped); Rk = e^{i k phi0}
@@ -1159,12 +1173,12 @@ static int ring_synthesis_t(const Plan *p, int nmaps, const c2<T> *phase, TOut *
         const size_t lds = ring_lds_bytes<T>(c.M);
         const c2<T> *tw = class_tw<T>(p, c);
         if (c.nrings_pow2)
-            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, false>), dim3(ring_grid(c.nrings_pow2, nmaps)),
+            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, false>), dim3(ring_grid_syn(c.nrings_pow2, nmaps)),
                                dim3(c.threads), lds, s, phase, pstride, maps, map_stride, c.d_rings_pow2,
                                p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
                                c.M, c.log2M, p->mmax, p->nrings, c.nrings_pow2, nmaps);
         if (c.nrings_blue)
-            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, true>), dim3(ring_grid(c.nrings_blue, nmaps)),
+            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, true>), dim3(ring_grid_syn(c.nrings_blue, nmaps)),
                                dim3(c.threads), lds, s, phase, pstride, maps, map_stride, c.d_rings_blue,
                                p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
                                c.M, c.log2M, p->mmax, p->nrings, c.nrings_blue, nmaps);

@@ -688,14 +688,15 @@ spin_synthesis_kernel(const c2<T> *__restrict__ scaled, int64_t nrow, c2<T> *__r
     const c2<T> gun = mk<T>((T)0.5 * (Pn.y - Mn.y), (T)-0.5 * (Pn.x - Mn.x));
     const c2<T> gqs = mk<T>((T)0.5 * (Ps.x + Ms.x), (T)0.5 * (Ps.y + Ms.y));
     const c2<T> gus = mk<T>((T)0.5 * (Ps.y - Ms.y), (T)-0.5 * (Ps.x - Ms.x));
-    c2<T> *oq = phase + (int64_t)(2 * b) * pstride + (int64_t)mb * nr * MB + w;
+    // synthesis layout: ring pair p, north and south halves
+    c2<T> *oq = phase + (int64_t)(2 * b) * pstride + syn_ofs(m, 0, p, npairs);
     c2<T> *ou = oq + pstride;
-    const int rs = nr - 1 - p;
-    oq[(int64_t)p * MB] = gqn;
-    ou[(int64_t)p * MB] = gun;
-    if (rs != p) {
-        oq[(int64_t)rs * MB] = gqs;
-        ou[(int64_t)rs * MB] = gus;
+    const int64_t south = (int64_t)pairs8(npairs) * MB;
+    oq[0] = gqn;
+    ou[0] = gun;
+    if (nr - 1 - p != p) {
+        oq[south] = gqs;
+        ou[south] = gus;
     }
 }
 
@@ -853,8 +854,10 @@ spin_synthesis_mfma_kernel(const float2 *__restrict__ scaled, int64_t nrow, floa
     if (jo >= npol) return;
     const float sigma = (s & 1) ? -1.0f : 1.0f;
     const float sg = ((l0 + m) & 1) ? -1.0f : 1.0f;
-    float2 *oq = phase + (int64_t)(2 * jo) * pstride + (int64_t)mb * nr * MB + (m & 7);
+    // synthesis layout: ring pair pp, north and south halves
+    float2 *oq = phase + (int64_t)(2 * jo) * pstride + syn_ofs(m, 0, 0, npairs);
     float2 *ou = oq + pstride;
+    const int64_t south = (int64_t)pairs8(npairs) * MB;
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int pp = pbase + 16 * t + (lane & 15);
@@ -864,13 +867,13 @@ spin_synthesis_mfma_kernel(const float2 *__restrict__ scaled, int64_t nrow, floa
         const float mnr = -sigma * (be[2] + bo[2]), mni = -sigma * (be[3] + bo[3]);
         const float psr = -sg * (be[0] - bo[0]), psi = -sg * (be[1] - bo[1]);
         const float msr = -sigma * sg * (ae[2] - ao[2]), msi = -sigma * sg * (ae[3] - ao[3]);
-        const int rn = pp, rs = nr - 1 - pp;
+        const int64_t o = (pp >> 3) * (8 * MB) + (pp & 7);
         // G_Q = (P + M)/2, G_U = -i (P - M)/2
-        oq[(int64_t)rn * MB] = make_float2(0.5f * (pnr + mnr), 0.5f * (pni + mni));
-        ou[(int64_t)rn * MB] = make_float2(0.5f * (pni - mni), -0.5f * (pnr - mnr));
-        if (rs != rn) {
-            oq[(int64_t)rs * MB] = make_float2(0.5f * (psr + msr), 0.5f * (psi + msi));
-            ou[(int64_t)rs * MB] = make_float2(0.5f * (psi - msi), -0.5f * (psr - msr));
+        oq[o] = make_float2(0.5f * (pnr + mnr), 0.5f * (pni + mni));
+        ou[o] = make_float2(0.5f * (pni - mni), -0.5f * (pnr - mnr));
+        if (nr - 1 - pp != pp) {
+            oq[south + o] = make_float2(0.5f * (psr + msr), 0.5f * (psi + msi));
+            ou[south + o] = make_float2(0.5f * (psi - msi), -0.5f * (psr - msr));
         }
     }
 }

@@ -118,6 +118,36 @@ def _map_dtype_code(dtype):
     raise TypeError(f"maps must be float64 or float32, got {dtype}")
 
 
+def phase_map_elems(nrings, mmax):
+    """Complex elements between consecutive maps of a phase buffer: m-blocks
+    of 8 m, each 2 * pairs8 * 8 elements (pairs8 = ceil(nrings/2) rounded up
+    to 8), the larger of the two layouts of phase_index()."""
+    pairs8 = ((nrings + 1) // 2 + 7) // 8 * 8
+    return (mmax + 8) // 8 * 2 * pairs8 * 8
+
+
+def phase_index(nrings, mmax, direction):
+    """Offsets [ring][m] (m <= mmax), in complex elements, of one map's phase
+    coefficients.  "analysis" (ring analysis -> Legendre analysis):
+    [m/8][ring][m%8].  "synthesis" (Legendre synthesis -> ring synthesis):
+    [m/8][half][pair/8][m%8][pair%8], ring r < npairs being half 0 of ring
+    pair r and ring r >= npairs half 1 of pair nrings-1-r (npairs =
+    ceil(nrings/2); halves pairs8 * 8 elements apart).  Maps lie
+    phase_map_elems() apart in both."""
+    nr = int(nrings)
+    m = np.arange(int(mmax) + 1, dtype=np.int64)[None, :]
+    r = np.arange(nr, dtype=np.int64)[:, None]
+    if direction == "analysis":
+        return (m >> 3) * (nr * 8) + r * 8 + (m & 7)
+    if direction != "synthesis":
+        raise ValueError(f"direction must be 'analysis' or 'synthesis', not {direction!r}")
+    npairs = (nr + 1) // 2
+    p8 = (npairs + 7) // 8 * 8
+    half = (r >= npairs).astype(np.int64)
+    p = np.where(half == 1, nr - 1 - r, r)
+    return (m >> 3) * (2 * p8 * 8) + half * (p8 * 8) + (p >> 3) * 64 + (m & 7) * 8 + (p & 7)
+
+
 class Plan:
     """A libhbx plan for one geometry, lmax, mmax, spin and precision on one GPU.
 
@@ -418,6 +448,15 @@ class Plan:
     # ------------------------------------------------------------------
     # stage entry points: maps <-> phase buffer <-> alm
     # ------------------------------------------------------------------
+    @property
+    def phase_map_elems(self):
+        """Complex elements between consecutive maps of a phase buffer."""
+        return phase_map_elems(self.geom.nrings, self.mmax)
+
+    def phase_index(self, direction):
+        """Offsets [ring][m] of one map's phase coefficients (phase_index())."""
+        return phase_index(self.geom.nrings, self.mmax, direction)
+
     def phase_buffer(self, nmaps):
         b = ctypes.c_uint64()
         _abi.check(_abi.lib().hbx_phase_bytes(self._h, int(nmaps), ctypes.byref(b)))

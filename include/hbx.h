@@ -217,10 +217,18 @@ int hbx_map2alm_spin_adjoint(const hbx_plan *plan, int32_t npol,
 
 /* Stage entry points (the two halves of every transform).  The phase
  * buffer holds F_r(m) / G_r(m) for every ring and m <= mmax as complex
- * (precision of the plan) in the m-blocked ring-major layout
- * [map][m/8][ring][m%8]; hbx_phase_bytes() gives its size.  They let a
- * caller time or overlap the stages, or exchange the phase buffer between
- * devices (ring-parallel FFT stage, m-parallel Legendre stage).
+ * (precision of the plan), maps S = nmb * 2 * pairs8 * 8 elements apart
+ * (nmb = ceil((mmax+1)/8), npairs = ceil(nrings/2), pairs8 = npairs rounded
+ * up to a multiple of 8), in m-blocks of 8 m laid out by direction so that
+ * each stage's writer fills whole 64-byte lines:
+ *   analysis (ring -> Legendre): [m/8][ring][m%8], blocks nrings * 8 apart;
+ *   synthesis (Legendre -> ring): [m/8][half][pair/8][m%8][pair%8], blocks
+ *     2 * pairs8 * 8 apart, half 0 (pairs8 * 8 elements) holding ring p and
+ *     half 1 ring nrings-1-p of ring pair p (ring r >= npairs is half 1 of
+ *     pair nrings-1-r).
+ * hbx_phase_bytes() gives the size.  The stages let a caller time or
+ * overlap them, or exchange the phase buffer between devices
+ * (ring-parallel FFT stage, m-parallel Legendre stage).
  *   hbx_ring_analysis      - per-ring FFT loop      sht_raw.py:38-43
  *   hbx_legendre_analysis  - per-m Legendre loop    sht_raw.py:46-56
  *   hbx_legendre_synthesis - per-m l-sum            isht.py:11-23

@@ -200,7 +200,7 @@ def test_h16_legendre_stages_scale_exactly(gpu, shift):
     phase = p32.phase_buffer(nb)
     p32.ring_analysis(maps32, phase)
     # FP32 phase coefficients first, then the FP64 band rows m < 8
-    main = nb * ((lmax + 8) // 8) * p32.geom.nrings * 8 * 8
+    main = nb * p32.phase_map_elems * 8
     band = (main + 255) // 256 * 256
     f32v = phase[:main].view(torch.float32)
     f64v = phase[band:band + nb * p32.geom.nrings * 8 * 16].view(torch.float64)

@@ -27,14 +27,27 @@ def _setup(precision, phi0="healpix"):
     return geom, get_plan(geom, LMAX, LMAX, precision)
 
 
-def _phase_view(plan, phase, nmaps):
-    """[map][ring][m] view of the m-blocked phase buffer [map][m/8][ring][m%8]."""
+def _phase_flat(plan, phase, nmaps):
+    """[map][element] complex view of a phase buffer (maps phase_map_elems apart)."""
     cdt = torch.complex128 if plan.alm_dtype == torch.complex128 else torch.complex64
-    nr, nmb = plan.geom.nrings, (LMAX + 1 + 7) // 8
-    n = nmaps * nmb * nr * 8
     csize = 16 if cdt == torch.complex128 else 8
-    v = phase[:n * csize].view(cdt).reshape(nmaps, nmb, nr, 8)
-    return v, nr, nmb
+    n = nmaps * plan.phase_map_elems
+    return phase[:n * csize].view(cdt).reshape(nmaps, plan.phase_map_elems)
+
+
+def _phase_get(plan, phase, nmaps, direction):
+    """[map][ring][m] coefficients read through the plan's layout index."""
+    ix = torch.as_tensor(plan.phase_index(direction).ravel(), device=phase.device)
+    v = _phase_flat(plan, phase, nmaps)[:, ix]
+    return v.reshape(nmaps, plan.geom.nrings, plan.mmax + 1).cpu().numpy()
+
+
+def _phase_put(plan, phase, G, direction):
+    """Write [map][ring][m] coefficients G through the plan's layout index."""
+    nmaps = G.shape[0]
+    flat = _phase_flat(plan, phase, nmaps)
+    ix = torch.as_tensor(plan.phase_index(direction).ravel(), device=phase.device)
+    flat[:, ix] = torch.as_tensor(G.reshape(nmaps, -1)).to(device=phase.device, dtype=flat.dtype)
 
 
 def _per_ring_rel(got, ref):
@@ -55,8 +68,7 @@ def test_ring_analysis_all_classes(gpu, precision, phi0):
     maps = torch.as_tensor(rng.standard_normal((nmaps, plan.npix)), dtype=rdt, device="cuda")
     phase = plan.phase_buffer(nmaps)
     plan.ring_analysis(maps, phase)
-    v, nr, nmb = _phase_view(plan, phase, nmaps)
-    got_all = v.permute(0, 2, 1, 3).reshape(nmaps, nr, nmb * 8)[:, :, :LMAX + 1].cpu().numpy()
+    got_all = _phase_get(plan, phase, nmaps, "analysis")
     f = maps.double().cpu().numpy()
     for b in range(nmaps):
         ref = OS.ring_phase(f[b], geom.ring_start, geom.nphi, LMAX, geom.phi0, geom.mcut) * geom.weight[:, None]
@@ -74,14 +86,10 @@ def test_ring_synthesis_all_classes(gpu, precision):
     nr = plan.geom.nrings
     G = rng.standard_normal((nmaps, nr, LMAX + 1)) + 1j * rng.standard_normal((nmaps, nr, LMAX + 1))
     phase = plan.phase_buffer(nmaps)
-    v, _, nmb = _phase_view(plan, phase, nmaps)
-    Gp = np.zeros((nmaps, nr, nmb * 8), dtype=np.complex128)
-    Gp[:, :, :LMAX + 1] = G
-    cdt = v.dtype
-    v.copy_(torch.as_tensor(Gp.reshape(nmaps, nr, nmb, 8).transpose(0, 2, 1, 3).copy()).to(cdt))
+    _phase_put(plan, phase, G, "synthesis")
     rdt = torch.float64 if precision == "f64" else torch.float32
     got = plan.ring_synthesis(phase, nmaps, dtype=rdt).double().cpu().numpy()
-    Gr = Gp[:, :, :LMAX + 1]
+    Gr = G
     if precision == "f32":
         Gr = Gr.astype(np.complex64).astype(np.complex128)     # the values the kernel reads
     for b in range(nmaps):
@@ -144,18 +152,15 @@ def test_ring_stage_nside4096_f32(gpu):
     f = rng.standard_normal(plan.npix)
     phase = plan.phase_buffer(1)
     plan.ring_analysis(torch.as_tensor(f, dtype=torch.float32, device="cuda")[None], phase)
-    nr, nmb = plan.geom.nrings, (mmax + 8) // 8
-    v = phase[:nmb * nr * 8 * 8].view(torch.complex64).reshape(nmb, nr, 8)
-    got = v.permute(1, 0, 2).reshape(nr, nmb * 8)[:, :mmax + 1].cpu().numpy()
+    nr = plan.geom.nrings
+    got = _phase_get(plan, phase, 1, "analysis")[0]
     f32 = f.astype(np.float32).astype(np.float64)
     ref = OS.ring_phase(f32, geom.ring_start, geom.nphi, mmax, geom.phi0, geom.mcut) * geom.weight[:, None]
     err = _per_ring_rel(got, ref)
     print(f"analysis per-ring rel err {err:.2e}", flush=True)
     assert err <= TOL["f32"]
     G = (rng.standard_normal((nr, mmax + 1)) + 1j * rng.standard_normal((nr, mmax + 1))).astype(np.complex64)
-    Gp = np.zeros((nr, nmb * 8), dtype=np.complex64)
-    Gp[:, :mmax + 1] = G
-    v.copy_(torch.as_tensor(Gp.reshape(nr, nmb, 8).transpose(1, 0, 2).copy()))
+    _phase_put(plan, phase, G[None], "synthesis")
     maps = plan.ring_synthesis(phase, 1, dtype=torch.float32)[0].double().cpu().numpy()
     refm = OS.ring_synthesis(G.astype(np.complex128), geom.ring_start, geom.nphi, mmax, geom.phi0, geom.mcut)
     worst = 0.0
