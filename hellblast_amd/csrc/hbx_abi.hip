@@ -198,13 +198,71 @@ static int upload(Plan *p, X **dst, const X *src, size_t n) {
 }
 
 DeviceAllocs::~DeviceAllocs() {
-    if (ptrs.empty()) return;
+    if (ptrs.empty() && !side) return;
     int cur = 0;
     const bool have = hipGetDevice(&cur) == hipSuccess;
     if (have && cur != device) (void)hipSetDevice(device);
     for (void *ptr : ptrs) (void)hipFree(ptr);
+    if (side) (void)hipStreamDestroy(side);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     if (have && cur != device) (void)hipSetDevice(cur);
 }
+
+// Side stream of a plan (non-blocking, default priority) and its fork/join
+// events; without them (creation failed) every call stays on its own stream.
+static void create_side_stream(DeviceAllocs *d) {
+    if (hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking) != hipSuccess) d->side = nullptr;
+    if (d->side && (hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                    hipEventCreateWithFlags(&d->ev_join, hipEventDisableTiming) != hipSuccess)) {
+        (void)hipStreamDestroy(d->side);
+        d->side = nullptr;
+    }
+    (void)hipGetLastError();
+}
+
+// Independent work of one call (the FP64 band of FP32 plans) on the plan's
+// side stream, concurrent with the main work on the caller's stream s: the
+// constructor makes the side stream wait for the work already queued on s,
+// join() (or the destructor) makes s wait for the side work.  The plan's
+// lock is held from fork to join because the two events are shared; while
+// s is being captured into a graph (or there is no side stream) side() is s
+// and nothing forks.
+class SideFork {
+  public:
+    SideFork(const Plan *p, hipStream_t s) : d_(p->allocations.s.get()), s_(s), side_(s) {
+        if (!d_->side) return;
+        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone) {
+            (void)hipGetLastError();
+            return;
+        }
+        lock_ = std::unique_lock<std::mutex>(d_->side_mu);
+        if (hipEventRecord(d_->ev_fork, s) != hipSuccess || hipStreamWaitEvent(d_->side, d_->ev_fork, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            lock_.unlock();
+            return;
+        }
+        side_ = d_->side;
+    }
+    SideFork(const SideFork &) = delete;
+    SideFork &operator=(const SideFork &) = delete;
+    ~SideFork() { (void)join(); }
+    hipStream_t side() const { return side_; }
+    int join() {
+        if (side_ == s_) return HBX_OK;
+        hipError_t e = hipEventRecord(d_->ev_join, side_);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s_, d_->ev_join, 0);
+        side_ = s_;
+        lock_.unlock();
+        return e == hipSuccess ? HBX_OK : hip_error(e, "side stream join");
+    }
+
+  private:
+    DeviceAllocs *d_;
+    hipStream_t s_, side_;
+    std::unique_lock<std::mutex> lock_;
+};
 
 // drops this plan's reference to the shared device tables
 static void destroy_plan(Plan *p) { p->allocations.s.reset(); }
@@ -279,9 +337,14 @@ static int check_exec(const hbx_plan *plan, int32_t nmaps, const void *a, const 
 // Ring stage of an analysis: FFTs for every m, plus the FP64 band of FP32 plans
 static int ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride, void *phase,
                          const double *scale, hipStream_t s) {
-    int rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, phase, scale, s);
-    if (rc || p->precision != HBX_F32) return rc;
-    return launch_ring_band(p, nmaps, maps, map_dtype, map_stride, band_phase_ptr(p, nmaps, phase), scale, s);
+    if (p->precision != HBX_F32) return launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, phase, scale, s);
+    // the band reads the same maps and writes its own buffer: side stream
+    SideFork fork(p, s);
+    int rc = launch_ring_band(p, nmaps, maps, map_dtype, map_stride, band_phase_ptr(p, nmaps, phase), scale,
+                              fork.side());
+    if (!rc) rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, phase, scale, s);
+    const int rj = fork.join();
+    return rc ? rc : rj;
 }
 
 // Legendre analysis: matrix cores for FP32 batches, VALU otherwise; FP32
@@ -290,15 +353,22 @@ static int legendre_analysis(const Plan *p, int nmaps, const void *phase, void *
                              void *tail, int adjoint, hipStream_t s) {
     const AnalysisWs w = analysis_ws(p, nmaps);
     char *t = static_cast<char *>(tail);
-    int rc;
-    if (p->precision == HBX_F32 && nmaps >= mfma_min_maps(p))
-        rc = launch_leg_analysis_mfma(p, nmaps, phase, alm, alm_stride, t + w.partial,
-                                      reinterpret_cast<int32_t *>(t + w.lfirst), adjoint, s);
-    else
-        rc = launch_leg_analysis(p, nmaps, phase, alm, alm_stride, t + w.partial, adjoint, s);
-    if (rc || p->precision != HBX_F32) return rc;
-    return launch_leg_analysis_band(p, nmaps, band_phase_ptr(p, nmaps, phase), alm, alm_stride, t + w.band_alm,
-                                    t + w.band_partial, adjoint, s);
+    if (p->precision != HBX_F32)
+        return launch_leg_analysis(p, nmaps, phase, alm, alm_stride, t + w.partial, adjoint, s);
+    // the band (alm rows m < M_LO, own workspace) on the side stream; the
+    // main kernels write rows m >= M_LO
+    SideFork fork(p, s);
+    int rc = launch_leg_analysis_band(p, nmaps, band_phase_ptr(p, nmaps, phase), alm, alm_stride, t + w.band_alm,
+                                      t + w.band_partial, adjoint, fork.side());
+    if (!rc) {
+        if (nmaps >= mfma_min_maps(p))
+            rc = launch_leg_analysis_mfma(p, nmaps, phase, alm, alm_stride, t + w.partial,
+                                          reinterpret_cast<int32_t *>(t + w.lfirst), adjoint, s);
+        else
+            rc = launch_leg_analysis(p, nmaps, phase, alm, alm_stride, t + w.partial, adjoint, s);
+    }
+    const int rj = fork.join();
+    return rc ? rc : rj;
 }
 
 // Legendre synthesis: matrix cores for FP32 batches, VALU otherwise
@@ -366,6 +436,7 @@ int hbx_plan_create(const hbx_geom *g, int32_t lmax, int32_t mmax, int32_t spin,
         delete hp;
         return set_error(rc, msg);
     }
+    if (p->precision == HBX_F32) create_side_stream(p->allocations.s.get());
     *out = hp;
     return HBX_OK;
 }
@@ -509,20 +580,37 @@ static int spin_analysis(const hbx_plan *plan, int32_t npol, const void *maps, i
     const SpinWs w = spin_ws(p, npol);
     char *base = static_cast<char *>(ws);
     const double *scale = adjoint ? nullptr : p->d_scale_ana;
-    if ((rc = launch_ring_analysis(p, 2 * npol, maps, map_dtype, map_stride, ws, scale, s))) return rc;
-    if (p->precision == HBX_F32 &&
-        (rc = launch_ring_band(p, 2 * npol, maps, map_dtype, map_stride, base + w.band_phase, scale, s)))
-        return rc;
-    if (spin_uses_mfma(p, npol))
-        rc = launch_spin_analysis_mfma(p, npol, ws, alm, alm_stride, base + w.tail,
-                                       reinterpret_cast<int32_t *>(base + w.lfirst), adjoint, s);
-    else
-        rc = launch_spin_analysis(p, npol, ws, alm, alm_stride, base + w.tail, adjoint, s);
-    if (rc) return rc;
-    if (p->precision != HBX_F32) return 0;
-    // m < M_LO in FP64 (the low-m rows carry most of the power of red spectra)
-    return launch_spin_analysis_band(p, npol, base + w.band_phase, alm, alm_stride, base + w.band_alm,
-                                     base + w.band_partial, adjoint, s);
+    const bool mf = spin_uses_mfma(p, npol);
+    auto main_work = [&]() {
+        int r = launch_ring_analysis(p, 2 * npol, maps, map_dtype, map_stride, ws, scale, s);
+        if (r) return r;
+        if (mf)
+            return launch_spin_analysis_mfma(p, npol, ws, alm, alm_stride, base + w.tail,
+                                             reinterpret_cast<int32_t *>(base + w.lfirst), adjoint, s);
+        return launch_spin_analysis(p, npol, ws, alm, alm_stride, base + w.tail, adjoint, s);
+    };
+    // FP32 plans: m < M_LO in FP64 (the low-m rows carry most of the power of
+    // red spectra): ring DFT of the maps, then E/B rows m < M_LO from the
+    // band's own workspace
+    auto band_work = [&](hipStream_t bs) {
+        int r = launch_ring_band(p, 2 * npol, maps, map_dtype, map_stride, base + w.band_phase, scale, bs);
+        if (r) return r;
+        return launch_spin_analysis_band(p, npol, base + w.band_phase, alm, alm_stride, base + w.band_alm,
+                                         base + w.band_partial, adjoint, bs);
+    };
+    if (p->precision != HBX_F32) return main_work();
+    if (!mf) {
+        // the VALU kernels write every row m, the band's rows then replace m < M_LO
+        if ((rc = main_work())) return rc;
+        return band_work(s);
+    }
+    // the matrix-core kernels write rows m >= M_LO only: the band runs whole
+    // on the side stream
+    SideFork fork(p, s);
+    rc = band_work(fork.side());
+    if (!rc) rc = main_work();
+    const int rj = fork.join();
+    return rc ? rc : rj;
 }
 
 static int spin_synthesis(const hbx_plan *plan, int32_t npol, const void *alm, int64_t alm_stride, void *maps,

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -78,10 +79,16 @@ struct FftClass {
 
 // Device allocations of a plan.  Plans derived from it with hbx_plan_derive
 // share them (read-only tables); the last plan destroyed frees them on their
-// device.
+// device.  They also own the side stream on which a call runs its
+// independent FP64 band work (FP32 plans) next to the main work on the
+// caller's stream, and the two events that fork and join it (used under
+// side_mu).
 struct DeviceAllocs {
     int device = 0;
     std::vector<void *> ptrs;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    std::mutex side_mu;
     DeviceAllocs() = default;
     DeviceAllocs(const DeviceAllocs &) = delete;
     DeviceAllocs &operator=(const DeviceAllocs &) = delete;

@@ -8,7 +8,10 @@
  * (contiguous, complex arrays interleaved re/im like torch complex64/128).
  * Execute functions never allocate and never synchronise: they enqueue on the
  * given hipStream_t (NULL = default stream) and are safe to capture in a HIP
- * graph.  Plans are immutable after creation (tiles included: another tile
+ * graph.  Analyses with FP32 plans run their FP64 low-m band on a stream of
+ * the plan, forked from the given stream and joined back to it: work the
+ * caller enqueues on the given stream after the call runs after the band
+ * (everything stays on the given stream while it is being captured).  Plans are immutable after creation (tiles included: another tile
  * choice is another plan, made by hbx_plan_derive) and bound to the device
  * that was current when they were created; one plan may be used from several
  * threads and streams at once.
