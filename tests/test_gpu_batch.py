@@ -192,3 +192,68 @@ def test_mixed_batch_graph_replay_equals_eager(gpu, tmp_path, monkeypatch):
         worst = max(worst, _eps_max(eager_alm[i].cpu().numpy(), ref, lmax))
     print(f"worst eps_max over the batch {worst:.2e}; tiles {batch.tiles}")
     assert worst <= 1e-5
+
+
+def test_fp32_analysis_band_joins_caller_stream(gpu):
+    """FP32 map2alm on a user stream: rows m < 8 come from the FP64 band,
+    which runs on the plan's side stream; work queued on the user stream
+    right after the call (a copy, no synchronisation between) sees them
+    complete, and the spin analysis likewise."""
+    print("[case] band join on a user stream", flush=True)
+    p = _plan(256, 511, "f32")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    maps = torch.randn((16, p.npix), generator=g, device="cuda")
+    ref = p.map2alm(maps)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(3):
+            out = p.map2alm(maps).clone()
+    s.synchronize()
+    assert torch.equal(out, ref)
+    ps = _plan(256, 511, "f32", spin=2)
+    qu = torch.randn((4, 2, ps.npix), generator=g, device="cuda")
+    ref_eb = ps.map2alm_spin(qu)
+    torch.cuda.synchronize()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        out_eb = ps.map2alm_spin(qu).clone()
+    s.synchronize()
+    assert torch.equal(out_eb, ref_eb)
+
+
+def test_fp32_plan_shared_by_threads(gpu):
+    """One FP32 plan used from two host threads on two streams at once (the
+    side stream and its events are shared under the plan's lock): each
+    thread's alm equal the same maps' alm computed alone."""
+    import threading
+    print("[case] FP32 plan from two threads", flush=True)
+    p = _plan(256, 511, "f32")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(6)
+    maps = [torch.randn((16, p.npix), generator=g, device="cuda") for _ in range(2)]
+    refs = [p.map2alm(m) for m in maps]
+    torch.cuda.synchronize()
+    outs = [None, None]
+    errs = []
+
+    def work(i):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for _ in range(4):
+                    outs[i] = p.map2alm(maps[i])
+            s.synchronize()
+        except Exception as e:          # reported by the main thread
+            errs.append(e)
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errs, errs
+    for i in range(2):
+        assert torch.equal(outs[i], refs[i]), i
