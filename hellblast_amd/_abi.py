@@ -77,6 +77,21 @@ class hbx_tiles(ctypes.Structure):
                 ("mfma_kind", ctypes.c_int32), ("mfma_groups", ctypes.c_int32)]
 
 
+class hbx_multi_tiles(ctypes.Structure):
+    _fields_ = [("valu_slots", ctypes.c_int32), ("syn_maps", ctypes.c_int32),
+                ("tiles_per_wave", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class hbx_multi_info(ctypes.Structure):
+    _fields_ = [("ngroups", ctypes.c_int32), ("precision", ctypes.c_int32), ("valu_slots", ctypes.c_int32),
+                ("syn_maps", ctypes.c_int32), ("tiles_per_wave", ctypes.c_int32),
+                ("analysis_tiles", ctypes.c_int32), ("synthesis_tiles", ctypes.c_int32),
+                ("analysis_workgroups", ctypes.c_int32), ("synthesis_workgroups", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("workspace_bytes", ctypes.c_uint64)]
+
+
+HBX_MULTI_MAX_GROUPS = 16
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -113,6 +128,15 @@ SIGNATURES = {
     "hbx_legendre_analysis": (ctypes.c_int, [_vp, _i32, _vp, _u64, _vp, _i64, _vp, _u64, _vp]),
     "hbx_legendre_synthesis": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _u64, _vp, _u64, _vp]),
     "hbx_ring_synthesis": (ctypes.c_int, [_vp, _i32, _vp, _u64, _vp, _i32, _i64, _vp]),
+    "hbx_multi_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(_i32), _i32,
+                                        ctypes.POINTER(hbx_multi_tiles), ctypes.POINTER(_vp)]),
+    "hbx_multi_destroy": (ctypes.c_int, [_vp]),
+    "hbx_multi_workspace_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_u64)]),
+    "hbx_multi_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(hbx_multi_info)]),
+    "hbx_multi_map2alm": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i64), _i32, ctypes.POINTER(_vp),
+                                         ctypes.POINTER(_i64), _vp, _u64, _vp]),
+    "hbx_multi_alm2map": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i64), ctypes.POINTER(_vp),
+                                         ctypes.POINTER(_i64), _i32, _vp, _u64, _vp]),
 }
 
 _lib = None

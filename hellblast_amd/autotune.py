@@ -124,3 +124,102 @@ def tune(plan, nmaps, reps=3, force=False, verbose=False):
     db[k] = entry
     _save(db)
     return entry
+
+
+def multi_key(plans, counts):
+    dev = torch.cuda.get_device_name(plans[0].device)
+    groups = ",".join(f"{p.geom.key()[:12]}:l{p.lmax}:m{p.mmax}:n{int(c)}" for p, c in zip(plans, counts))
+    return f"{dev}|multi|p{plans[0].precision}|{groups}"
+
+
+def _time_multi(plans, counts, tiles, reps):
+    from .batch import MultiPlan
+    mp = MultiPlan(plans, counts, **tiles)
+    dev = mp.device
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1)
+    rdt = torch.float64 if mp.alm_dtype == torch.complex128 else torch.float32
+    maps = [torch.randn((c, p.npix), generator=gen, device=dev, dtype=rdt) for p, c in zip(plans, counts)]
+    alm = [torch.empty((c, p.nalm), device=dev, dtype=mp.alm_dtype) for p, c in zip(plans, counts)]
+    out = [torch.empty_like(m) for m in maps]
+    ws = mp.workspace()
+
+    def step():
+        mp.map2alm(maps, alm, ws)
+        mp.alm2map(alm, out, ws)
+
+    step()
+    torch.cuda.synchronize(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        step()
+    e1.record()
+    torch.cuda.synchronize(dev)
+    return e0.elapsed_time(e1) / reps
+
+
+def tune_multi(plans, counts, reps=3, force=False, verbose=False):
+    """The best tiles of a persistent mixed-geometry batch (``MultiPlan``),
+    timing candidates unless cached: analysis slots x tile granularity
+    first, then maps per synthesis tile.  Returns {"tiles", "ms", "timed"}."""
+    db = _load()
+    k = multi_key(plans, counts)
+    if not force and k in db:
+        return db[k]
+    from . import _abi
+    slots = (2, 4, 8) if plans[0].precision == _abi.HBX_F32 else (4, 8)
+    results = []
+    for s in slots:
+        for tpw in (4, 8, 16):
+            cand = {"valu_slots": s, "syn_maps": 4, "tiles_per_wave": tpw}
+            results.append((_time_multi(plans, counts, cand, reps), cand))
+            if verbose:
+                print(f"[autotune] {k} {cand} {results[-1][0]:.3f} ms", flush=True)
+    best = dict(min(results, key=lambda r: r[0])[1])
+    for sm in (1, 2):
+        cand = dict(best, syn_maps=sm)
+        results.append((_time_multi(plans, counts, cand, reps), cand))
+        if verbose:
+            print(f"[autotune] {k} {cand} {results[-1][0]:.3f} ms", flush=True)
+    ms, best = min(results, key=lambda r: r[0])
+    entry = {"tiles": best, "ms": ms, "timed": [[m, c] for m, c in results]}
+    db = _load()
+    db[k] = entry
+    _save(db)
+    return entry
+
+
+def tune_mixed_mode(specs, precision="f32", weights="dz", phi0="reference", alias="reference", device=None,
+                    reps=3, force=False):
+    """"persistent" or "grouped", whichever runs a MixedBatch of these specs
+    faster (graph replay of map2alm + alm2map), timed once and cached."""
+    from .batch import MixedBatch
+    dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+    norm = sorted((int(n), 2 * int(n) if lm is None else int(lm)) for n, lm in
+                  ((s, None) if isinstance(s, int) else s for s in specs))
+    k = (f"{torch.cuda.get_device_name(dev)}|mixed-mode|{precision}|{weights}|{phi0}|{alias}|"
+         + ",".join(f"{n}:{lm}" for n, lm in norm))
+    db = _load()
+    if not force and k in db:
+        return db[k]["mode"]
+    timed = {}
+    for mode in ("persistent", "grouped"):
+        b = MixedBatch(specs, precision, True, weights, phi0, alias, device, mode=mode)
+        b.capture()
+        torch.cuda.synchronize(dev)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            b.replay()
+        e1.record()
+        torch.cuda.synchronize(dev)
+        timed[mode] = e0.elapsed_time(e1) / reps
+        del b
+    best = min(timed, key=timed.get)
+    db = _load()
+    db[k] = {"mode": best, "timed": timed}
+    _save(db)
+    return best

@@ -70,13 +70,13 @@ static uint64_t phase_main_bytes(const Plan *p, int nmaps) {
     return align256((uint64_t)nmaps * phase_elems_per_map(p) * csize(p));
 }
 
-static uint64_t phase_total_bytes(const Plan *p, int nmaps) {
+uint64_t phase_total_bytes(const Plan *p, int nmaps) {
     uint64_t b = phase_main_bytes(p, nmaps);
     if (p->precision == HBX_F32) b += align256((uint64_t)nmaps * band_phase_elems(p) * 16);
     return b;
 }
 
-static void *band_phase_ptr(const Plan *p, int nmaps, const void *phase) {
+void *band_phase_ptr(const Plan *p, int nmaps, const void *phase) {
     return static_cast<char *>(const_cast<void *>(phase)) + phase_main_bytes(p, nmaps);
 }
 
@@ -335,7 +335,7 @@ static int check_exec(const hbx_plan *plan, int32_t nmaps, const void *a, const 
 }
 
 // Ring stage of an analysis: FFTs for every m, plus the FP64 band of FP32 plans
-static int ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride, void *phase,
+int ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride, void *phase,
                          const double *scale, hipStream_t s) {
     if (p->precision != HBX_F32) return launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, phase, scale, s);
     // the band reads the same maps and writes its own buffer: side stream

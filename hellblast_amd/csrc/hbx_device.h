@@ -32,6 +32,36 @@ template <typename T> __device__ __forceinline__ c2<T> expipi(T a) {
     T s, c; sincospi_t(a, &s, &c); return mk<T>(c, s);
 }
 
+// Load through the constant address space: for memory no kernel running at
+// the same time writes (plan tables, inputs written by an earlier launch), so
+// a wave-uniform address becomes a scalar load even when the pointer itself
+// was read from memory (a table of per-group pointers) rather than passed
+// as a kernel argument.
+template <typename X> __device__ __forceinline__ X ldc(const X *p) {
+    return *(const __attribute__((address_space(4))) X *)(p);
+}
+// the same for a complex value, one component at a time
+template <typename T> __device__ __forceinline__ c2<T> ldc2(const c2<T> *p) {
+    const T *q = reinterpret_cast<const T *>(p);
+    return mk<T>(ldc(q), ldc(q + 1));
+}
+
+// Loads and stores through the global address space, for pointers read from
+// memory (a table of per-group pointers), which the compiler would otherwise
+// treat as generic (flat) addresses held whole in vector registers.
+#define HBX_GLOBAL __attribute__((address_space(1)))
+template <typename X> __device__ __forceinline__ X ldg(const X *p) { return *(const HBX_GLOBAL X *)(p); }
+template <typename T> __device__ __forceinline__ c2<T> ldg2(const c2<T> *p) {
+    const T *q = reinterpret_cast<const T *>(p);
+    return mk<T>(ldg(q), ldg(q + 1));
+}
+template <typename X> __device__ __forceinline__ void stg(X *p, X v) { *(HBX_GLOBAL X *)(p) = v; }
+template <typename T> __device__ __forceinline__ void stg2(c2<T> *p, c2<T> v) {
+    T *q = reinterpret_cast<T *>(p);
+    stg(q, v.x);
+    stg(q + 1, v.y);
+}
+
 // Wave-uniform copy of a value (lets the compiler keep it in an SGPR).
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 

@@ -234,6 +234,14 @@ inline int64_t band_nlm(const Plan *p) {
 }
 int64_t scaled_elems_per_map(const Plan *p);  // complex elements
 
+// hbx_abi.hip: phase buffer of nmaps maps (FP32 plans: plus the FP64 band
+// phase, which starts at band_phase_ptr) and the ring stage of an analysis
+// (FP32 plans: the band ring DFT on the plan's side stream)
+uint64_t phase_total_bytes(const Plan *p, int nmaps);
+void *band_phase_ptr(const Plan *p, int nmaps, const void *phase);
+int ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride, void *phase,
+                  const double *scale, hipStream_t s);
+
 }  // namespace hbx
 
 struct hbx_plan {

@@ -26,10 +26,10 @@
 
 #include "hbx_device.h"
 #include "hbx_internal.h"
+#include "legendre_valu.h"
 
 namespace hbx {
 
-constexpr int NEVER = 0x7fffffff;
 
 // ---------------------------------------------------------------------------
 // plan-time kernels
@@ -206,30 +206,6 @@ int plan_build_recursion(Plan *p, const std::vector<double> &pair_cos, const std
     return 0;
 }
 
-// ---------------------------------------------------------------------------
-// wave helpers (wave64)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return uniform(v);
-}
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return uniform(v);
-}
-
-// ---------------------------------------------------------------------------
-// analysis: phase -> alm.  One wave per (m, ring-pair group, map); each lane
-// holds R ring pairs ("slots", 64 consecutive pairs per slot), so a wave
-// covers 64*R pairs and runs with no workgroup barrier.  Per l-chunk the 16
-// partial sums (8 l x re/im) are reduced over the wave by an LDS transpose:
-// every lane writes its 16 values, lane j then sums value j/4 over 16
-// source lanes and two xor-shuffles finish the sum.  With a single group the
-// wave writes s_lm * sum straight into alm; otherwise it writes raw partial
-// sums per group and group_reduce_kernel adds them in a fixed order.
-// ---------------------------------------------------------------------------
 template <typename T, int R>
 __global__ void __launch_bounds__(64)
 leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__restrict__ alm,
@@ -239,10 +215,6 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
                     const double *__restrict__ sl, const int32_t *__restrict__ lseed,
                     const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
                     int64_t nlm, int adjoint, int mbeg) {
-    // transpose buffer: row v (one of the 2 CHUNK sums) of the 64 lanes, rows
-    // 65 elements apart so that the reads below (4 lanes per row, 16
-    // consecutive elements each) fall on distinct banks
-    constexpr int RED_ROW = 65;
     __shared__ T red[2 * CHUNK * RED_ROW];
     int mb, item;
     xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
@@ -250,124 +222,12 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
     const int grp = item % ngroups;
     if (m > mmax || m < mbeg) return;
     const int b = blockIdx.y;
-    const int lane = threadIdx.x;
-    const c2<T> *ph = phase + (int64_t)b * pstride + (int64_t)(m >> 3) * nr * MB + (m & 7);
-
-    double x[R], u1[R], u2[R];
-    c2<T> E[R], O[R];
-    int ls[R], sst[R], sinj[R];
-    const int p0 = grp * 64 * R + lane;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        const int p = p0 + 64 * k;
-        u1[k] = 0.0;
-        u2[k] = 0.0;
-        if (p < npairs) {
-            x[k] = px[p];
-            ls[k] = lseed[(int64_t)m * npairs + p];
-            const int rn = pair_n[p], rs = pair_s[p];
-            const c2<T> fn = ph[(int64_t)rn * MB];
-            const c2<T> fs = (rs != rn) ? ph[(int64_t)rs * MB] : mk<T>(0, 0);
-            E[k] = cadd<T>(fn, fs);
-            O[k] = csub<T>(fn, fs);
-        } else {
-            x[k] = 0.0;
-            ls[k] = NEVER;
-            E[k] = mk<T>(0, 0);
-            O[k] = mk<T>(0, 0);
-        }
-        sst[k] = wave_min(ls[k]);
-        sinj[k] = wave_max(ls[k] <= lmax ? ls[k] : -1);
-    }
-    constexpr int GS = R < 4 ? R : 4;
-    int gst[R / GS];
-    int lstart = NEVER;
-#pragma unroll
-    for (int k0 = 0; k0 < R; k0 += GS) {
-        int v = NEVER;
-#pragma unroll
-        for (int k = k0; k < k0 + GS; ++k) v = min(v, sst[k]);
-        gst[k0 / GS] = v;
-        lstart = min(lstart, v);
-    }
-
     const bool direct = (ngroups == 1);
     c2<T> *dst = direct ? alm + (int64_t)b * astride : partial + ((int64_t)b * ngroups + grp) * nlm;
-    const int64_t i0 = alm_index(m, m, lmax);
-    const int64_t ro = rofs[m];
-    // rows below every slot's switch-on are zero
-    for (int l = m + lane; l < (lstart > lmax ? lmax + 1 : lstart); l += 64) dst[i0 + (l - m)] = mk<T>(0, 0);
-
-    const int v_id = lane >> 2, q0 = (lane & 3) * 16;
-    for (int l0 = lstart; l0 <= lmax; l0 += CHUNK) {
-        double a[CHUNK];
-#pragma unroll
-        for (int i = 0; i < CHUNK; ++i) a[i] = alpha[ro + (l0 - m) + i];
-        T acc[2 * CHUNK];
-#pragma unroll
-        for (int i = 0; i < 2 * CHUNK; ++i) acc[i] = (T)0;
-        // slots advance in groups of GS with their steps interleaved, so
-        // GS independent recursion chains hide the FP64 latency
-#pragma unroll
-        for (int k0 = 0; k0 < R; k0 += GS) {
-            if (l0 < gst[k0 / GS]) continue;
-#pragma unroll
-            for (int k = k0; k < k0 + GS; ++k) {
-                if (l0 <= sinj[k] && ls[k] == l0) {
-                    const double2 sd = seed[(int64_t)m * npairs + p0 + 64 * k];
-                    u2[k] = sd.x;
-                    u1[k] = sd.y;
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < CHUNK; ++i) {
-#pragma unroll
-                for (int k = k0; k < k0 + GS; ++k) {
-                    const double u = fma(a[i] * x[k], u1[k], -u2[k]);
-                    u2[k] = u1[k];
-                    u1[k] = u;
-                    const T uu = (T)u;
-                    const c2<T> d = (i & 1) ? O[k] : E[k];
-                    acc[2 * i] = fma(uu, d.x, acc[2 * i]);
-                    acc[2 * i + 1] = fma(uu, d.y, acc[2 * i + 1]);
-                }
-            }
-        }
-        // wave-level transpose-reduce of the 16 values
-#pragma unroll
-        for (int v = 0; v < 2 * CHUNK; ++v) red[v * RED_ROW + lane] = acc[v];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        T s = (T)0;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) s += red[v_id * RED_ROW + q0 + q];
-        s += __shfl_xor(s, 1, 64);
-        s += __shfl_xor(s, 2, 64);
-        if ((lane & 3) == 0) {
-            const int i = v_id >> 1, c = v_id & 1;
-            const int l = l0 + i;
-            if (l <= lmax) {
-                T val = s;
-                if (direct) {
-                    double dv = (double)s * sl[ro + (l0 - m) + i];
-                    if (adjoint) {
-                        if (m > 0) dv *= 2.0;
-                        else if (c == 1) dv = 0.0;
-                    }
-                    val = (T)dv;
-                }
-                reinterpret_cast<T *>(dst + i0 + (l - m))[c] = val;
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
+    ana_wave<T, R>(phase + (int64_t)b * pstride + (int64_t)(m >> 3) * nr * MB + (m & 7), dst, direct, pair_n, pair_s,
+                   px, rofs, alpha, sl, lseed, seed, npairs, lmax, m, grp, adjoint, red, threadIdx.x);
 }
 
-// alm[idx(l,m)] = s_lm * sum_g partial[g][idx(l,m)] (fixed summation order);
-// with lfirst, group g's partial is read only for l >= lfirst[m][g] (zero below)
 template <typename T>
 __global__ void group_reduce_kernel(const c2<T> *__restrict__ partial, c2<T> *__restrict__ alm, int64_t astride,
                                     const int64_t *__restrict__ rofs, const double *__restrict__ sl, int lmax,
@@ -375,46 +235,18 @@ __global__ void group_reduce_kernel(const c2<T> *__restrict__ partial, c2<T> *__
                                     const int32_t *__restrict__ lfirst) {
     const int m = mbeg + blockIdx.x, b = blockIdx.y;
     const int64_t i0 = alm_index(m, m, lmax);
-    const int64_t ro = rofs[m];
-    const c2<T> *src = partial + (int64_t)b * ngroups * nlm + i0;
-    c2<T> *dst = alm + (int64_t)b * astride + i0;
-    for (int j = threadIdx.x; j <= lmax - m; j += blockDim.x) {
-        double re = 0.0, im = 0.0;
-        for (int g = 0; g < ngroups; ++g) {
-            if (lfirst && m + j < lfirst[(int64_t)m * ngroups + g]) continue;
-            const c2<T> v = src[(int64_t)g * nlm + j];
-            re += (double)v.x;
-            im += (double)v.y;
-        }
-        double s = sl[ro + j];
-        if (adjoint) {
-            if (m > 0) s *= 2.0;
-            else im = 0.0;
-        }
-        dst[j] = mk<T>((T)(re * s), (T)(im * s));
-    }
+    reduce_row<T>(partial + (int64_t)b * ngroups * nlm + i0, nlm, ngroups, alm + (int64_t)b * astride + i0,
+                  sl + rofs[m], lmax, m, adjoint, lfirst ? lfirst + (int64_t)m * ngroups : nullptr);
 }
 
-// ---------------------------------------------------------------------------
-// synthesis pre-pass: scaled[row m][l-m] = s_lm * a_lm (halved for m > 0 in
-// the map2alm adjoint), zero-padded rows so chunks never read past a row.
-// ---------------------------------------------------------------------------
 template <typename T>
 __global__ void prescale_kernel(const c2<T> *__restrict__ alm, int64_t astride, c2<T> *__restrict__ scaled,
                                 int64_t nrow, const int64_t *__restrict__ rofs, const double *__restrict__ sl,
                                 int lmax, int adjoint) {
     const int m = blockIdx.x, b = blockIdx.y;
     const int64_t o = rofs[m];
-    const int len = (int)(rofs[m + 1] - o);
-    const int64_t i0 = alm_index(m, m, lmax);
-    const c2<T> *a = alm + (int64_t)b * astride + i0;
-    c2<T> *d = scaled + (int64_t)b * nrow + o;
-    const double f = (adjoint && m > 0) ? 0.5 : 1.0;
-    for (int j = threadIdx.x; j < len; j += blockDim.x) {
-        c2<T> v = mk<T>(0, 0);
-        if (m + j <= lmax) v = cscale<T>(a[j], (T)(sl[o + j] * f));
-        d[j] = v;
-    }
+    prescale_row<T>(alm + (int64_t)b * astride + alm_index(m, m, lmax), scaled + (int64_t)b * nrow + o, sl + o,
+                    (int)(rofs[m + 1] - o), lmax, m, adjoint);
 }
 
 // ---------------------------------------------------------------------------
@@ -437,50 +269,12 @@ leg_synthesis_kernel(const c2<T> *__restrict__ scaled, int64_t nrow, c2<T> *__re
     const int m = mb * MB + w;
     if (m > mmax) return;
     const int p = pb * 64 + lane;
-    const bool valid = p < npairs;
-    const double x = valid ? px[p] : 0.0;
-    const int ls = valid ? lseed[(int64_t)m * npairs + p] : NEVER;
-    const int lstart = wave_min(ls);
-    const int linj = wave_max(ls <= lmax ? ls : -1);
-    c2<T> ge = mk<T>(0, 0), go = mk<T>(0, 0);
-    if (lstart <= lmax) {
-        const int64_t ro = rofs[m];
-        const c2<T> *ar = scaled + (int64_t)b * nrow + ro - m;
-        const double *al = alpha + ro - m;
-        double u2 = 0.0, u1 = 0.0;
-        for (int l0 = lstart; l0 <= lmax; l0 += CHUNK) {
-            if (l0 <= linj && ls == l0) {
-                const double2 sd = seed[(int64_t)m * npairs + p];
-                u2 = sd.x;
-                u1 = sd.y;
-            }
-#pragma unroll
-            for (int i = 0; i < CHUNK; ++i) {
-                const double u = fma(al[l0 + i] * x, u1, -u2);
-                u2 = u1;
-                u1 = u;
-                const T uu = (T)u;
-                const c2<T> av = ar[l0 + i];
-                if (i & 1) {
-                    go.x = fma(uu, av.x, go.x);
-                    go.y = fma(uu, av.y, go.y);
-                } else {
-                    ge.x = fma(uu, av.x, ge.x);
-                    ge.y = fma(uu, av.y, ge.y);
-                }
-            }
-        }
-    }
-    if (!valid) return;
-    // synthesis layout: ring pair p, north (ring pair_n[p]) and south (pair_s[p]) halves
-    c2<T> *out = phase + (int64_t)b * pstride + syn_ofs(m, 0, p, npairs);
-    out[0] = cadd<T>(ge, go);
-    if (pair_s[p] != pair_n[p]) out[(int64_t)pairs8(npairs) * MB] = csub<T>(ge, go);
+    const int64_t ro = rofs[m];
+    const c2<T> *ar[1] = {scaled + (int64_t)b * nrow + ro - m};
+    c2<T> *out[1] = {phase + (int64_t)b * pstride + syn_ofs(m, 0, p < npairs ? p : 0, npairs)};
+    syn_wave<T, 1>(ar, out, pair_n, pair_s, px, alpha + ro - m, lseed, seed, npairs, lmax, m, p);
 }
 
-// ---------------------------------------------------------------------------
-// alm2cl (alm_ops.py:19-32) on m-major alm
-// ---------------------------------------------------------------------------
 template <typename T>
 __global__ void alm2cl_kernel(const c2<T> *__restrict__ alm, int64_t astride, double *cl, int64_t cstride,
                               int lmax, int mmax) {

@@ -1,0 +1,664 @@
+// Mixed-geometry batches in persistent launches (no reference counterpart:
+// the reference transforms one map per call, sht_raw.py:30-57).
+//
+// A batch object (hbx_multi) holds up to HBX_MULTI_MAX_GROUPS plans
+// ("groups": one geometry, lmax and mmax each) and a map count per group.
+// One call transforms every map of every group.  The ring stage runs per
+// group (the plan's batched FFT launches); each Legendre direction is ONE
+// persistent launch whose workgroups pull tiles -- (group, m-block or part
+// of it, ring-pair block, map block) -- from work queues until every tile
+// is done, so the m-blocks of one large map and the tiles of many small
+// maps spread over the whole GPU in a single kernel.
+//
+// Queues: the tiles are split into 8 queues by m-block (all tiles of one
+// m-block, whose 8 m share the 64-byte lines of the analysis phase buffer,
+// sit in one queue), and each queue is ordered by estimated cost, largest
+// first (the cost counts the l steps every wave runs from its first
+// switched-on ring pair).  Workgroup w starts on queue w % 8 -- consecutive
+// workgroups are dispatched round-robin over the 8 XCDs, so a queue's tiles
+// mostly run on one XCD and share its L2 -- and pulls from the next queue in
+// turn once its own is empty.  A pull is one atomicAdd on the queue's
+// counter; a workgroup leaves after a failed pull on each of the 8 queues,
+// so every workgroup exits once all tiles are taken.  The counters live in
+// the call's workspace and are zeroed by the call's first kernel (no memset
+// node), so a call can be captured in a HIP graph and replayed.
+//
+// Tiles run the wave bodies of the per-plan VALU kernels (legendre_valu.h):
+// the analysis with R ring-pair slots per lane (64 R ring pairs of one m and
+// one map per single-wave workgroup, partial sums per ring-pair block reduced
+// in a fixed order), the synthesis with the 8 m of an m-block over 64 ring
+// pairs per 8-wave workgroup and up to BM maps of a group per tile sharing
+// each FP64 recursion step.  The results therefore equal the per-plan VALU
+// path bit for bit when it uses the same slots.  FP32 groups take m < 8 from
+// their plan's FP64 band kernels, as in hbx_map2alm.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "hbx_device.h"
+#include "hbx_internal.h"
+#include "legendre_valu.h"
+
+namespace hbx {
+
+constexpr int MULTI_MAXG = HBX_MULTI_MAX_GROUPS;
+constexpr int NQ = 8;                       // work queues (one per XCD)
+constexpr uint64_t HDR_CALL = 256;          // workspace: counters at 0, per-call pointers at 256
+constexpr uint64_t HDR_BYTES = 512;
+
+// one group's tables and workspace offsets (device copy made at creation)
+struct MultiGeomDev {
+    const int32_t *pair_n, *pair_s, *lseed;
+    const double *px, *alpha, *sl;
+    const int64_t *rofs;
+    const double2 *seed;
+    int npairs, nr, lmax, mmax, ngr, nmaps;
+    int64_t nlm, pstride, nrow;
+    uint64_t phase_off, aux_off;   // bytes into the workspace
+};
+
+// analysis tile: m0 .. m0 + mn - 1 (one m-block), ring-pair block rb of 64 R
+// pairs, map b0 (nb = 1).  Synthesis tile: the m-block starting at m0
+// (mn = 8), pair block rb of 64 pairs, maps b0 .. b0 + nb - 1.
+struct MultiTile {
+    int32_t g, m0, mn, rb, b0, nb;
+};
+
+// one (group, m, map) row of the reduction / pre-scaling passes
+struct MultiRow {
+    int32_t g, m, b, pad;
+};
+
+// per-call pointers, copied into the workspace by the call's first kernel
+struct MultiCall {
+    void *alm[MULTI_MAXG];
+    int64_t astride[MULTI_MAXG];
+};
+static_assert(sizeof(MultiCall) <= HDR_BYTES - HDR_CALL, "per-call header");
+
+__global__ void multi_setup_kernel(MultiCall c, char *ws) {
+    if (threadIdx.x < 2 * NQ) reinterpret_cast<unsigned *>(ws)[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) *reinterpret_cast<MultiCall *>(ws + HDR_CALL) = c;
+}
+
+// next tile index for the calling thread: queues home, home+1, ... in turn;
+// qi (the thread's current queue offset) only grows; -1 once all are empty
+__device__ __forceinline__ int pull_tile(unsigned *queue, const int32_t *__restrict__ qrange, int home, int &qi) {
+    while (qi < NQ) {
+        const int q = (home + qi) & (NQ - 1);
+        const int t = qrange[q] + (int)atomicAdd(queue + q, 1u);
+        if (t < qrange[NQ + q]) return t;
+        ++qi;
+    }
+    return -1;
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64)
+multi_analysis_kernel(const MultiGeomDev *__restrict__ geo, const MultiTile *__restrict__ tiles,
+                      const int32_t *__restrict__ qrange, char *__restrict__ ws) {
+    __shared__ T red[2 * CHUNK * RED_ROW];
+    unsigned *queue = reinterpret_cast<unsigned *>(ws);
+    const MultiCall *call = reinterpret_cast<const MultiCall *>(ws + HDR_CALL);
+    const int lane = threadIdx.x;
+    const int home = blockIdx.x & (NQ - 1);
+    int qi = 0;
+    // one m per trip: the tile's m range is walked by this loop rather than
+    // an inner one, so per-lane addresses are not kept live across it
+    MultiTile tl{0, 0, 0, 0, 0, 0};
+    int m = 0;
+    for (;;) {
+        if (m >= tl.m0 + tl.mn) {
+            int t = 0;
+            if (lane == 0) t = pull_tile(queue, qrange, home, qi);
+            t = uniform(t);   // lane 0's value
+            if (t < 0) break;
+            tl = tiles[t];
+            m = tl.m0;
+        }
+        const MultiGeomDev G = geo[tl.g];
+        const c2<T> *ph = reinterpret_cast<const c2<T> *>(ws + G.phase_off) + (int64_t)tl.b0 * G.pstride;
+        const bool direct = G.ngr == 1;
+        c2<T> *dst = direct ? reinterpret_cast<c2<T> *>(call->alm[tl.g]) + (int64_t)tl.b0 * call->astride[tl.g]
+                            : reinterpret_cast<c2<T> *>(ws + G.aux_off) + ((int64_t)tl.b0 * G.ngr + tl.rb) * G.nlm;
+        ana_wave<T, R>(ph + (int64_t)(m >> 3) * G.nr * MB + (m & 7), dst, direct, G.pair_n, G.pair_s, G.px, G.rofs,
+                       G.alpha, G.sl, G.lseed, G.seed, G.npairs, G.lmax, m, tl.rb, 0, red, lane);
+        ++m;
+    }
+}
+
+template <typename T>
+__global__ void multi_reduce_kernel(const MultiGeomDev *__restrict__ geo, const MultiRow *__restrict__ rows,
+                                    char *__restrict__ ws) {
+    const MultiRow r = rows[blockIdx.x];
+    const MultiGeomDev G = geo[r.g];
+    const MultiCall *call = reinterpret_cast<const MultiCall *>(ws + HDR_CALL);
+    const int64_t i0 = alm_index(r.m, r.m, G.lmax);
+    const c2<T> *src = reinterpret_cast<const c2<T> *>(ws + G.aux_off) + (int64_t)r.b * G.ngr * G.nlm + i0;
+    c2<T> *dst = reinterpret_cast<c2<T> *>(call->alm[r.g]) + (int64_t)r.b * call->astride[r.g] + i0;
+    reduce_row<T>(src, G.nlm, G.ngr, dst, G.sl + G.rofs[r.m], G.lmax, r.m, 0, nullptr);
+}
+
+template <typename T>
+__global__ void multi_prescale_kernel(const MultiGeomDev *__restrict__ geo, const MultiRow *__restrict__ rows,
+                                      char *__restrict__ ws) {
+    const MultiRow r = rows[blockIdx.x];
+    const MultiGeomDev G = geo[r.g];
+    const MultiCall *call = reinterpret_cast<const MultiCall *>(ws + HDR_CALL);
+    const int64_t o = G.rofs[r.m];
+    const c2<T> *a =
+        reinterpret_cast<const c2<T> *>(call->alm[r.g]) + (int64_t)r.b * call->astride[r.g] + alm_index(r.m, r.m, G.lmax);
+    c2<T> *d = reinterpret_cast<c2<T> *>(ws + G.aux_off) + (int64_t)r.b * G.nrow + o;
+    prescale_row<T>(a, d, G.sl + o, (int)(G.rofs[r.m + 1] - o), G.lmax, r.m, 0);
+}
+
+template <typename T, int NB>
+__device__ __forceinline__ void syn_tile(const MultiGeomDev &G, const MultiTile &tl, char *ws, int m, int p) {
+    const int64_t ro = ldc(G.rofs + m);
+    const c2<T> *sc = reinterpret_cast<const c2<T> *>(ws + G.aux_off) + (int64_t)tl.b0 * G.nrow + ro - m;
+    c2<T> *ph = reinterpret_cast<c2<T> *>(ws + G.phase_off) + (int64_t)tl.b0 * G.pstride +
+                syn_ofs(m, 0, p < G.npairs ? p : 0, G.npairs);
+    const c2<T> *ar[NB];
+    c2<T> *out[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        ar[j] = sc + (int64_t)j * G.nrow;
+        out[j] = ph + (int64_t)j * G.pstride;
+    }
+    syn_wave<T, NB>(ar, out, G.pair_n, G.pair_s, G.px, G.alpha + ro - m, G.lseed, G.seed, G.npairs, G.lmax, m, p);
+}
+
+template <typename T, int BM>
+__global__ void __launch_bounds__(512)
+multi_synthesis_kernel(const MultiGeomDev *__restrict__ geo, const MultiTile *__restrict__ tiles,
+                       const int32_t *__restrict__ qrange, char *__restrict__ ws) {
+    __shared__ int s_tile;
+    unsigned *queue = reinterpret_cast<unsigned *>(ws);
+    const int w = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int home = blockIdx.x & (NQ - 1);
+    int qi = 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = pull_tile(queue, qrange, home, qi);
+        __syncthreads();
+        const int t = uniform(s_tile);
+        __syncthreads();   // every wave has read s_tile before thread 0 pulls again
+        if (t < 0) break;
+        const MultiTile tl = tiles[t];
+        const MultiGeomDev G = geo[tl.g];
+        const int m = tl.m0 + w;
+        if (m > G.mmax) continue;
+        const int p = tl.rb * 64 + lane;
+        switch (tl.nb) {
+        case 1: syn_tile<T, 1>(G, tl, ws, m, p); break;
+        case 2: if constexpr (BM >= 2) syn_tile<T, 2>(G, tl, ws, m, p); break;
+        case 3: if constexpr (BM >= 3) syn_tile<T, 3>(G, tl, ws, m, p); break;
+        case 4: if constexpr (BM >= 4) syn_tile<T, 4>(G, tl, ws, m, p); break;
+        default: break;
+        }
+    }
+}
+
+static uint64_t align256(uint64_t v) { return (v + 255) & ~uint64_t(255); }
+
+}  // namespace hbx
+
+using namespace hbx;
+
+struct hbx_multi {
+    int device = 0, precision = 0, ngroups = 0;
+    int slots = 0, syn_maps = 0, tiles_per_wave = 0;
+    std::vector<Plan> plans;               // copies: share each plan's device tables
+    std::vector<int> nmaps, ngr;
+    std::vector<uint64_t> phase_off, aux_off;
+    uint64_t ws_bytes = 0;
+    MultiGeomDev *d_geo = nullptr;
+    MultiTile *d_atiles = nullptr, *d_stiles = nullptr;
+    int32_t *d_aq = nullptr, *d_sq = nullptr;      // [2 * NQ] queue ranges: begin, end
+    MultiRow *d_rrows = nullptr, *d_prows = nullptr;
+    int n_atiles = 0, n_stiles = 0, n_rrows = 0, n_prows = 0;
+    int grid_ana = 0, grid_syn = 0;
+    std::vector<void *> ptrs;
+    ~hbx_multi() {
+        int cur = 0;
+        const bool have = hipGetDevice(&cur) == hipSuccess;
+        if (have && cur != device) (void)hipSetDevice(device);
+        for (void *p : ptrs) (void)hipFree(p);
+        if (have && cur != device) (void)hipSetDevice(cur);
+    }
+};
+
+namespace hbx {
+
+template <typename X> static int upload_vec(hbx_multi *mb, X **dst, const std::vector<X> &v) {
+    *dst = nullptr;
+    if (v.empty()) return HBX_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipMalloc(&ptr, v.size() * sizeof(X));
+    if (e != hipSuccess) return set_error(HBX_ENOMEM, "hipMalloc for the batch tables failed");
+    mb->ptrs.push_back(ptr);
+    HBX_HIP(hipMemcpy(ptr, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
+    *dst = static_cast<X *>(ptr);
+    return HBX_OK;
+}
+
+// tiles sorted into the 8 queues (by m-block), each by cost, largest first
+static void sort_queues(std::vector<MultiTile> &tiles, const std::vector<double> &cost, std::vector<int32_t> &qrange) {
+    std::vector<size_t> idx(tiles.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    auto q_of = [&](size_t i) { return (tiles[i].m0 / MB + 3 * tiles[i].g) & (NQ - 1); };
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+        const int qa = q_of(a), qb = q_of(b);
+        if (qa != qb) return qa < qb;
+        return cost[a] > cost[b];
+    });
+    std::vector<MultiTile> out(tiles.size());
+    qrange.assign(2 * NQ, 0);
+    for (int q = 0; q < NQ; ++q) qrange[q] = qrange[NQ + q] = 0;
+    std::vector<int> cnt(NQ, 0);
+    for (size_t i = 0; i < idx.size(); ++i) {
+        out[i] = tiles[idx[i]];
+        ++cnt[q_of(idx[i])];
+    }
+    int o = 0;
+    for (int q = 0; q < NQ; ++q) {
+        qrange[q] = o;
+        o += cnt[q];
+        qrange[NQ + q] = o;
+    }
+    tiles.swap(out);
+}
+
+static int resident_blocks(const void *kernel, int threads, int *out) {
+    int dev = 0, cus = 0, per = 0;
+    HBX_HIP(hipGetDevice(&dev));
+    HBX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HBX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0));
+    int n = std::max(1, per) * std::max(1, cus);
+    *out = std::max(NQ, n / NQ * NQ);
+    return HBX_OK;
+}
+
+template <typename T> static const void *ana_kernel(int R) {
+    switch (R) {
+    case 1: return reinterpret_cast<const void *>(multi_analysis_kernel<T, 1>);
+    case 2: return reinterpret_cast<const void *>(multi_analysis_kernel<T, 2>);
+    case 4: return reinterpret_cast<const void *>(multi_analysis_kernel<T, 4>);
+    case 8: return reinterpret_cast<const void *>(multi_analysis_kernel<T, 8>);
+    default: return nullptr;
+    }
+}
+template <typename T> static const void *syn_kernel(int BM) {
+    switch (BM) {
+    case 1: return reinterpret_cast<const void *>(multi_synthesis_kernel<T, 1>);
+    case 2: return reinterpret_cast<const void *>(multi_synthesis_kernel<T, 2>);
+    case 4: return reinterpret_cast<const void *>(multi_synthesis_kernel<T, 4>);
+    default: return nullptr;
+    }
+}
+
+// first switched-on l of each wave (64 consecutive ring pairs of one m): the
+// l at which its recursion starts
+static int wave_starts(const Plan *p, std::vector<int> &ws0) {
+    const int np = p->npairs, nw = (np + 63) / 64;
+    std::vector<int32_t> ls((size_t)(p->mmax + 1) * np);
+    HBX_HIP(hipMemcpy(ls.data(), p->d_lseed, ls.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    ws0.assign((size_t)(p->mmax + 1) * nw, NEVER);
+    for (int m = 0; m <= p->mmax; ++m)
+        for (int q = 0; q < np; ++q) {
+            int &v = ws0[(size_t)m * nw + q / 64];
+            v = std::min(v, (int)ls[(size_t)m * np + q]);
+        }
+    return HBX_OK;
+}
+
+static int build_multi(hbx_multi *mb) {
+    const bool f64 = mb->precision == HBX_F64;
+    const int R = mb->slots, BM = mb->syn_maps;
+    const void *ka = f64 ? ana_kernel<double>(R) : ana_kernel<float>(R);
+    const void *ks = f64 ? syn_kernel<double>(BM) : syn_kernel<float>(BM);
+    int rc;
+    if ((rc = resident_blocks(ka, 64, &mb->grid_ana))) return rc;
+    if ((rc = resident_blocks(ks, 512, &mb->grid_syn))) return rc;
+
+    const int G = mb->ngroups;
+    const uint64_t cs = f64 ? 16 : 8;
+    std::vector<MultiGeomDev> geo(G);
+    std::vector<MultiTile> at, st;
+    std::vector<double> acost, scost;
+    std::vector<MultiRow> rrows, prows;
+    uint64_t off = HDR_BYTES;
+    mb->ngr.assign(G, 1);
+    mb->phase_off.assign(G, 0);
+    mb->aux_off.assign(G, 0);
+    // analysis cost per (group, m, ring-pair block): l steps from each
+    // wave's start, summed over the block's waves
+    std::vector<std::vector<double>> cm(G);
+    std::vector<std::vector<int>> wst(G);
+    double total = 0.0;
+    for (int g = 0; g < G; ++g) {
+        const Plan *p = &mb->plans[g];
+        const int n = mb->nmaps[g];
+        const int ngr = (p->npairs + 64 * R - 1) / (64 * R);
+        mb->ngr[g] = ngr;
+        if ((rc = wave_starts(p, wst[g]))) return rc;
+        const std::vector<int> &w0 = wst[g];
+        const int nw = (p->npairs + 63) / 64;
+        cm[g].assign((size_t)(p->mmax + 1) * ngr, 0.0);
+        for (int m = 0; m <= p->mmax; ++m)
+            for (int w = 0; w < nw; ++w) {
+                const int s = std::max(w0[(size_t)m * nw + w], m);
+                if (s <= p->lmax) cm[g][(size_t)m * ngr + w / R] += 64.0 * (p->lmax + 1 - s);
+            }
+        const int mlo = f64 ? 0 : std::min(M_LO, p->mmax + 1);
+        for (int m = mlo; m <= p->mmax; ++m)
+            for (int r = 0; r < ngr; ++r) total += cm[g][(size_t)m * ngr + r] * n;
+        // workspace: phase buffer (+ FP32 band phase), then the aux region:
+        // analysis partial sums | FP32 band alm | band partials, or the
+        // synthesis scaled rows
+        mb->phase_off[g] = off;
+        off += align256(phase_total_bytes(p, n));
+        mb->aux_off[g] = off;
+        uint64_t ana = align256(ngr > 1 ? (uint64_t)n * ngr * p->nlm * cs : 0);
+        if (!f64) {
+            const int g64 = std::max(band_groups(p), band_mfma_groups(p));
+            ana += align256((uint64_t)n * band_nlm(p) * 16);
+            ana += align256(g64 > 1 ? (uint64_t)n * g64 * band_nlm(p) * 16 : 0);
+        }
+        const uint64_t syn = align256((uint64_t)n * p->nrow * cs);
+        off += std::max(ana, syn);
+
+        MultiGeomDev &d = geo[g];
+        d.pair_n = p->d_pair_n;
+        d.pair_s = p->d_pair_s;
+        d.lseed = p->d_lseed;
+        d.px = p->d_px;
+        d.alpha = p->d_alpha;
+        d.sl = p->d_sl;
+        d.rofs = p->d_rofs;
+        d.seed = p->d_seed;
+        d.npairs = p->npairs;
+        d.nr = p->nrings;
+        d.lmax = p->lmax;
+        d.mmax = p->mmax;
+        d.ngr = ngr;
+        d.nmaps = n;
+        d.nlm = p->nlm;
+        d.pstride = phase_elems_per_map(p);
+        d.nrow = p->nrow;
+        d.phase_off = mb->phase_off[g];
+        d.aux_off = mb->aux_off[g];
+    }
+    mb->ws_bytes = off;
+    // analysis tiles: consecutive m of one m-block, about total / (waves *
+    // tiles_per_wave) l-steps each (at least one m)
+    const double target = total / ((double)mb->grid_ana * mb->tiles_per_wave);
+    for (int g = 0; g < G; ++g) {
+        const Plan *p = &mb->plans[g];
+        const int n = mb->nmaps[g], ngr = mb->ngr[g];
+        const int mlo = f64 ? 0 : std::min(M_LO, p->mmax + 1);
+        for (int b = 0; b < n; ++b)
+            for (int r = 0; r < ngr; ++r)
+                for (int m = mlo; m <= p->mmax;) {
+                    const int mend = std::min(p->mmax, (m / MB) * MB + MB - 1);
+                    MultiTile t{g, m, 0, r, b, 1};
+                    double c = 0.0;
+                    while (m <= mend && (t.mn == 0 || c + cm[g][(size_t)m * ngr + r] <= target)) {
+                        c += cm[g][(size_t)m * ngr + r];
+                        ++t.mn;
+                        ++m;
+                    }
+                    at.push_back(t);
+                    acost.push_back(c);
+                }
+        if (ngr > 1)
+            for (int b = 0; b < n; ++b)
+                for (int m = mlo; m <= p->mmax; ++m) rrows.push_back(MultiRow{g, m, b, 0});
+        for (int b = 0; b < n; ++b)
+            for (int m = 0; m <= p->mmax; ++m) prows.push_back(MultiRow{g, m, b, 0});
+        // synthesis tiles: one m-block x 64 ring pairs x up to BM maps
+        const int nw = (p->npairs + 63) / 64;
+        const std::vector<int> &w0 = wst[g];
+        for (int b0 = 0; b0 < n; b0 += BM) {
+            const int nb = std::min(BM, n - b0);
+            for (int m0 = 0; m0 <= p->mmax; m0 += MB)
+                for (int w = 0; w < nw; ++w) {
+                    double c = 0.0;
+                    for (int m = m0; m <= std::min(p->mmax, m0 + MB - 1); ++m) {
+                        const int s = std::max(w0[(size_t)m * nw + w], m);
+                        if (s <= p->lmax) c += 64.0 * (p->lmax + 1 - s) * (1 + nb);
+                    }
+                    st.push_back(MultiTile{g, m0, MB, w, b0, nb});
+                    scost.push_back(c);
+                }
+        }
+    }
+    std::vector<int32_t> aq, sq;
+    sort_queues(at, acost, aq);
+    sort_queues(st, scost, sq);
+    mb->n_atiles = (int)at.size();
+    mb->n_stiles = (int)st.size();
+    mb->n_rrows = (int)rrows.size();
+    mb->n_prows = (int)prows.size();
+    if ((rc = upload_vec(mb, &mb->d_geo, geo))) return rc;
+    if ((rc = upload_vec(mb, &mb->d_atiles, at))) return rc;
+    if ((rc = upload_vec(mb, &mb->d_stiles, st))) return rc;
+    if ((rc = upload_vec(mb, &mb->d_aq, aq))) return rc;
+    if ((rc = upload_vec(mb, &mb->d_sq, sq))) return rc;
+    if ((rc = upload_vec(mb, &mb->d_rrows, rrows))) return rc;
+    if ((rc = upload_vec(mb, &mb->d_prows, prows))) return rc;
+    return HBX_OK;
+}
+
+static int check_call(const hbx_multi *mb, const void *const *a, const int64_t *astr, const void *const *b,
+                      const int64_t *bstr, int32_t map_dtype, void *ws, uint64_t wsb, bool a_is_maps) {
+    if (!mb) return set_error(HBX_EINVAL, "batch is NULL");
+    if (!a || !astr || !b || !bstr) return set_error(HBX_EINVAL, "pointer or stride array is NULL");
+    if (map_dtype != HBX_DT_F64 && map_dtype != HBX_DT_F32) return set_error(HBX_EINVAL, "map_dtype");
+    if (!ws || wsb < mb->ws_bytes)
+        return set_error(HBX_EWORKSPACE, "workspace too small: need " + std::to_string(mb->ws_bytes) + " bytes");
+    int dev = -1;
+    HBX_HIP(hipGetDevice(&dev));
+    if (dev != mb->device)
+        return set_error(HBX_EPLAN, "batch was created on device " + std::to_string(mb->device) + " but device " +
+                                        std::to_string(dev) + " is current");
+    for (int g = 0; g < mb->ngroups; ++g) {
+        if (!a[g] || !b[g]) return set_error(HBX_EINVAL, "data pointer of group " + std::to_string(g) + " is NULL");
+        const int64_t ms = a_is_maps ? astr[g] : bstr[g], as = a_is_maps ? bstr[g] : astr[g];
+        if (ms < mb->plans[g].npix) return set_error(HBX_EINVAL, "map stride of group " + std::to_string(g));
+        if (as < mb->plans[g].nlm) return set_error(HBX_EINVAL, "alm stride of group " + std::to_string(g));
+    }
+    return HBX_OK;
+}
+
+static int launch_setup(const hbx_multi *mb, void *const *alm, const int64_t *astr, char *ws, hipStream_t s) {
+    MultiCall c;
+    std::memset(&c, 0, sizeof(c));
+    for (int g = 0; g < mb->ngroups; ++g) {
+        c.alm[g] = alm[g];
+        c.astride[g] = astr[g];
+    }
+    hipLaunchKernelGGL(multi_setup_kernel, dim3(1), dim3(64), 0, s, c, ws);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "multi_setup_kernel launch");
+    return HBX_OK;
+}
+
+template <typename T>
+static int multi_legendre_analysis(const hbx_multi *mb, char *ws, hipStream_t s) {
+#define HBX_MA(RV)                                                                                        \
+    hipLaunchKernelGGL((multi_analysis_kernel<T, RV>), dim3(mb->grid_ana), dim3(64), 0, s, mb->d_geo,     \
+                       mb->d_atiles, mb->d_aq, ws)
+    timer_mark(HBX_TIMER_LEGENDRE_ANALYSIS, true, s);
+    switch (mb->slots) {
+    case 1: HBX_MA(1); break;
+    case 2: HBX_MA(2); break;
+    case 4: HBX_MA(4); break;
+    case 8: HBX_MA(8); break;
+    default: return set_error(HBX_EUNSUPPORTED, "batch analysis slots");
+    }
+#undef HBX_MA
+    timer_mark(HBX_TIMER_LEGENDRE_ANALYSIS, false, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "multi_analysis_kernel launch");
+    if (mb->n_rrows > 0) {
+        hipLaunchKernelGGL(multi_reduce_kernel<T>, dim3(mb->n_rrows), dim3(256), 0, s, mb->d_geo, mb->d_rrows, ws);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "multi_reduce_kernel launch");
+    }
+    return HBX_OK;
+}
+
+template <typename T>
+static int multi_legendre_synthesis(const hbx_multi *mb, char *ws, hipStream_t s) {
+    hipLaunchKernelGGL(multi_prescale_kernel<T>, dim3(mb->n_prows), dim3(256), 0, s, mb->d_geo, mb->d_prows, ws);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "multi_prescale_kernel launch");
+#define HBX_MS(BV)                                                                                        \
+    hipLaunchKernelGGL((multi_synthesis_kernel<T, BV>), dim3(mb->grid_syn), dim3(512), 0, s, mb->d_geo,   \
+                       mb->d_stiles, mb->d_sq, ws)
+    timer_mark(HBX_TIMER_LEGENDRE_SYNTHESIS, true, s);
+    switch (mb->syn_maps) {
+    case 1: HBX_MS(1); break;
+    case 2: HBX_MS(2); break;
+    case 4: HBX_MS(4); break;
+    default: return set_error(HBX_EUNSUPPORTED, "batch synthesis maps per tile");
+    }
+#undef HBX_MS
+    timer_mark(HBX_TIMER_LEGENDRE_SYNTHESIS, false, s);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "multi_synthesis_kernel launch");
+    return HBX_OK;
+}
+
+}  // namespace hbx
+
+extern "C" {
+
+int hbx_multi_create(const hbx_plan *const *plans, const int32_t *nmaps, int32_t ngroups,
+                     const hbx_multi_tiles *tiles, hbx_multi **out) {
+    if (!out) return set_error(HBX_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!plans || !nmaps) return set_error(HBX_EINVAL, "plans or nmaps is NULL");
+    if (ngroups < 1 || ngroups > MULTI_MAXG)
+        return set_error(HBX_EINVAL, "ngroups must be in [1, " + std::to_string(MULTI_MAXG) + "]");
+    int dev = -1;
+    HBX_HIP(hipGetDevice(&dev));
+    const int prec = plans[0] ? plans[0]->impl.precision : 0;
+    for (int g = 0; g < ngroups; ++g) {
+        if (!plans[g]) return set_error(HBX_EINVAL, "plan of group " + std::to_string(g) + " is NULL");
+        const Plan &p = plans[g]->impl;
+        if (p.spin != 0) return set_error(HBX_EPLAN, "batches take spin-0 plans");
+        if (p.precision != prec) return set_error(HBX_EINVAL, "every plan of a batch must have the same precision");
+        if (p.device != dev)
+            return set_error(HBX_EPLAN, "plan of group " + std::to_string(g) + " was created on device " +
+                                            std::to_string(p.device) + " but device " + std::to_string(dev) +
+                                            " is current");
+        if (nmaps[g] < 1 || nmaps[g] > 65535)
+            return set_error(HBX_EINVAL, "nmaps of group " + std::to_string(g) + " must be in [1, 65535]");
+    }
+    hbx_multi_tiles t{};
+    if (tiles) t = *tiles;
+    const int slots = t.valu_slots > 0 ? t.valu_slots : (prec == HBX_F64 ? 8 : 4);
+    const int syn_maps = t.syn_maps > 0 ? t.syn_maps : 4;
+    const int tpw = t.tiles_per_wave > 0 ? t.tiles_per_wave : 8;
+    if (slots != 1 && slots != 2 && slots != 4 && slots != 8)
+        return set_error(HBX_EINVAL, "valu_slots must be 1, 2, 4 or 8");
+    if (syn_maps != 1 && syn_maps != 2 && syn_maps != 4) return set_error(HBX_EINVAL, "syn_maps must be 1, 2 or 4");
+    if (tpw > 1024) return set_error(HBX_EINVAL, "tiles_per_wave must be <= 1024");
+    auto *mb = new (std::nothrow) hbx_multi;
+    if (!mb) return set_error(HBX_ENOMEM, "host allocation failed");
+    mb->device = dev;
+    mb->precision = prec;
+    mb->ngroups = ngroups;
+    mb->slots = slots;
+    mb->syn_maps = syn_maps;
+    mb->tiles_per_wave = tpw;
+    for (int g = 0; g < ngroups; ++g) {
+        mb->plans.push_back(plans[g]->impl);
+        mb->nmaps.push_back(nmaps[g]);
+    }
+    const int rc = build_multi(mb);
+    if (rc) {
+        delete mb;
+        return rc;
+    }
+    *out = mb;
+    return HBX_OK;
+}
+
+int hbx_multi_destroy(hbx_multi *mb) {
+    delete mb;
+    return HBX_OK;
+}
+
+int hbx_multi_workspace_bytes(const hbx_multi *mb, uint64_t *bytes) {
+    if (!mb || !bytes) return set_error(HBX_EINVAL, "NULL argument");
+    *bytes = mb->ws_bytes;
+    return HBX_OK;
+}
+
+int hbx_multi_get_info(const hbx_multi *mb, hbx_multi_info *info) {
+    if (!mb || !info) return set_error(HBX_EINVAL, "NULL argument");
+    std::memset(info, 0, sizeof(*info));
+    info->ngroups = mb->ngroups;
+    info->precision = mb->precision;
+    info->valu_slots = mb->slots;
+    info->syn_maps = mb->syn_maps;
+    info->tiles_per_wave = mb->tiles_per_wave;
+    info->analysis_tiles = mb->n_atiles;
+    info->synthesis_tiles = mb->n_stiles;
+    info->analysis_workgroups = mb->grid_ana;
+    info->synthesis_workgroups = mb->grid_syn;
+    info->workspace_bytes = mb->ws_bytes;
+    return HBX_OK;
+}
+
+int hbx_multi_map2alm(const hbx_multi *mb, const void *const *maps, const int64_t *map_stride, int32_t map_dtype,
+                      void *const *alm, const int64_t *alm_stride, void *ws, uint64_t ws_bytes, void *stream) {
+    int rc = check_call(mb, maps, map_stride, alm, alm_stride, map_dtype, ws, ws_bytes, true);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    char *w = static_cast<char *>(ws);
+    if ((rc = launch_setup(mb, alm, alm_stride, w, s))) return rc;
+    const bool f64 = mb->precision == HBX_F64;
+    for (int g = 0; g < mb->ngroups; ++g) {
+        const Plan *p = &mb->plans[g];
+        if ((rc = ring_analysis(p, mb->nmaps[g], maps[g], map_dtype, map_stride[g], w + mb->phase_off[g],
+                                p->d_scale_ana, s)))
+            return rc;
+    }
+    if (!f64)
+        for (int g = 0; g < mb->ngroups; ++g) {
+            const Plan *p = &mb->plans[g];
+            const int n = mb->nmaps[g];
+            const uint64_t cs = 8;
+            char *aux = w + mb->aux_off[g];
+            char *band_alm = aux + align256(mb->ngr[g] > 1 ? (uint64_t)n * mb->ngr[g] * p->nlm * cs : 0);
+            char *band_part = band_alm + align256((uint64_t)n * band_nlm(p) * 16);
+            if ((rc = launch_leg_analysis_band(p, n, band_phase_ptr(p, n, w + mb->phase_off[g]), alm[g],
+                                               alm_stride[g], band_alm, band_part, 0, s)))
+                return rc;
+        }
+    return f64 ? multi_legendre_analysis<double>(mb, w, s) : multi_legendre_analysis<float>(mb, w, s);
+}
+
+int hbx_multi_alm2map(const hbx_multi *mb, const void *const *alm, const int64_t *alm_stride, void *const *maps,
+                      const int64_t *map_stride, int32_t map_dtype, void *ws, uint64_t ws_bytes, void *stream) {
+    int rc = check_call(mb, alm, alm_stride, maps, map_stride, map_dtype, ws, ws_bytes, false);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    char *w = static_cast<char *>(ws);
+    if ((rc = launch_setup(mb, const_cast<void *const *>(alm), alm_stride, w, s))) return rc;
+    const bool f64 = mb->precision == HBX_F64;
+    rc = f64 ? multi_legendre_synthesis<double>(mb, w, s) : multi_legendre_synthesis<float>(mb, w, s);
+    if (rc) return rc;
+    for (int g = 0; g < mb->ngroups; ++g) {
+        const Plan *p = &mb->plans[g];
+        if ((rc = launch_ring_synthesis(p, mb->nmaps[g], w + mb->phase_off[g], maps[g], map_dtype, map_stride[g],
+                                        nullptr, s)))
+            return rc;
+    }
+    return HBX_OK;
+}
+
+}  // extern "C"

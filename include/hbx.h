@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define HBX_ABI_VERSION 2
+#define HBX_ABI_VERSION 3
 
 typedef enum hbx_status {
     HBX_OK = 0,
@@ -274,6 +274,53 @@ int hbx_pix2ang_ring(int64_t n, const int64_t *ipix, int32_t nrings,
                      const double *theta_centers, const int64_t *ring_start,
                      const int32_t *nphi, const double *phi0, double *theta,
                      double *phi, void *stream);
+
+/* Mixed-geometry batches in persistent launches (no reference counterpart:
+ * the reference transforms one map per call, sht_raw.py:30-57 and
+ * isht.py:5-26; this is the batched form of those two entry points).  A
+ * batch holds ngroups spin-0 plans of one precision (each its own geometry,
+ * lmax and mmax) with nmaps[g] maps each; it copies what it needs, so the
+ * plans may be destroyed afterwards.  hbx_multi_map2alm / hbx_multi_alm2map
+ * transform every map of every group: the ring stage runs per group, and
+ * each Legendre direction is ONE persistent launch whose workgroups take
+ * (group, m-block, ring-pair block, map block) tiles from work queues, so
+ * the work of large and small maps spreads over the whole GPU in one
+ * kernel.  The Legendre tiles run the VALU kernels' arithmetic: results
+ * equal hbx_map2alm / hbx_alm2map of each group with those kernels
+ * (valu_slots equal to the plan's, mfma_min_maps above the batch) bit for
+ * bit.  Per group g: maps[g] (nmaps[g] maps, map_stride[g] apart, dtype
+ * map_dtype) and alm[g] (alm_stride[g] apart).  ws: at least
+ * hbx_multi_workspace_bytes(); it also holds the call's work-queue counters,
+ * so calls that overlap in time need their own workspaces.  Calls never
+ * allocate or synchronise and can be captured in a HIP graph.
+ * Tiles (0 = default):
+ *   valu_slots      analysis ring-pair slots of 64 pairs per lane: 1, 2, 4, 8
+ *                   (default 4 for FP32, 8 for FP64)
+ *   syn_maps        maps of one group per synthesis tile sharing each
+ *                   recursion step: 1, 2, 4 (default 4)
+ *   tiles_per_wave  analysis tiles per resident wave the m ranges are cut
+ *                   for (default 8; larger = smaller tiles, finer balance) */
+#define HBX_MULTI_MAX_GROUPS 16
+typedef struct hbx_multi hbx_multi;
+typedef struct hbx_multi_tiles {
+    int32_t valu_slots, syn_maps, tiles_per_wave, reserved;
+} hbx_multi_tiles;
+typedef struct hbx_multi_info {
+    int32_t ngroups, precision, valu_slots, syn_maps, tiles_per_wave;
+    int32_t analysis_tiles, synthesis_tiles, analysis_workgroups, synthesis_workgroups, reserved;
+    uint64_t workspace_bytes;
+} hbx_multi_info;
+int hbx_multi_create(const hbx_plan *const *plans, const int32_t *nmaps, int32_t ngroups,
+                     const hbx_multi_tiles *tiles, hbx_multi **out);
+int hbx_multi_destroy(hbx_multi *batch);
+int hbx_multi_workspace_bytes(const hbx_multi *batch, uint64_t *bytes);
+int hbx_multi_get_info(const hbx_multi *batch, hbx_multi_info *info);
+int hbx_multi_map2alm(const hbx_multi *batch, const void *const *maps, const int64_t *map_stride,
+                      int32_t map_dtype, void *const *alm, const int64_t *alm_stride, void *ws,
+                      uint64_t ws_bytes, void *stream);
+int hbx_multi_alm2map(const hbx_multi *batch, const void *const *alm, const int64_t *alm_stride,
+                      void *const *maps, const int64_t *map_stride, int32_t map_dtype, void *ws,
+                      uint64_t ws_bytes, void *stream);
 
 /* Kernel timers (diagnostics; no reference counterpart).  hbx_timer_arm(mask)
  * arms the kernels whose bits (1 << HBX_TIMER_*) are set and clears their

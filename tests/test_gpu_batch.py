@@ -151,17 +151,19 @@ def _c3_specs(rng):
     return [(n, 2 * n) for n in specs]
 
 
-def test_mixed_batch_graph_replay_equals_eager(gpu, tmp_path, monkeypatch):
+@pytest.mark.parametrize("mode", ["persistent", "grouped"])
+def test_mixed_batch_graph_replay_equals_eager(gpu, tmp_path, monkeypatch, mode):
     """The 64-map mixed batch (lmax = 2 nside, C_l = (l+10)^-2), FP32, tuned
-    tiles, one graph launch: replays equal the eager run bit for bit and every
-    map's alm is within eps_max 1e-5 of its FP64 transform."""
+    tiles, one graph launch (persistent Legendre launches over all groups, or
+    one batched call per group): replays equal the eager run bit for bit and
+    every map's alm is within eps_max 1e-5 of its FP64 transform."""
     from hellblast_amd import autotune
     from hellblast_amd.batch import MixedBatch
     monkeypatch.setenv(autotune.CACHE_ENV, str(tmp_path / "tiles.json"))
     rng = np.random.default_rng(3)
     specs = _c3_specs(rng)
-    print(f"[case] mixed batch of {len(specs)} maps", flush=True)
-    batch = MixedBatch(specs, precision="f32", tune=True)
+    print(f"[case] mixed batch of {len(specs)} maps, {mode}", flush=True)
+    batch = MixedBatch(specs, precision="f32", tune=True, mode=mode)
     # inputs: FP64 syntheses of C_l = (l+10)^-2 alm, stored as FP32 maps
     maps64 = []
     for i, (nside, lmax) in enumerate(specs):
@@ -190,7 +192,7 @@ def test_mixed_batch_graph_replay_equals_eager(gpu, tmp_path, monkeypatch):
     for i, (nside, lmax) in enumerate(specs):
         ref = _plan(nside, lmax).map2alm(batch.map(i).double()).cpu().numpy()
         worst = max(worst, _eps_max(eager_alm[i].cpu().numpy(), ref, lmax))
-    print(f"worst eps_max over the batch {worst:.2e}; tiles {batch.tiles}")
+    print(f"worst eps_max over the batch {worst:.2e}; tiles {batch.tiles} {getattr(batch, 'multi_tiles', '')}")
     assert worst <= 1e-5
 
 
