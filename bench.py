@@ -527,6 +527,27 @@ def mixed_main(args):
         runs[mode] = r
         del batch
         torch.cuda.empty_cache()
+    # each group alone: Legendre kernels of its plan (HIP events), with the
+    # VALU kernels (4 slots, the persistent tiles' arithmetic) and with the
+    # plan's default kernels (matrix cores for 3+ maps)
+    groups = {}
+    for nside, count in alloc.items():
+        lmax = 2 * nside
+        p = _plan(nside, lmax, lmax, "f32", "dz", "reference", "reference")
+        mp = torch.randn((count, p.npix), device=dev)
+        row = {"maps": count}
+        for name, q in (("valu", p.with_tiles(valu_slots=4, mfma_min_maps=100000)), ("default", p)):
+            a = q.map2alm(mp)
+            o = torch.empty_like(mp)
+            with KernelTimer("legendre_analysis", "legendre_synthesis") as kt:
+                for _ in range(args.steps):
+                    q.map2alm(mp, out=a)
+                    q.alm2map(a, dtype=torch.float32, out=o)
+                torch.cuda.synchronize()
+            row[f"{name}_analysis_ms"] = round(kt.mean_ms("legendre_analysis"), 3)
+            row[f"{name}_synthesis_ms"] = round(kt.mean_ms("legendre_synthesis"), 3)
+        groups[f"nside{nside}_lmax{lmax}"] = row
+    log(f"groups alone: {groups}")
     best = min(runs, key=lambda k: runs[k]["graph_ms"])
     rb = runs[best]
     npix = sum(12 * n * n for n in sides)
@@ -541,7 +562,8 @@ def mixed_main(args):
                       "tiles": rb["tiles"],
                       "modes": {k: {"graph_ms_per_step": round(v["graph_ms"], 3),
                                     "eager_ms_per_step": round(v["eager_ms"], 3), "eps_max_worst_map": v["eps"],
-                                    **v.get("timers", {})} for k, v in runs.items()}}}
+                                    **v.get("timers", {})} for k, v in runs.items()},
+                      "groups_alone": groups}}
     print(json.dumps(res), flush=True)
 
 
