@@ -32,6 +32,7 @@
 // path bit for bit when it uses the same slots.  FP32 groups take m < 8 from
 // their plan's FP64 band kernels, as in hbx_map2alm.
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <new>
 #include <string>
@@ -434,6 +435,9 @@ static int build_multi(hbx_multi *mb) {
                 }
         }
     }
+    const size_t lim = (size_t)INT32_MAX;
+    if (at.size() > lim || st.size() > lim || rrows.size() > lim || prows.size() > lim)
+        return set_error(HBX_EUNSUPPORTED, "batch too large: more than 2^31 tiles or rows");
     std::vector<int32_t> aq, sq;
     sort_queues(at, acost, aq);
     sort_queues(st, scost, sq);
