@@ -80,6 +80,8 @@ void *band_phase_ptr(const Plan *p, int nmaps, const void *phase) {
     return static_cast<char *>(const_cast<void *>(phase)) + phase_main_bytes(p, nmaps);
 }
 
+uint64_t band_phase_offset(const Plan *p, int nmaps) { return phase_main_bytes(p, nmaps); }
+
 struct AnalysisWs {
     uint64_t partial = 0, band_alm = 0, band_partial = 0, lfirst = 0, total = 0;
 };

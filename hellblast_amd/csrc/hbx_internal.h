@@ -155,6 +155,42 @@ int launch_ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dty
 int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *maps,
                           int map_dtype, int64_t map_stride, const double *scale,
                           hipStream_t s);
+
+// Fused ring launches of mixed-geometry batches: one launch runs the rings
+// of every plan ("group") of the batch whose size classes share a workgroup
+// size.  A segment is one group's rings of one class (or, for the FP64 band
+// DFT, all of its rings) and owns blocks block0 .. of the launch; pixels
+// and per-call pointers come from tables in device memory (maps_tab[g],
+// mstride_tab[g]), phase buffers from the batch workspace.
+struct RingSeg {
+    const int32_t *rings;          // ring indices of the class (null: band DFT)
+    const int64_t *ring_start;
+    const int32_t *nphi, *mcut;
+    const double *phi0, *scale;
+    const double2 *rot;
+    const void *tw, *V;            // pass twiddles and chirp spectra, plan precision
+    const int64_t *vofs;
+    int64_t pstride;               // phase elements per map
+    uint64_t phase_off;            // bytes into the workspace
+    int32_t block0, ncls, nmaps, g, M, log2M, mmax, nr;
+};
+struct RingLaunch {
+    int threads = 0, blue = 0, nseg = 0;
+    size_t lds = 0;
+    unsigned grid_ana = 0, grid_syn = 0;
+    RingSeg *d_ana = nullptr, *d_syn = nullptr;
+};
+// launches for the groups' FFT classes, largest workgroups first, and (FP32
+// batches) the band DFT launch; device tables are appended to allocs
+int build_ring_launches(const std::vector<Plan> &plans, const std::vector<int> &nmaps,
+                        const std::vector<uint64_t> &phase_off, std::vector<RingLaunch> &out,
+                        RingLaunch &band, std::vector<void *> &allocs);
+int launch_multi_ring_analysis(const RingLaunch &L, int precision, int map_dtype, const void *const *maps_tab,
+                               const int64_t *mstride_tab, char *ws, hipStream_t s);
+int launch_multi_ring_synthesis(const RingLaunch &L, int precision, int map_dtype, void *const *maps_tab,
+                                const int64_t *mstride_tab, char *ws, hipStream_t s);
+int launch_multi_ring_band(const RingLaunch &L, int map_dtype, const void *const *maps_tab,
+                           const int64_t *mstride_tab, char *ws, hipStream_t s);
 // recursion rows: row m holds l = m .. m + len - 1 (len a multiple of CHUNK);
 // spin plans start their chunks at l0 = max(m, |s|) and get longer rows
 int plan_build_recursion(Plan *p, const std::vector<double> &pair_cos, const std::vector<double> &pair_sin,
@@ -239,6 +275,7 @@ int64_t scaled_elems_per_map(const Plan *p);  // complex elements
 // (FP32 plans: the band ring DFT on the plan's side stream)
 uint64_t phase_total_bytes(const Plan *p, int nmaps);
 void *band_phase_ptr(const Plan *p, int nmaps, const void *phase);
+uint64_t band_phase_offset(const Plan *p, int nmaps);   // bytes from the phase buffer's start
 int ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride, void *phase,
                   const double *scale, hipStream_t s);
 

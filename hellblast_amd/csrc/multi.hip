@@ -3,12 +3,14 @@
 //
 // A batch object (hbx_multi) holds up to HBX_MULTI_MAX_GROUPS plans
 // ("groups": one geometry, lmax and mmax each) and a map count per group.
-// One call transforms every map of every group.  The ring stage runs per
-// group (the plan's batched FFT launches); each Legendre direction is ONE
+// One call transforms every map of every group.  The ring stage runs in
+// fused launches, one per workgroup size of the FFT size classes, each over
+// the rings of every group (ring_fft.hip); each Legendre direction is ONE
 // persistent launch whose workgroups pull tiles -- (group, m-block or part
 // of it, ring-pair block, map block) -- from work queues until every tile
 // is done, so the m-blocks of one large map and the tiles of many small
-// maps spread over the whole GPU in a single kernel.
+// maps spread over the whole GPU in a single kernel.  A call therefore
+// issues a fixed number of launches, whatever the number of groups.
 //
 // Queues: the tiles are split into 8 queues by m-block (all tiles of one
 // m-block, whose 8 m share the 64-byte lines of the analysis phase buffer,
@@ -29,11 +31,16 @@
 // in a fixed order), the synthesis with the 8 m of an m-block over 64 ring
 // pairs per 8-wave workgroup and up to BM maps of a group per tile sharing
 // each FP64 recursion step.  The results therefore equal the per-plan VALU
-// path bit for bit when it uses the same slots.  FP32 groups take m < 8 from
-// their plan's FP64 band kernels, as in hbx_map2alm.
+// path bit for bit when it uses the same slots.  FP32 groups take m < 8 in
+// FP64, as in hbx_map2alm: one band-DFT launch over every group's rings
+// (the per-plan VALU band DFT) and one persistent FP64 launch over band
+// tiles with min(4, slots) ring-pair slots (the per-plan VALU band), which
+// takes the second set of 8 queue counters.
 #include <algorithm>
 #include <climits>
+#include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -47,7 +54,7 @@ namespace hbx {
 constexpr int MULTI_MAXG = HBX_MULTI_MAX_GROUPS;
 constexpr int NQ = 8;                       // work queues (one per XCD)
 constexpr uint64_t HDR_CALL = 256;          // workspace: counters at 0, per-call pointers at 256
-constexpr uint64_t HDR_BYTES = 512;
+constexpr uint64_t HDR_BYTES = 1024;
 
 // one group's tables and workspace offsets (device copy made at creation)
 struct MultiGeomDev {
@@ -58,6 +65,7 @@ struct MultiGeomDev {
     int npairs, nr, lmax, mmax, ngr, nmaps;
     int64_t nlm, pstride, nrow;
     uint64_t phase_off, aux_off;   // bytes into the workspace
+    uint64_t out_off;              // analysis result: 0 = the call's alm, else rows of nlm in the workspace
 };
 
 // analysis tile: m0 .. m0 + mn - 1 (one m-block), ring-pair block rb of 64 R
@@ -76,6 +84,8 @@ struct MultiRow {
 struct MultiCall {
     void *alm[MULTI_MAXG];
     int64_t astride[MULTI_MAXG];
+    void *maps[MULTI_MAXG];
+    int64_t mstride[MULTI_MAXG];
 };
 static_assert(sizeof(MultiCall) <= HDR_BYTES - HDR_CALL, "per-call header");
 
@@ -99,9 +109,9 @@ __device__ __forceinline__ int pull_tile(unsigned *queue, const int32_t *__restr
 template <typename T, int R>
 __global__ void __launch_bounds__(64)
 multi_analysis_kernel(const MultiGeomDev *__restrict__ geo, const MultiTile *__restrict__ tiles,
-                      const int32_t *__restrict__ qrange, char *__restrict__ ws) {
+                      const int32_t *__restrict__ qrange, char *__restrict__ ws, int qoff) {
     __shared__ T red[2 * CHUNK * RED_ROW];
-    unsigned *queue = reinterpret_cast<unsigned *>(ws);
+    unsigned *queue = reinterpret_cast<unsigned *>(ws) + qoff;
     const MultiCall *call = reinterpret_cast<const MultiCall *>(ws + HDR_CALL);
     const int lane = threadIdx.x;
     const int home = blockIdx.x & (NQ - 1);
@@ -122,8 +132,9 @@ multi_analysis_kernel(const MultiGeomDev *__restrict__ geo, const MultiTile *__r
         const MultiGeomDev G = geo[tl.g];
         const c2<T> *ph = reinterpret_cast<const c2<T> *>(ws + G.phase_off) + (int64_t)tl.b0 * G.pstride;
         const bool direct = G.ngr == 1;
-        c2<T> *dst = direct ? reinterpret_cast<c2<T> *>(call->alm[tl.g]) + (int64_t)tl.b0 * call->astride[tl.g]
-                            : reinterpret_cast<c2<T> *>(ws + G.aux_off) + ((int64_t)tl.b0 * G.ngr + tl.rb) * G.nlm;
+        c2<T> *dst = !direct   ? reinterpret_cast<c2<T> *>(ws + G.aux_off) + ((int64_t)tl.b0 * G.ngr + tl.rb) * G.nlm
+                     : G.out_off ? reinterpret_cast<c2<T> *>(ws + G.out_off) + (int64_t)tl.b0 * G.nlm
+                                 : reinterpret_cast<c2<T> *>(call->alm[tl.g]) + (int64_t)tl.b0 * call->astride[tl.g];
         ana_wave<T, R>(ph + (int64_t)(m >> 3) * G.nr * MB + (m & 7), dst, direct, G.pair_n, G.pair_s, G.px, G.rofs,
                        G.alpha, G.sl, G.lseed, G.seed, G.npairs, G.lmax, m, tl.rb, 0, red, lane);
         ++m;
@@ -138,8 +149,25 @@ __global__ void multi_reduce_kernel(const MultiGeomDev *__restrict__ geo, const 
     const MultiCall *call = reinterpret_cast<const MultiCall *>(ws + HDR_CALL);
     const int64_t i0 = alm_index(r.m, r.m, G.lmax);
     const c2<T> *src = reinterpret_cast<const c2<T> *>(ws + G.aux_off) + (int64_t)r.b * G.ngr * G.nlm + i0;
-    c2<T> *dst = reinterpret_cast<c2<T> *>(call->alm[r.g]) + (int64_t)r.b * call->astride[r.g] + i0;
+    c2<T> *dst = (G.out_off ? reinterpret_cast<c2<T> *>(ws + G.out_off) + (int64_t)r.b * G.nlm
+                            : reinterpret_cast<c2<T> *>(call->alm[r.g]) + (int64_t)r.b * call->astride[r.g]) +
+                 i0;
     reduce_row<T>(src, G.nlm, G.ngr, dst, G.sl + G.rofs[r.m], G.lmax, r.m, 0, nullptr);
+}
+
+// FP32 batches: the FP64 band alm (rows m < M_LO, the first nlm entries of
+// the m-major layout) of one (group, map) row rounded into the call's alm
+__global__ void multi_band_convert_kernel(const MultiGeomDev *__restrict__ bgeo, const MultiRow *__restrict__ rows,
+                                          char *__restrict__ ws) {
+    const MultiRow r = rows[blockIdx.x];
+    const MultiGeomDev G = bgeo[r.g];
+    const MultiCall *call = reinterpret_cast<const MultiCall *>(ws + HDR_CALL);
+    const double2 *src = reinterpret_cast<const double2 *>(ws + G.out_off) + (int64_t)r.b * G.nlm;
+    float2 *dst = reinterpret_cast<float2 *>(call->alm[r.g]) + (int64_t)r.b * call->astride[r.g];
+    for (int64_t i = threadIdx.x; i < G.nlm; i += blockDim.x) {
+        const double2 v = src[i];
+        dst[i] = make_float2((float)v.x, (float)v.y);
+    }
 }
 
 template <typename T>
@@ -220,12 +248,31 @@ struct hbx_multi {
     MultiRow *d_rrows = nullptr, *d_prows = nullptr;
     int n_atiles = 0, n_stiles = 0, n_rrows = 0, n_prows = 0;
     int grid_ana = 0, grid_syn = 0;
+    // ring stage: fused launches over every group's size classes
+    std::vector<RingLaunch> rings;
+    RingLaunch band_dft;
+    // FP32 batches, FP64 band (m < M_LO): persistent tiles, reduction rows,
+    // conversion rows (one per group and map)
+    int band_slots = 0, grid_band = 0, n_btiles = 0, n_brrows = 0, n_bcrows = 0;
+    MultiGeomDev *d_bgeo = nullptr;
+    MultiTile *d_btiles = nullptr;
+    int32_t *d_bq = nullptr;
+    MultiRow *d_brrows = nullptr, *d_bcrows = nullptr;
+    // side stream of the band chain (non-blocking, default priority) and the
+    // events that fork it from and join it to the caller's stream, used
+    // under side_mu from fork to join
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    mutable std::mutex side_mu;
     std::vector<void *> ptrs;
     ~hbx_multi() {
         int cur = 0;
         const bool have = hipGetDevice(&cur) == hipSuccess;
         if (have && cur != device) (void)hipSetDevice(device);
         for (void *p : ptrs) (void)hipFree(p);
+        if (side) (void)hipStreamDestroy(side);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         if (have && cur != device) (void)hipSetDevice(cur);
     }
 };
@@ -314,44 +361,96 @@ static int wave_starts(const Plan *p, std::vector<int> &ws0) {
     return HBX_OK;
 }
 
+// Analysis cost per (m, ring-pair block of 64 R pairs): the l steps every
+// wave of the block runs from its first switched-on l
+static std::vector<double> block_costs(const Plan *p, const std::vector<int> &w0, int R) {
+    const int nw = (p->npairs + 63) / 64, ngr = (p->npairs + 64 * R - 1) / (64 * R);
+    std::vector<double> c((size_t)(p->mmax + 1) * ngr, 0.0);
+    for (int m = 0; m <= p->mmax; ++m)
+        for (int w = 0; w < nw; ++w) {
+            const int s = std::max(w0[(size_t)m * nw + w], m);
+            if (s <= p->lmax) c[(size_t)m * ngr + w / R] += 64.0 * (p->lmax + 1 - s);
+        }
+    return c;
+}
+
+// analysis tiles of one group for m in [mlo, mhi]: consecutive m of one
+// m-block, about `target` l-steps each (at least one m)
+static void analysis_tiles(int g, int nmaps, int ngr, int mlo, int mhi, const std::vector<double> &cm,
+                           double target, std::vector<MultiTile> &at, std::vector<double> &acost) {
+    for (int b = 0; b < nmaps; ++b)
+        for (int r = 0; r < ngr; ++r)
+            for (int m = mlo; m <= mhi;) {
+                const int mend = std::min(mhi, (m / MB) * MB + MB - 1);
+                MultiTile t{g, m, 0, r, b, 1};
+                double c = 0.0;
+                while (m <= mend && (t.mn == 0 || c + cm[(size_t)m * ngr + r] <= target)) {
+                    c += cm[(size_t)m * ngr + r];
+                    ++t.mn;
+                    ++m;
+                }
+                at.push_back(t);
+                acost.push_back(c);
+            }
+}
+
+static MultiGeomDev geom_of(const Plan *p, int nmaps, int ngr) {
+    MultiGeomDev d{};
+    d.pair_n = p->d_pair_n;
+    d.pair_s = p->d_pair_s;
+    d.lseed = p->d_lseed;
+    d.px = p->d_px;
+    d.alpha = p->d_alpha;
+    d.sl = p->d_sl;
+    d.rofs = p->d_rofs;
+    d.seed = p->d_seed;
+    d.npairs = p->npairs;
+    d.nr = p->nrings;
+    d.lmax = p->lmax;
+    d.mmax = p->mmax;
+    d.ngr = ngr;
+    d.nmaps = nmaps;
+    d.nlm = p->nlm;
+    d.pstride = phase_elems_per_map(p);
+    d.nrow = p->nrow;
+    return d;
+}
+
 static int build_multi(hbx_multi *mb) {
     const bool f64 = mb->precision == HBX_F64;
     const int R = mb->slots, BM = mb->syn_maps;
+    // FP64 band of FP32 batches: at most 4 slots (the per-plan VALU band's
+    // limit), so a plan tiled with the batch's slots runs the same band
+    const int RB = std::min(4, R);
+    mb->band_slots = f64 ? 0 : RB;
     const void *ka = f64 ? ana_kernel<double>(R) : ana_kernel<float>(R);
     const void *ks = f64 ? syn_kernel<double>(BM) : syn_kernel<float>(BM);
     int rc;
     if ((rc = resident_blocks(ka, 64, &mb->grid_ana))) return rc;
     if ((rc = resident_blocks(ks, 512, &mb->grid_syn))) return rc;
+    if (!f64 && (rc = resident_blocks(ana_kernel<double>(RB), 64, &mb->grid_band))) return rc;
 
     const int G = mb->ngroups;
     const uint64_t cs = f64 ? 16 : 8;
-    std::vector<MultiGeomDev> geo(G);
-    std::vector<MultiTile> at, st;
-    std::vector<double> acost, scost;
-    std::vector<MultiRow> rrows, prows;
+    std::vector<MultiGeomDev> geo(G), bgeo;
+    std::vector<MultiTile> at, st, bt;
+    std::vector<double> acost, scost, bcost;
+    std::vector<MultiRow> rrows, prows, brrows, bcrows;
     uint64_t off = HDR_BYTES;
     mb->ngr.assign(G, 1);
     mb->phase_off.assign(G, 0);
     mb->aux_off.assign(G, 0);
-    // analysis cost per (group, m, ring-pair block): l steps from each
-    // wave's start, summed over the block's waves
-    std::vector<std::vector<double>> cm(G);
+    std::vector<std::vector<double>> cm(G), cb(G);
     std::vector<std::vector<int>> wst(G);
-    double total = 0.0;
+    std::vector<int> ngb(G, 1);
+    double total = 0.0, btotal = 0.0;
     for (int g = 0; g < G; ++g) {
         const Plan *p = &mb->plans[g];
         const int n = mb->nmaps[g];
         const int ngr = (p->npairs + 64 * R - 1) / (64 * R);
         mb->ngr[g] = ngr;
         if ((rc = wave_starts(p, wst[g]))) return rc;
-        const std::vector<int> &w0 = wst[g];
-        const int nw = (p->npairs + 63) / 64;
-        cm[g].assign((size_t)(p->mmax + 1) * ngr, 0.0);
-        for (int m = 0; m <= p->mmax; ++m)
-            for (int w = 0; w < nw; ++w) {
-                const int s = std::max(w0[(size_t)m * nw + w], m);
-                if (s <= p->lmax) cm[g][(size_t)m * ngr + w / R] += 64.0 * (p->lmax + 1 - s);
-            }
+        cm[g] = block_costs(p, wst[g], R);
         const int mlo = f64 ? 0 : std::min(M_LO, p->mmax + 1);
         for (int m = mlo; m <= p->mmax; ++m)
             for (int r = 0; r < ngr; ++r) total += cm[g][(size_t)m * ngr + r] * n;
@@ -362,62 +461,51 @@ static int build_multi(hbx_multi *mb) {
         off += align256(phase_total_bytes(p, n));
         mb->aux_off[g] = off;
         uint64_t ana = align256(ngr > 1 ? (uint64_t)n * ngr * p->nlm * cs : 0);
+        geo[g] = geom_of(p, n, ngr);
+        geo[g].phase_off = mb->phase_off[g];
+        geo[g].aux_off = mb->aux_off[g];
         if (!f64) {
-            const int g64 = std::max(band_groups(p), band_mfma_groups(p));
+            ngb[g] = (p->npairs + 64 * RB - 1) / (64 * RB);
+            cb[g] = block_costs(p, wst[g], RB);
+            const int mt = std::min(M_LO - 1, p->mmax);
+            for (int m = 0; m <= mt; ++m)
+                for (int r = 0; r < ngb[g]; ++r) btotal += cb[g][(size_t)m * ngb[g] + r] * n;
+            MultiGeomDev d = geom_of(p, n, ngb[g]);
+            d.mmax = mt;
+            d.nlm = band_nlm(p);
+            d.pstride = band_phase_elems(p);
+            d.phase_off = mb->phase_off[g] + band_phase_offset(p, n);
+            d.out_off = mb->aux_off[g] + ana;                       // band alm (FP64)
             ana += align256((uint64_t)n * band_nlm(p) * 16);
-            ana += align256(g64 > 1 ? (uint64_t)n * g64 * band_nlm(p) * 16 : 0);
+            d.aux_off = mb->aux_off[g] + ana;                       // band partial sums
+            ana += align256(ngb[g] > 1 ? (uint64_t)n * ngb[g] * band_nlm(p) * 16 : 0);
+            bgeo.push_back(d);
         }
         const uint64_t syn = align256((uint64_t)n * p->nrow * cs);
         off += std::max(ana, syn);
-
-        MultiGeomDev &d = geo[g];
-        d.pair_n = p->d_pair_n;
-        d.pair_s = p->d_pair_s;
-        d.lseed = p->d_lseed;
-        d.px = p->d_px;
-        d.alpha = p->d_alpha;
-        d.sl = p->d_sl;
-        d.rofs = p->d_rofs;
-        d.seed = p->d_seed;
-        d.npairs = p->npairs;
-        d.nr = p->nrings;
-        d.lmax = p->lmax;
-        d.mmax = p->mmax;
-        d.ngr = ngr;
-        d.nmaps = n;
-        d.nlm = p->nlm;
-        d.pstride = phase_elems_per_map(p);
-        d.nrow = p->nrow;
-        d.phase_off = mb->phase_off[g];
-        d.aux_off = mb->aux_off[g];
     }
     mb->ws_bytes = off;
-    // analysis tiles: consecutive m of one m-block, about total / (waves *
-    // tiles_per_wave) l-steps each (at least one m)
     const double target = total / ((double)mb->grid_ana * mb->tiles_per_wave);
+    const double btarget = f64 ? 0.0 : btotal / ((double)mb->grid_band * mb->tiles_per_wave);
     for (int g = 0; g < G; ++g) {
         const Plan *p = &mb->plans[g];
         const int n = mb->nmaps[g], ngr = mb->ngr[g];
         const int mlo = f64 ? 0 : std::min(M_LO, p->mmax + 1);
-        for (int b = 0; b < n; ++b)
-            for (int r = 0; r < ngr; ++r)
-                for (int m = mlo; m <= p->mmax;) {
-                    const int mend = std::min(p->mmax, (m / MB) * MB + MB - 1);
-                    MultiTile t{g, m, 0, r, b, 1};
-                    double c = 0.0;
-                    while (m <= mend && (t.mn == 0 || c + cm[g][(size_t)m * ngr + r] <= target)) {
-                        c += cm[g][(size_t)m * ngr + r];
-                        ++t.mn;
-                        ++m;
-                    }
-                    at.push_back(t);
-                    acost.push_back(c);
-                }
+        analysis_tiles(g, n, ngr, mlo, p->mmax, cm[g], target, at, acost);
         if (ngr > 1)
             for (int b = 0; b < n; ++b)
                 for (int m = mlo; m <= p->mmax; ++m) rrows.push_back(MultiRow{g, m, b, 0});
         for (int b = 0; b < n; ++b)
             for (int m = 0; m <= p->mmax; ++m) prows.push_back(MultiRow{g, m, b, 0});
+        if (!f64) {
+            const int mt = std::min(M_LO - 1, p->mmax);
+            analysis_tiles(g, n, ngb[g], 0, mt, cb[g], btarget, bt, bcost);
+            for (int b = 0; b < n; ++b) {
+                if (ngb[g] > 1)
+                    for (int m = 0; m <= mt; ++m) brrows.push_back(MultiRow{g, m, b, 0});
+                bcrows.push_back(MultiRow{g, 0, b, 0});
+            }
+        }
         // synthesis tiles: one m-block x 64 ring pairs x up to BM maps
         const int nw = (p->npairs + 63) / 64;
         const std::vector<int> &w0 = wst[g];
@@ -436,9 +524,10 @@ static int build_multi(hbx_multi *mb) {
         }
     }
     const size_t lim = (size_t)INT32_MAX;
-    if (at.size() > lim || st.size() > lim || rrows.size() > lim || prows.size() > lim)
+    if (at.size() > lim || st.size() > lim || rrows.size() > lim || prows.size() > lim || bt.size() > lim ||
+        brrows.size() > lim)
         return set_error(HBX_EUNSUPPORTED, "batch too large: more than 2^31 tiles or rows");
-    std::vector<int32_t> aq, sq;
+    std::vector<int32_t> aq, sq, bq;
     sort_queues(at, acost, aq);
     sort_queues(st, scost, sq);
     mb->n_atiles = (int)at.size();
@@ -452,7 +541,18 @@ static int build_multi(hbx_multi *mb) {
     if ((rc = upload_vec(mb, &mb->d_sq, sq))) return rc;
     if ((rc = upload_vec(mb, &mb->d_rrows, rrows))) return rc;
     if ((rc = upload_vec(mb, &mb->d_prows, prows))) return rc;
-    return HBX_OK;
+    if (!f64) {
+        sort_queues(bt, bcost, bq);
+        mb->n_btiles = (int)bt.size();
+        mb->n_brrows = (int)brrows.size();
+        mb->n_bcrows = (int)bcrows.size();
+        if ((rc = upload_vec(mb, &mb->d_bgeo, bgeo))) return rc;
+        if ((rc = upload_vec(mb, &mb->d_btiles, bt))) return rc;
+        if ((rc = upload_vec(mb, &mb->d_bq, bq))) return rc;
+        if ((rc = upload_vec(mb, &mb->d_brrows, brrows))) return rc;
+        if ((rc = upload_vec(mb, &mb->d_bcrows, bcrows))) return rc;
+    }
+    return build_ring_launches(mb->plans, mb->nmaps, mb->phase_off, mb->rings, mb->band_dft, mb->ptrs);
 }
 
 static int check_call(const hbx_multi *mb, const void *const *a, const int64_t *astr, const void *const *b,
@@ -476,12 +576,15 @@ static int check_call(const hbx_multi *mb, const void *const *a, const int64_t *
     return HBX_OK;
 }
 
-static int launch_setup(const hbx_multi *mb, void *const *alm, const int64_t *astr, char *ws, hipStream_t s) {
+static int launch_setup(const hbx_multi *mb, void *const *alm, const int64_t *astr, void *const *maps,
+                        const int64_t *mstr, char *ws, hipStream_t s) {
     MultiCall c;
     std::memset(&c, 0, sizeof(c));
     for (int g = 0; g < mb->ngroups; ++g) {
         c.alm[g] = alm[g];
         c.astride[g] = astr[g];
+        c.maps[g] = maps[g];
+        c.mstride[g] = mstr[g];
     }
     hipLaunchKernelGGL(multi_setup_kernel, dim3(1), dim3(64), 0, s, c, ws);
     hipError_t e = hipGetLastError();
@@ -489,11 +592,57 @@ static int launch_setup(const hbx_multi *mb, void *const *alm, const int64_t *as
     return HBX_OK;
 }
 
+// per-call map pointers and strides in the workspace header
+static void *const *maps_tab(char *ws) {
+    return reinterpret_cast<void *const *>(ws + HDR_CALL + offsetof(MultiCall, maps));
+}
+static const int64_t *mstride_tab(char *ws) {
+    return reinterpret_cast<const int64_t *>(ws + HDR_CALL + offsetof(MultiCall, mstride));
+}
+
+static int multi_ring_analysis(const hbx_multi *mb, int map_dtype, char *ws, hipStream_t s) {
+    timer_mark(HBX_TIMER_RING_ANALYSIS, true, s);
+    for (const RingLaunch &L : mb->rings) {
+        const int rc = launch_multi_ring_analysis(L, mb->precision, map_dtype, maps_tab(ws), mstride_tab(ws), ws, s);
+        if (rc) return rc;
+    }
+    timer_mark(HBX_TIMER_RING_ANALYSIS, false, s);
+    return HBX_OK;
+}
+
+// FP32 batches: m < M_LO in FP64 -- band DFT of every group's rings (one
+// launch), then the band's persistent Legendre launch, its reduction and the
+// rounding into the call's alm
+static int multi_band_analysis(const hbx_multi *mb, int map_dtype, char *ws, hipStream_t s) {
+    int rc = launch_multi_ring_band(mb->band_dft, map_dtype, maps_tab(ws), mstride_tab(ws), ws, s);
+    if (rc) return rc;
+#define HBX_MB(RV)                                                                                          \
+    hipLaunchKernelGGL((multi_analysis_kernel<double, RV>), dim3(mb->grid_band), dim3(64), 0, s, mb->d_bgeo, \
+                       mb->d_btiles, mb->d_bq, ws, NQ)
+    switch (mb->band_slots) {
+    case 1: HBX_MB(1); break;
+    case 2: HBX_MB(2); break;
+    case 4: HBX_MB(4); break;
+    default: return set_error(HBX_EUNSUPPORTED, "batch band slots");
+    }
+#undef HBX_MB
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "multi_analysis_kernel (band) launch");
+    if (mb->n_brrows > 0) {
+        hipLaunchKernelGGL(multi_reduce_kernel<double>, dim3(mb->n_brrows), dim3(256), 0, s, mb->d_bgeo,
+                           mb->d_brrows, ws);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "multi_reduce_kernel (band) launch");
+    }
+    hipLaunchKernelGGL(multi_band_convert_kernel, dim3(mb->n_bcrows), dim3(256), 0, s, mb->d_bgeo, mb->d_bcrows, ws);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "multi_band_convert_kernel launch");
+    return HBX_OK;
+}
+
 template <typename T>
 static int multi_legendre_analysis(const hbx_multi *mb, char *ws, hipStream_t s) {
 #define HBX_MA(RV)                                                                                        \
     hipLaunchKernelGGL((multi_analysis_kernel<T, RV>), dim3(mb->grid_ana), dim3(64), 0, s, mb->d_geo,     \
-                       mb->d_atiles, mb->d_aq, ws)
+                       mb->d_atiles, mb->d_aq, ws, 0)
     timer_mark(HBX_TIMER_LEGENDRE_ANALYSIS, true, s);
     switch (mb->slots) {
     case 1: HBX_MA(1); break;
@@ -581,7 +730,13 @@ int hbx_multi_create(const hbx_plan *const *plans, const int32_t *nmaps, int32_t
         mb->plans.push_back(plans[g]->impl);
         mb->nmaps.push_back(nmaps[g]);
     }
-    const int rc = build_multi(mb);
+    int rc = build_multi(mb);
+    if (!rc && prec == HBX_F32) {
+        hipError_t e = hipStreamCreateWithFlags(&mb->side, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&mb->ev_fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&mb->ev_join, hipEventDisableTiming);
+        if (e != hipSuccess) rc = hip_error(e, "batch side stream");
+    }
     if (rc) {
         delete mb;
         return rc;
@@ -623,27 +778,25 @@ int hbx_multi_map2alm(const hbx_multi *mb, const void *const *maps, const int64_
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     char *w = static_cast<char *>(ws);
-    if ((rc = launch_setup(mb, alm, alm_stride, w, s))) return rc;
-    const bool f64 = mb->precision == HBX_F64;
-    for (int g = 0; g < mb->ngroups; ++g) {
-        const Plan *p = &mb->plans[g];
-        if ((rc = ring_analysis(p, mb->nmaps[g], maps[g], map_dtype, map_stride[g], w + mb->phase_off[g],
-                                p->d_scale_ana, s)))
-            return rc;
+    if ((rc = launch_setup(mb, alm, alm_stride, const_cast<void *const *>(maps), map_stride, w, s))) return rc;
+    if (mb->precision == HBX_F64) {
+        if ((rc = multi_ring_analysis(mb, map_dtype, w, s))) return rc;
+        return multi_legendre_analysis<double>(mb, w, s);
     }
-    if (!f64)
-        for (int g = 0; g < mb->ngroups; ++g) {
-            const Plan *p = &mb->plans[g];
-            const int n = mb->nmaps[g];
-            const uint64_t cs = 8;
-            char *aux = w + mb->aux_off[g];
-            char *band_alm = aux + align256(mb->ngr[g] > 1 ? (uint64_t)n * mb->ngr[g] * p->nlm * cs : 0);
-            char *band_part = band_alm + align256((uint64_t)n * band_nlm(p) * 16);
-            if ((rc = launch_leg_analysis_band(p, n, band_phase_ptr(p, n, w + mb->phase_off[g]), alm[g],
-                                               alm_stride[g], band_alm, band_part, 0, s)))
-                return rc;
-        }
-    return f64 ? multi_legendre_analysis<double>(mb, w, s) : multi_legendre_analysis<float>(mb, w, s);
+    // FP32: the FP64 band chain (its own buffers, the second queue counters)
+    // on the side stream next to the ring FFTs and the FP32 Legendre launch;
+    // forked after the setup kernel and joined before the call returns, also
+    // while s is being captured (the side stream joins the capture)
+    std::lock_guard<std::mutex> lock(mb->side_mu);
+    HBX_HIP(hipEventRecord(mb->ev_fork, s));
+    HBX_HIP(hipStreamWaitEvent(mb->side, mb->ev_fork, 0));
+    const int rb = multi_band_analysis(mb, map_dtype, w, mb->side);
+    HBX_HIP(hipEventRecord(mb->ev_join, mb->side));
+    rc = rb ? rb : multi_ring_analysis(mb, map_dtype, w, s);
+    if (!rc) rc = multi_legendre_analysis<float>(mb, w, s);
+    const hipError_t ej = hipStreamWaitEvent(s, mb->ev_join, 0);
+    if (rc) return rc;
+    return ej == hipSuccess ? HBX_OK : hip_error(ej, "batch side stream join");
 }
 
 int hbx_multi_alm2map(const hbx_multi *mb, const void *const *alm, const int64_t *alm_stride, void *const *maps,
@@ -652,16 +805,15 @@ int hbx_multi_alm2map(const hbx_multi *mb, const void *const *alm, const int64_t
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     char *w = static_cast<char *>(ws);
-    if ((rc = launch_setup(mb, const_cast<void *const *>(alm), alm_stride, w, s))) return rc;
+    if ((rc = launch_setup(mb, const_cast<void *const *>(alm), alm_stride, maps, map_stride, w, s))) return rc;
     const bool f64 = mb->precision == HBX_F64;
     rc = f64 ? multi_legendre_synthesis<double>(mb, w, s) : multi_legendre_synthesis<float>(mb, w, s);
     if (rc) return rc;
-    for (int g = 0; g < mb->ngroups; ++g) {
-        const Plan *p = &mb->plans[g];
-        if ((rc = launch_ring_synthesis(p, mb->nmaps[g], w + mb->phase_off[g], maps[g], map_dtype, map_stride[g],
-                                        nullptr, s)))
+    timer_mark(HBX_TIMER_RING_SYNTHESIS, true, s);
+    for (const RingLaunch &L : mb->rings)
+        if ((rc = launch_multi_ring_synthesis(L, mb->precision, map_dtype, maps_tab(w), mstride_tab(w), w, s)))
             return rc;
-    }
+    timer_mark(HBX_TIMER_RING_SYNTHESIS, false, s);
     return HBX_OK;
 }
 

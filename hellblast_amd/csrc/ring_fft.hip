@@ -419,24 +419,22 @@ inline unsigned ring_grid_syn(int ncls, int nmaps) { return (unsigned)(((ncls + 
 // TP: precision of the phase written (FP64 transforms of FP32 spin plans
 // round their output once to FP32).
 // ---------------------------------------------------------------------------
-template <typename T, typename TIn, bool BLUE, typename TP = T>
-__global__ void __launch_bounds__(ring_max_threads<T>())
-ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<TP> *__restrict__ phase,
-                     int64_t phase_stride, const int32_t *__restrict__ rings,
-                     const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
-                     const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
-                     const double *__restrict__ scale, const double2 *__restrict__ rot,
-                     const c2<T> *__restrict__ tw, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
-                     int M, int log2M, int mmax, int nr, int ncls, int nmaps) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
-    int slot, b;
-    ring_block(blockIdx.x, nmaps, &slot, &b);
-    if (slot >= ncls) return;
-    const int r = rings[slot];
+// One ring r of one map: mapb / phb are that map's pixels and phase buffer.
+// Shared by the per-plan launch below and the fused launches of
+// mixed-geometry batches (one launch per size class over every group).
+template <typename T, typename TIn, bool BLUE, typename TP>
+__device__ __forceinline__ void ring_analysis_ring(c2<T> *buf, const TIn *__restrict__ mapb,
+                                                   c2<TP> *__restrict__ phb, int r,
+                                                   const int64_t *__restrict__ ring_start,
+                                                   const int32_t *__restrict__ nphi,
+                                                   const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
+                                                   const double *__restrict__ scale,
+                                                   const double2 *__restrict__ rot, const c2<T> *__restrict__ tw,
+                                                   const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
+                                                   int M, int log2M, int mmax, int nr) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = nphi[r], h = n >> 1;
-    const TIn *f = maps + (int64_t)b * map_stride + ring_start[r];
+    const TIn *f = mapb + ring_start[r];
     auto pixel_pair = [&](int q) { return mk<T>(load_as<T>(f + 2 * q), load_as<T>(f + 2 * q + 1)); };
     auto lds = [&](int, int pq) { return buf[pq]; };
     if constexpr (!BLUE) {
@@ -493,7 +491,7 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<TP> *_
     const double ph0 = phi0 ? phi0[r] : 0.0;
     const bool rotp = (ph0 != 0.0);
     const T sc = (T)scd;
-    c2<TP> *o = phase + (int64_t)b * phase_stride + ((int64_t)(tid >> 3) * nr + r) * MB + (tid & 7);
+    c2<TP> *o = phb + ((int64_t)(tid >> 3) * nr + r) * MB + (tid & 7);
     const int64_t ostep = (int64_t)(nt >> 3) * nr * MB;
     double2 P = make_double2(scd, 0.0);
     if (rotp) {
@@ -519,24 +517,39 @@ ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<TP> *_
     }
 }
 
+template <typename T, typename TIn, bool BLUE, typename TP = T>
+__global__ void __launch_bounds__(ring_max_threads<T>())
+ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<TP> *__restrict__ phase,
+                     int64_t phase_stride, const int32_t *__restrict__ rings,
+                     const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
+                     const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
+                     const double *__restrict__ scale, const double2 *__restrict__ rot,
+                     const c2<T> *__restrict__ tw, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
+                     int M, int log2M, int mmax, int nr, int ncls, int nmaps) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int slot, b;
+    ring_block(blockIdx.x, nmaps, &slot, &b);
+    if (slot >= ncls) return;
+    ring_analysis_ring<T, TIn, BLUE, TP>(reinterpret_cast<c2<T> *>(smem_raw), maps + (int64_t)b * map_stride,
+                                         phase + (int64_t)b * phase_stride, rings[slot], ring_start, nphi, mcut,
+                                         phi0, scale, rot, tw, V, vofs, M, log2M, mmax, nr);
+}
+
 // ---------------------------------------------------------------------------
 // synthesis: phase -> maps
 // ---------------------------------------------------------------------------
+// One ring r of one map: phb / mapb are that map's phase buffer and pixels.
 template <typename T, typename TOut, bool BLUE>
-__global__ void __launch_bounds__(ring_max_threads<T>())
-ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOut *__restrict__ maps,
-                      int64_t map_stride, const int32_t *__restrict__ rings,
-                      const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
-                      const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
-                      const double *__restrict__ scale, const double2 *__restrict__ rot,
-                      const c2<T> *__restrict__ tw, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
-                      int M, int log2M, int mmax, int nr, int ncls, int nmaps) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    c2<T> *buf = reinterpret_cast<c2<T> *>(smem_raw);
-    int slot, b;
-    ring_block_syn(blockIdx.x, nmaps, (ncls + 63) >> 6, &slot, &b);
-    if (slot >= ncls) return;
-    const int r = rings[slot];
+__device__ __forceinline__ void ring_synthesis_ring(c2<T> *buf, const c2<T> *__restrict__ phb,
+                                                    TOut *__restrict__ mapb, int r,
+                                                    const int64_t *__restrict__ ring_start,
+                                                    const int32_t *__restrict__ nphi,
+                                                    const int32_t *__restrict__ mcut,
+                                                    const double *__restrict__ phi0,
+                                                    const double *__restrict__ scale,
+                                                    const double2 *__restrict__ rot, const c2<T> *__restrict__ tw,
+                                                    const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
+                                                    int M, int log2M, int mmax, int nr) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = nphi[r], h = n >> 1;
     const int top = min(mmax, mcut[r] - 1);
@@ -548,7 +561,7 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
     const int npairs = (nr + 1) >> 1;
     const int half = r >= npairs ? 1 : 0, pr = half ? nr - 1 - r : r;
     const int64_t sbe = syn_block_elems(npairs);
-    const c2<T> *G = phase + (int64_t)b * phase_stride + syn_ofs(0, half, pr, npairs);
+    const c2<T> *G = phb + syn_ofs(0, half, pr, npairs);
     const double2 Rn = rot[nr + r];
     auto gload = [&](int m) { return G[(int64_t)(m >> 3) * sbe + (m & 7) * 8]; };
 
@@ -635,7 +648,7 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
         for (int j = h + tid; j < M; j += nt) buf[pad_idx<T>(j)] = mk<T>(0, 0);
     __syncthreads();
     const T sc = (T)(scale ? scale[r] : 1.0);
-    TOut *f = maps + (int64_t)b * map_stride + ring_start[r];
+    TOut *f = mapb + ring_start[r];
     auto lds = [&](int, int pq) { return buf[pq]; };
     if constexpr (!BLUE) {
         fft_run<T, true>(buf, M, log2M, tw, lds, [&](int j, int, c2<T> z) {
@@ -653,6 +666,24 @@ ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOu
             }
         });
     }
+}
+
+template <typename T, typename TOut, bool BLUE>
+__global__ void __launch_bounds__(ring_max_threads<T>())
+ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOut *__restrict__ maps,
+                      int64_t map_stride, const int32_t *__restrict__ rings,
+                      const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
+                      const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
+                      const double *__restrict__ scale, const double2 *__restrict__ rot,
+                      const c2<T> *__restrict__ tw, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
+                      int M, int log2M, int mmax, int nr, int ncls, int nmaps) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int slot, b;
+    ring_block_syn(blockIdx.x, nmaps, (ncls + 63) >> 6, &slot, &b);
+    if (slot >= ncls) return;
+    ring_synthesis_ring<T, TOut, BLUE>(reinterpret_cast<c2<T> *>(smem_raw), phase + (int64_t)b * phase_stride,
+                                       maps + (int64_t)b * map_stride, rings[slot], ring_start, nphi, mcut, phi0,
+                                       scale, rot, tw, V, vofs, M, log2M, mmax, nr);
 }
 
 // ---------------------------------------------------------------------------
@@ -766,17 +797,18 @@ ring_band_kernel(const TIn *__restrict__ maps, int64_t map_stride, double2 *__re
 // pixel, the higher m by complex multiplication), then a fixed-order
 // reduction over the wave (xor shuffles) and the 4 waves (LDS).  Same
 // conventions and output as ring_band_kernel.
+// One ring r of one map (mapb: its pixels, bandb: its band phase); every
+// thread of a 256-thread workgroup calls it.
 template <typename TIn>
-__global__ void __launch_bounds__(256)
-ring_band_valu_kernel(const TIn *__restrict__ maps, int64_t map_stride, double2 *__restrict__ band,
-                      int64_t band_stride, const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
-                      const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
-                      const double *__restrict__ scale, int mtop, int nr) {
+__device__ __forceinline__ void ring_band_valu_ring(const TIn *__restrict__ mapb, double2 *__restrict__ bandb, int r,
+                                                    const int64_t *__restrict__ ring_start,
+                                                    const int32_t *__restrict__ nphi,
+                                                    const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
+                                                    const double *__restrict__ scale, int mtop, int nr) {
     __shared__ double part[4][2 * M_LO];
-    const int r = blockIdx.x, b = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int n = nphi[r];
-    const TIn *f = maps + (int64_t)b * map_stride + ring_start[r];
+    const TIn *f = mapb + ring_start[r];
     double acc[2 * M_LO];
 #pragma unroll
     for (int i = 0; i < 2 * M_LO; ++i) acc[i] = 0.0;
@@ -824,8 +856,19 @@ ring_band_valu_kernel(const TIn *__restrict__ maps, int64_t map_stride, double2 
                 F = make_double2(x, y);
             }
         }
-        band[(int64_t)b * band_stride + ((int64_t)(mm >> 3) * nr + r) * MB + (mm & 7)] = F;
+        bandb[((int64_t)(mm >> 3) * nr + r) * MB + (mm & 7)] = F;
     }
+}
+
+template <typename TIn>
+__global__ void __launch_bounds__(256)
+ring_band_valu_kernel(const TIn *__restrict__ maps, int64_t map_stride, double2 *__restrict__ band,
+                      int64_t band_stride, const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
+                      const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
+                      const double *__restrict__ scale, int mtop, int nr) {
+    const int r = blockIdx.x, b = blockIdx.y;
+    ring_band_valu_ring<TIn>(maps + (int64_t)b * map_stride, band + (int64_t)b * band_stride, r, ring_start, nphi,
+                             mcut, phi0, scale, mtop, nr);
 }
 
 int launch_ring_band(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride, void *band,
@@ -1222,6 +1265,263 @@ int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *map
     if (map_dtype == HBX_DT_F64)
         return ring_synthesis_t<float, double>(p, nmaps, ph, (double *)maps, map_stride, scale, p->d_V32, s);
     return ring_synthesis_t<float, float>(p, nmaps, ph, (float *)maps, map_stride, scale, p->d_V32, s);
+}
+
+
+// ---------------------------------------------------------------------------
+// Fused ring launches of mixed-geometry batches.  Block -> segment by a
+// binary search over the segments' first blocks (scalar loads), then the
+// per-plan block mapping inside the segment: segments start at multiples of
+// 8 (analysis, band) or 64 (synthesis) blocks, so the XCD grouping of the
+// per-plan launches is kept.  The rings run the per-plan code
+// (ring_analysis_ring / ring_synthesis_ring / ring_band_valu_ring) with the
+// group's tables, so every group's phases and pixels equal its plan's own
+// launches bit for bit.  Table pointers read from the segment are cast to
+// the global address space so their loads and stores stay global ones.
+// ---------------------------------------------------------------------------
+template <typename X> __device__ __forceinline__ X *gptr(X *p) { return (X *)(HBX_GLOBAL X *)p; }
+
+__device__ __forceinline__ const RingSeg *find_seg(const RingSeg *segs, int nseg, int blk) {
+    int lo = 0, hi = nseg - 1;                      // last segment with block0 <= blk
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ldc(&segs[mid].block0) <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return segs + lo;
+}
+
+template <typename T, typename TIn, bool BLUE>
+__global__ void __launch_bounds__(ring_max_threads<T>())
+multi_ring_analysis_kernel(const RingSeg *__restrict__ segs, int nseg, const void *const *maps_tab,
+                           const int64_t *mstride_tab, char *ws) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int blk = blockIdx.x;
+    const RingSeg *sg = find_seg(segs, nseg, blk);
+    int slot, b;
+    ring_block(blk - ldc(&sg->block0), ldc(&sg->nmaps), &slot, &b);
+    if (slot >= ldc(&sg->ncls)) return;
+    const int g = ldc(&sg->g);
+    const TIn *mapb = gptr((const TIn *)ldc(maps_tab + g)) + (int64_t)b * ldc(mstride_tab + g);
+    c2<T> *phb = gptr(reinterpret_cast<c2<T> *>(ws + ldc(&sg->phase_off))) + (int64_t)b * ldc(&sg->pstride);
+    const int r = ldc(ldc(&sg->rings) + slot);
+    ring_analysis_ring<T, TIn, BLUE, T>(
+        reinterpret_cast<c2<T> *>(smem_raw), mapb, phb, r, gptr(ldc(&sg->ring_start)), gptr(ldc(&sg->nphi)),
+        gptr(ldc(&sg->mcut)), gptr(ldc(&sg->phi0)), gptr(ldc(&sg->scale)), gptr(ldc(&sg->rot)),
+        gptr((const c2<T> *)ldc(&sg->tw)), gptr((const c2<T> *)ldc(&sg->V)), gptr(ldc(&sg->vofs)), ldc(&sg->M),
+        ldc(&sg->log2M), ldc(&sg->mmax), ldc(&sg->nr));
+}
+
+template <typename T, typename TOut, bool BLUE>
+__global__ void __launch_bounds__(ring_max_threads<T>())
+multi_ring_synthesis_kernel(const RingSeg *__restrict__ segs, int nseg, void *const *maps_tab,
+                            const int64_t *mstride_tab, const char *ws) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int blk = blockIdx.x;
+    const RingSeg *sg = find_seg(segs, nseg, blk);
+    const int ncls = ldc(&sg->ncls);
+    int slot, b;
+    ring_block_syn(blk - ldc(&sg->block0), ldc(&sg->nmaps), (ncls + 63) >> 6, &slot, &b);
+    if (slot >= ncls) return;
+    const int g = ldc(&sg->g);
+    TOut *mapb = gptr((TOut *)ldc(maps_tab + g)) + (int64_t)b * ldc(mstride_tab + g);
+    const c2<T> *phb =
+        gptr(reinterpret_cast<const c2<T> *>(ws + ldc(&sg->phase_off))) + (int64_t)b * ldc(&sg->pstride);
+    const int r = ldc(ldc(&sg->rings) + slot);
+    ring_synthesis_ring<T, TOut, BLUE>(
+        reinterpret_cast<c2<T> *>(smem_raw), phb, mapb, r, gptr(ldc(&sg->ring_start)), gptr(ldc(&sg->nphi)),
+        gptr(ldc(&sg->mcut)), gptr(ldc(&sg->phi0)), gptr(ldc(&sg->scale)), gptr(ldc(&sg->rot)),
+        gptr((const c2<T> *)ldc(&sg->tw)), gptr((const c2<T> *)ldc(&sg->V)), gptr(ldc(&sg->vofs)), ldc(&sg->M),
+        ldc(&sg->log2M), ldc(&sg->mmax), ldc(&sg->nr));
+}
+
+// band DFT (FP32 batches): segment = one group, blocks (map, ring) map-major
+template <typename TIn>
+__global__ void __launch_bounds__(256)
+multi_ring_band_kernel(const RingSeg *__restrict__ segs, int nseg, const void *const *maps_tab,
+                       const int64_t *mstride_tab, char *ws) {
+    const int blk = blockIdx.x;
+    const RingSeg *sg = find_seg(segs, nseg, blk);
+    const int nr = ldc(&sg->nr), local = blk - ldc(&sg->block0);
+    const int b = local / nr, r = local - b * nr;
+    if (b >= ldc(&sg->nmaps)) return;
+    const int g = ldc(&sg->g);
+    const TIn *mapb = gptr((const TIn *)ldc(maps_tab + g)) + (int64_t)b * ldc(mstride_tab + g);
+    double2 *bandb =
+        gptr(reinterpret_cast<double2 *>(ws + ldc(&sg->phase_off))) + (int64_t)b * ldc(&sg->pstride);
+    const int mmax = ldc(&sg->mmax);
+    ring_band_valu_ring<TIn>(mapb, bandb, r, gptr(ldc(&sg->ring_start)), gptr(ldc(&sg->nphi)),
+                             gptr(ldc(&sg->mcut)), gptr(ldc(&sg->phi0)), gptr(ldc(&sg->scale)),
+                             mmax < M_LO - 1 ? mmax : M_LO - 1, nr);
+}
+
+template <typename T> static int allow_multi_lds(size_t lds) {
+    int rc;
+    if ((rc = allow_lds<T>(multi_ring_analysis_kernel<T, float, false>, lds))) return rc;
+    if ((rc = allow_lds<T>(multi_ring_analysis_kernel<T, double, false>, lds))) return rc;
+    if ((rc = allow_lds<T>(multi_ring_analysis_kernel<T, float, true>, lds))) return rc;
+    if ((rc = allow_lds<T>(multi_ring_analysis_kernel<T, double, true>, lds))) return rc;
+    if ((rc = allow_lds<T>(multi_ring_synthesis_kernel<T, float, false>, lds))) return rc;
+    if ((rc = allow_lds<T>(multi_ring_synthesis_kernel<T, double, false>, lds))) return rc;
+    if ((rc = allow_lds<T>(multi_ring_synthesis_kernel<T, float, true>, lds))) return rc;
+    return allow_lds<T>(multi_ring_synthesis_kernel<T, double, true>, lds);
+}
+
+static int upload_segs(const std::vector<RingSeg> &v, RingSeg **d, std::vector<void *> &allocs) {
+    void *ptr = nullptr;
+    if (hipMalloc(&ptr, v.size() * sizeof(RingSeg)) != hipSuccess)
+        return set_error(HBX_ENOMEM, "hipMalloc for the fused ring tables failed");
+    allocs.push_back(ptr);
+    HBX_HIP(hipMemcpy(ptr, v.data(), v.size() * sizeof(RingSeg), hipMemcpyHostToDevice));
+    *d = static_cast<RingSeg *>(ptr);
+    return HBX_OK;
+}
+
+int build_ring_launches(const std::vector<Plan> &plans, const std::vector<int> &nmaps,
+                        const std::vector<uint64_t> &phase_off, std::vector<RingLaunch> &out, RingLaunch &band,
+                        std::vector<void *> &allocs) {
+    out.clear();
+    band = RingLaunch{};
+    const bool f64 = plans[0].precision == HBX_F64;
+    // (workgroup size, Bluestein) -> segments, in group order
+    std::map<std::pair<int, int>, std::vector<RingSeg>> by;
+    for (size_t g = 0; g < plans.size(); ++g) {
+        const Plan &p = plans[g];
+        if (p.ring_f64) return set_error(HBX_EPLAN, "batches take plans whose ring stage runs at their precision");
+        for (const FftClass &c : p.classes)
+            for (int blue = 0; blue < 2; ++blue) {
+                const int n = blue ? c.nrings_blue : c.nrings_pow2;
+                if (n == 0) continue;
+                RingSeg sg{};
+                sg.rings = blue ? c.d_rings_blue : c.d_rings_pow2;
+                sg.ring_start = p.d_ring_start;
+                sg.nphi = p.d_nphi;
+                sg.mcut = p.d_mcut;
+                sg.phi0 = p.d_phi0;
+                sg.scale = p.d_scale_ana;
+                sg.rot = p.d_rot;
+                sg.tw = f64 ? (const void *)(p.d_ftw64 + c.tw_off64) : (const void *)(p.d_ftw32 + c.tw_off32);
+                sg.V = f64 ? (const void *)p.d_V64 : (const void *)p.d_V32;
+                sg.vofs = p.d_vofs;
+                sg.pstride = phase_elems_per_map(&p);
+                sg.phase_off = phase_off[g];
+                sg.ncls = n;
+                sg.nmaps = nmaps[g];
+                sg.g = (int32_t)g;
+                sg.M = c.M;
+                sg.log2M = c.log2M;
+                sg.mmax = p.mmax;
+                sg.nr = p.nrings;
+                by[{c.threads, blue}].push_back(sg);
+            }
+    }
+    const int64_t lim = INT32_MAX;
+    for (auto it = by.rbegin(); it != by.rend(); ++it) {
+        RingLaunch L;
+        L.threads = it->first.first;
+        L.blue = it->first.second;
+        std::vector<RingSeg> ana = it->second, syn = it->second;
+        int64_t ba = 0, bs = 0;
+        for (size_t i = 0; i < ana.size(); ++i) {
+            ana[i].block0 = (int32_t)ba;
+            syn[i].block0 = (int32_t)bs;
+            syn[i].scale = nullptr;                 // batch synthesis: unit scale
+            ba += ring_grid(ana[i].ncls, ana[i].nmaps);
+            bs += ring_grid_syn(syn[i].ncls, syn[i].nmaps);
+            if (ba > lim || bs > lim) return set_error(HBX_EUNSUPPORTED, "batch too large for one ring launch");
+            L.lds = std::max(L.lds, f64 ? ring_lds_bytes<double>(ana[i].M) : ring_lds_bytes<float>(ana[i].M));
+        }
+        L.nseg = (int)ana.size();
+        L.grid_ana = (unsigned)ba;
+        L.grid_syn = (unsigned)bs;
+        int rc;
+        if ((rc = upload_segs(ana, &L.d_ana, allocs))) return rc;
+        if ((rc = upload_segs(syn, &L.d_syn, allocs))) return rc;
+        if ((rc = f64 ? allow_multi_lds<double>(L.lds) : allow_multi_lds<float>(L.lds))) return rc;
+        out.push_back(L);
+    }
+    if (!f64) {
+        std::vector<RingSeg> segs;
+        int64_t bb = 0;
+        for (size_t g = 0; g < plans.size(); ++g) {
+            const Plan &p = plans[g];
+            RingSeg sg{};
+            sg.ring_start = p.d_ring_start;
+            sg.nphi = p.d_nphi;
+            sg.mcut = p.d_mcut;
+            sg.phi0 = p.d_phi0;
+            sg.scale = p.d_scale_ana;
+            sg.pstride = band_phase_elems(&p);
+            sg.phase_off = phase_off[g] + band_phase_offset(&p, nmaps[g]);
+            sg.ncls = p.nrings;
+            sg.nmaps = nmaps[g];
+            sg.g = (int32_t)g;
+            sg.mmax = p.mmax;
+            sg.nr = p.nrings;
+            sg.block0 = (int32_t)bb;
+            bb += ((int64_t)p.nrings * nmaps[g] + 7) / 8 * 8;
+            if (bb > lim) return set_error(HBX_EUNSUPPORTED, "batch too large for one band launch");
+            segs.push_back(sg);
+        }
+        band.threads = 256;
+        band.nseg = (int)segs.size();
+        band.grid_ana = (unsigned)bb;
+        int rc;
+        if ((rc = upload_segs(segs, &band.d_ana, allocs))) return rc;
+    }
+    return HBX_OK;
+}
+
+int launch_multi_ring_analysis(const RingLaunch &L, int precision, int map_dtype, const void *const *maps_tab,
+                               const int64_t *mstride_tab, char *ws, hipStream_t s) {
+#define HBX_MRA(T, TI, B)                                                                                   \
+    hipLaunchKernelGGL((multi_ring_analysis_kernel<T, TI, B>), dim3(L.grid_ana), dim3(L.threads), L.lds, s, \
+                       L.d_ana, L.nseg, maps_tab, mstride_tab, ws)
+    const bool din = map_dtype == HBX_DT_F64;
+    if (precision == HBX_F64) {
+        if (L.blue) { if (din) HBX_MRA(double, double, true); else HBX_MRA(double, float, true); }
+        else { if (din) HBX_MRA(double, double, false); else HBX_MRA(double, float, false); }
+    } else {
+        if (L.blue) { if (din) HBX_MRA(float, double, true); else HBX_MRA(float, float, true); }
+        else { if (din) HBX_MRA(float, double, false); else HBX_MRA(float, float, false); }
+    }
+#undef HBX_MRA
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "multi_ring_analysis_kernel launch");
+    return HBX_OK;
+}
+
+int launch_multi_ring_synthesis(const RingLaunch &L, int precision, int map_dtype, void *const *maps_tab,
+                                const int64_t *mstride_tab, char *ws, hipStream_t s) {
+#define HBX_MRS(T, TO, B)                                                                                    \
+    hipLaunchKernelGGL((multi_ring_synthesis_kernel<T, TO, B>), dim3(L.grid_syn), dim3(L.threads), L.lds, s, \
+                       L.d_syn, L.nseg, maps_tab, mstride_tab, ws)
+    const bool dout = map_dtype == HBX_DT_F64;
+    if (precision == HBX_F64) {
+        if (L.blue) { if (dout) HBX_MRS(double, double, true); else HBX_MRS(double, float, true); }
+        else { if (dout) HBX_MRS(double, double, false); else HBX_MRS(double, float, false); }
+    } else {
+        if (L.blue) { if (dout) HBX_MRS(float, double, true); else HBX_MRS(float, float, true); }
+        else { if (dout) HBX_MRS(float, double, false); else HBX_MRS(float, float, false); }
+    }
+#undef HBX_MRS
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "multi_ring_synthesis_kernel launch");
+    return HBX_OK;
+}
+
+int launch_multi_ring_band(const RingLaunch &L, int map_dtype, const void *const *maps_tab,
+                           const int64_t *mstride_tab, char *ws, hipStream_t s) {
+    if (L.nseg == 0) return HBX_OK;
+    if (map_dtype == HBX_DT_F64)
+        hipLaunchKernelGGL(multi_ring_band_kernel<double>, dim3(L.grid_ana), dim3(256), 0, s, L.d_ana, L.nseg,
+                           maps_tab, mstride_tab, ws);
+    else
+        hipLaunchKernelGGL(multi_ring_band_kernel<float>, dim3(L.grid_ana), dim3(256), 0, s, L.d_ana, L.nseg,
+                           maps_tab, mstride_tab, ws);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "multi_ring_band_kernel launch");
+    return HBX_OK;
 }
 
 }  // namespace hbx

@@ -11,7 +11,9 @@
  * graph.  Analyses with FP32 plans run their FP64 low-m band on a stream of
  * the plan, forked from the given stream and joined back to it: work the
  * caller enqueues on the given stream after the call runs after the band
- * (everything stays on the given stream while it is being captured).  Plans are immutable after creation (tiles included: another tile
+ * (everything stays on the given stream while it is being captured; FP32
+ * mixed-geometry batches fork their band also then, see hbx_multi_create).
+ * Plans are immutable after creation (tiles included: another tile
  * choice is another plan, made by hbx_plan_derive) and bound to the device
  * that was current when they were created; one plan may be used from several
  * threads and streams at once.
@@ -281,11 +283,18 @@ int hbx_pix2ang_ring(int64_t n, const int64_t *ipix, int32_t nrings,
  * batch holds ngroups spin-0 plans of one precision (each its own geometry,
  * lmax and mmax) with nmaps[g] maps each; it copies what it needs, so the
  * plans may be destroyed afterwards.  hbx_multi_map2alm / hbx_multi_alm2map
- * transform every map of every group: the ring stage runs per group, and
- * each Legendre direction is ONE persistent launch whose workgroups take
- * (group, m-block, ring-pair block, map block) tiles from work queues, so
- * the work of large and small maps spreads over the whole GPU in one
- * kernel.  The Legendre tiles run the VALU kernels' arithmetic: results
+ * transform every map of every group in a fixed number of launches: the
+ * ring stage runs in fused launches over the rings of every group (one per
+ * FFT workgroup size), and each Legendre direction is ONE persistent launch
+ * whose workgroups take (group, m-block, ring-pair block, map block) tiles
+ * from work queues, so the work of large and small maps spreads over the
+ * whole GPU in one kernel.  FP32 batches compute m < 8 in FP64 (band DFT
+ * and one persistent FP64 launch) on a stream of the batch, forked from the
+ * given stream after the call's first kernel and joined back to it before
+ * the call returns, by two events under the batch's lock; this also holds
+ * while the given stream is being captured (the fork and join become graph
+ * edges).  The ring stages and Legendre tiles run the per-plan VALU
+ * kernels' arithmetic: results
  * equal hbx_map2alm / hbx_alm2map of each group with those kernels
  * (valu_slots equal to the plan's, mfma_min_maps above the batch) bit for
  * bit.  Per group g: maps[g] (nmaps[g] maps, map_stride[g] apart, dtype
