@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--mixed", action="store_true",
                     help="instead of the headline: 64 FP32 maps of mixed nside (128..2048, lmax 2 nside) "
                          "in one HIP graph with autotuned tiles; prints its own JSON line")
+    ap.add_argument("--mixed-workload", default="nside128-2048", choices=["nside128-2048", "groups16"],
+                    help="with --mixed: the 64-map batch, or 16 FP32 maps at nside 512 in 16 (nside, lmax) "
+                         "groups, lmax 512, 544, .., 992 (one map each)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB -> 1 GiB elementwise copy kernel (HBM counter calibration)")
     return ap.parse_args()
@@ -449,7 +452,9 @@ def configs_main(args):
 def mixed_main(args):
     """64 FP32 maps, nside {2048:1, 1024:2, 512:5, 256:14, 128:42} (seeded
     shuffle), lmax = 2 nside, alm ~ C_l = (l+10)^-2: map2alm + alm2map of the
-    whole batch per step, replayed from one HIP graph (MixedBatch)."""
+    whole batch per step, replayed from one HIP graph (MixedBatch).  With
+    --mixed-workload groups16: 16 maps at nside 512, lmax 512 + 32 k (16
+    groups of one map), the same spectrum."""
     import numpy as np
     import torch
     from hellblast_amd.batch import MixedBatch
@@ -458,10 +463,15 @@ def mixed_main(args):
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
     rng = np.random.default_rng(3)
+    many = args.mixed_workload == "groups16"
     alloc = {2048: 1, 1024: 2, 512: 5, 256: 14, 128: 42}
-    sides = [n for n, k in alloc.items() for _ in range(k)]
-    rng.shuffle(sides)
-    specs = [(n, 2 * n) for n in sides]
+    if many:
+        sides = [512] * 16
+        specs = [(512, 512 + 32 * k) for k in range(16)]
+    else:
+        sides = [n for n, k in alloc.items() for _ in range(k)]
+        rng.shuffle(sides)
+        specs = [(n, 2 * n) for n in sides]
     t0 = time.time()
     modes = ["persistent", "grouped"] if args.mixed_mode == "auto" else [args.mixed_mode]
     maps = []
@@ -531,7 +541,7 @@ def mixed_main(args):
     # VALU kernels (4 slots, the persistent tiles' arithmetic) and with the
     # plan's default kernels (matrix cores for 3+ maps)
     groups = {}
-    for nside, count in alloc.items():
+    for nside, count in ({} if many else alloc).items():
         lmax = 2 * nside
         p = _plan(nside, lmax, lmax, "f32", "dz", "reference", "reference")
         mp = torch.randn((count, p.npix), device=dev)
@@ -551,12 +561,17 @@ def mixed_main(args):
     best = min(runs, key=lambda k: runs[k]["graph_ms"])
     rb = runs[best]
     npix = sum(12 * n * n for n in sides)
-    res = {"metric": "mixed-batch SHT pairs/s: 64 FP32 maps, nside 128..2048, lmax 2 nside, one HIP graph",
+    metric = ("mixed-batch SHT pairs/s: 16 FP32 maps, nside 512, 16 groups lmax 512..992, one HIP graph" if many else
+              "mixed-batch SHT pairs/s: 64 FP32 maps, nside 128..2048, lmax 2 nside, one HIP graph")
+    workload = ("16 maps nside 512, lmax 512+32k (k=0..15, one map per group), C_l=(l+10)^-2, map2alm+alm2map "
+                "per map" if many else
+                "64 maps nside {2048:1,1024:2,512:5,256:14,128:42}, lmax 2 nside, C_l=(l+10)^-2, map2alm+alm2map "
+                "per map")
+    res = {"metric": metric,
            "value": round(len(specs) / (rb["graph_ms"] / 1e3), 3), "unit": "pairs/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(rb["graph_ms"], 3),
            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "64 maps nside {2048:1,1024:2,512:5,256:14,128:42}, lmax 2 nside, "
-                                  "C_l=(l+10)^-2, map2alm+alm2map per map",
+           "config": {"workload": workload,
                       "pixels_per_step": npix, "mode": best, "eager_ms_per_step": round(rb["eager_ms"], 3),
                       "graph_over_eager": round(rb["eager_ms"] / rb["graph_ms"], 3), "eps_max_worst_map": rb["eps"],
                       "tiles": rb["tiles"],

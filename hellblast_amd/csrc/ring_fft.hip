@@ -792,39 +792,48 @@ ring_band_kernel(const TIn *__restrict__ maps, int64_t map_stride, double2 *__re
 }
 
 // The band for batches below the matrix-core threshold (single maps): one
-// workgroup per (ring, map) on the VALU, each thread summing its pixels
-// j = tid + 256 t against e^{-2 pi i m j / n}, m < M_LO (one sincospi per
-// pixel, the higher m by complex multiplication), then a fixed-order
-// reduction over the wave (xor shuffles) and the 4 waves (LDS).  Same
+// wave per (ring, map) on the VALU, each lane summing its pixels
+// j = lane + 64 t against e^{-2 pi i m j / n}, m < M_LO (e^{-2 pi i j / n}
+// from one sincospi per lane, advanced to the lane's next pixel by an FP64
+// rotation step: at most n / 64 <= 256 steps, a relative error of order
+// 1e-14; the higher m by complex multiplication), then a fixed-order xor
+// reduction over the wave, after which every lane holds the same sums.  Same
 // conventions and output as ring_band_kernel.
+constexpr int RBV_LANES = 64;
+
 // One ring r of one map (mapb: its pixels, bandb: its band phase); every
-// thread of a 256-thread workgroup calls it.
+// lane of a one-wave workgroup calls it.
 template <typename TIn>
 __device__ __forceinline__ void ring_band_valu_ring(const TIn *__restrict__ mapb, double2 *__restrict__ bandb, int r,
                                                     const int64_t *__restrict__ ring_start,
                                                     const int32_t *__restrict__ nphi,
                                                     const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
                                                     const double *__restrict__ scale, int mtop, int nr) {
-    __shared__ double part[4][2 * M_LO];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int lane = threadIdx.x;
     const int n = nphi[r];
     const TIn *f = mapb + ring_start[r];
     double acc[2 * M_LO];
 #pragma unroll
     for (int i = 0; i < 2 * M_LO; ++i) acc[i] = 0.0;
-    for (int j = tid; j < n; j += 256) {
+    double s1, c1, ss, cs;                          // e^{-2 pi i j / n}, j = lane; the step to j + 64
+    sincospi(-2.0 * (double)(lane % n) / (double)n, &s1, &c1);
+    sincospi(-2.0 * (double)(RBV_LANES % n) / (double)n, &ss, &cs);
+    for (int j = lane; j < n; j += RBV_LANES) {
         const double v = (double)f[j];
-        double s1, c1;                              // e^{-2 pi i j / n}
-        sincospi(-2.0 * (double)j / (double)n, &s1, &c1);
         double cr = 1.0, ci = 0.0;
 #pragma unroll
         for (int m = 0; m < M_LO; ++m) {
             acc[2 * m] = fma(v, cr, acc[2 * m]);
             acc[2 * m + 1] = fma(v, ci, acc[2 * m + 1]);
-            const double t = cr * c1 - ci * s1;
-            ci = cr * s1 + ci * c1;
-            cr = t;
+            if (m + 1 < M_LO) {
+                const double t = cr * c1 - ci * s1;
+                ci = cr * s1 + ci * c1;
+                cr = t;
+            }
         }
+        const double t = c1 * cs - s1 * ss;
+        s1 = c1 * ss + s1 * cs;
+        c1 = t;
     }
 #pragma unroll
     for (int i = 0; i < 2 * M_LO; ++i) {
@@ -833,15 +842,15 @@ __device__ __forceinline__ void ring_band_valu_ring(const TIn *__restrict__ mapb
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
         acc[i] = v;
     }
-    if (lane == 0) {
+    if (lane <= mtop) {
+        const int mm = lane;
+        double x = 0.0, y = 0.0;
 #pragma unroll
-        for (int i = 0; i < 2 * M_LO; ++i) part[w][i] = acc[i];
-    }
-    __syncthreads();
-    if (tid <= mtop) {
-        const int mm = tid;
-        double x = ((part[0][2 * mm] + part[1][2 * mm]) + part[2][2 * mm]) + part[3][2 * mm];
-        double y = ((part[0][2 * mm + 1] + part[1][2 * mm + 1]) + part[2][2 * mm + 1]) + part[3][2 * mm + 1];
+        for (int i = 0; i < M_LO; ++i)
+            if (i == mm) {
+                x = acc[2 * i];
+                y = acc[2 * i + 1];
+            }
         double2 F = make_double2(0.0, 0.0);
         if (mm < mcut[r]) {
             const double sc = scale ? scale[r] : 1.0;
@@ -849,9 +858,9 @@ __device__ __forceinline__ void ring_band_valu_ring(const TIn *__restrict__ mapb
             y *= sc;
             const double ph = phi0 ? phi0[r] : 0.0;
             if (ph != 0.0) {
-                double sn, cs;
-                sincos(-(double)mm * ph, &sn, &cs);
-                F = make_double2(x * cs - y * sn, x * sn + y * cs);
+                double sn, csn;
+                sincos(-(double)mm * ph, &sn, &csn);
+                F = make_double2(x * csn - y * sn, x * sn + y * csn);
             } else {
                 F = make_double2(x, y);
             }
@@ -861,7 +870,7 @@ __device__ __forceinline__ void ring_band_valu_ring(const TIn *__restrict__ mapb
 }
 
 template <typename TIn>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(RBV_LANES)
 ring_band_valu_kernel(const TIn *__restrict__ maps, int64_t map_stride, double2 *__restrict__ band,
                       int64_t band_stride, const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
                       const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
@@ -877,11 +886,11 @@ int launch_ring_band(const Plan *p, int nmaps, const void *maps, int map_dtype, 
     if (nmaps < mfma_min_maps(p)) {
         dim3 grid(p->nrings, nmaps);
         if (map_dtype == HBX_DT_F64)
-            hipLaunchKernelGGL(ring_band_valu_kernel<double>, grid, dim3(256), 0, s, (const double *)maps,
+            hipLaunchKernelGGL(ring_band_valu_kernel<double>, grid, dim3(RBV_LANES), 0, s, (const double *)maps,
                                map_stride, (double2 *)band, band_phase_elems(p), p->d_ring_start, p->d_nphi,
                                p->d_mcut, p->d_phi0, scale, mtop, p->nrings);
         else
-            hipLaunchKernelGGL(ring_band_valu_kernel<float>, grid, dim3(256), 0, s, (const float *)maps,
+            hipLaunchKernelGGL(ring_band_valu_kernel<float>, grid, dim3(RBV_LANES), 0, s, (const float *)maps,
                                map_stride, (double2 *)band, band_phase_elems(p), p->d_ring_start, p->d_nphi,
                                p->d_mcut, p->d_phi0, scale, mtop, p->nrings);
         hipError_t e = hipGetLastError();
@@ -1337,7 +1346,7 @@ multi_ring_synthesis_kernel(const RingSeg *__restrict__ segs, int nseg, void *co
 
 // band DFT (FP32 batches): segment = one group, blocks (map, ring) map-major
 template <typename TIn>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(RBV_LANES)
 multi_ring_band_kernel(const RingSeg *__restrict__ segs, int nseg, const void *const *maps_tab,
                        const int64_t *mstride_tab, char *ws) {
     const int blk = blockIdx.x;
@@ -1463,7 +1472,7 @@ int build_ring_launches(const std::vector<Plan> &plans, const std::vector<int> &
             if (bb > lim) return set_error(HBX_EUNSUPPORTED, "batch too large for one band launch");
             segs.push_back(sg);
         }
-        band.threads = 256;
+        band.threads = RBV_LANES;
         band.nseg = (int)segs.size();
         band.grid_ana = (unsigned)bb;
         int rc;
@@ -1514,10 +1523,10 @@ int launch_multi_ring_band(const RingLaunch &L, int map_dtype, const void *const
                            const int64_t *mstride_tab, char *ws, hipStream_t s) {
     if (L.nseg == 0) return HBX_OK;
     if (map_dtype == HBX_DT_F64)
-        hipLaunchKernelGGL(multi_ring_band_kernel<double>, dim3(L.grid_ana), dim3(256), 0, s, L.d_ana, L.nseg,
+        hipLaunchKernelGGL(multi_ring_band_kernel<double>, dim3(L.grid_ana), dim3(RBV_LANES), 0, s, L.d_ana, L.nseg,
                            maps_tab, mstride_tab, ws);
     else
-        hipLaunchKernelGGL(multi_ring_band_kernel<float>, dim3(L.grid_ana), dim3(256), 0, s, L.d_ana, L.nseg,
+        hipLaunchKernelGGL(multi_ring_band_kernel<float>, dim3(L.grid_ana), dim3(RBV_LANES), 0, s, L.d_ana, L.nseg,
                            maps_tab, mstride_tab, ws);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "multi_ring_band_kernel launch");

@@ -101,6 +101,70 @@ def test_multi_equals_per_plan_valu_bit_for_bit(gpu, precision, slots):
             assert torch.equal(out[g], refs_m[g]), (g, syn_maps, tpw)
 
 
+# 16 groups (the batch maximum): geometries whose FFT size classes repeat
+# across groups, so the fused ring launches hold many segments each
+GROUPS16 = [(1, 2, 2, 1), (2, 4, 4, 2), (2, 6, 3, 1), (4, 8, 8, 1), (4, 12, 6, 2), (8, 16, 16, 1),
+            (8, 20, 12, 3), (8, 24, 24, 1), (16, 32, 32, 2), (16, 40, 20, 1), (16, 48, 48, 1), (32, 64, 64, 1),
+            (32, 80, 40, 2), (32, 96, 96, 1), (64, 128, 128, 1), (64, 160, 100, 1)]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_multi_16_groups_other_map_dtype_bit_for_bit(gpu, precision):
+    """A batch of 16 groups whose maps have the other float type (FP64 maps
+    into an FP32 batch, FP32 maps into an FP64 batch) equals each group's
+    per-plan VALU transform with the same map type bit for bit."""
+    from hellblast_amd.batch import MultiPlan
+    print(f"[case] multi 16 groups {precision}, other map dtype", flush=True)
+    plans, maps, alm_in = _inputs(GROUPS16, precision, 14)
+    mdt = torch.float32 if precision == "f64" else torch.float64
+    maps = [m.to(mdt) for m in maps]
+    plans = [p.with_tiles(valu_slots=4, mfma_min_maps=1000) for p in plans]
+    refs_a = [q.map2alm(m) for q, m in zip(plans, maps)]
+    refs_m = [q.alm2map(a, dtype=mdt) for q, a in zip(plans, alm_in)]
+    mp = MultiPlan(plans, [m.shape[0] for m in maps], valu_slots=4, syn_maps=2)
+    alm, out = _run(mp, maps, alm_in)
+    for g in range(len(plans)):
+        assert out[g].dtype == mdt
+        assert torch.equal(alm[g], refs_a[g]), g
+        assert torch.equal(out[g], refs_m[g]), g
+
+
+def test_multi_shared_by_threads(gpu):
+    """One FP32 batch used from two host threads on two streams at once, each
+    with its own workspace (the batch's side stream and events are shared
+    under its lock): each thread's alm and maps equal a lone call's."""
+    import threading
+    from hellblast_amd.batch import MultiPlan
+    print("[case] multi batch from two threads", flush=True)
+    plans, maps, alm_in = _inputs(GROUPS, "f32", 15)
+    mp = MultiPlan(plans, [m.shape[0] for m in maps])
+    ref_a, ref_o = _run(mp, maps, alm_in)
+    outs = [None, None]
+    errs = []
+
+    def work(i):
+        try:
+            st = torch.cuda.Stream()
+            with torch.cuda.stream(st):
+                for _ in range(4):
+                    outs[i] = _run(mp, maps, alm_in)
+            st.synchronize()
+        except Exception as e:          # reported by the main thread
+            errs.append(e)
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errs, errs
+    for i in range(2):
+        a, o = outs[i]
+        for g in range(len(plans)):
+            assert torch.equal(a[g], ref_a[g]), (i, g)
+            assert torch.equal(o[g], ref_o[g]), (i, g)
+
+
 def test_multi_f32_c3_shape_within_gate(gpu):
     """FP32 batch over nside 128..1024 (lmax 2 nside, C_l = (l+10)^-2, the
     mixed-batch spectrum): eps_max <= 1e-5 of every map against the FP64
