@@ -129,6 +129,26 @@ def test_multi_16_groups_other_map_dtype_bit_for_bit(gpu, precision):
         assert torch.equal(out[g], refs_m[g]), g
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_multi_large_geometries_bit_for_bit(gpu, precision):
+    """Groups at nside 128..512, whose size classes run in fused launches of
+    64 to 512 threads (Bluestein and power-of-two rings, several groups per
+    launch), equal their per-plan VALU transforms bit for bit."""
+    from hellblast_amd.batch import MultiPlan
+    print(f"[case] multi nside 128..512 {precision}", flush=True)
+    groups = [(256, 512, 512, 2), (512, 1023, 1023, 1), (128, 256, 200, 3), (512, 700, 700, 1)]
+    plans, maps, alm_in = _inputs(groups, precision, 16)
+    plans = [p.with_tiles(valu_slots=4, mfma_min_maps=1000) for p in plans]
+    rdt = maps[0].dtype
+    refs_a = [q.map2alm(m) for q, m in zip(plans, maps)]
+    refs_m = [q.alm2map(a, dtype=rdt) for q, a in zip(plans, alm_in)]
+    mp = MultiPlan(plans, [m.shape[0] for m in maps], valu_slots=4)
+    alm, out = _run(mp, maps, alm_in)
+    for g in range(len(plans)):
+        assert torch.equal(alm[g], refs_a[g]), g
+        assert torch.equal(out[g], refs_m[g]), g
+
+
 def test_multi_shared_by_threads(gpu):
     """One FP32 batch used from two host threads on two streams at once, each
     with its own workspace (the batch's side stream and events are shared
