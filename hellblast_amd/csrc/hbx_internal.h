@@ -77,6 +77,45 @@ struct FftClass {
     int64_t tw_off64 = 0, tw_off32 = 0;   // offsets of this class's FP64 / FP32 twiddle tables
 };
 
+// Fused ring launches (ring_fft.hip): one launch runs every FFT size class
+// of one workgroup size, for one plan or for every plan ("group") of a
+// mixed-geometry batch.  A segment is one plan's rings of one class (or, for
+// a batch's FP64 band DFT, all of a group's rings) and owns the blocks from
+// block0 (times the call's unit) on.
+struct RingSeg {
+    const int32_t *rings;          // ring indices of the class (null: band DFT)
+    const int64_t *ring_start;
+    const int32_t *nphi, *mcut;
+    const double *phi0, *scale;    // scale: batches only (one plan: the call's)
+    const double2 *rot;
+    const void *tw, *V;            // pass twiddles and chirp spectra of the transform's precision
+    const int64_t *vofs;
+    int64_t pstride;               // phase elements per map
+    uint64_t phase_off;            // batches: bytes into the workspace
+    int32_t block0, ncls, nmaps, g, M, log2M, mmax, nr;
+};
+// size_ana / size_syn: blocks (batches) or units of 8 / 64 blocks per map
+// (one plan, whose grid scales with the call's map count)
+struct RingLaunch {
+    int threads = 0, blue = 0, nseg = 0;
+    size_t lds = 0;
+    int64_t size_ana = 0, size_syn = 0;
+    RingSeg *d_ana = nullptr, *d_syn = nullptr;
+};
+// Per-call arguments: a batch passes maps_tab / mstride_tab (device tables,
+// one entry per group) and its workspace, unit 1; one plan passes its maps,
+// phase buffer, scale, map count and unit = 8 or 64 blocks per map.
+struct RingCall {
+    const void *const *maps_tab;
+    const int64_t *mstride_tab;
+    char *ws;
+    void *maps;
+    int64_t mstride;
+    void *phase;
+    const double *scale;
+    int nmaps, unit;
+};
+
 // Device allocations of a plan.  Plans derived from it with hbx_plan_derive
 // share them (read-only tables); the last plan destroyed frees them on their
 // device.  They also own the side stream on which a call runs its
@@ -137,6 +176,9 @@ struct Plan {
     double2 *d_V64 = nullptr;  float2 *d_V32 = nullptr;
     int64_t vtotal = 0;
     std::vector<FftClass> classes;
+    // ring stage launches: the plan's precision, and (ring_f64) the FP64
+    // analysis transforms of an FP32 plan
+    std::vector<RingLaunch> fused, fused64;
     AllocRef allocations;
     // tiles (fixed when the plan is created or derived); 0 = size-based default
     int tile_valu_slots = 0;      // VALU analysis ring-pair slots per lane
@@ -156,32 +198,9 @@ int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *map
                           int map_dtype, int64_t map_stride, const double *scale,
                           hipStream_t s);
 
-// Fused ring launches of mixed-geometry batches: one launch runs the rings
-// of every plan ("group") of the batch whose size classes share a workgroup
-// size.  A segment is one group's rings of one class (or, for the FP64 band
-// DFT, all of its rings) and owns blocks block0 .. of the launch; pixels
-// and per-call pointers come from tables in device memory (maps_tab[g],
-// mstride_tab[g]), phase buffers from the batch workspace.
-struct RingSeg {
-    const int32_t *rings;          // ring indices of the class (null: band DFT)
-    const int64_t *ring_start;
-    const int32_t *nphi, *mcut;
-    const double *phi0, *scale;
-    const double2 *rot;
-    const void *tw, *V;            // pass twiddles and chirp spectra, plan precision
-    const int64_t *vofs;
-    int64_t pstride;               // phase elements per map
-    uint64_t phase_off;            // bytes into the workspace
-    int32_t block0, ncls, nmaps, g, M, log2M, mmax, nr;
-};
-struct RingLaunch {
-    int threads = 0, blue = 0, nseg = 0;
-    size_t lds = 0;
-    unsigned grid_ana = 0, grid_syn = 0;
-    RingSeg *d_ana = nullptr, *d_syn = nullptr;
-};
-// launches for the groups' FFT classes, largest workgroups first, and (FP32
-// batches) the band DFT launch; device tables are appended to allocs
+// a batch's ring launches over every group's classes, largest workgroups
+// first, and (FP32 batches) its band DFT launch; device tables are appended
+// to allocs
 int build_ring_launches(const std::vector<Plan> &plans, const std::vector<int> &nmaps,
                         const std::vector<uint64_t> &phase_off, std::vector<RingLaunch> &out,
                         RingLaunch &band, std::vector<void *> &allocs);

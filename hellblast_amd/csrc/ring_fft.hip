@@ -517,24 +517,6 @@ __device__ __forceinline__ void ring_analysis_ring(c2<T> *buf, const TIn *__rest
     }
 }
 
-template <typename T, typename TIn, bool BLUE, typename TP = T>
-__global__ void __launch_bounds__(ring_max_threads<T>())
-ring_analysis_kernel(const TIn *__restrict__ maps, int64_t map_stride, c2<TP> *__restrict__ phase,
-                     int64_t phase_stride, const int32_t *__restrict__ rings,
-                     const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
-                     const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
-                     const double *__restrict__ scale, const double2 *__restrict__ rot,
-                     const c2<T> *__restrict__ tw, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
-                     int M, int log2M, int mmax, int nr, int ncls, int nmaps) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    int slot, b;
-    ring_block(blockIdx.x, nmaps, &slot, &b);
-    if (slot >= ncls) return;
-    ring_analysis_ring<T, TIn, BLUE, TP>(reinterpret_cast<c2<T> *>(smem_raw), maps + (int64_t)b * map_stride,
-                                         phase + (int64_t)b * phase_stride, rings[slot], ring_start, nphi, mcut,
-                                         phi0, scale, rot, tw, V, vofs, M, log2M, mmax, nr);
-}
-
 // ---------------------------------------------------------------------------
 // synthesis: phase -> maps
 // ---------------------------------------------------------------------------
@@ -668,28 +650,10 @@ __device__ __forceinline__ void ring_synthesis_ring(c2<T> *buf, const c2<T> *__r
     }
 }
 
-template <typename T, typename TOut, bool BLUE>
-__global__ void __launch_bounds__(ring_max_threads<T>())
-ring_synthesis_kernel(const c2<T> *__restrict__ phase, int64_t phase_stride, TOut *__restrict__ maps,
-                      int64_t map_stride, const int32_t *__restrict__ rings,
-                      const int64_t *__restrict__ ring_start, const int32_t *__restrict__ nphi,
-                      const int32_t *__restrict__ mcut, const double *__restrict__ phi0,
-                      const double *__restrict__ scale, const double2 *__restrict__ rot,
-                      const c2<T> *__restrict__ tw, const c2<T> *__restrict__ V, const int64_t *__restrict__ vofs,
-                      int M, int log2M, int mmax, int nr, int ncls, int nmaps) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    int slot, b;
-    ring_block_syn(blockIdx.x, nmaps, (ncls + 63) >> 6, &slot, &b);
-    if (slot >= ncls) return;
-    ring_synthesis_ring<T, TOut, BLUE>(reinterpret_cast<c2<T> *>(smem_raw), phase + (int64_t)b * phase_stride,
-                                       maps + (int64_t)b * map_stride, rings[slot], ring_start, nphi, mcut, phi0,
-                                       scale, rot, tw, V, vofs, M, log2M, mmax, nr);
-}
-
 // ---------------------------------------------------------------------------
 // FP64 low-m band of FP32 plans: direct DFT in FP64 for m < M_LO,
 //   F_r(m) = [m < mcut_r] scale_r e^{-i m phi0_r} sum_j f_j e^{-2 pi i m j / n}
-// (same conventions as ring_analysis_kernel), on the FP64 matrix cores: per
+// (same conventions as the ring analysis), on the FP64 matrix cores: per
 // ring and group of 16 maps, v_mfma_f64_16x16x4f64 contracts 4 pixels per
 // instruction,
 //   D[row = 2 m + c][col = map] += A[row][k] B[k][col],
@@ -979,6 +943,219 @@ static int build_chirp_spectra(const std::vector<int32_t> &hs_host, const std::v
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Fused ring launches: one launch runs every size class of one workgroup size
+// (FP32: 64 threads for M <= 1024, then 128 .. 1024; power-of-two and
+// Bluestein rings apart), for one plan or for every group of a mixed-geometry
+// batch.  A segment is one plan's rings of one class; a block finds its
+// segment by a binary search over the segments' first blocks (scalar loads),
+// then maps to (ring slot, map) as a launch of that class alone would: the
+// maps of one ring run back to back on one XCD (analysis), and each XCD
+// takes a contiguous run of ring octets (synthesis).  Segments start at
+// multiples of 8 (analysis) or 64 (synthesis) blocks, so this grouping
+// holds in every segment.  The rings run ring_analysis_ring /
+// ring_synthesis_ring with the segment's tables; table pointers read from a
+// segment are cast to the global address space so their loads and stores
+// stay global ones.
+// ---------------------------------------------------------------------------
+template <typename X> __device__ __forceinline__ X *gptr(X *p) { return (X *)(HBX_GLOBAL X *)p; }
+
+// last segment whose first block (block0 * unit) is <= blk
+__device__ __forceinline__ const RingSeg *find_seg(const RingSeg *segs, int nseg, int blk, int unit) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ldc(&segs[mid].block0) * unit <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return segs + lo;
+}
+
+// Where a block's map and phase buffer are: a batch (maps_tab != null) reads
+// them per group from device tables and the workspace; one plan takes the
+// call's arguments.
+template <typename TM, typename TPh>
+__device__ __forceinline__ void seg_buffers(const RingCall &c, const RingSeg *sg, int b, TM *&map, TPh *&ph,
+                                            const double *&scale) {
+    const int64_t pst = ldc(&sg->pstride);
+    if (c.maps_tab != nullptr) {
+        const int g = ldc(&sg->g);
+        map = gptr((TM *)ldc(c.maps_tab + g)) + (int64_t)b * ldc(c.mstride_tab + g);
+        ph = gptr((TPh *)(c.ws + ldc(&sg->phase_off))) + (int64_t)b * pst;
+        scale = gptr(ldc(&sg->scale));
+    } else {
+        map = gptr((TM *)c.maps) + (int64_t)b * c.mstride;
+        ph = gptr((TPh *)c.phase) + (int64_t)b * pst;
+        scale = gptr(c.scale);
+    }
+}
+
+template <typename T, typename TIn, bool BLUE, typename TP>
+__global__ void __launch_bounds__(ring_max_threads<T>())
+fused_ring_analysis_kernel(const RingSeg *__restrict__ segs, int nseg, RingCall c) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int blk = blockIdx.x;
+    const RingSeg *sg = find_seg(segs, nseg, blk, c.unit);
+    const int nmaps = c.maps_tab != nullptr ? ldc(&sg->nmaps) : c.nmaps;
+    int slot, b;
+    ring_block(blk - ldc(&sg->block0) * c.unit, nmaps, &slot, &b);
+    if (slot >= ldc(&sg->ncls)) return;
+    const TIn *mapb;
+    c2<TP> *phb;
+    const double *scale;
+    seg_buffers<const TIn, c2<TP>>(c, sg, b, mapb, phb, scale);
+    ring_analysis_ring<T, TIn, BLUE, TP>(
+        reinterpret_cast<c2<T> *>(smem_raw), mapb, phb, ldc(ldc(&sg->rings) + slot), gptr(ldc(&sg->ring_start)),
+        gptr(ldc(&sg->nphi)), gptr(ldc(&sg->mcut)), gptr(ldc(&sg->phi0)), scale, gptr(ldc(&sg->rot)),
+        gptr((const c2<T> *)ldc(&sg->tw)), gptr((const c2<T> *)ldc(&sg->V)), gptr(ldc(&sg->vofs)), ldc(&sg->M),
+        ldc(&sg->log2M), ldc(&sg->mmax), ldc(&sg->nr));
+}
+
+template <typename T, typename TOut, bool BLUE>
+__global__ void __launch_bounds__(ring_max_threads<T>())
+fused_ring_synthesis_kernel(const RingSeg *__restrict__ segs, int nseg, RingCall c) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int blk = blockIdx.x;
+    const RingSeg *sg = find_seg(segs, nseg, blk, c.unit);
+    const int nmaps = c.maps_tab != nullptr ? ldc(&sg->nmaps) : c.nmaps;
+    const int ncls = ldc(&sg->ncls);
+    int slot, b;
+    ring_block_syn(blk - ldc(&sg->block0) * c.unit, nmaps, (ncls + 63) >> 6, &slot, &b);
+    if (slot >= ncls) return;
+    TOut *mapb;
+    const c2<T> *phb;
+    const double *scale;
+    seg_buffers<TOut, const c2<T>>(c, sg, b, mapb, phb, scale);
+    ring_synthesis_ring<T, TOut, BLUE>(
+        reinterpret_cast<c2<T> *>(smem_raw), phb, mapb, ldc(ldc(&sg->rings) + slot), gptr(ldc(&sg->ring_start)),
+        gptr(ldc(&sg->nphi)), gptr(ldc(&sg->mcut)), gptr(ldc(&sg->phi0)), scale, gptr(ldc(&sg->rot)),
+        gptr((const c2<T> *)ldc(&sg->tw)), gptr((const c2<T> *)ldc(&sg->V)), gptr(ldc(&sg->vofs)), ldc(&sg->M),
+        ldc(&sg->log2M), ldc(&sg->mmax), ldc(&sg->nr));
+}
+
+// band DFT of a batch (FP32): segment = one group, blocks (map, ring) map-major
+template <typename TIn>
+__global__ void __launch_bounds__(RBV_LANES)
+multi_ring_band_kernel(const RingSeg *__restrict__ segs, int nseg, RingCall c) {
+    const int blk = blockIdx.x;
+    const RingSeg *sg = find_seg(segs, nseg, blk, 1);
+    const int nr = ldc(&sg->nr), local = blk - ldc(&sg->block0);
+    const int b = local / nr, r = local - b * nr;
+    if (b >= ldc(&sg->nmaps)) return;
+    const TIn *mapb;
+    double2 *bandb;
+    const double *scale;
+    seg_buffers<const TIn, double2>(c, sg, b, mapb, bandb, scale);
+    const int mmax = ldc(&sg->mmax);
+    ring_band_valu_ring<TIn>(mapb, bandb, r, gptr(ldc(&sg->ring_start)), gptr(ldc(&sg->nphi)),
+                             gptr(ldc(&sg->mcut)), gptr(ldc(&sg->phi0)), scale, mmax < M_LO - 1 ? mmax : M_LO - 1,
+                             nr);
+}
+
+template <typename T, typename TP> static int allow_fused_lds(size_t lds) {
+    int rc;
+    if ((rc = allow_lds<T>(fused_ring_analysis_kernel<T, float, false, TP>, lds))) return rc;
+    if ((rc = allow_lds<T>(fused_ring_analysis_kernel<T, double, false, TP>, lds))) return rc;
+    if ((rc = allow_lds<T>(fused_ring_analysis_kernel<T, float, true, TP>, lds))) return rc;
+    if ((rc = allow_lds<T>(fused_ring_analysis_kernel<T, double, true, TP>, lds))) return rc;
+    if constexpr (sizeof(T) == sizeof(TP)) {
+        if ((rc = allow_lds<T>(fused_ring_synthesis_kernel<T, float, false>, lds))) return rc;
+        if ((rc = allow_lds<T>(fused_ring_synthesis_kernel<T, double, false>, lds))) return rc;
+        if ((rc = allow_lds<T>(fused_ring_synthesis_kernel<T, float, true>, lds))) return rc;
+        if ((rc = allow_lds<T>(fused_ring_synthesis_kernel<T, double, true>, lds))) return rc;
+    }
+    return 0;
+}
+
+// one class (kind blue) of plan p as a segment; t64: the FP64 tables (FP64
+// plans, or the FP64 analysis transforms of an FP32 plan)
+static RingSeg class_seg(const Plan *p, const FftClass &c, int blue, bool t64, const double2 *rot) {
+    RingSeg sg{};
+    sg.rings = blue ? c.d_rings_blue : c.d_rings_pow2;
+    sg.ring_start = p->d_ring_start;
+    sg.nphi = p->d_nphi;
+    sg.mcut = p->d_mcut;
+    sg.phi0 = p->d_phi0;
+    sg.scale = p->d_scale_ana;
+    sg.rot = rot;
+    sg.tw = t64 ? (const void *)(p->d_ftw64 + c.tw_off64) : (const void *)(p->d_ftw32 + c.tw_off32);
+    sg.V = t64 ? (const void *)p->d_V64 : (const void *)p->d_V32;
+    sg.vofs = p->d_vofs;
+    sg.pstride = phase_elems_per_map(p);
+    sg.ncls = blue ? c.nrings_blue : c.nrings_pow2;
+    sg.M = c.M;
+    sg.log2M = c.log2M;
+    sg.mmax = p->mmax;
+    sg.nr = p->nrings;
+    return sg;
+}
+
+// Launches from segments keyed by (workgroup size, Bluestein), largest
+// workgroups first.  absolute: block0 / the grids count blocks (batches,
+// each segment with its own map count); otherwise they count units of 8
+// (analysis) or 64 (synthesis) blocks per map of the call (one plan).
+// alloc(bytes) returns device memory owned by the caller's object.
+template <class Alloc>
+static int make_launches(const std::map<std::pair<int, int>, std::vector<RingSeg>> &by, bool t64, bool tp32,
+                         bool absolute, std::vector<RingLaunch> &out, Alloc alloc) {
+    out.clear();
+    const int64_t lim = INT32_MAX;
+    for (auto it = by.rbegin(); it != by.rend(); ++it) {
+        RingLaunch L;
+        L.threads = it->first.first;
+        L.blue = it->first.second;
+        std::vector<RingSeg> ana = it->second, syn = it->second;
+        int64_t ba = 0, bs = 0;
+        for (size_t i = 0; i < ana.size(); ++i) {
+            ana[i].block0 = (int32_t)ba;
+            syn[i].block0 = (int32_t)bs;
+            syn[i].scale = nullptr;                 // batch synthesis: unit scale (one plan: the call's)
+            const int n = ana[i].ncls;
+            ba += absolute ? (int64_t)ring_grid(n, ana[i].nmaps) : (n + 7) / 8;
+            bs += absolute ? (int64_t)ring_grid_syn(n, syn[i].nmaps) : (n + 63) / 64;
+            if (ba > lim || bs > lim) return set_error(HBX_EUNSUPPORTED, "too many rings for one ring launch");
+            L.lds = std::max(L.lds, t64 ? ring_lds_bytes<double>(ana[i].M) : ring_lds_bytes<float>(ana[i].M));
+        }
+        L.nseg = (int)ana.size();
+        L.size_ana = ba;
+        L.size_syn = bs;
+        void *da = alloc(ana.size() * sizeof(RingSeg)), *ds = alloc(syn.size() * sizeof(RingSeg));
+        if (!da || !ds) return set_error(HBX_ENOMEM, "hipMalloc for the ring launch tables failed");
+        HBX_HIP(hipMemcpy(da, ana.data(), ana.size() * sizeof(RingSeg), hipMemcpyHostToDevice));
+        HBX_HIP(hipMemcpy(ds, syn.data(), syn.size() * sizeof(RingSeg), hipMemcpyHostToDevice));
+        L.d_ana = static_cast<RingSeg *>(da);
+        L.d_syn = static_cast<RingSeg *>(ds);
+        int rc = t64 ? (tp32 ? allow_fused_lds<double, float>(L.lds) : allow_fused_lds<double, double>(L.lds))
+                     : allow_fused_lds<float, float>(L.lds);
+        if (rc) return rc;
+        out.push_back(L);
+    }
+    return HBX_OK;
+}
+
+// the plan's own launches: its precision, and (FP32 plans whose analysis
+// runs in FP64) the FP64 analysis launches
+static int plan_build_fused(Plan *p) {
+    auto alloc = [p](size_t bytes) -> void * {
+        void *ptr = nullptr;
+        if (hipMalloc(&ptr, bytes) != hipSuccess) return nullptr;
+        p->allocations.push_back(ptr);
+        p->device_bytes += bytes;
+        return ptr;
+    };
+    const bool f64 = p->precision == HBX_F64;
+    std::map<std::pair<int, int>, std::vector<RingSeg>> by, by64;
+    for (const FftClass &c : p->classes)
+        for (int blue = 0; blue < 2; ++blue) {
+            if ((blue ? c.nrings_blue : c.nrings_pow2) == 0) continue;
+            by[{c.threads, blue}].push_back(class_seg(p, c, blue, f64, p->d_rot));
+            if (p->ring_f64) by64[{class_threads<double>(c.M), blue}].push_back(class_seg(p, c, blue, true, p->d_rot64));
+        }
+    int rc = make_launches(by, f64, false, false, p->fused, alloc);
+    if (!rc && p->ring_f64) rc = make_launches(by64, true, true, false, p->fused64, alloc);
+    return rc;
+}
+
 int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const double *phi0) {
     const int nr = p->nrings;
     // class per ring
@@ -1149,93 +1326,62 @@ int plan_build_fft_tables(Plan *p, const std::vector<int32_t> &nphi, const doubl
         if (rc) return rc;
         if (e != hipSuccess) return hip_error(e, "FP32 chirp spectra");
     }
-    // raise the dynamic-LDS limit of every kernel instance we may launch
-    for (const auto &c : p->classes) {
-        if (p->precision == HBX_F64) {
-            const size_t lds = ring_lds_bytes<double>(c.M);
-            if ((rc = allow_lds<double>(ring_analysis_kernel<double, double, false>, lds))) return rc;
-            if ((rc = allow_lds<double>(ring_analysis_kernel<double, float, false>, lds))) return rc;
-            if ((rc = allow_lds<double>(ring_analysis_kernel<double, double, true>, lds))) return rc;
-            if ((rc = allow_lds<double>(ring_analysis_kernel<double, float, true>, lds))) return rc;
-            if ((rc = allow_lds<double>(ring_synthesis_kernel<double, double, false>, lds))) return rc;
-            if ((rc = allow_lds<double>(ring_synthesis_kernel<double, float, false>, lds))) return rc;
-            if ((rc = allow_lds<double>(ring_synthesis_kernel<double, double, true>, lds))) return rc;
-            if ((rc = allow_lds<double>(ring_synthesis_kernel<double, float, true>, lds))) return rc;
-        } else {
-            const size_t lds = ring_lds_bytes<float>(c.M);
-            if ((rc = allow_lds<float>(ring_analysis_kernel<float, double, false>, lds))) return rc;
-            if ((rc = allow_lds<float>(ring_analysis_kernel<float, float, false>, lds))) return rc;
-            if ((rc = allow_lds<float>(ring_analysis_kernel<float, double, true>, lds))) return rc;
-            if ((rc = allow_lds<float>(ring_analysis_kernel<float, float, true>, lds))) return rc;
-            if ((rc = allow_lds<float>(ring_synthesis_kernel<float, double, false>, lds))) return rc;
-            if ((rc = allow_lds<float>(ring_synthesis_kernel<float, float, false>, lds))) return rc;
-            if ((rc = allow_lds<float>(ring_synthesis_kernel<float, double, true>, lds))) return rc;
-            if ((rc = allow_lds<float>(ring_synthesis_kernel<float, float, true>, lds))) return rc;
-            if (p->ring_f64) {
-                const size_t l64 = ring_lds_bytes<double>(c.M);
-                if ((rc = allow_lds<double>(ring_analysis_kernel<double, double, false, float>, l64))) return rc;
-                if ((rc = allow_lds<double>(ring_analysis_kernel<double, float, false, float>, l64))) return rc;
-                if ((rc = allow_lds<double>(ring_analysis_kernel<double, double, true, float>, l64))) return rc;
-                if ((rc = allow_lds<double>(ring_analysis_kernel<double, float, true, float>, l64))) return rc;
-            }
-        }
-    }
+    if ((rc = plan_build_fused(p))) return rc;
     HBX_HIP(hipDeviceSynchronize());
     return 0;
 }
 
-template <typename T> static const c2<T> *class_tw(const Plan *p, const FftClass &c);
-template <> const double2 *class_tw<double>(const Plan *p, const FftClass &c) { return p->d_ftw64 + c.tw_off64; }
-template <> const float2 *class_tw<float>(const Plan *p, const FftClass &c) { return p->d_ftw32 + c.tw_off32; }
-
 template <typename T, typename TIn, typename TP = T>
-static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t map_stride,
-                           c2<TP> *phase, const double *scale, const c2<T> *V, hipStream_t s) {
-    const int64_t pstride = phase_elems_per_map(p);
-    // FP64 transforms of an FP32 plan use their own rotation steps (thread count)
-    const double2 *rot = (sizeof(T) != sizeof(TP)) ? p->d_rot64 : p->d_rot;
+static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t map_stride, c2<TP> *phase,
+                           const double *scale, hipStream_t s) {
+    // FP64 transforms of an FP32 plan run their own launches (FP64 thread counts and tables)
+    const std::vector<RingLaunch> &Ls = (sizeof(T) != sizeof(TP)) ? p->fused64 : p->fused;
+    RingCall c{};
+    c.maps = const_cast<TIn *>(maps);
+    c.mstride = map_stride;
+    c.phase = phase;
+    c.scale = scale;
+    c.nmaps = nmaps;
+    c.unit = 8 * nmaps;
     timer_mark(HBX_TIMER_RING_ANALYSIS, true, s);
-    for (const auto &c : p->classes) {
-        const size_t lds = ring_lds_bytes<T>(c.M);
-        const c2<T> *tw = class_tw<T>(p, c);
-        const int nt = class_threads<T>(c.M);
-        if (c.nrings_pow2)
-            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, false, TP>), dim3(ring_grid(c.nrings_pow2, nmaps)),
-                               dim3(nt), lds, s, maps, map_stride, phase, pstride, c.d_rings_pow2,
-                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, rot, tw, V, p->d_vofs,
-                               c.M, c.log2M, p->mmax, p->nrings, c.nrings_pow2, nmaps);
-        if (c.nrings_blue)
-            hipLaunchKernelGGL((ring_analysis_kernel<T, TIn, true, TP>), dim3(ring_grid(c.nrings_blue, nmaps)),
-                               dim3(nt), lds, s, maps, map_stride, phase, pstride, c.d_rings_blue,
-                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, rot, tw, V, p->d_vofs,
-                               c.M, c.log2M, p->mmax, p->nrings, c.nrings_blue, nmaps);
+    for (const RingLaunch &L : Ls) {
+        const int64_t grid = L.size_ana * 8 * (int64_t)nmaps;
+        if (grid > INT32_MAX) return set_error(HBX_EUNSUPPORTED, "too many maps for one ring launch");
+        if (L.blue)
+            hipLaunchKernelGGL((fused_ring_analysis_kernel<T, TIn, true, TP>), dim3((unsigned)grid), dim3(L.threads),
+                               L.lds, s, L.d_ana, L.nseg, c);
+        else
+            hipLaunchKernelGGL((fused_ring_analysis_kernel<T, TIn, false, TP>), dim3((unsigned)grid),
+                               dim3(L.threads), L.lds, s, L.d_ana, L.nseg, c);
         hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return hip_error(e, "ring_analysis_kernel launch");
+        if (e != hipSuccess) return hip_error(e, "fused_ring_analysis_kernel launch");
     }
     timer_mark(HBX_TIMER_RING_ANALYSIS, false, s);
     return 0;
 }
 
 template <typename T, typename TOut>
-static int ring_synthesis_t(const Plan *p, int nmaps, const c2<T> *phase, TOut *maps,
-                            int64_t map_stride, const double *scale, const c2<T> *V, hipStream_t s) {
-    const int64_t pstride = phase_elems_per_map(p);
+static int ring_synthesis_t(const Plan *p, int nmaps, const c2<T> *phase, TOut *maps, int64_t map_stride,
+                            const double *scale, hipStream_t s) {
+    RingCall c{};
+    c.maps = maps;
+    c.mstride = map_stride;
+    c.phase = const_cast<c2<T> *>(phase);
+    c.scale = scale;
+    c.nmaps = nmaps;
+    c.unit = 64 * nmaps;
     timer_mark(HBX_TIMER_RING_SYNTHESIS, true, s);
-    for (const auto &c : p->classes) {
-        const size_t lds = ring_lds_bytes<T>(c.M);
-        const c2<T> *tw = class_tw<T>(p, c);
-        if (c.nrings_pow2)
-            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, false>), dim3(ring_grid_syn(c.nrings_pow2, nmaps)),
-                               dim3(c.threads), lds, s, phase, pstride, maps, map_stride, c.d_rings_pow2,
-                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
-                               c.M, c.log2M, p->mmax, p->nrings, c.nrings_pow2, nmaps);
-        if (c.nrings_blue)
-            hipLaunchKernelGGL((ring_synthesis_kernel<T, TOut, true>), dim3(ring_grid_syn(c.nrings_blue, nmaps)),
-                               dim3(c.threads), lds, s, phase, pstride, maps, map_stride, c.d_rings_blue,
-                               p->d_ring_start, p->d_nphi, p->d_mcut, p->d_phi0, scale, p->d_rot, tw, V, p->d_vofs,
-                               c.M, c.log2M, p->mmax, p->nrings, c.nrings_blue, nmaps);
+    for (const RingLaunch &L : p->fused) {
+        const int64_t grid = L.size_syn * 64 * (int64_t)nmaps;
+        if (grid > INT32_MAX) return set_error(HBX_EUNSUPPORTED, "too many maps for one ring launch");
+        if (L.blue)
+            hipLaunchKernelGGL((fused_ring_synthesis_kernel<T, TOut, true>), dim3((unsigned)grid), dim3(L.threads),
+                               L.lds, s, L.d_syn, L.nseg, c);
+        else
+            hipLaunchKernelGGL((fused_ring_synthesis_kernel<T, TOut, false>), dim3((unsigned)grid), dim3(L.threads),
+                               L.lds, s, L.d_syn, L.nseg, c);
         hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return hip_error(e, "ring_synthesis_kernel launch");
+        if (e != hipSuccess) return hip_error(e, "fused_ring_synthesis_kernel launch");
     }
     timer_mark(HBX_TIMER_RING_SYNTHESIS, false, s);
     return 0;
@@ -1246,20 +1392,18 @@ int launch_ring_analysis(const Plan *p, int nmaps, const void *maps, int map_dty
     if (p->precision == HBX_F64) {
         auto *ph = static_cast<double2 *>(phase);
         if (map_dtype == HBX_DT_F64)
-            return ring_analysis_t<double, double>(p, nmaps, (const double *)maps, map_stride, ph, scale, p->d_V64, s);
-        return ring_analysis_t<double, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_V64, s);
+            return ring_analysis_t<double, double>(p, nmaps, (const double *)maps, map_stride, ph, scale, s);
+        return ring_analysis_t<double, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, s);
     }
     auto *ph = static_cast<float2 *>(phase);
     if (p->ring_f64) {
         if (map_dtype == HBX_DT_F64)
-            return ring_analysis_t<double, double, float>(p, nmaps, (const double *)maps, map_stride, ph, scale,
-                                                          p->d_V64, s);
-        return ring_analysis_t<double, float, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_V64,
-                                                     s);
+            return ring_analysis_t<double, double, float>(p, nmaps, (const double *)maps, map_stride, ph, scale, s);
+        return ring_analysis_t<double, float, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, s);
     }
     if (map_dtype == HBX_DT_F64)
-        return ring_analysis_t<float, double>(p, nmaps, (const double *)maps, map_stride, ph, scale, p->d_V32, s);
-    return ring_analysis_t<float, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, p->d_V32, s);
+        return ring_analysis_t<float, double>(p, nmaps, (const double *)maps, map_stride, ph, scale, s);
+    return ring_analysis_t<float, float>(p, nmaps, (const float *)maps, map_stride, ph, scale, s);
 }
 
 int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *maps, int map_dtype,
@@ -1267,225 +1411,93 @@ int launch_ring_synthesis(const Plan *p, int nmaps, const void *phase, void *map
     if (p->precision == HBX_F64) {
         auto *ph = static_cast<const double2 *>(phase);
         if (map_dtype == HBX_DT_F64)
-            return ring_synthesis_t<double, double>(p, nmaps, ph, (double *)maps, map_stride, scale, p->d_V64, s);
-        return ring_synthesis_t<double, float>(p, nmaps, ph, (float *)maps, map_stride, scale, p->d_V64, s);
+            return ring_synthesis_t<double, double>(p, nmaps, ph, (double *)maps, map_stride, scale, s);
+        return ring_synthesis_t<double, float>(p, nmaps, ph, (float *)maps, map_stride, scale, s);
     }
     auto *ph = static_cast<const float2 *>(phase);
     if (map_dtype == HBX_DT_F64)
-        return ring_synthesis_t<float, double>(p, nmaps, ph, (double *)maps, map_stride, scale, p->d_V32, s);
-    return ring_synthesis_t<float, float>(p, nmaps, ph, (float *)maps, map_stride, scale, p->d_V32, s);
+        return ring_synthesis_t<float, double>(p, nmaps, ph, (double *)maps, map_stride, scale, s);
+    return ring_synthesis_t<float, float>(p, nmaps, ph, (float *)maps, map_stride, scale, s);
 }
-
 
 // ---------------------------------------------------------------------------
-// Fused ring launches of mixed-geometry batches.  Block -> segment by a
-// binary search over the segments' first blocks (scalar loads), then the
-// per-plan block mapping inside the segment: segments start at multiples of
-// 8 (analysis, band) or 64 (synthesis) blocks, so the XCD grouping of the
-// per-plan launches is kept.  The rings run the per-plan code
-// (ring_analysis_ring / ring_synthesis_ring / ring_band_valu_ring) with the
-// group's tables, so every group's phases and pixels equal its plan's own
-// launches bit for bit.  Table pointers read from the segment are cast to
-// the global address space so their loads and stores stay global ones.
+// mixed-geometry batches: the groups' classes merged by workgroup size, and
+// (FP32 batches) the band DFT over every group's rings
 // ---------------------------------------------------------------------------
-template <typename X> __device__ __forceinline__ X *gptr(X *p) { return (X *)(HBX_GLOBAL X *)p; }
-
-__device__ __forceinline__ const RingSeg *find_seg(const RingSeg *segs, int nseg, int blk) {
-    int lo = 0, hi = nseg - 1;                      // last segment with block0 <= blk
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (ldc(&segs[mid].block0) <= blk) lo = mid;
-        else hi = mid - 1;
-    }
-    return segs + lo;
-}
-
-template <typename T, typename TIn, bool BLUE>
-__global__ void __launch_bounds__(ring_max_threads<T>())
-multi_ring_analysis_kernel(const RingSeg *__restrict__ segs, int nseg, const void *const *maps_tab,
-                           const int64_t *mstride_tab, char *ws) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int blk = blockIdx.x;
-    const RingSeg *sg = find_seg(segs, nseg, blk);
-    int slot, b;
-    ring_block(blk - ldc(&sg->block0), ldc(&sg->nmaps), &slot, &b);
-    if (slot >= ldc(&sg->ncls)) return;
-    const int g = ldc(&sg->g);
-    const TIn *mapb = gptr((const TIn *)ldc(maps_tab + g)) + (int64_t)b * ldc(mstride_tab + g);
-    c2<T> *phb = gptr(reinterpret_cast<c2<T> *>(ws + ldc(&sg->phase_off))) + (int64_t)b * ldc(&sg->pstride);
-    const int r = ldc(ldc(&sg->rings) + slot);
-    ring_analysis_ring<T, TIn, BLUE, T>(
-        reinterpret_cast<c2<T> *>(smem_raw), mapb, phb, r, gptr(ldc(&sg->ring_start)), gptr(ldc(&sg->nphi)),
-        gptr(ldc(&sg->mcut)), gptr(ldc(&sg->phi0)), gptr(ldc(&sg->scale)), gptr(ldc(&sg->rot)),
-        gptr((const c2<T> *)ldc(&sg->tw)), gptr((const c2<T> *)ldc(&sg->V)), gptr(ldc(&sg->vofs)), ldc(&sg->M),
-        ldc(&sg->log2M), ldc(&sg->mmax), ldc(&sg->nr));
-}
-
-template <typename T, typename TOut, bool BLUE>
-__global__ void __launch_bounds__(ring_max_threads<T>())
-multi_ring_synthesis_kernel(const RingSeg *__restrict__ segs, int nseg, void *const *maps_tab,
-                            const int64_t *mstride_tab, const char *ws) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int blk = blockIdx.x;
-    const RingSeg *sg = find_seg(segs, nseg, blk);
-    const int ncls = ldc(&sg->ncls);
-    int slot, b;
-    ring_block_syn(blk - ldc(&sg->block0), ldc(&sg->nmaps), (ncls + 63) >> 6, &slot, &b);
-    if (slot >= ncls) return;
-    const int g = ldc(&sg->g);
-    TOut *mapb = gptr((TOut *)ldc(maps_tab + g)) + (int64_t)b * ldc(mstride_tab + g);
-    const c2<T> *phb =
-        gptr(reinterpret_cast<const c2<T> *>(ws + ldc(&sg->phase_off))) + (int64_t)b * ldc(&sg->pstride);
-    const int r = ldc(ldc(&sg->rings) + slot);
-    ring_synthesis_ring<T, TOut, BLUE>(
-        reinterpret_cast<c2<T> *>(smem_raw), phb, mapb, r, gptr(ldc(&sg->ring_start)), gptr(ldc(&sg->nphi)),
-        gptr(ldc(&sg->mcut)), gptr(ldc(&sg->phi0)), gptr(ldc(&sg->scale)), gptr(ldc(&sg->rot)),
-        gptr((const c2<T> *)ldc(&sg->tw)), gptr((const c2<T> *)ldc(&sg->V)), gptr(ldc(&sg->vofs)), ldc(&sg->M),
-        ldc(&sg->log2M), ldc(&sg->mmax), ldc(&sg->nr));
-}
-
-// band DFT (FP32 batches): segment = one group, blocks (map, ring) map-major
-template <typename TIn>
-__global__ void __launch_bounds__(RBV_LANES)
-multi_ring_band_kernel(const RingSeg *__restrict__ segs, int nseg, const void *const *maps_tab,
-                       const int64_t *mstride_tab, char *ws) {
-    const int blk = blockIdx.x;
-    const RingSeg *sg = find_seg(segs, nseg, blk);
-    const int nr = ldc(&sg->nr), local = blk - ldc(&sg->block0);
-    const int b = local / nr, r = local - b * nr;
-    if (b >= ldc(&sg->nmaps)) return;
-    const int g = ldc(&sg->g);
-    const TIn *mapb = gptr((const TIn *)ldc(maps_tab + g)) + (int64_t)b * ldc(mstride_tab + g);
-    double2 *bandb =
-        gptr(reinterpret_cast<double2 *>(ws + ldc(&sg->phase_off))) + (int64_t)b * ldc(&sg->pstride);
-    const int mmax = ldc(&sg->mmax);
-    ring_band_valu_ring<TIn>(mapb, bandb, r, gptr(ldc(&sg->ring_start)), gptr(ldc(&sg->nphi)),
-                             gptr(ldc(&sg->mcut)), gptr(ldc(&sg->phi0)), gptr(ldc(&sg->scale)),
-                             mmax < M_LO - 1 ? mmax : M_LO - 1, nr);
-}
-
-template <typename T> static int allow_multi_lds(size_t lds) {
-    int rc;
-    if ((rc = allow_lds<T>(multi_ring_analysis_kernel<T, float, false>, lds))) return rc;
-    if ((rc = allow_lds<T>(multi_ring_analysis_kernel<T, double, false>, lds))) return rc;
-    if ((rc = allow_lds<T>(multi_ring_analysis_kernel<T, float, true>, lds))) return rc;
-    if ((rc = allow_lds<T>(multi_ring_analysis_kernel<T, double, true>, lds))) return rc;
-    if ((rc = allow_lds<T>(multi_ring_synthesis_kernel<T, float, false>, lds))) return rc;
-    if ((rc = allow_lds<T>(multi_ring_synthesis_kernel<T, double, false>, lds))) return rc;
-    if ((rc = allow_lds<T>(multi_ring_synthesis_kernel<T, float, true>, lds))) return rc;
-    return allow_lds<T>(multi_ring_synthesis_kernel<T, double, true>, lds);
-}
-
-static int upload_segs(const std::vector<RingSeg> &v, RingSeg **d, std::vector<void *> &allocs) {
-    void *ptr = nullptr;
-    if (hipMalloc(&ptr, v.size() * sizeof(RingSeg)) != hipSuccess)
-        return set_error(HBX_ENOMEM, "hipMalloc for the fused ring tables failed");
-    allocs.push_back(ptr);
-    HBX_HIP(hipMemcpy(ptr, v.data(), v.size() * sizeof(RingSeg), hipMemcpyHostToDevice));
-    *d = static_cast<RingSeg *>(ptr);
-    return HBX_OK;
-}
-
 int build_ring_launches(const std::vector<Plan> &plans, const std::vector<int> &nmaps,
                         const std::vector<uint64_t> &phase_off, std::vector<RingLaunch> &out, RingLaunch &band,
                         std::vector<void *> &allocs) {
-    out.clear();
     band = RingLaunch{};
+    auto alloc = [&allocs](size_t bytes) -> void * {
+        void *ptr = nullptr;
+        if (hipMalloc(&ptr, bytes) != hipSuccess) return nullptr;
+        allocs.push_back(ptr);
+        return ptr;
+    };
     const bool f64 = plans[0].precision == HBX_F64;
-    // (workgroup size, Bluestein) -> segments, in group order
     std::map<std::pair<int, int>, std::vector<RingSeg>> by;
     for (size_t g = 0; g < plans.size(); ++g) {
         const Plan &p = plans[g];
         if (p.ring_f64) return set_error(HBX_EPLAN, "batches take plans whose ring stage runs at their precision");
         for (const FftClass &c : p.classes)
             for (int blue = 0; blue < 2; ++blue) {
-                const int n = blue ? c.nrings_blue : c.nrings_pow2;
-                if (n == 0) continue;
-                RingSeg sg{};
-                sg.rings = blue ? c.d_rings_blue : c.d_rings_pow2;
-                sg.ring_start = p.d_ring_start;
-                sg.nphi = p.d_nphi;
-                sg.mcut = p.d_mcut;
-                sg.phi0 = p.d_phi0;
-                sg.scale = p.d_scale_ana;
-                sg.rot = p.d_rot;
-                sg.tw = f64 ? (const void *)(p.d_ftw64 + c.tw_off64) : (const void *)(p.d_ftw32 + c.tw_off32);
-                sg.V = f64 ? (const void *)p.d_V64 : (const void *)p.d_V32;
-                sg.vofs = p.d_vofs;
-                sg.pstride = phase_elems_per_map(&p);
-                sg.phase_off = phase_off[g];
-                sg.ncls = n;
+                if ((blue ? c.nrings_blue : c.nrings_pow2) == 0) continue;
+                RingSeg sg = class_seg(&p, c, blue, f64, p.d_rot);
                 sg.nmaps = nmaps[g];
                 sg.g = (int32_t)g;
-                sg.M = c.M;
-                sg.log2M = c.log2M;
-                sg.mmax = p.mmax;
-                sg.nr = p.nrings;
+                sg.phase_off = phase_off[g];
                 by[{c.threads, blue}].push_back(sg);
             }
     }
-    const int64_t lim = INT32_MAX;
-    for (auto it = by.rbegin(); it != by.rend(); ++it) {
-        RingLaunch L;
-        L.threads = it->first.first;
-        L.blue = it->first.second;
-        std::vector<RingSeg> ana = it->second, syn = it->second;
-        int64_t ba = 0, bs = 0;
-        for (size_t i = 0; i < ana.size(); ++i) {
-            ana[i].block0 = (int32_t)ba;
-            syn[i].block0 = (int32_t)bs;
-            syn[i].scale = nullptr;                 // batch synthesis: unit scale
-            ba += ring_grid(ana[i].ncls, ana[i].nmaps);
-            bs += ring_grid_syn(syn[i].ncls, syn[i].nmaps);
-            if (ba > lim || bs > lim) return set_error(HBX_EUNSUPPORTED, "batch too large for one ring launch");
-            L.lds = std::max(L.lds, f64 ? ring_lds_bytes<double>(ana[i].M) : ring_lds_bytes<float>(ana[i].M));
-        }
-        L.nseg = (int)ana.size();
-        L.grid_ana = (unsigned)ba;
-        L.grid_syn = (unsigned)bs;
-        int rc;
-        if ((rc = upload_segs(ana, &L.d_ana, allocs))) return rc;
-        if ((rc = upload_segs(syn, &L.d_syn, allocs))) return rc;
-        if ((rc = f64 ? allow_multi_lds<double>(L.lds) : allow_multi_lds<float>(L.lds))) return rc;
-        out.push_back(L);
+    int rc = make_launches(by, f64, false, true, out, alloc);
+    if (rc || f64) return rc;
+    std::vector<RingSeg> segs;
+    int64_t bb = 0;
+    for (size_t g = 0; g < plans.size(); ++g) {
+        const Plan &p = plans[g];
+        RingSeg sg{};
+        sg.ring_start = p.d_ring_start;
+        sg.nphi = p.d_nphi;
+        sg.mcut = p.d_mcut;
+        sg.phi0 = p.d_phi0;
+        sg.scale = p.d_scale_ana;
+        sg.pstride = band_phase_elems(&p);
+        sg.phase_off = phase_off[g] + band_phase_offset(&p, nmaps[g]);
+        sg.ncls = p.nrings;
+        sg.nmaps = nmaps[g];
+        sg.g = (int32_t)g;
+        sg.mmax = p.mmax;
+        sg.nr = p.nrings;
+        sg.block0 = (int32_t)bb;
+        bb += ((int64_t)p.nrings * nmaps[g] + 7) / 8 * 8;
+        if (bb > INT32_MAX) return set_error(HBX_EUNSUPPORTED, "batch too large for one band launch");
+        segs.push_back(sg);
     }
-    if (!f64) {
-        std::vector<RingSeg> segs;
-        int64_t bb = 0;
-        for (size_t g = 0; g < plans.size(); ++g) {
-            const Plan &p = plans[g];
-            RingSeg sg{};
-            sg.ring_start = p.d_ring_start;
-            sg.nphi = p.d_nphi;
-            sg.mcut = p.d_mcut;
-            sg.phi0 = p.d_phi0;
-            sg.scale = p.d_scale_ana;
-            sg.pstride = band_phase_elems(&p);
-            sg.phase_off = phase_off[g] + band_phase_offset(&p, nmaps[g]);
-            sg.ncls = p.nrings;
-            sg.nmaps = nmaps[g];
-            sg.g = (int32_t)g;
-            sg.mmax = p.mmax;
-            sg.nr = p.nrings;
-            sg.block0 = (int32_t)bb;
-            bb += ((int64_t)p.nrings * nmaps[g] + 7) / 8 * 8;
-            if (bb > lim) return set_error(HBX_EUNSUPPORTED, "batch too large for one band launch");
-            segs.push_back(sg);
-        }
-        band.threads = RBV_LANES;
-        band.nseg = (int)segs.size();
-        band.grid_ana = (unsigned)bb;
-        int rc;
-        if ((rc = upload_segs(segs, &band.d_ana, allocs))) return rc;
-    }
+    band.threads = RBV_LANES;
+    band.nseg = (int)segs.size();
+    band.size_ana = bb;
+    void *d = alloc(segs.size() * sizeof(RingSeg));
+    if (!d) return set_error(HBX_ENOMEM, "hipMalloc for the band launch table failed");
+    HBX_HIP(hipMemcpy(d, segs.data(), segs.size() * sizeof(RingSeg), hipMemcpyHostToDevice));
+    band.d_ana = static_cast<RingSeg *>(d);
     return HBX_OK;
+}
+
+static RingCall batch_call(const void *const *maps_tab, const int64_t *mstride_tab, char *ws) {
+    RingCall c{};
+    c.maps_tab = maps_tab;
+    c.mstride_tab = mstride_tab;
+    c.ws = ws;
+    c.unit = 1;
+    return c;
 }
 
 int launch_multi_ring_analysis(const RingLaunch &L, int precision, int map_dtype, const void *const *maps_tab,
                                const int64_t *mstride_tab, char *ws, hipStream_t s) {
-#define HBX_MRA(T, TI, B)                                                                                   \
-    hipLaunchKernelGGL((multi_ring_analysis_kernel<T, TI, B>), dim3(L.grid_ana), dim3(L.threads), L.lds, s, \
-                       L.d_ana, L.nseg, maps_tab, mstride_tab, ws)
+    const RingCall c = batch_call(maps_tab, mstride_tab, ws);
+#define HBX_MRA(T, TI, B)                                                                                          \
+    hipLaunchKernelGGL((fused_ring_analysis_kernel<T, TI, B, T>), dim3((unsigned)L.size_ana), dim3(L.threads), L.lds, \
+                       s, L.d_ana, L.nseg, c)
     const bool din = map_dtype == HBX_DT_F64;
     if (precision == HBX_F64) {
         if (L.blue) { if (din) HBX_MRA(double, double, true); else HBX_MRA(double, float, true); }
@@ -1496,15 +1508,16 @@ int launch_multi_ring_analysis(const RingLaunch &L, int precision, int map_dtype
     }
 #undef HBX_MRA
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_error(e, "multi_ring_analysis_kernel launch");
+    if (e != hipSuccess) return hip_error(e, "fused_ring_analysis_kernel (batch) launch");
     return HBX_OK;
 }
 
 int launch_multi_ring_synthesis(const RingLaunch &L, int precision, int map_dtype, void *const *maps_tab,
                                 const int64_t *mstride_tab, char *ws, hipStream_t s) {
-#define HBX_MRS(T, TO, B)                                                                                    \
-    hipLaunchKernelGGL((multi_ring_synthesis_kernel<T, TO, B>), dim3(L.grid_syn), dim3(L.threads), L.lds, s, \
-                       L.d_syn, L.nseg, maps_tab, mstride_tab, ws)
+    const RingCall c = batch_call(const_cast<const void *const *>(maps_tab), mstride_tab, ws);
+#define HBX_MRS(T, TO, B)                                                                                       \
+    hipLaunchKernelGGL((fused_ring_synthesis_kernel<T, TO, B>), dim3((unsigned)L.size_syn), dim3(L.threads), L.lds, \
+                       s, L.d_syn, L.nseg, c)
     const bool dout = map_dtype == HBX_DT_F64;
     if (precision == HBX_F64) {
         if (L.blue) { if (dout) HBX_MRS(double, double, true); else HBX_MRS(double, float, true); }
@@ -1515,19 +1528,20 @@ int launch_multi_ring_synthesis(const RingLaunch &L, int precision, int map_dtyp
     }
 #undef HBX_MRS
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_error(e, "multi_ring_synthesis_kernel launch");
+    if (e != hipSuccess) return hip_error(e, "fused_ring_synthesis_kernel (batch) launch");
     return HBX_OK;
 }
 
 int launch_multi_ring_band(const RingLaunch &L, int map_dtype, const void *const *maps_tab,
                            const int64_t *mstride_tab, char *ws, hipStream_t s) {
     if (L.nseg == 0) return HBX_OK;
+    const RingCall c = batch_call(maps_tab, mstride_tab, ws);
     if (map_dtype == HBX_DT_F64)
-        hipLaunchKernelGGL(multi_ring_band_kernel<double>, dim3(L.grid_ana), dim3(RBV_LANES), 0, s, L.d_ana, L.nseg,
-                           maps_tab, mstride_tab, ws);
+        hipLaunchKernelGGL(multi_ring_band_kernel<double>, dim3((unsigned)L.size_ana), dim3(RBV_LANES), 0, s,
+                           L.d_ana, L.nseg, c);
     else
-        hipLaunchKernelGGL(multi_ring_band_kernel<float>, dim3(L.grid_ana), dim3(RBV_LANES), 0, s, L.d_ana, L.nseg,
-                           maps_tab, mstride_tab, ws);
+        hipLaunchKernelGGL(multi_ring_band_kernel<float>, dim3((unsigned)L.size_ana), dim3(RBV_LANES), 0, s,
+                           L.d_ana, L.nseg, c);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "multi_ring_band_kernel launch");
     return HBX_OK;
