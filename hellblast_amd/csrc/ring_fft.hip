@@ -13,9 +13,10 @@
 //   * the half-length spectrum is untangled into X[k], k <= h, and written as
 //     F_r(m) = scale_r * e^{-i m phi0_r} X[m mod n]  for m < mcut_r, else 0,
 //     into the m-blocked ring-major phase buffer [map][m/8][ring][m%8].
-// Rings are grouped into size classes (one launch per power-of-two M and
-// kind, power of two or Bluestein) so that each launch reserves exactly the
-// LDS it needs and runs one code path.
+// Rings are grouped into size classes (power-of-two M, and kind: power of
+// two or Bluestein); one launch runs every class of one workgroup size and
+// kind, so a ring stage is a handful of launches, each reserving the LDS of
+// its largest class.
 //
 // The FFT engine keeps 16 points per thread in registers: radix-16 passes
 // (4 radix-2 levels each, one LDS exchange per pass) and one radix-2/4/8
