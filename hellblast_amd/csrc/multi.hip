@@ -95,12 +95,22 @@ __global__ void multi_setup_kernel(MultiCall c, char *ws) {
 }
 
 // next tile index for the calling thread: queues home, home+1, ... in turn;
-// qi (the thread's current queue offset) only grows; -1 once all are empty
+// qi (the thread's current queue offset) only grows; -1 once all are empty.
+// The home queue is pulled directly; another queue is first checked with a
+// coherent load of its counter (which only grows), so a drained queue costs
+// a load instead of an atomic on a counter every workgroup hits at the end
+// of the launch.  Measured (16 FP32 maps at nside 64 in 16 groups / 16 at
+// nside 512 / the 64-map nside 128..2048 batch, ms per call pair): atomics
+// on every queue 0.86 / 2.88 / 13.8, a load before every pull 0.18 / 3.68 /
+// 15.7, a load before steals only 0.27 / 2.56 / 13.6.
 __device__ __forceinline__ int pull_tile(unsigned *queue, const int32_t *__restrict__ qrange, int home, int &qi) {
     while (qi < NQ) {
         const int q = (home + qi) & (NQ - 1);
-        const int t = qrange[q] + (int)atomicAdd(queue + q, 1u);
-        if (t < qrange[NQ + q]) return t;
+        const unsigned n = (unsigned)(qrange[NQ + q] - qrange[q]);
+        if (qi == 0 || __hip_atomic_load(queue + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < n) {
+            const int t = qrange[q] + (int)atomicAdd(queue + q, 1u);
+            if (t < qrange[NQ + q]) return t;
+        }
         ++qi;
     }
     return -1;
@@ -527,6 +537,13 @@ static int build_multi(hbx_multi *mb) {
     if (at.size() > lim || st.size() > lim || rrows.size() > lim || prows.size() > lim || bt.size() > lim ||
         brrows.size() > lim)
         return set_error(HBX_EUNSUPPORTED, "batch too large: more than 2^31 tiles or rows");
+    // launch no more workgroups than there are tiles (small batches)
+    auto cap = [](int grid, size_t ntiles) {
+        return (int)std::max<size_t>(NQ, std::min<size_t>((size_t)grid, (ntiles + NQ - 1) / NQ * NQ));
+    };
+    mb->grid_ana = cap(mb->grid_ana, at.size());
+    mb->grid_syn = cap(mb->grid_syn, st.size());
+    if (!f64) mb->grid_band = cap(mb->grid_band, bt.size());
     std::vector<int32_t> aq, sq, bq;
     sort_queues(at, acost, aq);
     sort_queues(st, scost, sq);
