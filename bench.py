@@ -56,8 +56,10 @@ def parse():
                     help="instead of the headline: 64 FP32 maps of mixed nside (128..2048, lmax 2 nside) "
                          "in one HIP graph with autotuned tiles; prints its own JSON line")
     ap.add_argument("--mixed-workload", default="nside128-2048", choices=["nside128-2048", "groups16"],
-                    help="with --mixed: the 64-map batch, or 16 FP32 maps at nside 512 in 16 (nside, lmax) "
-                         "groups, lmax 512, 544, .., 992 (one map each)")
+                    help="with --mixed: the 64-map batch, or 16 FP32 maps at one nside in 16 (nside, lmax) "
+                         "groups, lmax = nside (1 + k/16), k = 0..15 (one map each)")
+    ap.add_argument("--mixed-nside", type=int, default=128,
+                    help="with --mixed-workload groups16: the maps' nside (a power of two, at least 16)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB -> 1 GiB elementwise copy kernel (HBM counter calibration)")
     return ap.parse_args()
@@ -453,8 +455,8 @@ def mixed_main(args):
     """64 FP32 maps, nside {2048:1, 1024:2, 512:5, 256:14, 128:42} (seeded
     shuffle), lmax = 2 nside, alm ~ C_l = (l+10)^-2: map2alm + alm2map of the
     whole batch per step, replayed from one HIP graph (MixedBatch).  With
-    --mixed-workload groups16: 16 maps at nside 512, lmax 512 + 32 k (16
-    groups of one map), the same spectrum."""
+    --mixed-workload groups16: 16 maps at nside N (--mixed-nside, default
+    128), lmax N + k N/16 (16 groups of one map), the same spectrum."""
     import numpy as np
     import torch
     from hellblast_amd.batch import MixedBatch
@@ -465,9 +467,12 @@ def mixed_main(args):
     rng = np.random.default_rng(3)
     many = args.mixed_workload == "groups16"
     alloc = {2048: 1, 1024: 2, 512: 5, 256: 14, 128: 42}
+    ns = args.mixed_nside
     if many:
-        sides = [512] * 16
-        specs = [(512, 512 + 32 * k) for k in range(16)]
+        if ns < 16 or ns & (ns - 1):
+            raise SystemExit("--mixed-nside must be a power of two, at least 16")
+        sides = [ns] * 16
+        specs = [(ns, ns + ns // 16 * k) for k in range(16)]
     else:
         sides = [n for n, k in alloc.items() for _ in range(k)]
         rng.shuffle(sides)
@@ -561,10 +566,11 @@ def mixed_main(args):
     best = min(runs, key=lambda k: runs[k]["graph_ms"])
     rb = runs[best]
     npix = sum(12 * n * n for n in sides)
-    metric = ("mixed-batch SHT pairs/s: 16 FP32 maps, nside 512, 16 groups lmax 512..992, one HIP graph" if many else
+    metric = (f"mixed-batch SHT pairs/s: 16 FP32 maps, nside {ns}, 16 groups lmax {specs[0][1]}..{specs[-1][1]}, "
+              "one HIP graph" if many else
               "mixed-batch SHT pairs/s: 64 FP32 maps, nside 128..2048, lmax 2 nside, one HIP graph")
-    workload = ("16 maps nside 512, lmax 512+32k (k=0..15, one map per group), C_l=(l+10)^-2, map2alm+alm2map "
-                "per map" if many else
+    workload = (f"16 maps nside {ns}, lmax {ns}+{ns // 16}k (k=0..15, one map per group), C_l=(l+10)^-2, "
+                "map2alm+alm2map per map" if many else
                 "64 maps nside {2048:1,1024:2,512:5,256:14,128:42}, lmax 2 nside, C_l=(l+10)^-2, map2alm+alm2map "
                 "per map")
     res = {"metric": metric,

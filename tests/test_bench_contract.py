@@ -23,7 +23,8 @@ def test_flags_parse():
         for argv in (["bench.py"], ["bench.py", "--spin", "--steps", "3"], ["bench.py", "--mixed", "--warmup", "0"],
                      ["bench.py", "--batch", "8", "--nside", "512", "--no-cpu", "--copy-calibration"],
                      ["bench.py", "--configs", "--no-cpu"], ["bench.py", "--cpu-full"],
-                     ["bench.py", "--mixed", "--mixed-workload", "groups16", "--mixed-mode", "persistent"]):
+                     ["bench.py", "--mixed", "--mixed-workload", "groups16", "--mixed-mode", "persistent"],
+                     ["bench.py", "--mixed", "--mixed-workload", "groups16", "--mixed-nside", "512"]):
             sys.argv = argv
             a = bench.parse()
             assert a.gpus == 1 and a.steps >= 1
