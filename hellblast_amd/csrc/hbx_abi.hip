@@ -519,16 +519,47 @@ int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes) {
     return HBX_OK;
 }
 
+// Whole analysis (map2alm and its adjoint).  FP32 plans run the FP64 band
+// chain -- band DFT, band Legendre, rounding into alm rows m < M_LO -- on the
+// side stream beside the ring FFTs, whose barrier-bound workgroups leave
+// issue slots free; forked at the Legendre stage it would sit beside the
+// split-fp16 analysis, which fills the GPU, for that kernel's whole run.
+static int analysis(const Plan *p, int nmaps, const void *maps, int map_dtype, int64_t map_stride, void *alm,
+                    int64_t alm_stride, void *ws, const double *scale, int adjoint, hipStream_t s) {
+    void *tail = static_cast<char *>(ws) + ws_tail_offset(p, nmaps);
+    if (p->precision != HBX_F32) {
+        int rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, scale, s);
+        if (rc) return rc;
+        return legendre_analysis(p, nmaps, ws, alm, alm_stride, tail, adjoint, s);
+    }
+    const AnalysisWs w = analysis_ws(p, nmaps);
+    char *t = static_cast<char *>(tail);
+    void *band = band_phase_ptr(p, nmaps, ws);
+    SideFork fork(p, s);
+    int rc = launch_ring_band(p, nmaps, maps, map_dtype, map_stride, band, scale, fork.side());
+    if (!rc)
+        rc = launch_leg_analysis_band(p, nmaps, band, alm, alm_stride, t + w.band_alm, t + w.band_partial, adjoint,
+                                      fork.side());
+    if (!rc) rc = launch_ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, scale, s);
+    if (!rc) {
+        if (nmaps >= mfma_min_maps(p))
+            rc = launch_leg_analysis_mfma(p, nmaps, ws, alm, alm_stride, t + w.partial,
+                                          reinterpret_cast<int32_t *>(t + w.lfirst), adjoint, s);
+        else
+            rc = launch_leg_analysis(p, nmaps, ws, alm, alm_stride, t + w.partial, adjoint, s);
+    }
+    const int rj = fork.join();
+    return rc ? rc : rj;
+}
+
 int hbx_map2alm(const hbx_plan *plan, int32_t nmaps, const void *maps, int32_t map_dtype, int64_t map_stride,
                 void *alm, int64_t alm_stride, void *ws, uint64_t ws_bytes_, void *stream) {
     int rc = check_exec(plan, nmaps, maps, alm, ws, ws_bytes_, 1);
     if (rc || nmaps == 0) return rc;
     const Plan *p = &plan->impl;
     if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    void *tail = static_cast<char *>(ws) + ws_tail_offset(p, nmaps);
-    if ((rc = ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, p->d_scale_ana, s))) return rc;
-    return legendre_analysis(p, nmaps, ws, alm, alm_stride, tail, 0, s);
+    return analysis(p, nmaps, maps, map_dtype, map_stride, alm, alm_stride, ws, p->d_scale_ana, 0,
+                    (hipStream_t)stream);
 }
 
 int hbx_alm2map_adjoint(const hbx_plan *plan, int32_t nmaps, const void *maps, int32_t map_dtype,
@@ -538,10 +569,7 @@ int hbx_alm2map_adjoint(const hbx_plan *plan, int32_t nmaps, const void *maps, i
     if (rc || nmaps == 0) return rc;
     const Plan *p = &plan->impl;
     if ((rc = check_strides(p, map_dtype, map_stride, alm_stride))) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    void *tail = static_cast<char *>(ws) + ws_tail_offset(p, nmaps);
-    if ((rc = ring_analysis(p, nmaps, maps, map_dtype, map_stride, ws, nullptr, s))) return rc;
-    return legendre_analysis(p, nmaps, ws, alm, alm_stride, tail, 1, s);
+    return analysis(p, nmaps, maps, map_dtype, map_stride, alm, alm_stride, ws, nullptr, 1, (hipStream_t)stream);
 }
 
 static int synth(const hbx_plan *plan, int32_t nmaps, const void *alm, int64_t alm_stride, void *maps,
