@@ -1344,10 +1344,12 @@ static int ring_analysis_t(const Plan *p, int nmaps, const TIn *maps, int64_t ma
     c.scale = scale;
     c.nmaps = nmaps;
     c.unit = 8 * nmaps;
+    for (const RingLaunch &L : Ls)      // every grid checked before the first launch
+        if (L.size_ana * 8 * (int64_t)nmaps > INT32_MAX)
+            return set_error(HBX_EUNSUPPORTED, "too many maps for one ring launch");
     timer_mark(HBX_TIMER_RING_ANALYSIS, true, s);
     for (const RingLaunch &L : Ls) {
         const int64_t grid = L.size_ana * 8 * (int64_t)nmaps;
-        if (grid > INT32_MAX) return set_error(HBX_EUNSUPPORTED, "too many maps for one ring launch");
         if (L.blue)
             hipLaunchKernelGGL((fused_ring_analysis_kernel<T, TIn, true, TP>), dim3((unsigned)grid), dim3(L.threads),
                                L.lds, s, L.d_ana, L.nseg, c);
@@ -1371,10 +1373,12 @@ static int ring_synthesis_t(const Plan *p, int nmaps, const c2<T> *phase, TOut *
     c.scale = scale;
     c.nmaps = nmaps;
     c.unit = 64 * nmaps;
+    for (const RingLaunch &L : p->fused)
+        if (L.size_syn * 64 * (int64_t)nmaps > INT32_MAX)
+            return set_error(HBX_EUNSUPPORTED, "too many maps for one ring launch");
     timer_mark(HBX_TIMER_RING_SYNTHESIS, true, s);
     for (const RingLaunch &L : p->fused) {
         const int64_t grid = L.size_syn * 64 * (int64_t)nmaps;
-        if (grid > INT32_MAX) return set_error(HBX_EUNSUPPORTED, "too many maps for one ring launch");
         if (L.blue)
             hipLaunchKernelGGL((fused_ring_synthesis_kernel<T, TOut, true>), dim3((unsigned)grid), dim3(L.threads),
                                L.lds, s, L.d_syn, L.nseg, c);
