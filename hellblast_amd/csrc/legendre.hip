@@ -215,7 +215,7 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
                     const double *__restrict__ sl, const int32_t *__restrict__ lseed,
                     const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
                     int64_t nlm, int adjoint, int mbeg) {
-    __shared__ T red[2 * CHUNK * RED_ROW];
+    __shared__ T red[red_elems<T>()];
     int mb, item;
     xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
     const int m = mb * MB + item / ngroups;

@@ -37,14 +37,26 @@ __device__ __forceinline__ int wave_max(int v) {
 // sums per group and group_reduce_kernel adds them in a fixed order.
 // ---------------------------------------------------------------------------
 // transpose buffer of the VALU analysis: row v (one of the 2 CHUNK sums) of
-// the 64 lanes, rows 65 elements apart so that the reads of the reduction (4
-// lanes per row, 16 consecutive elements each) fall on distinct banks
-constexpr int RED_ROW = 65;
+// the 64 lanes.  The reduction reads 16 consecutive columns per lane, 4 lanes
+// per row.  4-byte elements: rows 65 elements apart put those reads on
+// distinct banks.  8-byte elements (two banks each): the 4 lanes of one row
+// would meet on the same banks, so every 16 columns are followed by 4 pad
+// elements and rows are 81 elements apart.
+template <typename T> struct RedLayout;
+template <> struct RedLayout<float> {
+    static constexpr int ROW = 65;
+    __device__ static int at(int v, int j) { return v * ROW + j; }
+};
+template <> struct RedLayout<double> {
+    static constexpr int ROW = 81;
+    __device__ static int at(int v, int j) { return v * ROW + j + 4 * (j >> 4); }
+};
+template <typename T> constexpr int red_elems() { return 2 * CHUNK * RedLayout<T>::ROW; }
 
 // One wave's share of the VALU analysis: ring pairs grp*64R .. grp*64R+64R-1
 // of one m and one map.  ph: the map's phase at column m of its m-block;
 // dst: the map's alm (direct) or its partial-sum row of ring-pair group grp.
-// red: 2*CHUNK*RED_ROW elements of LDS owned by this wave.
+// red: red_elems<T>() elements of LDS owned by this wave.
 template <typename T, int R>
 __device__ __forceinline__ void
 ana_wave(const c2<T> *__restrict__ ph, c2<T> *__restrict__ dst, bool direct, const int32_t *__restrict__ pair_n,
@@ -131,13 +143,13 @@ ana_wave(const c2<T> *__restrict__ ph, c2<T> *__restrict__ dst, bool direct, con
         }
         // wave-level transpose-reduce of the 16 values
 #pragma unroll
-        for (int v = 0; v < 2 * CHUNK; ++v) red[v * RED_ROW + lane] = acc[v];
+        for (int v = 0; v < 2 * CHUNK; ++v) red[RedLayout<T>::at(v, lane)] = acc[v];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         T s = (T)0;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) s += red[v_id * RED_ROW + q0 + q];
+        for (int q = 0; q < 16; ++q) s += red[RedLayout<T>::at(v_id, q0 + q)];
         s += __shfl_xor(s, 1, 64);
         s += __shfl_xor(s, 2, 64);
         if ((lane & 3) == 0) {

@@ -120,7 +120,7 @@ template <typename T, int R>
 __global__ void __launch_bounds__(64)
 multi_analysis_kernel(const MultiGeomDev *__restrict__ geo, const MultiTile *__restrict__ tiles,
                       const int32_t *__restrict__ qrange, char *__restrict__ ws, int qoff) {
-    __shared__ T red[2 * CHUNK * RED_ROW];
+    __shared__ T red[red_elems<T>()];
     unsigned *queue = reinterpret_cast<unsigned *>(ws) + qoff;
     const MultiCall *call = reinterpret_cast<const MultiCall *>(ws + HDR_CALL);
     const int lane = threadIdx.x;
