@@ -5,12 +5,14 @@ Maps of different nside / lmax are grouped by geometry.  Two execution modes
 (``mode="auto"``, the default, times both once and keeps the faster):
 
 - ``"persistent"``: one libhbx batch object (``MultiPlan``,
-  ``hbx_multi_*``) over all groups.  The ring stage runs per group; each
-  Legendre direction is ONE persistent kernel whose workgroups pull
-  (group, m-block, ring-pair block, map block) tiles from work queues, so the
-  m-blocks of a large map and the tiles of the small maps share the whole GPU.
-  Tile shapes (ring-pair slots, maps per synthesis tile, tile granularity)
-  come from the autotuner (``autotune.tune_multi``).
+  ``hbx_multi_*``) over all groups, a fixed number of launches per call: the
+  ring stage in fused launches over every group's rings; each Legendre
+  direction ONE persistent kernel whose workgroups pull (group, m-block,
+  ring-pair block, map block) tiles from work queues, so the m-blocks of a
+  large map and the tiles of the small maps share the whole GPU.  Tile
+  shapes (ring-pair slots, maps per synthesis tile, tile granularity) come
+  from the autotuner (``autotune.tune_multi``).  The faster form for many
+  small groups (whose own launches cannot fill the GPU).
 - ``"grouped"``: each group is one batched call of its plan (matrix-core
   Legendre kernels for FP32 groups of 3+ maps), each group on its own stream
   branch.

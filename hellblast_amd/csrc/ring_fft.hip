@@ -421,8 +421,8 @@ inline unsigned ring_grid_syn(int ncls, int nmaps) { return (unsigned)(((ncls + 
 // round their output once to FP32).
 // ---------------------------------------------------------------------------
 // One ring r of one map: mapb / phb are that map's pixels and phase buffer.
-// Shared by the per-plan launch below and the fused launches of
-// mixed-geometry batches (one launch per size class over every group).
+// Run by the fused ring launches (below), for one plan or for every group of
+// a mixed-geometry batch.
 template <typename T, typename TIn, bool BLUE, typename TP>
 __device__ __forceinline__ void ring_analysis_ring(c2<T> *buf, const TIn *__restrict__ mapb,
                                                    c2<TP> *__restrict__ phb, int r,
