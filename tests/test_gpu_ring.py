@@ -57,14 +57,17 @@ def _per_ring_rel(got, ref):
     return float(np.max(d / s))
 
 
-@pytest.mark.parametrize("phi0", ["healpix", "reference"])
-@pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_ring_analysis_all_classes(gpu, precision, phi0):
-    print(f"[case] ring analysis nside={NSIDE} mmax={LMAX} {precision} phi0={phi0}", flush=True)
+@pytest.mark.parametrize("precision,phi0,other", [("f64", "healpix", False), ("f64", "reference", False),
+                                                  ("f32", "healpix", False), ("f32", "reference", False),
+                                                  ("f64", "healpix", True), ("f32", "healpix", True)])
+def test_ring_analysis_all_classes(gpu, precision, phi0, other):
+    """other: maps of the other float type (FP32 maps into an FP64 plan, FP64
+    maps into an FP32 plan)."""
+    print(f"[case] ring analysis nside={NSIDE} mmax={LMAX} {precision} phi0={phi0} other-dtype={other}", flush=True)
     geom, plan = _setup(precision, phi0)
     nmaps = 2
     rng = np.random.default_rng(7)
-    rdt = torch.float64 if precision == "f64" else torch.float32
+    rdt = torch.float64 if (precision == "f64") != other else torch.float32
     maps = torch.as_tensor(rng.standard_normal((nmaps, plan.npix)), dtype=rdt, device="cuda")
     phase = plan.phase_buffer(nmaps)
     plan.ring_analysis(maps, phase)
@@ -77,9 +80,11 @@ def test_ring_analysis_all_classes(gpu, precision, phi0):
         assert err <= TOL[precision]
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_ring_synthesis_all_classes(gpu, precision):
-    print(f"[case] ring synthesis nside={NSIDE} mmax={LMAX} {precision}", flush=True)
+@pytest.mark.parametrize("precision,other", [("f64", False), ("f32", False), ("f64", True), ("f32", True)])
+def test_ring_synthesis_all_classes(gpu, precision, other):
+    """other: pixels written in the other float type (an FP64 plan's FP32
+    output is held to FP32 rounding, 1e-7)."""
+    print(f"[case] ring synthesis nside={NSIDE} mmax={LMAX} {precision} other-dtype={other}", flush=True)
     geom, plan = _setup(precision)
     nmaps = 2
     rng = np.random.default_rng(8)
@@ -87,7 +92,8 @@ def test_ring_synthesis_all_classes(gpu, precision):
     G = rng.standard_normal((nmaps, nr, LMAX + 1)) + 1j * rng.standard_normal((nmaps, nr, LMAX + 1))
     phase = plan.phase_buffer(nmaps)
     _phase_put(plan, phase, G, "synthesis")
-    rdt = torch.float64 if precision == "f64" else torch.float32
+    rdt = torch.float64 if (precision == "f64") != other else torch.float32
+    tol = 1e-7 if (precision == "f64" and other) else TOL[precision]
     got = plan.ring_synthesis(phase, nmaps, dtype=rdt).double().cpu().numpy()
     Gr = G
     if precision == "f32":
@@ -99,7 +105,7 @@ def test_ring_synthesis_all_classes(gpu, precision):
             s, e = geom.ring_start[r], geom.ring_start[r + 1]
             worst = max(worst, float(np.max(np.abs(got[b, s:e] - ref[s:e])) / np.max(np.abs(ref[s:e]))))
         print(f"map {b}: per-ring rel err {worst:.2e}", flush=True)
-        assert worst <= TOL[precision]
+        assert worst <= tol
 
 
 def test_f32_gate_full_size(gpu):
