@@ -216,10 +216,11 @@ def test_multi_f32_c3_shape_within_gate(gpu):
     assert worst <= 1e-5
 
 
-def test_multi_graph_replay_equals_eager(gpu):
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_multi_graph_replay_equals_eager(gpu, precision):
     from hellblast_amd.batch import MultiPlan
-    print("[case] multi graph replay", flush=True)
-    plans, maps, alm_in = _inputs(GROUPS, "f32", 13)
+    print(f"[case] multi graph replay {precision}", flush=True)
+    plans, maps, alm_in = _inputs(GROUPS + [(128, 256, 256, 3)], precision, 13)
     mp = MultiPlan(plans, [m.shape[0] for m in maps])
     ws = mp.workspace()
     alm = [torch.zeros((m.shape[0], p.nalm), dtype=mp.alm_dtype, device="cuda") for m, p in zip(maps, plans)]
