@@ -1095,37 +1095,42 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
 
 // ---------------------------------------------------------------------------
 // Split-fp16 analysis with two 16-map groups per workgroup (32 maps; tile
-// mfma_groups = 2, not the automatic choice): the
-// contraction of leg_analysis_h16_kernel<2> (512 ring pairs of one m, two
-// K-halves of 256) for both groups, 16 waves = (map group u, n-tile t,
-// parity par, K-half kh), sharing one lambda tile per 32-l chunk, so every
-// FP64 recursion step feeds twice the matrix work.  The recursion runs on
-// the 8 waves of group 0 (two per SIMD), one ring pair per lane; adjacent
+// mfma_groups = 2): the contraction of leg_analysis_h16_kernel<Q> (256 Q
+// ring pairs of one m, Q K-parts of 256) for both groups, 8 Q waves = (map
+// group u, n-tile t, parity par, K-part kh), sharing one lambda tile per
+// 32-l chunk, so every FP64 recursion step feeds twice the matrix work.  The
+// recursion runs on waves 0 .. 4Q-1, one ring pair per lane; adjacent
 // lanes trade their fp16 (hi, lo) halves with one DPP move, and the even lane
 // stores the two pairs' hi halves, the odd lane their lo halves, as one
-// dword each (the tile layout of leg_analysis_h16_kernel<2>).  The A
+// dword each (the tile layout of leg_analysis_h16_kernel<Q>).  The A
 // operands of each wave are loaded and scaled as there (the north/south
 // exchange in four rounds of 16 K-steps, so that it fits in the idle tiles).
+// Q = 2: one 16-wave workgroup per CU; Q = 1: 8-wave workgroups over 256
+// ring pairs, two per CU.
 // ---------------------------------------------------------------------------
-struct H16x2Geom {
-    static constexpr int W = 16;                  // waves per workgroup
-    static constexpr int P = 512;                 // ring pairs per workgroup
-    static constexpr int RW = 8;                  // recursion waves (64 ring pairs each)
-    static constexpr int COLB = H16Geom<2>::COLB, PARB = H16Geom<2>::PARB, TILEB = H16Geom<2>::TILEB;
-    static constexpr int XCH = 8 * 256;           // floats: the K-half 1 tiles of the 8 (u, t, par)
+template <int Q> struct H16x2Geom {
+    static constexpr int W = 8 * Q;               // waves per workgroup
+    static constexpr int P = 256 * Q;             // ring pairs per workgroup
+    static constexpr int RW = 4 * Q;              // recursion waves (64 ring pairs each)
+    static constexpr int COLB = H16Geom<Q>::COLB, PARB = H16Geom<Q>::PARB, TILEB = H16Geom<Q>::TILEB;
+    // floats: the K-part 1 tiles of the 8 (u, t, par) (Q = 2), and the
+    // per-wave row maxima before the chunk loop
+    static constexpr int XCH = Q > 1 ? 8 * 256 : W * 16;
     static constexpr size_t LDS = 2 * (size_t)TILEB + XCH * sizeof(float) + 4 * sizeof(int);
 };
-static_assert((size_t)H16x2Geom::W * 16 * 64 * sizeof(float) <= 2 * (size_t)H16x2Geom::TILEB,
+static_assert((size_t)H16x2Geom<2>::W * 16 * 64 * sizeof(float) <= 2 * (size_t)H16x2Geom<2>::TILEB &&
+                  (size_t)H16x2Geom<1>::W * 16 * 64 * sizeof(float) <= 2 * (size_t)H16x2Geom<1>::TILEB,
               "the A-operand exchange fits in the lambda tiles");
 
-__global__ void __launch_bounds__(H16x2Geom::W * 64) __attribute__((amdgpu_waves_per_eu(4)))
+template <int Q>
+__global__ void __launch_bounds__(H16x2Geom<Q>::W * 64) __attribute__((amdgpu_waves_per_eu(4)))
 leg_analysis_h16x2_kernel(const float2 *__restrict__ phase, int64_t pstride, float2 *__restrict__ alm,
                           int64_t astride, float2 *__restrict__ partial, const double *__restrict__ px,
                           const int64_t *__restrict__ rofs, const double *__restrict__ alpha,
                           const double *__restrict__ sl, const int32_t *__restrict__ lseed,
                           const double2 *__restrict__ seed, int npairs, int nr, int lmax, int mmax, int ngroups,
                           int64_t nlm, int nmaps, int adjoint, int mbeg, float neg1, int32_t *__restrict__ lfirst) {
-    using G = H16x2Geom;
+    using G = H16x2Geom<Q>;
     extern __shared__ __align__(16) unsigned char hsm[];
     unsigned char *thi = hsm, *tlo = hsm + G::TILEB;
     float *xch = reinterpret_cast<float *>(hsm + 2 * G::TILEB);
@@ -1140,7 +1145,7 @@ leg_analysis_h16x2_kernel(const float2 *__restrict__ phase, int64_t pstride, flo
     const int g = blockIdx.y;                         // group of 32 maps
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int t = w & 1, par = (w >> 1) & 1, kh = (w >> 2) & 1, u = w >> 3;
+    const int t = w & 1, par = (w >> 1) & 1, kh = (w >> 2) & (Q - 1), u = w / (4 * Q);
     const int P0 = grp * G::P;
     const bool rec = (w < G::RW);
 
@@ -1203,7 +1208,10 @@ leg_analysis_h16x2_kernel(const float2 *__restrict__ phase, int64_t pstride, flo
         float *am = xch;                                // [W][16], idle until the chunk loop
         if (lane < 16) am[w * 16 + lane] = amax;
         __syncthreads();
-        amax = fmaxf(am[(w & ~4) * 16 + row], am[(w | 4) * 16 + row]);
+        float am_all = 0.0f;
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) am_all = fmaxf(am_all, am[((w & 3) + 4 * k2 + 4 * Q * u) * 16 + row]);
+        amax = am_all;
         // map maximum scaled to [2^13, 2^14), applied in two exact power-of-two steps
         int e = 0;
         (void)frexpf(amax, &e);
@@ -1225,11 +1233,11 @@ leg_analysis_h16x2_kernel(const float2 *__restrict__ phase, int64_t pstride, flo
 #pragma unroll
         for (int j = 0; j < 4; ++j) kr[j] = __shfl(k, 4 * q + j, 64);
     }
-    if (tid < 3) starts[tid] = NEVER_L;
+    if (tid < 1 + Q) starts[tid] = NEVER_L;
     __syncthreads();
     if (rec && lane == 0) {
         atomicMin(&starts[0], wstart);
-        atomicMin(&starts[1 + (w >> 2)], wstart);     // recursion waves 4j .. 4j+3 cover K-half j
+        atomicMin(&starts[1 + (w >> 2)], wstart);     // recursion waves 4j .. 4j+3 cover K-part j
     }
     __syncthreads();
     const int lstart = starts[0];
@@ -1341,14 +1349,16 @@ leg_analysis_h16x2_kernel(const float2 *__restrict__ phase, int64_t pstride, flo
         floatx4 D = D0 + D1;
         // 3. K-half 1 hands its tile to K-half 0 through LDS (same row scales)
         const int xo = ((u * 2 + t) * 2 + par) * 256;
-        if (kh == 1) {
+        if (Q > 1 && kh == 1) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) xch[xo + r * 64 + lane] = D[r];
         }
         __syncthreads();
         if (kh == 0) {
+            if (Q > 1) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) D[r] += xch[xo + r * 64 + lane];
+                for (int r = 0; r < 4; ++r) D[r] += xch[xo + r * 64 + lane];
+            }
             // D[row][col]: lane = col + 16 (row/4), reg = row%4; row = n - 16 t, col -> l
             // (rows 4q, 4q+1 are re, im of one map and share its scale)
             const int l = c0 + 2 * (lane & 15) + par;
@@ -1794,17 +1804,27 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
                        (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,       \
                        p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,     \
                        p->mmax, G, p->nlm, nmaps, adjoint, mbeg, -1.0f, lfirst)
-        // two 16-map groups per workgroup (8-wave geometry) only on request
-        // (tile mfma_groups = 2): at nside 2048 its single 16-wave workgroup
-        // per CU alternates recursion and matrix phases with nothing to
-        // overlap them (22.9 ms for 32 maps against 20.6 ms as two 16-map
-        // groups), so the automatic choice keeps one group
-        if (W == 8 && nmaps > 16 && p->tile_syn_groups == 2) {
+        // two 16-map groups per workgroup (tile mfma_groups = 2; results
+        // bit-identical to one group): automatic only where one 256-pair
+        // workgroup covers every ring pair (nside <= 128) and the 16-map
+        // groups pair up (map2alm of 32 maps at nside 128: 0.128 against
+        // 0.137 ms).  Over several ring-pair groups it does not pay: the
+        // 8-wave geometry's single 16-wave workgroup per CU alternates
+        // recursion and matrix phases with nothing to overlap them (nside
+        // 2048, 32 maps: 24.9 against 21.3 ms), and the 256-pair geometry
+        // halves the recursion for a 6 % shorter kernel (21.1 against 22.4
+        // ms) but doubles the partial sums (map2alm 38.1 against 35.6 ms).
+#define HBX_AH2(QV)                                                                                             \
+    hipLaunchKernelGGL(leg_analysis_h16x2_kernel<QV>, grid2, dim3(H16x2Geom<QV>::W * 64), H16x2Geom<QV>::LDS, s, \
+                       (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,       \
+                       p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,     \
+                       p->mmax, G, p->nlm, nmaps, adjoint, mbeg, -1.0f, lfirst)
+        const bool pair_groups = ((nmaps + 15) / 16) % 2 == 0;
+        const bool two = p->tile_syn_groups == 2 || (p->tile_syn_groups == 0 && W == 4 && G == 1 && pair_groups);
+        if ((W == 4 || W == 8) && nmaps > 16 && two) {
             dim3 grid2(grid.x, (nmaps + 31) / 32);
-            hipLaunchKernelGGL(leg_analysis_h16x2_kernel, grid2, dim3(H16x2Geom::W * 64), H16x2Geom::LDS, s,
-                               (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,
-                               p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings,
-                               p->lmax, p->mmax, G, p->nlm, nmaps, adjoint, mbeg, -1.0f, lfirst);
+            if (W == 4) HBX_AH2(1);
+            else HBX_AH2(2);
         } else switch (W) {
         case 4: HBX_AH(1); break;
         case 8: HBX_AH(2); break;
@@ -1812,6 +1832,7 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
         default: return set_error(HBX_EUNSUPPORTED, "split-fp16 analysis waves must be 4, 8 or 16");
         }
 #undef HBX_AH
+#undef HBX_AH2
     } else switch (W) {
     case 4: HBX_AM(1); break;
     case 8: HBX_AM(2); break;
@@ -1850,8 +1871,11 @@ int prepare_mfma_kernels() {
         e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_h16_kernel<4>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)H16Geom<4>::LDS);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_h16x2_kernel),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)H16x2Geom::LDS);
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_h16x2_kernel<1>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)H16x2Geom<1>::LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_h16x2_kernel<2>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)H16x2Geom<2>::LDS);
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void *>(leg_analysis_mfma_pipe_kernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)PipeGeom::LDS);

@@ -140,11 +140,13 @@ int hbx_workspace_bytes(const hbx_plan *plan, int32_t nmaps, uint64_t *bytes);
  *                  default); with HBX_MFMA_F16X2, mfma_waves is 4, 8 or 16
  *   mfma_groups    groups of 16 maps per split-fp16 workgroup: 1, or 2
  *                  (each lambda value feeds twice the matrix work, at two
- *                  thirds of the occupancy in the synthesis, in 16-wave
- *                  analysis workgroups with mfma_waves 8); 0 (the default):
- *                  the synthesis takes 2 for batches of an even number of
- *                  16-map groups, else 1, and the analysis 1.  The results
- *                  do not depend on it (bit for bit). */
+ *                  thirds of the occupancy in the synthesis; in the analysis
+ *                  16-wave workgroups with mfma_waves 8, 8-wave workgroups
+ *                  with mfma_waves 4); 0 (the default): the synthesis takes
+ *                  2 for batches of an even number of 16-map groups, else
+ *                  1, and the analysis 2 for such batches where one 256-pair
+ *                  workgroup covers every ring pair (nside <= 128), else 1.
+ *                  The results do not depend on it (bit for bit). */
 #define HBX_MFMA_F32 1
 #define HBX_MFMA_F16X2 2
 typedef struct hbx_tiles {
