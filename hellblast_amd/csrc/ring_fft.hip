@@ -404,13 +404,24 @@ __device__ __forceinline__ void ring_block(int g, int nmaps, int *slot, int *map
 }
 inline unsigned ring_grid(int ncls, int nmaps) { return (unsigned)(((ncls + 7) / 8) * 8 * nmaps); }
 // synthesis: each XCD takes a contiguous run of slots (octets q of C per
-// XCD), map-major, the 8 slots of an octet next to each other in time, so
-// that the 64-byte lines of the synthesis layout (8 ring pairs of one m,
-// shared by 8 consecutive rings of a class) meet in that XCD's L2
+// XCD), the 8 slots of an octet next to each other in time, so that the
+// 64-byte lines of the synthesis layout (8 ring pairs of one m, shared by 8
+// consecutive rings of a class) meet in that XCD's L2.  Bluestein classes
+// run octet-major, the maps of one octet back to back, so the octet's chirp
+// spectra and chirps (up to 8 x 96 KB) stay in L2 from one map to the next
+// (FETCH per 32 maps at nside 2048, 8192-point class: 3.7 GB against 9.7
+// map-major, for 2.1 GB of coefficients); power-of-two classes, which have
+// no chirps, run map-major (4.4 GB against 6.7 octet-major, for 4.3 GB)
+template <bool BLUE>
 __device__ __forceinline__ void ring_block_syn(int g, int nmaps, int C, int *slot, int *map) {
     const int xcd = g & 7, k = g >> 3;
-    *map = (k >> 3) / C;
-    *slot = ((xcd * C + (k >> 3) % C) << 3) + (k & 7);
+    if constexpr (BLUE) {
+        *map = (k >> 3) % nmaps;
+        *slot = ((xcd * C + (k >> 3) / nmaps) << 3) + (k & 7);
+    } else {
+        *map = (k >> 3) / C;
+        *slot = ((xcd * C + (k >> 3) % C) << 3) + (k & 7);
+    }
 }
 inline unsigned ring_grid_syn(int ncls, int nmaps) { return (unsigned)(((ncls + 63) / 64) * 64 * nmaps); }
 
@@ -1021,7 +1032,7 @@ fused_ring_synthesis_kernel(const RingSeg *__restrict__ segs, int nseg, RingCall
     const int nmaps = c.maps_tab != nullptr ? ldc(&sg->nmaps) : c.nmaps;
     const int ncls = ldc(&sg->ncls);
     int slot, b;
-    ring_block_syn(blk - ldc(&sg->block0) * c.unit, nmaps, (ncls + 63) >> 6, &slot, &b);
+    ring_block_syn<BLUE>(blk - ldc(&sg->block0) * c.unit, nmaps, (ncls + 63) >> 6, &slot, &b);
     if (slot >= ncls) return;
     TOut *mapb;
     const c2<T> *phb;
