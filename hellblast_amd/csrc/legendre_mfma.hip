@@ -363,9 +363,11 @@ leg_synthesis_h16_kernel(const float2 *__restrict__ alm, int64_t astride, float 
     __shared__ int wg_start;
     int mb, item;
     xcd_map(blockIdx.x, MB * nsb, &mb, &item);
-    // the 8 m of one m-block (sharing 64-byte phase lines) run next to each other
-    const int m = mb * MB + item % MB;
-    const int sb = item / MB;
+    // the ring-pair blocks of one m run next to each other on one XCD, so the
+    // prepared alm chunks and recursion coefficients of that m are read into
+    // its L2 once (each 64-byte line of the synthesis layout belongs to one m)
+    const int m = mb * MB + item / nsb;
+    const int sb = item % nsb;
     if (m > mmax) return;
     const int g0 = blockIdx.y * 16 * NG;            // first map of the workgroup
     const int tid = threadIdx.x, lane = tid & 63;
