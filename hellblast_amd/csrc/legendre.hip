@@ -228,12 +228,17 @@ leg_analysis_kernel(const c2<T> *__restrict__ phase, int64_t pstride, c2<T> *__r
                    px, rofs, alpha, sl, lseed, seed, npairs, lmax, m, grp, adjoint, red, threadIdx.x);
 }
 
+// One workgroup per (m, map); the maps of one m run on one XCD, so the
+// row's s_lm (read by every map) comes into that XCD's L2 once.
 template <typename T>
 __global__ void group_reduce_kernel(const c2<T> *__restrict__ partial, c2<T> *__restrict__ alm, int64_t astride,
                                     const int64_t *__restrict__ rofs, const double *__restrict__ sl, int lmax,
-                                    int ngroups, int64_t nlm, int adjoint, int mbeg,
+                                    int ngroups, int64_t nlm, int adjoint, int mbeg, int nrows, int nmaps,
                                     const int32_t *__restrict__ lfirst) {
-    const int m = mbeg + blockIdx.x, b = blockIdx.y;
+    int mi, b;
+    xcd_map(blockIdx.x, nmaps, &mi, &b);
+    if (mi >= nrows) return;
+    const int m = mbeg + mi;
     const int64_t i0 = alm_index(m, m, lmax);
     reduce_row<T>(partial + (int64_t)b * ngroups * nlm + i0, nlm, ngroups, alm + (int64_t)b * astride + i0,
                   sl + rofs[m], lmax, m, adjoint, lfirst ? lfirst + (int64_t)m * ngroups : nullptr);
@@ -362,15 +367,16 @@ static int leg_analysis_t(const Plan *p, int nmaps, const c2<T> *phase, int64_t 
 int launch_group_reduce(const Plan *p, int nmaps, void *partial, void *alm, int64_t astride, int ngroups,
                         int64_t nlm_stride, int mbeg, int mend, bool f64, int adjoint, hipStream_t s,
                         const int32_t *lfirst) {
-    dim3 grid(mend - mbeg + 1, nmaps);
+    const int nrows = mend - mbeg + 1;
+    dim3 grid((unsigned)((nrows + 7) / 8) * 8 * (unsigned)nmaps);
     if (f64)
         hipLaunchKernelGGL(group_reduce_kernel<double>, grid, dim3(256), 0, s, (const double2 *)partial,
                            (double2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups, nlm_stride, adjoint, mbeg,
-                           lfirst);
+                           nrows, nmaps, lfirst);
     else
         hipLaunchKernelGGL(group_reduce_kernel<float>, grid, dim3(256), 0, s, (const float2 *)partial,
                            (float2 *)alm, astride, p->d_rofs, p->d_sl, p->lmax, ngroups, nlm_stride, adjoint, mbeg,
-                           lfirst);
+                           nrows, nmaps, lfirst);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "group_reduce_kernel launch");
     return 0;
