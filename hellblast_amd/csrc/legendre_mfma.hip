@@ -861,13 +861,17 @@ leg_analysis_h16_kernel(const float2 *__restrict__ phase, int64_t pstride, float
     double *alds = reinterpret_cast<double *>(xch + G::XCH);          // [2][ANA_L]
     int *starts = reinterpret_cast<int *>(alds + 2 * ANA_L);         // [0]: workgroup, [1 + kh]: K-half
 
+    // one-dimensional grid over (m-block, ring-pair group, group of 16 maps,
+    // m % 8): the 8 m of one m-block (sharing 64-byte phase lines) run next
+    // to each other, and the map groups of one (m, ring-pair group) on the
+    // same XCD, so its recursion rows and switch-on seeds are read into L2 once
+    const int ngr = (nmaps + 15) / 16;
     int mb, item;
-    xcd_map(blockIdx.x, MB * ngroups, &mb, &item);
-    // the 8 m of one m-block (sharing 64-byte phase lines) run next to each other
+    xcd_map(blockIdx.x, MB * ngroups * ngr, &mb, &item);
     const int m = mb * MB + item % MB;
-    const int grp = item / MB;
+    const int g = (item / MB) % ngr;                  // group of 16 maps
+    const int grp = item / (MB * ngr);
     if (m > mmax || m < mbeg) return;
-    const int g = blockIdx.y;                         // group of 16 maps
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int t = w & 1, par = (w >> 1) & 1, kh = w >> 2;    // output tile, parity, K-half
@@ -1801,8 +1805,9 @@ int launch_leg_analysis_mfma(const Plan *p, int nmaps, const void *phase, void *
                        p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,     \
                        p->mmax, G, p->nlm, nmaps, adjoint, mbeg)
     if (mfma_kind_of(p) == HBX_MFMA_F16X2) {
+        const dim3 gridh(grid.x * grid.y);            // map groups folded into x (see the kernel)
 #define HBX_AH(QV)                                                                                              \
-    hipLaunchKernelGGL(leg_analysis_h16_kernel<QV>, grid, dim3(H16Geom<QV>::W * 64), H16Geom<QV>::LDS, s,      \
+    hipLaunchKernelGGL(leg_analysis_h16_kernel<QV>, gridh, dim3(H16Geom<QV>::W * 64), H16Geom<QV>::LDS, s,     \
                        (const float2 *)phase, pstride, (float2 *)alm, astride, (float2 *)partial, p->d_px,       \
                        p->d_rofs, p->d_alpha, p->d_sl, p->d_lseed, p->d_seed, p->npairs, p->nrings, p->lmax,     \
                        p->mmax, G, p->nlm, nmaps, adjoint, mbeg, -1.0f, lfirst)
